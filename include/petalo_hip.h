@@ -1,0 +1,210 @@
+/*
+ * petalo_hip.h -- C ABI of the MI355X (gfx950) implementation of petalo's
+ * image-reconstruction hot path: the Siddon projector, its TOF-weighted
+ * adjoint, the MLEM/OSEM multiplicative update and the sensitivity image.
+ *
+ * Conventions
+ *  - Every function returns 0 (PETALO_OK) or a PETALO_ERR_* code; the
+ *    message of the last failure on the calling thread is available from
+ *    petalo_last_error().  Nothing unwinds across this boundary.  Contract
+ *    violations that panic in the reference (wrong image length, mismatched
+ *    sensitivity image) are reported as PETALO_ERR_SIZE.
+ *  - Units are mm and ps (the reference's base units, units/src/lib.rs:36-47).
+ *  - Host pointers in, host pointers out.  The library copies inputs to
+ *    device memory; the caller keeps ownership of its arrays.
+ *  - A context is single-threaded (one HIP stream each); separate contexts
+ *    may be used from separate threads.  Calls that return host data
+ *    synchronise before returning.
+ *  - Image layout: x fastest, i = ix + (iy + iz*ny)*nx (src/index.rs:15-20).
+ */
+#ifndef PETALO_HIP_H
+#define PETALO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PETALO_OK 0
+#define PETALO_ERR_ARG 1     /* invalid argument                       */
+#define PETALO_ERR_HIP 2     /* HIP runtime failure / no GPU            */
+#define PETALO_ERR_NOMEM 3   /* allocation failed                       */
+#define PETALO_ERR_IO 4      /* file read/write failed                  */
+#define PETALO_ERR_PARSE 5   /* config / FOV string could not be parsed */
+#define PETALO_ERR_SIZE 6    /* image size mismatch (reference panics)  */
+
+/* Field of view: full extent and voxel counts.  The library derives
+ * half_width = full/2 and voxel_size = (half_width*2)/n in binary32 exactly
+ * as FOV::new does.  Replaces petalo::FOV (src/fov.rs:13-39). */
+typedef struct {
+  float full_size_mm[3];
+  uint32_t n[3];
+} petalo_fov_t;
+
+/* Line of response, 32 bytes: endpoints (mm), dt = t2 - t1 (ps), and the
+ * multiplicative additive-correction factor.  Replaces petalo::LOR
+ * (src/lor.rs:11-25). */
+typedef struct {
+  float p1[3];
+  float p2[3];
+  float dt_ps;
+  float additive_correction;
+} petalo_lor_t;
+
+/* TOF parameters as in the [tof] config table (src/config/mlem.rs:177-188):
+ * sigma is the time sigma in ps (the kernel width is sigma*C), cutoff in
+ * sigmas (default 3).  Replaces Option<Tof> / Siddon::new (siddon.rs:47). */
+typedef struct {
+  int32_t enabled;
+  float sigma_ps;
+  float cutoff;
+} petalo_tof_t;
+
+/* OSEM progress counter, 1-based (src/mlem.rs:70-96). */
+typedef struct {
+  uint64_t n_subsets;
+  uint64_t iteration;
+  uint64_t subset;
+} petalo_osem_t;
+
+/* Work counters of the last projection pass. */
+typedef struct {
+  uint64_t n_lors;     /* LORs processed                                     */
+  uint64_t n_steps;    /* iterations of the reference walk loop (full chord) */
+  uint64_t n_nz;       /* pushed system-matrix elements (weight > 0)         */
+  uint64_t n_skipped;  /* LORs dropped by the index check (projector.rs:103-118) */
+  uint64_t n_missed;   /* LORs missing the FOV                               */
+  uint64_t n_runaway;  /* walks stopped at the step cap                      */
+} petalo_stats_t;
+
+/* Backprojection schemes for petalo_mlem_options_t.scheme. */
+#define PETALO_SCHEME_AUTO 0     /* autotuned choice                        */
+#define PETALO_SCHEME_ATOMIC 1   /* plain: one lane per LOR, f32 global atomics */
+#define PETALO_SCHEME_SLAB 2     /* LDS z-slab accumulation, FP64 flush     */
+
+typedef struct {
+  int32_t scheme;        /* PETALO_SCHEME_*                                   */
+  int32_t sort_lors;     /* 1: reorder LORs inside each subset for locality   */
+  int32_t device;        /* HIP device ordinal                                */
+  int32_t reserved;
+} petalo_mlem_options_t;
+
+/* ---- library ----------------------------------------------------------- */
+const char* petalo_last_error(void);
+int petalo_version(void);
+/* Number of visible HIP devices (0 without a GPU; never fails for that). */
+int petalo_device_count(int* count);
+/* Blocks until all work on the given device has finished. */
+int petalo_device_synchronize(int device);
+
+/* ---- FOV (src/fov.rs:22-48,75-91) ------------------------------------- */
+int petalo_fov_derive(const petalo_fov_t* fov, float half_width[3], float voxel_size[3]);
+/* Parses '1 mm  2 mm  0.2 cm  100 100 100' (FOV::from_str, src/fov.rs:75-91). */
+int petalo_fov_parse(const char* text, petalo_fov_t* out);
+/* Centre of voxel (ix,iy,iz) (FOV::voxel_centre, src/fov.rs:42-48). */
+int petalo_fov_voxel_centre(const petalo_fov_t* fov, const uint32_t index3[3], float centre[3]);
+
+/* ---- MLEM / OSEM iterator (replaces petalo::mlem::mlem, src/mlem.rs:8-31) */
+typedef struct petalo_mlem petalo_mlem_t;
+
+/* Copies LORs (and the sensitivity image, NULL = ones) to the device and
+ * initialises the image to ones (mlem.rs:17-19).  options may be NULL. */
+int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petalo_tof_t* tof_or_null,
+                       const petalo_lor_t* lors, size_t n_lors, const float* sensitivity_or_null,
+                       size_t n_subsets, const petalo_mlem_options_t* options);
+/* One subset update = one Iterator::next (mlem.rs:25-30).  id_out receives
+ * the Osem counter *before* it advances; image_out (N_vox floats, may be
+ * NULL) receives the updated image. */
+int petalo_mlem_next(petalo_mlem_t* ctx, float* image_out_or_null, petalo_osem_t* id_out);
+/* Copies the current image to the host. */
+int petalo_mlem_get_image(petalo_mlem_t* ctx, float* image_out);
+/* Replaces the current image (for resuming from a written subset image). */
+int petalo_mlem_set_image(petalo_mlem_t* ctx, const float* image);
+/* Exact walk statistics of the LOR set (full reference walk per LOR), for
+ * subset `subset` (1-based; 0 = all LORs). */
+int petalo_mlem_walk_stats(petalo_mlem_t* ctx, size_t subset, petalo_stats_t* out);
+/* LORs skipped by the index check during the last update. */
+int petalo_mlem_last_skipped(petalo_mlem_t* ctx, uint64_t* skipped);
+/* Per-kernel device time of the last update in ms, measured with HIP events
+ * on the context's stream when timing is on. */
+int petalo_mlem_set_timing(petalo_mlem_t* ctx, int on);
+int petalo_mlem_last_timing(petalo_mlem_t* ctx, double* project_ms, double* update_ms);
+int petalo_mlem_destroy(petalo_mlem_t* ctx);
+
+/* ---- Sensitivity image (continuous.rs:3-47, main.rs:77) ---------------
+ * Backprojects n_lors LORs through the attenuation image mu [1/mm] on
+ * att_fov, adapting each projection p to exp(-p) (projector.rs:72-74), into
+ * out_fov (NULL = att_fov), normalised by n_lors.  With lors == NULL the
+ * LORs are generated on the device: both ends uniform on the detector
+ * cylinder (length, diameter), rejected until they enter att_fov, from a
+ * Philox4x32-10 stream keyed by seed and counted by LOR index.
+ * Accumulation is in FP64. */
+int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu,
+                                  const petalo_fov_t* out_fov_or_null, const petalo_lor_t* lors_or_null,
+                                  size_t n_lors, uint64_t seed, float detector_length_mm,
+                                  float detector_diameter_mm, float* out, petalo_stats_t* stats_or_null);
+/* The LORs petalo_make_sensitivity_image generates for (seed, fov, cylinder). */
+int petalo_sensitivity_lors(const petalo_fov_t* att_fov, size_t n_lors, uint64_t seed,
+                            float detector_length_mm, float detector_diameter_mm, petalo_lor_t* out);
+/* density [kg/m^3] -> mu [1/mm] in place, rho_to_mu in cm^2/g
+ * (density_image_into_attenuation_image, main.rs:11-17,80-94). */
+int petalo_density_to_attenuation(float* image, size_t n_voxels, float rho_to_mu_cm2_per_g);
+
+/* ---- System-matrix rows (Siddon::new_system_matrix_row, siddon.rs:69-73)
+ * For each LOR i: up to cap elements at idx/w[i*cap ...], count[i] = number
+ * of pushed elements (may exceed cap: then only cap are written), steps[i] =
+ * walk-loop iterations (UINT32_MAX for a walk stopped at the step cap). */
+int petalo_system_matrix_rows(const petalo_fov_t* fov, const petalo_tof_t* tof_or_null,
+                              const petalo_lor_t* lors, size_t n_lors, size_t cap, int32_t* idx,
+                              float* w, uint32_t* count, uint32_t* steps);
+
+/* ---- Image3D raw format (src/io/raw.rs:78-126) -------------------------
+ * Big-endian: u16 n[3], f32 full_size_mm[3], then f32 data. */
+int petalo_image3d_write(const char* path, const petalo_fov_t* fov, const float* data);
+int petalo_image3d_read_header(const char* path, petalo_fov_t* fov_out);
+int petalo_image3d_read(const char* path, petalo_fov_t* fov_out, float* data, size_t capacity);
+
+/* ---- MLEM config TOML (src/config/mlem.rs:73-253) ----------------------
+ * Supported tables: [input] (file, dataset, energy/charge/events bounds),
+ * [iterations], [fov], [tof], [attenuation_correction]; [scatter_correction],
+ * [smear_energy] and [detector_full_axial_length] are parsed and checked but
+ * not used on this path.  Unknown keys are rejected (deny_unknown_fields). */
+typedef struct {
+  char input_file[512];
+  char input_dataset[256];
+  uint64_t iterations_number;
+  uint64_t iterations_subsets;
+  petalo_fov_t fov;
+  int32_t has_tof;
+  petalo_tof_t tof;
+  int32_t has_attenuation_correction;
+  char sensitivity_image[512];
+  int32_t has_energy_min, has_energy_max, has_charge_min, has_charge_max, has_events_min, has_events_max;
+  float energy_min, energy_max, charge_min, charge_max;
+  uint64_t events_min, events_max;
+  int32_t has_scatter_correction;
+} petalo_config_t;
+
+int petalo_config_parse(const char* toml_text, petalo_config_t* out);
+int petalo_config_read(const char* path, petalo_config_t* out);
+/* Parses a quantity string such as "200 ps", "3 cm", "1.5 mm" into base
+ * units (mm or ps); kind: 'L' length, 'T' time. */
+int petalo_parse_quantity(const char* text, char kind, float* value);
+
+/* ---- Synthetic workloads (seeded, Philox4x32-10 counter-based) ---------
+ * Water cylinder (r 80 mm, |z| <= 75 mm, activity 1) with six hot spheres
+ * (activity 4, radii 5..15 mm on a 50 mm ring), isotropic back-to-back
+ * emission, both ends on the detector cylinder (r 350 mm, |z| <= 150 mm);
+ * TOF dt with Gaussian timing noise of the given FWHM (0 = exact dt).
+ * Deterministic for a given (seed, n) regardless of thread count. */
+int petalo_synth_phantom_lors(uint64_t seed, size_t n_lors, float tof_fwhm_ps, petalo_lor_t* out);
+/* Density map [kg/m^3] for the attenuation workload: water cylinder
+ * (1000) with a bone insert (1791.67) of r 15 mm at (-40, 0), |z| <= 75. */
+int petalo_synth_density(const petalo_fov_t* fov, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

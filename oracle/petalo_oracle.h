@@ -1,0 +1,142 @@
+/*
+ * petalo_oracle.h -- CPU oracle for the petalo reconstruction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * implementation in petalo_amd/ is compared against.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product never links or calls it.
+ *
+ * It restates, operation for operation in IEEE binary32 (and again in
+ * binary64 as the "FP64 oracle"), the Rust reference:
+ *   walk/setup      src/projectors/siddon.rs:81-340
+ *   FOV             src/fov.rs:22-71   (ray/box entry per ncollide3d 0.33.0)
+ *   norm/normalize  geometry/src/vector.rs:160-167 (nalgebra 0.30.1)
+ *   TOF Gaussian    src/gauss.rs:31-60
+ *   projection      src/projector.rs:33-153
+ *   MLEM update     src/mlem.rs:33-66
+ *   sensitivity     src/bin/make_sensitivity_image/continuous.rs:3-17,
+ *                   main.rs:77-94
+ * plus an independent classic Siddon (parametric alpha merge, Siddon 1985)
+ * and a brute-force per-voxel clipper, both in binary64, used only to pin
+ * the restatement against the mathematics.
+ */
+#ifndef PETALO_ORACLE_H
+#define PETALO_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 32-byte LOR record: endpoints in mm, dt = t2 - t1 in ps (src/lor.rs:11-25). */
+typedef struct {
+  float p1[3];
+  float p2[3];
+  float dt_ps;
+  float additive_correction;
+} orc_lor_t;
+
+/* TOF parameters as in the config (src/config/mlem.rs:177-188). */
+typedef struct {
+  float sigma_ps;
+  float cutoff;
+} orc_tof_t;
+
+/* Work counters for one projection pass. */
+typedef struct {
+  uint64_t n_lors;       /* LORs visited                                  */
+  uint64_t n_steps;      /* iterations of the walk loop (siddon.rs:96-129) */
+  uint64_t n_nz;         /* pushed row elements (weight > 0)              */
+  uint64_t n_skipped;    /* LORs dropped by the index check (projector.rs:103-118) */
+  uint64_t n_missed;     /* LORs that miss the FOV (empty row)            */
+  uint64_t n_runaway;    /* walks stopped at the step cap (reference would not terminate) */
+} orc_stats_t;
+
+/* FOV derivation, binary32 (src/fov.rs:22-39). */
+void orc_fov_derive_f32(const float full_mm[3], const uint32_t n[3],
+                        float half_width[3], float voxel_size[3]);
+
+/* FOV::entry (src/fov.rs:55-71); returns 1 and the entry point on a hit. */
+int orc_fov_entry_f32(const float full_mm[3], const uint32_t n[3],
+                      const float p1[3], const float p2[3], float entry[3]);
+
+/* TOF Gaussian g(dx) in 1/mm (src/gauss.rs:31-60). */
+float orc_gauss_f32(float sigma_ps, float cutoff, float dx_mm);
+double orc_gauss_f64(double sigma_ps, double cutoff, double dx_mm);
+
+/* One system-matrix row (siddon.rs:50-130).  tof may be NULL.  Writes at most
+ * cap elements; returns the number of pushed elements (may exceed cap, in
+ * which case only cap were written).  *steps gets the walk-loop count. */
+int64_t orc_row_f32(const float full_mm[3], const uint32_t n[3],
+                    const orc_tof_t* tof, const orc_lor_t* lor,
+                    int64_t* idx, float* w, int64_t cap, uint64_t* steps);
+int64_t orc_row_f64(const float full_mm[3], const uint32_t n[3],
+                    const orc_tof_t* tof, const orc_lor_t* lor,
+                    int64_t* idx, double* w, int64_t cap, uint64_t* steps);
+
+/* Independent classic Siddon in binary64: returns the number of segments
+ * (voxel index in the reference's 1-D layout, and length in mm), in
+ * traversal order p1 -> p2, zero-length segments dropped. */
+int64_t orc_classic_siddon_f64(const double full_mm[3], const uint32_t n[3],
+                               const double p1[3], const double p2[3],
+                               int64_t* idx, double* len, int64_t cap);
+
+/* Brute force: length of segment [p1,p2] inside every voxel (binary64),
+ * written to len_per_voxel[N_vox]. */
+void orc_bruteforce_lengths_f64(const double full_mm[3], const uint32_t n[3],
+                                const double p1[3], const double p2[3],
+                                double* len_per_voxel);
+
+/* Chord length of the segment [p1,p2] inside the FOV box (binary64 slab). */
+double orc_chord_f64(const double full_mm[3], const double p1[3], const double p2[3]);
+
+/* One MLEM/OSEM subset update over lors[0..n_lors) (src/mlem.rs:33-52).
+ * sensitivity may be NULL (ones).  image is updated in place.  Work is cut
+ * into 10 000-LOR jobs spread over n_threads threads, each folding into a
+ * private backprojection that is then summed in thread order
+ * (src/mlem.rs:57-66, src/projector.rs:48-62). */
+int orc_mlem_update_f32(const float full_mm[3], const uint32_t n[3],
+                        const orc_tof_t* tof, const orc_lor_t* lors, size_t n_lors,
+                        const float* sensitivity, float* image, int n_threads,
+                        orc_stats_t* stats);
+int orc_mlem_update_f64(const float full_mm[3], const uint32_t n[3],
+                        const orc_tof_t* tof, const orc_lor_t* lors, size_t n_lors,
+                        const double* sensitivity, double* image, int n_threads,
+                        orc_stats_t* stats);
+
+/* Backprojection only (no update): bp[j] = sum over LORs of w_ij / (p_i * ac_i)
+ * with p_i the forward projection of image. */
+int orc_backproject_mlem_f32(const float full_mm[3], const uint32_t n[3],
+                             const orc_tof_t* tof, const orc_lor_t* lors, size_t n_lors,
+                             const float* image, float* bp, int n_threads,
+                             orc_stats_t* stats);
+
+/* Sensitivity image (continuous.rs:3-17 with project_one_lor_sens,
+ * projector.rs:72-74): forward walk on the attenuation FOV, backward walk on
+ * out FOV (NULL = same), normalised by n_lors (main.rs:77). */
+int orc_sensitivity_f32(const float att_full_mm[3], const uint32_t att_n[3],
+                        const float* mu,
+                        const float* out_full_mm, const uint32_t* out_n,
+                        const orc_lor_t* lors, size_t n_lors, int n_threads,
+                        float* out, orc_stats_t* stats);
+int orc_sensitivity_f64(const float att_full_mm[3], const uint32_t att_n[3],
+                        const float* mu,
+                        const float* out_full_mm, const uint32_t* out_n,
+                        const orc_lor_t* lors, size_t n_lors, int n_threads,
+                        double* out, orc_stats_t* stats);
+
+/* density [kg/m^3] -> mu [1/mm] factor, binary32, as
+ * src/bin/make_sensitivity_image/main.rs:11-17,80-94 compute it. */
+float orc_rho_to_mu_factor_f32(float rho_to_mu_cm2_per_g);
+
+/* Walk statistics only (no image): full reference walk step counts. */
+int orc_walk_stats_f32(const float full_mm[3], const uint32_t n[3],
+                       const orc_tof_t* tof, const orc_lor_t* lors, size_t n_lors,
+                       int n_threads, orc_stats_t* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

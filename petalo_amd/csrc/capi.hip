@@ -1,0 +1,537 @@
+// capi.hip -- C ABI of the MI355X reconstruction hot path (include/petalo_hip.h).
+//
+// Device work: the kernels in kernels.hpp.  Host work here: argument checks,
+// device memory, the OSEM schedule (src/mlem.rs:70-96) and the LOR ordering
+// applied inside each subset.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/petalo_hip.h"
+#include "capi_internal.hpp"
+#include "kernels.hpp"
+
+using namespace petalo;
+
+namespace petalo {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+}  // namespace petalo
+
+namespace {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define HIP_CHECK(expr)                                                                              \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess) throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+template <class F> int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const HipError& e) {
+    return fail(PETALO_ERR_HIP, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(PETALO_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(PETALO_ERR_ARG, e.what());
+  } catch (...) {
+    return fail(PETALO_ERR_ARG, "unknown error");
+  }
+}
+
+bool fov_ok(const petalo_fov_t* f, std::string& why) {
+  if (!f) { why = "fov is NULL"; return false; }
+  for (int i = 0; i < 3; ++i) {
+    if (f->n[i] == 0) { why = "fov has zero voxels along an axis"; return false; }
+    if (!(f->full_size_mm[i] > 0.0f) || !std::isfinite(f->full_size_mm[i])) {
+      why = "fov size must be positive and finite";
+      return false;
+    }
+  }
+  const uint64_t nv = (uint64_t)f->n[0] * f->n[1] * f->n[2];
+  if (nv >= (1ull << 31)) { why = "fov has 2^31 or more voxels"; return false; }
+  return true;
+}
+
+DevFov to_dev(const petalo_fov_t& f) { return make_fov(f.full_size_mm, f.n); }
+
+uint64_t nvox(const petalo_fov_t& f) { return (uint64_t)f.n[0] * f.n[1] * f.n[2]; }
+
+DevTof to_dev(const petalo_tof_t* t) {
+  if (!t || !t->enabled) return DevTof{1.0f, INFINITY, 0.0f, 0};
+  return make_tof(true, t->sigma_ps, t->cutoff);
+}
+
+constexpr int kBlock = 256;
+inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <class T> struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+void require_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw HipError("no HIP device available");
+}
+
+// Ordering key of a LOR: azimuth of its direction (folded to [0, pi)), then
+// signed transaxial distance from the axis, then axial midpoint.  LORs close
+// in this order are near-parallel neighbours: their walks cross the same
+// voxels, so a wavefront's gathers and atomics share cache lines and its
+// lanes take similar numbers of steps.
+uint64_t lor_order_key(const petalo_lor_t& l) {
+  const double dx = (double)l.p2[0] - l.p1[0], dy = (double)l.p2[1] - l.p1[1];
+  double phi = std::atan2(dy, dx);
+  double ux = dx, uy = dy;
+  if (phi < 0) { phi += M_PI; ux = -ux; uy = -uy; }
+  if (phi >= M_PI) phi -= M_PI;
+  const double ln = std::sqrt(ux * ux + uy * uy);
+  double s = 0.0;
+  if (ln > 0) s = ((double)l.p1[0] * uy - (double)l.p1[1] * ux) / ln;  // signed distance
+  const double zm = 0.5 * ((double)l.p1[2] + l.p2[2]);
+  auto q = [](double v, double lo, double hi, int bits) -> uint64_t {
+    double t = (v - lo) / (hi - lo);
+    t = std::min(std::max(t, 0.0), 1.0 - 1e-12);
+    return (uint64_t)(t * (double)(1ull << bits));
+  };
+  return (q(phi, 0.0, M_PI, 10) << 40) | (q(s, -400.0, 400.0, 20) << 20) | q(zm, -400.0, 400.0, 20);
+}
+
+void order_lors(std::vector<petalo_lor_t>& lors, size_t lo, size_t hi) {
+  const size_t m = hi - lo;
+  if (m < 2) return;
+  std::vector<std::pair<uint64_t, uint32_t>> key(m);
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        for (size_t k = t; k < m; k += nt) key[k] = {lor_order_key(lors[lo + k]), (uint32_t)k};
+      });
+    for (auto& x : th) x.join();
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<petalo_lor_t> tmp(lors.begin() + lo, lors.begin() + hi);
+  for (size_t k = 0; k < m; ++k) lors[lo + k] = tmp[key[k].second];
+}
+
+}  // namespace
+
+// ============================================================================
+struct petalo_mlem {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  petalo_fov_t fov{};
+  DevFov dfov{};
+  DevTof dtof{};
+  bool tof_on = false;
+  size_t n_lors = 0;
+  size_t n_subsets = 1;
+  size_t set_size = 0;
+  int scheme = PETALO_SCHEME_ATOMIC;
+  DevBuf<Lor> lors;
+  DevBuf<float> image, bp, sens;
+  DevBuf<unsigned long long> counters;
+  petalo_osem_t osem{};
+  bool timing = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  double t_project = 0, t_update = 0;
+  bool have_timing = false;
+};
+
+namespace {
+
+void launch_project(petalo_mlem* c, size_t lo, size_t count) {
+  if (count == 0) return;
+  const Lor* L = c->lors.p + lo;
+  const unsigned g = grid_for(count);
+  if (c->tof_on)
+    k_mlem_project<true><<<g, kBlock, 0, c->stream>>>(L, (uint32_t)count, c->dfov, c->dtof, c->image.p, c->bp.p,
+                                                       c->counters.p);
+  else
+    k_mlem_project<false><<<g, kBlock, 0, c->stream>>>(L, (uint32_t)count, c->dfov, c->dtof, c->image.p, c->bp.p,
+                                                        c->counters.p);
+  HIP_CHECK(hipGetLastError());
+}
+
+void fill_f32(float* p, float v, size_t n, hipStream_t s) {
+  if (!n) return;
+  k_fill_f32<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void fill_f64(double* p, double v, size_t n, hipStream_t s) {
+  if (!n) return;
+  k_fill_f64<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+void zero_counters(unsigned long long* p, hipStream_t s) {
+  k_zero_u64<<<1, 64, 0, s>>>(p, CNT_N);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* petalo_last_error(void) { return petalo::g_last_error.c_str(); }
+
+int petalo_version(void) { return 100; }
+
+int petalo_device_count(int* count) {
+  if (!count) return fail(PETALO_ERR_ARG, "count is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return PETALO_OK;
+}
+
+int petalo_device_synchronize(int device) {
+  return guarded([&] {
+    require_device();
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipDeviceSynchronize());
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petalo_tof_t* tof,
+                       const petalo_lor_t* lors, size_t n_lors, const float* sensitivity, size_t n_subsets,
+                       const petalo_mlem_options_t* options) {
+  return guarded([&] {
+    if (!out) return fail(PETALO_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    std::string why;
+    if (!fov_ok(fov, why)) return fail(PETALO_ERR_ARG, why);
+    if (n_subsets == 0) return fail(PETALO_ERR_ARG, "n_subsets must be at least 1");
+    if (n_lors && !lors) return fail(PETALO_ERR_ARG, "lors is NULL");
+    if (n_lors >= (1ull << 32)) return fail(PETALO_ERR_ARG, "more than 2^32-1 LORs");
+    if (tof && tof->enabled && !(tof->sigma_ps > 0.0f)) return fail(PETALO_ERR_ARG, "TOF sigma must be positive");
+    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, 0};
+    if (options) opt = *options;
+    if (opt.scheme != PETALO_SCHEME_AUTO && opt.scheme != PETALO_SCHEME_ATOMIC)
+      return fail(PETALO_ERR_ARG, "unsupported backprojection scheme");
+    require_device();
+    HIP_CHECK(hipSetDevice(opt.device));
+
+    auto* c = new petalo_mlem();
+    try {
+      c->device = opt.device;
+      c->fov = *fov;
+      c->dfov = to_dev(*fov);
+      c->dtof = to_dev(tof);
+      c->tof_on = tof && tof->enabled;
+      c->n_lors = n_lors;
+      c->n_subsets = n_subsets;
+      c->set_size = n_lors / n_subsets;  // remainder LORs ignored (mlem.rs:91)
+      c->scheme = PETALO_SCHEME_ATOMIC;
+      c->osem = petalo_osem_t{n_subsets, 1, 1};
+      HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      const size_t nv = nvox(*fov);
+
+      // LORs: reorder inside each subset (subset membership is unchanged).
+      std::vector<petalo_lor_t> h(lors, lors + n_lors);
+      if (opt.sort_lors)
+        for (size_t s = 0; s < n_subsets; ++s) order_lors(h, s * c->set_size, (s + 1) * c->set_size);
+      c->lors.alloc(std::max<size_t>(n_lors, 1));
+      if (n_lors) HIP_CHECK(hipMemcpy(c->lors.p, h.data(), n_lors * sizeof(Lor), hipMemcpyHostToDevice));
+
+      c->image.alloc(nv);
+      c->bp.alloc(nv);
+      c->counters.alloc(CNT_N);
+      fill_f32(c->image.p, 1.0f, nv, c->stream);  // Image::ones (mlem.rs:17)
+      fill_f32(c->bp.p, 0.0f, nv, c->stream);
+      zero_counters(c->counters.p, c->stream);
+      if (sensitivity) {
+        c->sens.alloc(nv);
+        HIP_CHECK(hipMemcpy(c->sens.p, sensitivity, nv * sizeof(float), hipMemcpyHostToDevice));
+      }
+      for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      petalo_mlem_destroy(c);
+      throw;
+    }
+    *out = c;
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_next(petalo_mlem_t* c, float* image_out, petalo_osem_t* id_out) {
+  return guarded([&] {
+    if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
+    HIP_CHECK(hipSetDevice(c->device));
+    const size_t nv = nvox(c->fov);
+    const size_t lo = (c->osem.subset - 1) * c->set_size;  // Osem::subset (mlem.rs:90-95)
+    zero_counters(c->counters.p, c->stream);
+    if (c->timing) HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
+    launch_project(c, lo, c->set_size);
+    if (c->timing) HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
+    k_update<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp.p, c->sens.p, (uint32_t)nv);
+    HIP_CHECK(hipGetLastError());
+    if (c->timing) {
+      HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
+      c->have_timing = true;
+    }
+    if (id_out) *id_out = c->osem;
+    // Osem::advance (mlem.rs:82-88)
+    c->osem.subset += 1;
+    if (c->osem.subset > c->osem.n_subsets) {
+      c->osem.subset = 1;
+      c->osem.iteration += 1;
+    }
+    if (image_out) {
+      HIP_CHECK(hipMemcpyAsync(image_out, c->image.p, nv * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_get_image(petalo_mlem_t* c, float* out) {
+  return guarded([&] {
+    if (!c || !out) return fail(PETALO_ERR_ARG, "NULL argument");
+    HIP_CHECK(hipSetDevice(c->device));
+    HIP_CHECK(hipMemcpyAsync(out, c->image.p, nvox(c->fov) * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_set_image(petalo_mlem_t* c, const float* img) {
+  return guarded([&] {
+    if (!c || !img) return fail(PETALO_ERR_ARG, "NULL argument");
+    HIP_CHECK(hipSetDevice(c->device));
+    HIP_CHECK(hipMemcpyAsync(c->image.p, img, nvox(c->fov) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_walk_stats(petalo_mlem_t* c, size_t subset, petalo_stats_t* out) {
+  return guarded([&] {
+    if (!c || !out) return fail(PETALO_ERR_ARG, "NULL argument");
+    if (subset > c->n_subsets) return fail(PETALO_ERR_ARG, "subset out of range");
+    HIP_CHECK(hipSetDevice(c->device));
+    DevBuf<unsigned long long> cnt(CNT_N);
+    zero_counters(cnt.p, c->stream);
+    const size_t lo = subset ? (subset - 1) * c->set_size : 0;
+    const size_t m = subset ? c->set_size : c->n_lors;
+    if (m) {
+      if (c->tof_on)
+        k_walk_stats<true><<<grid_for(m), kBlock, 0, c->stream>>>(c->lors.p + lo, (uint32_t)m, c->dfov, c->dtof,
+                                                                  cnt.p);
+      else
+        k_walk_stats<false><<<grid_for(m), kBlock, 0, c->stream>>>(c->lors.p + lo, (uint32_t)m, c->dfov, c->dtof,
+                                                                   cnt.p);
+      HIP_CHECK(hipGetLastError());
+    }
+    unsigned long long h[CNT_N];
+    HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    *out = petalo_stats_t{h[CNT_LORS], h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED], h[CNT_RUNAWAY]};
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_last_skipped(petalo_mlem_t* c, uint64_t* skipped) {
+  return guarded([&] {
+    if (!c || !skipped) return fail(PETALO_ERR_ARG, "NULL argument");
+    HIP_CHECK(hipSetDevice(c->device));
+    unsigned long long h[CNT_N];
+    HIP_CHECK(hipMemcpyAsync(h, c->counters.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    *skipped = h[CNT_SKIPPED];
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_set_timing(petalo_mlem_t* c, int on) {
+  if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
+  c->timing = on != 0;
+  c->have_timing = false;
+  return PETALO_OK;
+}
+
+int petalo_mlem_last_timing(petalo_mlem_t* c, double* project_ms, double* update_ms) {
+  return guarded([&] {
+    if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
+    if (!c->have_timing) return fail(PETALO_ERR_ARG, "timing is off or no update has run");
+    HIP_CHECK(hipSetDevice(c->device));
+    HIP_CHECK(hipEventSynchronize(c->ev[2]));
+    float a = 0, b = 0;
+    HIP_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    if (project_ms) *project_ms = a;
+    if (update_ms) *update_ms = b;
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_destroy(petalo_mlem_t* c) {
+  if (!c) return PETALO_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  c->lors.release();
+  c->image.release();
+  c->bp.release();
+  c->sens.release();
+  c->counters.release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PETALO_OK;
+}
+
+// ---------------------------------------------------------------------------
+int petalo_system_matrix_rows(const petalo_fov_t* fov, const petalo_tof_t* tof, const petalo_lor_t* lors,
+                              size_t n_lors, size_t cap, int32_t* idx, float* w, uint32_t* count,
+                              uint32_t* steps) {
+  return guarded([&] {
+    std::string why;
+    if (!fov_ok(fov, why)) return fail(PETALO_ERR_ARG, why);
+    if (n_lors && (!lors || !count || !steps || (cap && (!idx || !w)))) return fail(PETALO_ERR_ARG, "NULL argument");
+    if (n_lors >= (1ull << 32) || cap >= (1ull << 32)) return fail(PETALO_ERR_ARG, "too many LORs or too large cap");
+    if (n_lors == 0) return PETALO_OK;
+    require_device();
+    const DevFov f = to_dev(*fov);
+    const DevTof t = to_dev(tof);
+    DevBuf<Lor> dl(n_lors);
+    DevBuf<int32_t> di(std::max<size_t>(n_lors * cap, 1));
+    DevBuf<float> dw(std::max<size_t>(n_lors * cap, 1));
+    DevBuf<uint32_t> dc(n_lors), ds(n_lors);
+    HIP_CHECK(hipMemcpy(dl.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice));
+    if (t.on)
+      k_rows<true><<<grid_for(n_lors), kBlock>>>(dl.p, (uint32_t)n_lors, f, t, (uint32_t)cap, di.p, dw.p, dc.p, ds.p);
+    else
+      k_rows<false><<<grid_for(n_lors), kBlock>>>(dl.p, (uint32_t)n_lors, f, t, (uint32_t)cap, di.p, dw.p, dc.p, ds.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    if (cap) {
+      HIP_CHECK(hipMemcpy(idx, di.p, n_lors * cap * sizeof(int32_t), hipMemcpyDeviceToHost));
+      HIP_CHECK(hipMemcpy(w, dw.p, n_lors * cap * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    HIP_CHECK(hipMemcpy(count, dc.p, n_lors * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(steps, ds.p, n_lors * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return PETALO_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------
+int petalo_sensitivity_lors(const petalo_fov_t* fov, size_t n_lors, uint64_t seed, float L, float D,
+                            petalo_lor_t* out) {
+  return guarded([&] {
+    std::string why;
+    if (!fov_ok(fov, why)) return fail(PETALO_ERR_ARG, why);
+    if (n_lors && !out) return fail(PETALO_ERR_ARG, "out is NULL");
+    if (n_lors >= (1ull << 32)) return fail(PETALO_ERR_ARG, "more than 2^32-1 LORs");
+    if (!(L > 0.0f) || !(D > 0.0f)) return fail(PETALO_ERR_ARG, "detector length and diameter must be positive");
+    if (n_lors == 0) return PETALO_OK;
+    require_device();
+    DevBuf<Lor> d(n_lors);
+    k_sens_lors<<<grid_for(n_lors), kBlock>>>((uint32_t)n_lors, seed, to_dev(*fov), L, D / 2.0f, d.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(out, d.p, n_lors * sizeof(Lor), hipMemcpyDeviceToHost));
+    return PETALO_OK;
+  });
+}
+
+int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, const petalo_fov_t* out_fov,
+                                  const petalo_lor_t* lors, size_t n_lors, uint64_t seed, float L, float D,
+                                  float* out, petalo_stats_t* stats) {
+  return guarded([&] {
+    std::string why;
+    if (!fov_ok(att_fov, why)) return fail(PETALO_ERR_ARG, why);
+    if (out_fov && !fov_ok(out_fov, why)) return fail(PETALO_ERR_ARG, why);
+    if (!mu || !out) return fail(PETALO_ERR_ARG, "NULL argument");
+    if (n_lors >= (1ull << 32)) return fail(PETALO_ERR_ARG, "more than 2^32-1 LORs");
+    if (!lors && (!(L > 0.0f) || !(D > 0.0f)))
+      return fail(PETALO_ERR_ARG, "detector length and diameter must be positive");
+    require_device();
+    const DevFov fa = to_dev(*att_fov);
+    const DevFov fo = to_dev(out_fov ? *out_fov : *att_fov);
+    const size_t na = nvox(*att_fov), no = nvox(out_fov ? *out_fov : *att_fov);
+    hipStream_t s;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    DevBuf<float> dmu(na), dout(no);
+    DevBuf<double> acc(no);
+    DevBuf<unsigned long long> cnt(CNT_N);
+    DevBuf<Lor> dl;
+    HIP_CHECK(hipMemcpyAsync(dmu.p, mu, na * sizeof(float), hipMemcpyHostToDevice, s));
+    if (lors && n_lors) {
+      dl.alloc(n_lors);
+      HIP_CHECK(hipMemcpyAsync(dl.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice, s));
+    }
+    fill_f64(acc.p, 0.0, no, s);
+    zero_counters(cnt.p, s);
+    const bool two = out_fov != nullptr;
+    if (n_lors) {
+      const unsigned g = grid_for(n_lors);
+      const uint32_t n = (uint32_t)n_lors;
+      const float R = D / 2.0f;
+      if (lors) {
+        if (two) k_sens<false, true><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
+        else     k_sens<false, false><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
+      } else {
+        if (two) k_sens<true, true><<<g, kBlock, 0, s>>>(nullptr, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
+        else     k_sens<true, false><<<g, kBlock, 0, s>>>(nullptr, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
+      }
+      HIP_CHECK(hipGetLastError());
+    }
+    k_sens_finalize<<<grid_for(no), kBlock, 0, s>>>(acc.p, dout.p, (float)n_lors, no);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(out, dout.p, no * sizeof(float), hipMemcpyDeviceToHost, s));
+    unsigned long long h[CNT_N];
+    HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (stats)
+      *stats = petalo_stats_t{h[CNT_LORS], h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED], h[CNT_RUNAWAY]};
+    return PETALO_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,347 @@
+// kernels.hpp -- HIP kernels of the reconstruction hot path (gfx950).
+//
+// Layout in HBM
+//   LORs          32-byte records (Lor), one lane per LOR, read once per
+//                 launch as two 16-byte loads: fully coalesced.
+//   image         binary32, x fastest (src/index.rs:15-20).
+//   backprojection binary32 (MLEM) or binary64 (sensitivity) accumulators of
+//                 the same layout, reset by the update kernel.
+//
+// The MLEM projection is one fused kernel per subset update:
+//   walk #1   full reference walk from the FOV entry (bit-exact voxel
+//             sequence), gathers image values for pushed elements and forms
+//             p = sum w*x in walk order (projector.rs:135-141).  With TOF the
+//             walk stops at the first step whose entry distance is already
+//             past the Gaussian cutoff: every later weight is exactly 0.
+//   walk #2   restarts from the state saved just before the first pushed
+//             element and replays exactly the steps of walk #1, adding
+//             w * (1/(p*ac)) to the backprojection (projector.rs:144-149).
+// A LOR with a pushed index outside the image is dropped whole
+// (projector.rs:103-118).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.hpp"
+#include "siddon_walk.hpp"
+
+namespace petalo {
+
+// Counter slots.
+enum : int { CNT_LORS = 0, CNT_STEPS, CNT_NZ, CNT_SKIPPED, CNT_MISSED, CNT_RUNAWAY, CNT_N };
+
+// Sum a per-lane value over the wavefront; lane 0 adds it to *dst.  Every
+// lane of the wavefront must call it.
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
+  unsigned long long s = v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(dst, s);
+}
+
+__device__ __forceinline__ Lor load_lor(const Lor* __restrict__ lors, uint32_t i) {
+  const float4* p = reinterpret_cast<const float4*>(lors + i);
+  const float4 a = p[0];
+  const float4 b = p[1];
+  return Lor{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+
+// Weight of the step about to be taken (siddon.rs:99-111).
+template <bool TOF>
+__device__ __forceinline__ float step_weight(const Walk& w, const Step& s, const DevTof& t) {
+  float wt = s.bnd - w.here;
+  if (TOF) wt *= tof_factor(t, w.here, w.tof_peak);
+  return wt;
+}
+
+// ---------------------------------------------------------------------------
+// Plain MLEM projection: fused forward + back projection, f32 atomics.
+template <bool TOF>
+__global__ __launch_bounds__(256) void k_mlem_project(const Lor* __restrict__ lors, uint32_t n, DevFov f,
+                                                      DevTof t, const float* __restrict__ img,
+                                                      float* __restrict__ bp,
+                                                      unsigned long long* __restrict__ counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t skipped = 0;
+  if (i < n) {
+    const Lor l = load_lor(lors, i);
+    Walk w;
+    if (walk_setup(l, f, w)) {
+      // With every remaining count positive the walk provably ends within
+      // nx+ny+nz-2 steps; otherwise it is walked in full against the cap.
+      const bool bounded = w.r_x > 0 && w.r_y > 0 && w.r_z > 0;
+      const uint32_t cap = step_cap(f);
+      Walk start = w;
+      uint32_t steps = 0, start_step = 0;
+      bool started = false, bad = false;
+      float p = 0.0f;
+      for (;;) {
+        if (!bounded && steps >= cap) { bad = true; break; }
+        if (TOF && bounded && (w.here - w.tof_peak) >= t.cutoff) break;
+        const Step s = next_boundary(w);
+        const float wt = step_weight<TOF>(w, s, t);
+        if (wt > 0.0f) {
+          if (!started) { start = w; start_step = steps; started = true; }
+          if ((uint32_t)w.index >= (uint32_t)f.nvox) { bad = true; break; }
+          p += wt * img[w.index];
+        }
+        ++steps;
+        if (advance(w, s)) break;
+      }
+      if (bad) {
+        skipped = 1;
+      } else if (started) {
+        const float r = 1.0f / (p * l.ac);
+        w = start;
+        for (uint32_t k = start_step; k < steps; ++k) {
+          const Step s = next_boundary(w);
+          const float wt = step_weight<TOF>(w, s, t);
+          if (wt > 0.0f) unsafeAtomicAdd(bp + w.index, wt * r);
+          advance(w, s);
+        }
+      }
+    }
+  }
+  wave_add_u64(counters + CNT_SKIPPED, skipped);
+}
+
+// MLEM update (src/mlem.rs:45-52) fused with the reset of the
+// backprojection: x = s > 0 ? x * (b * s) : 0.
+__global__ __launch_bounds__(256) void k_update(float* __restrict__ x, float* __restrict__ bp,
+                                                const float* __restrict__ s, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float b = bp[i];
+  bp[i] = 0.0f;
+  const float sj = s ? s[i] : 1.0f;
+  x[i] = sj > 0.0f ? x[i] * (b * sj) : 0.0f;
+}
+
+__global__ void k_fill_f32(float* __restrict__ x, float v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void k_fill_f64(double* __restrict__ x, double v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ x, int n) {
+  if ((int)threadIdx.x < n) x[threadIdx.x] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// Full reference walk per LOR, counting only (no memory traffic besides the
+// LOR): steps, pushed elements, skipped, missed and runaway LORs.
+template <bool TOF>
+__global__ __launch_bounds__(256) void k_walk_stats(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
+                                                    unsigned long long* __restrict__ counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t c_lor = 0, c_steps = 0, c_nz = 0, c_skip = 0, c_miss = 0, c_run = 0;
+  if (i < n) {
+    c_lor = 1;
+    const Lor l = load_lor(lors, i);
+    Walk w;
+    if (!walk_setup(l, f, w)) {
+      c_miss = 1;
+    } else {
+      const uint32_t cap = step_cap(f);
+      bool bad = false;
+      for (;;) {
+        if (c_steps >= cap) { c_run = 1; bad = true; break; }
+        ++c_steps;
+        const Step s = next_boundary(w);
+        const float wt = step_weight<TOF>(w, s, t);
+        if (wt > 0.0f) {
+          ++c_nz;
+          if ((uint32_t)w.index >= (uint32_t)f.nvox) bad = true;
+        }
+        if (advance(w, s)) break;
+      }
+      c_skip = bad ? 1 : 0;
+    }
+  }
+  wave_add_u64(counters + CNT_LORS, c_lor);
+  wave_add_u64(counters + CNT_STEPS, c_steps);
+  wave_add_u64(counters + CNT_NZ, c_nz);
+  wave_add_u64(counters + CNT_SKIPPED, c_skip);
+  wave_add_u64(counters + CNT_MISSED, c_miss);
+  wave_add_u64(counters + CNT_RUNAWAY, c_run);
+}
+
+// ---------------------------------------------------------------------------
+// System-matrix rows (siddon.rs:69-73): element k of LOR i at [i*cap + k].
+template <bool TOF>
+__global__ __launch_bounds__(256) void k_rows(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
+                                              uint32_t cap, int32_t* __restrict__ idx, float* __restrict__ wout,
+                                              uint32_t* __restrict__ count, uint32_t* __restrict__ steps_out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Lor l = load_lor(lors, i);
+  Walk w;
+  uint32_t m = 0, steps = 0;
+  if (walk_setup(l, f, w)) {
+    const uint32_t scap = step_cap(f);
+    const size_t base = (size_t)i * cap;
+    for (;;) {
+      if (steps >= scap) { steps = 0xffffffffu; break; }
+      ++steps;
+      const Step s = next_boundary(w);
+      const float wt = step_weight<TOF>(w, s, t);
+      if (wt > 0.0f) {
+        if (m < cap) {
+          idx[base + m] = w.index;
+          wout[base + m] = wt;
+        }
+        ++m;
+      }
+      if (advance(w, s)) break;
+    }
+  }
+  count[i] = m;
+  steps_out[i] = steps;
+}
+
+// ---------------------------------------------------------------------------
+// Sensitivity-image LORs (continuous.rs:20-47): both ends uniform on the
+// detector cylinder, z = L*(u - 1/2), theta = TAU*u; rejected until the LOR
+// enters the FOV.  Attempt k of LOR i draws Philox block (i, k, 0, tag).
+constexpr uint32_t kSensTag = 0x5e45u;
+constexpr uint32_t kSensMaxAttempts = 4096;
+
+__device__ __forceinline__ Lor sens_lor(uint64_t seed, uint32_t i, const DevFov& f, float L, float R,
+                                        bool& found) {
+  for (uint32_t k = 0; k < kSensMaxAttempts; ++k) {
+    const U32x4 r = philox(seed, i, k, 0, kSensTag);
+    const float z1 = L * (unit_f32(r.x) - 0.5f);
+    const float t1 = kTau * unit_f32(r.y);
+    const float z2 = L * (unit_f32(r.z) - 0.5f);
+    const float t2 = kTau * unit_f32(r.w);
+    float s1, c1, s2, c2;
+    sincosf(t1, &s1, &c1);
+    sincosf(t2, &s2, &c2);
+    const float x1 = R * c1, y1 = R * s1, x2 = R * c2, y2 = R * s2;
+    float ex, ey, ez;
+    if (fov_entry(f, x1, y1, z1, x2, y2, z2, ex, ey, ez)) {
+      found = true;
+      return Lor{x1, y1, z1, x2, y2, z2, 0.0f, 1.0f};
+    }
+  }
+  found = false;
+  return Lor{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+}
+
+__global__ __launch_bounds__(256) void k_sens_lors(uint32_t n, uint64_t seed, DevFov f, float L, float R,
+                                                   Lor* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool found;
+  const Lor l = sens_lor(seed, i, f, L, R, found);
+  float4* p = reinterpret_cast<float4*>(out + i);
+  p[0] = make_float4(l.x1, l.y1, l.z1, l.x2);
+  p[1] = make_float4(l.y2, l.z2, l.dt, l.ac);
+}
+
+// Sensitivity backprojection (project_one_lor_sens, projector.rs:72-74):
+// forward walk on the attenuation FOV, p = sum w*mu, adapted to exp(-p);
+// backward walk on the output FOV (TWO) or the same row, adding
+// w * (1/exp(-p)) into binary64 accumulators.
+template <bool GEN, bool TWO>
+__global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint32_t n, uint64_t seed, float L,
+                                              float R, DevFov fa, const float* __restrict__ mu, DevFov fo,
+                                              double* __restrict__ acc, unsigned long long* __restrict__ counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t c_lor = 0, c_steps = 0, c_nz = 0, c_skip = 0, c_miss = 0, c_run = 0;
+  const DevTof notof{};
+  if (i < n) {
+    c_lor = 1;
+    Lor l;
+    if (GEN) {
+      bool found;
+      l = sens_lor(seed, i, fa, L, R, found);
+    } else {
+      l = load_lor(lors, i);
+    }
+    Walk w;
+    const bool hit = walk_setup(l, fa, w);
+    bool bad = false;
+    float p = 0.0f;
+    Walk start = w;
+    uint32_t start_step = 0;
+    bool started = false;
+    if (!hit) {
+      c_miss = 1;
+    } else {
+      const uint32_t cap = step_cap(fa);
+      for (;;) {
+        if (c_steps >= cap) { c_run = 1; bad = true; break; }
+        const Step s = next_boundary(w);
+        const float wt = step_weight<false>(w, s, notof);
+        if (wt > 0.0f) {
+          if (!started) { start = w; start_step = c_steps; started = true; }
+          ++c_nz;
+          if ((uint32_t)w.index >= (uint32_t)fa.nvox) { bad = true; break; }
+          p += wt * mu[w.index];
+        }
+        ++c_steps;
+        if (advance(w, s)) break;
+      }
+    }
+    const float r = 1.0f / expf(-p);
+    if (!TWO) {
+      if (bad) {
+        c_skip = 1;
+      } else if (started) {
+        w = start;
+        for (uint32_t k = start_step; k < c_steps; ++k) {
+          const Step s = next_boundary(w);
+          const float wt = step_weight<false>(w, s, notof);
+          if (wt > 0.0f) unsafeAtomicAdd(acc + w.index, (double)(wt * r));
+          advance(w, s);
+        }
+      }
+    } else {
+      Walk wb;
+      if (!bad && walk_setup(l, fo, wb)) {
+        // check pass over the backward row, then the scatter pass
+        const uint32_t cap = step_cap(fo);
+        const Walk b0 = wb;
+        uint32_t nb = 0;
+        for (;;) {
+          if (nb >= cap) { bad = true; break; }
+          const Step s = next_boundary(wb);
+          const float wt = step_weight<false>(wb, s, notof);
+          ++nb;
+          if (wt > 0.0f && (uint32_t)wb.index >= (uint32_t)fo.nvox) { bad = true; break; }
+          if (advance(wb, s)) break;
+        }
+        if (!bad) {
+          wb = b0;
+          for (uint32_t k = 0; k < nb; ++k) {
+            const Step s = next_boundary(wb);
+            const float wt = step_weight<false>(wb, s, notof);
+            if (wt > 0.0f) unsafeAtomicAdd(acc + wb.index, (double)(wt * r));
+            advance(wb, s);
+          }
+        }
+      }
+      c_skip = bad ? 1 : 0;
+    }
+  }
+  wave_add_u64(counters + CNT_LORS, c_lor);
+  wave_add_u64(counters + CNT_STEPS, c_steps);
+  wave_add_u64(counters + CNT_NZ, c_nz);
+  wave_add_u64(counters + CNT_SKIPPED, c_skip);
+  wave_add_u64(counters + CNT_MISSED, c_miss);
+  wave_add_u64(counters + CNT_RUNAWAY, c_run);
+}
+
+// normalize (main.rs:77): out = (float)acc / (float)n.
+__global__ void k_sens_finalize(const double* __restrict__ acc, float* __restrict__ out, float nl, size_t nv) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) out[i] = (float)acc[i] / nl;
+}
+
+}  // namespace petalo

@@ -1,0 +1,244 @@
+// siddon_walk.hpp -- device-side Siddon setup and voxel walk for gfx950.
+//
+// Semantics follow the reference walk exactly (src/projectors/siddon.rs):
+// the same binary32 operations in the same order, so that the voxel-index
+// sequence of every LOR and every geometric weight is bit-identical to the
+// reference's.  This translation unit must be compiled with
+// -ffp-contract=off (no FMA contraction) and with hipcc's default correctly
+// rounded binary32 division and square root.
+//
+// The state of a walk lives in scalars (no dynamically indexed arrays) so
+// that it stays in VGPRs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace petalo {
+
+// mm/ps, units/src/lib.rs:156 (binary32 rounds it to 0.29979247).
+constexpr float kC = 0.299792458f;
+// siddon.rs:134: entry-point components closer than this to 0 become 0.
+constexpr float kEntryEps = 1e-5f;
+// siddon.rs:106-108: factor applied to the TOF Gaussian.
+constexpr float kTofFactor = 666.0f;
+// std::f32::consts::TAU
+constexpr float kTau = 6.2831855f;
+
+struct DevFov {
+  float hw_x, hw_y, hw_z;  // half widths
+  float vs_x, vs_y, vs_z;  // voxel sizes
+  int32_t nx, ny, nz;
+  int32_t nvox;
+};
+
+struct DevTof {
+  float sigma;   // mm (sigma_t * C)
+  float cutoff;  // mm (cutoff * sigma)
+  float peak;    // 1 / (sigma * sqrt(TAU))
+  int32_t on;
+};
+
+struct Lor {
+  float x1, y1, z1, x2, y2, z2, dt, ac;
+};
+
+// Walk state.
+struct Walk {
+  float nb_x, nb_y, nb_z;  // distance along the LOR to the next boundary
+  float vl_x, vl_y, vl_z;  // voxel size in LOR-distance units
+  int32_t index;           // current 1-D voxel index
+  int32_t d_x, d_y, d_z;   // index change per axis
+  int32_t r_x, r_y, r_z;   // voxels remaining per axis
+  float here;              // distance travelled since the FOV entry
+  float tof_peak;          // distance from the entry to the TOF peak
+};
+
+// nalgebra 0.30.1 Vector3::norm = sqrt((x*x + y*y) + z*z).
+__host__ __device__ __forceinline__ float norm3(float x, float y, float z) {
+  return sqrtf((x * x + y * y) + z * z);
+}
+
+// FOV::new (src/fov.rs:22-39).
+inline DevFov make_fov(const float full[3], const uint32_t n[3]) {
+  DevFov f;
+  f.hw_x = full[0] / 2.0f;
+  f.hw_y = full[1] / 2.0f;
+  f.hw_z = full[2] / 2.0f;
+  f.vs_x = (f.hw_x * 2.0f) / (float)n[0];
+  f.vs_y = (f.hw_y * 2.0f) / (float)n[1];
+  f.vs_z = (f.hw_z * 2.0f) / (float)n[2];
+  f.nx = (int32_t)n[0];
+  f.ny = (int32_t)n[1];
+  f.nz = (int32_t)n[2];
+  f.nvox = f.nx * f.ny * f.nz;
+  return f;
+}
+
+// Gaussian::new (src/gauss.rs:31-45, 58-60).
+inline DevTof make_tof(bool on, float sigma_ps, float cutoff) {
+  DevTof t;
+  t.on = on ? 1 : 0;
+  t.sigma = sigma_ps * kC;
+  const float root_two_pi = sqrtf(kTau);
+  t.peak = 1.0f / (t.sigma * root_two_pi);
+  t.cutoff = cutoff * t.sigma;
+  return t;
+}
+
+// One slab of the ray/box cast (ncollide3d 0.33.0 AABB cast, called from
+// FOV::entry, src/fov.rs:55-71).  Returns false on a miss.
+__host__ __device__ __forceinline__ bool slab(float o, float dir, float hw, float& tmin, float& tmax) {
+  if (dir == 0.0f) return !(o < -hw || o > hw);
+  const float denom = 1.0f / dir;
+  float a = (-hw - o) * denom;
+  float b = (hw - o) * denom;
+  if (a > b) {
+    const float t = a;
+    a = b;
+    b = t;
+  }
+  tmin = fmaxf(tmin, a);
+  tmax = fminf(tmax, b);
+  return !(tmin > tmax);
+}
+
+// FOV::entry: ray from p1 along normalize(p2 - p1), max_toi = |p2 - p1|,
+// solid cast; entry = p1 + dir * toi.
+__host__ __device__ __forceinline__ bool fov_entry(const DevFov& f, float x1, float y1, float z1, float x2,
+                                                   float y2, float z2, float& ex, float& ey, float& ez) {
+  const float dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+  const float len = norm3(dx, dy, dz);
+  const float ux = dx / len, uy = dy / len, uz = dz / len;
+  float tmin = 0.0f, tmax = len;
+  if (!slab(x1, ux, f.hw_x, tmin, tmax)) return false;
+  if (!slab(y1, uy, f.hw_y, tmin, tmax)) return false;
+  if (!slab(z1, uz, f.hw_z, tmin, tmax)) return false;
+  ex = x1 + ux * tmin;
+  ey = y1 + uy * tmin;
+  ez = z1 + uz * tmin;
+  return true;
+}
+
+// Rust `x.floor() as usize as i32`: saturating to [0, 2^64-1], then the low
+// 32 bits.
+__device__ __forceinline__ int32_t floor_index(float e) {
+  const float fl = floorf(e);
+  if (!(fl > 0.0f)) return 0;
+  if (fl >= 18446744073709551616.0f) return -1;
+  if (fl < 2147483648.0f) return (int32_t)fl;
+  return (int32_t)(uint32_t)((unsigned long long)fl & 0xffffffffull);
+}
+
+// lor_fov_hit (siddon.rs:190-229) with flip_axes (:322-340),
+// find_tof_peak (:156-163), find_entry_point (:138-153),
+// index_trackers (:233-273), voxel_size (:179-185), first_boundaries
+// (:166-174).  Returns false when the LOR misses the FOV.
+__device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, Walk& w) {
+  float x1 = l.x1, y1 = l.y1, z1 = l.z1, x2 = l.x2, y2 = l.y2, z2 = l.z2;
+  const bool fx = (x2 - x1) < 0.0f, fy = (y2 - y1) < 0.0f, fz = (z2 - z1) < 0.0f;
+  if (fx) { x1 = -x1; x2 = -x2; }
+  if (fy) { y1 = -y1; y2 = -y2; }
+  if (fz) { z1 = -z1; z2 = -z2; }
+
+  float ex, ey, ez;
+  if (!fov_entry(f, x1, y1, z1, x2, y2, z2, ex, ey, ez)) return false;
+
+  const float half_len = norm3(x1 - x2, y1 - y2, z1 - z2) / 2.0f;
+  const float tof_shift = kC * l.dt / 2.0f;
+  const float p1_to_peak = half_len - tof_shift;
+  const float p1_to_entry = norm3(ex - x1, ey - y1, ez - z1);
+  w.tof_peak = p1_to_peak - p1_to_entry;
+
+  float vx = (ex + f.hw_x) / f.vs_x;
+  float vy = (ey + f.hw_y) / f.vs_y;
+  float vz = (ez + f.hw_z) / f.vs_z;
+  if (fabsf(vx) < kEntryEps) vx = 0.0f;
+  if (fabsf(vy) < kEntryEps) vy = 0.0f;
+  if (fabsf(vz) < kEntryEps) vz = 0.0f;
+
+  const int32_t ix = floor_index(vx), iy = floor_index(vy), iz = floor_index(vz);
+  w.d_x = fx ? -1 : 1;
+  w.d_y = fy ? -f.nx : f.nx;
+  w.d_z = fz ? -(f.nx * f.ny) : f.nx * f.ny;
+  w.r_x = f.nx - ix;
+  w.r_y = f.ny - iy;
+  w.r_z = f.nz - iz;
+  const uint32_t jx = (uint32_t)(fx ? f.nx - 1 - ix : ix);
+  const uint32_t jy = (uint32_t)(fy ? f.ny - 1 - iy : iy);
+  const uint32_t jz = (uint32_t)(fz ? f.nz - 1 - iz : iz);
+  w.index = (int32_t)(jx + (jy + jz * (uint32_t)f.ny) * (uint32_t)f.nx);
+
+  const float dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+  const float len = norm3(dx, dy, dz);
+  w.vl_x = f.vs_x / (dx / len);
+  w.vl_y = f.vs_y / (dy / len);
+  w.vl_z = f.vs_z / (dz / len);
+
+  w.nb_x = (1.0f - (vx - floorf(vx))) * w.vl_x;
+  w.nb_y = (1.0f - (vy - floorf(vy))) * w.vl_y;
+  w.nb_z = (1.0f - (vz - floorf(vz))) * w.vl_z;
+  w.here = 0.0f;
+  return true;
+}
+
+// Step cap: a legitimate walk ends within nx+ny+nz-2 steps.  The reference
+// never ends an axis-parallel walk that starts exactly on the far face; such
+// a walk is stopped here and its LOR dropped.
+__host__ __device__ __forceinline__ uint32_t step_cap(const DevFov& f) {
+  return (uint32_t)(2 * (f.nx + f.ny + f.nz) + 16);
+}
+
+// The boundary crossed next (Vector::argmin, geometry/src/vector.rs:150-158:
+// ties go to x, then y) and its position.
+struct Step {
+  float bnd;
+  int32_t axis;  // 0, 1, 2
+};
+
+__device__ __forceinline__ Step next_boundary(const Walk& w) {
+  Step s;
+  if (w.nb_x <= w.nb_y) {
+    if (w.nb_x <= w.nb_z) { s.axis = 0; s.bnd = w.nb_x; }
+    else                  { s.axis = 2; s.bnd = w.nb_z; }
+  } else {
+    if (w.nb_y <= w.nb_z) { s.axis = 1; s.bnd = w.nb_y; }
+    else                  { s.axis = 2; s.bnd = w.nb_z; }
+  }
+  return s;
+}
+
+// TOF Gaussian (src/gauss.rs:47-55) times the 666 mm factor (siddon.rs:106-108),
+// evaluated at the voxel entry distance `here`.
+__device__ __forceinline__ float tof_factor(const DevTof& t, float here, float tof_peak) {
+  const float dx = here - tof_peak;
+  if (fabsf(dx) < t.cutoff) {
+    const float y = dx / t.sigma;
+    const float z = y * y;
+    return kTofFactor * (t.peak * expf(-0.5f * z));
+  }
+  return 0.0f;
+}
+
+// Advance across the boundary of step s (siddon.rs:118-128).  Returns true
+// when the walk has left the FOV.
+__device__ __forceinline__ bool advance(Walk& w, const Step& s) {
+  w.here = s.bnd;
+  bool done;
+  if (s.axis == 0) {
+    w.nb_x += w.vl_x;
+    w.index += w.d_x;
+    done = (--w.r_x) == 0;
+  } else if (s.axis == 1) {
+    w.nb_y += w.vl_y;
+    w.index += w.d_y;
+    done = (--w.r_y) == 0;
+  } else {
+    w.nb_z += w.vl_z;
+    w.index += w.d_z;
+    done = (--w.r_z) == 0;
+  }
+  return done;
+}
+
+}  // namespace petalo

@@ -1,0 +1,239 @@
+"""Parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bars: voxel-index sequences identical; binary32 geometric weights
+bit-identical (the walk is the same IEEE operation sequence); TOF weights
+within 8 ulp (device expf vs the host's); images within 1e-4 relative L2
+after 10 iterations (the atomic backprojection sums in another order).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import petalo_amd as P
+from helpers import max_rel, proptest_lors, random_lors, rel_l2, ulps
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def oracle_rows(full, n, lors, tof=None):
+    return [O.row(full, n, l, tof=tof) for l in lors]
+
+
+def test_rows_hand_picked(gpu):
+    kats = json.load(open(os.path.join(GOLDEN, "reference_kats.json")))["hand_picked"]["cases"]
+    for c in kats:
+        _say(f"hand_picked {c['p1']} -> {c['p2']} n={c['n']}")
+        fov = P.FOV([c["size"][0], c["size"][1], 1.0], [c["n"][0], c["n"][1], 1])
+        lor = P.make_lors([c["p1"][0], c["p1"][1], 0.0], [c["p2"][0], c["p2"][1], 0.0])
+        idx, w, cnt, steps = P.system_matrix_rows(fov, lor)
+        m = int(cnt[0])
+        vox = [[int(i) % c["n"][0], (int(i) // c["n"][0]) % c["n"][1]] for i in idx[0, :m]]
+        assert vox == c["voxels"]
+        total = np.float32(0)
+        for x in w[0, :m]:
+            total = np.float32(total + x)
+        assert ulps(total, np.float32(c["length"])) <= 1
+
+
+@pytest.mark.parametrize("tof", [None, (42.47, 3.0)])
+def test_rows_random_match_oracle(gpu, tof):
+    rng = np.random.default_rng(7)
+    cases = proptest_lors(rng, 300)
+    n_elems = n_ties = 0
+    for k, (full, nv, p1, p2) in enumerate(cases):
+        lor = P.make_lors(p1, p2)
+        if tof:
+            lor["dt"] = np.float32(rng.normal(0, 200))
+        fov = P.FOV(full, nv)
+        idx, w, cnt, steps = P.system_matrix_rows(fov, lor, tof=P.Tof(*tof) if tof else None)
+        oi, ow, osteps = O.row(full, nv, lor[0], tof=tof)
+        m = int(cnt[0])
+        assert m == len(oi), f"case {k}: element count {m} vs oracle {len(oi)}"
+        assert int(steps[0]) == osteps, f"case {k}: steps"
+        assert np.array_equal(idx[0, :m].astype(np.int64), oi), f"case {k}: voxel sequence"
+        if tof is None:
+            assert np.array_equal(w[0, :m], ow), f"case {k}: weights not bit-identical"
+        else:
+            assert ulps(w[0, :m], ow).max(initial=0) <= 8, f"case {k}: TOF weights"
+        n_elems += m
+        n_ties += int(steps[0]) - m if tof is None else 0
+    _say(f"rows: {n_elems} elements, {n_ties} zero-length tie steps, all identical")
+
+
+def test_rows_phantom_bitexact(gpu):
+    """Scanner-geometry LORs on the 60^3 FOV: every row identical to the oracle's."""
+    lors = P.synth_phantom_lors(11, 2000, 0.0)
+    full, n = [180.0] * 3, [60] * 3
+    fov = P.FOV(full, n)
+    idx, w, cnt, steps = P.system_matrix_rows(fov, lors)
+    mism = 0
+    for i, l in enumerate(lors):
+        oi, ow, os_ = O.row(full, n, l)
+        m = int(cnt[i])
+        ok = m == len(oi) and np.array_equal(idx[i, :m], oi) and np.array_equal(w[i, :m], ow) and steps[i] == os_
+        mism += 0 if ok else 1
+    assert mism == 0, f"{mism} of {len(lors)} rows differ"
+
+
+def _run_gpu(fov, lors, iters, tof=None, sens=None, subsets=1, **kw):
+    it = P.mlem(fov, lors, tof=tof, sensitivity=sens, n_subsets=subsets, **kw)
+    imgs = []
+    for _ in range(iters * subsets):
+        img, osem = next(it)
+        imgs.append(img.data.copy())
+    it.close()
+    return imgs
+
+
+def _run_oracle(full, n, lors, iters, tof=None, sens=None, subsets=1, threads=16, f64=False):
+    nv = int(np.prod(n))
+    x = np.ones(nv, np.float64 if f64 else np.float32)
+    s = None if sens is None else np.asarray(sens, np.float64 if f64 else np.float32)
+    size = len(lors) // subsets
+    out = []
+    for it in range(iters):
+        for sub in range(subsets):
+            O.mlem_update(full, n, lors[sub * size:(sub + 1) * size], x, sens=s, tof=tof, threads=threads,
+                          f64=f64)
+            out.append(x.copy())
+    return out
+
+
+@pytest.mark.parametrize("tof", [None, (42.47, 3.0)])
+def test_mlem_small(gpu, tof):
+    full, n = [48.0, 48.0, 40.0], [12, 12, 10]
+    rng = np.random.default_rng(3)
+    lors = random_lors(rng, 20000, dt_sigma_ps=84.93 if tof else 0.0)
+    fov = P.FOV(full, n)
+    g = _run_gpu(fov, lors, 5, tof=P.Tof(*tof) if tof else None)
+    o = _run_oracle(full, n, lors, 5, tof=tof)
+    for k in range(5):
+        e = rel_l2(g[k], o[k])
+        _say(f"small tof={tof} iter {k + 1}: relL2 {e:.2e}")
+        assert e <= 1e-5
+
+
+def test_mlem_config1_full(gpu):
+    """60^3 x 3 mm, 1e6 phantom LORs, non-TOF, 10 iterations: <= 1e-4 relative L2."""
+    full, n = [180.0] * 3, [60] * 3
+    lors = P.synth_phantom_lors(1, 1_000_000, 0.0)
+    fov = P.FOV(full, n)
+    _say("config1: GPU 10 iterations")
+    g = _run_gpu(fov, lors, 10)
+    _say("config1: oracle 10 iterations")
+    o = _run_oracle(full, n, lors, 10)
+    o64 = _run_oracle(full, n, lors, 10, f64=True)
+    e, m, e64 = rel_l2(g[-1], o[-1]), max_rel(g[-1], o[-1]), rel_l2(g[-1], o64[-1])
+    _say(f"config1: relL2 vs f32 oracle {e:.2e}, max {m:.2e}; vs f64 oracle {e64:.2e}")
+    assert e <= 1e-4
+
+
+def test_mlem_tof_120(gpu):
+    """120^3 x 1.5 mm TOF (200 ps FWHM data, sigma 42.47 ps, cutoff 3) with 1e6 LORs, 10 iterations."""
+    full, n = [180.0] * 3, [120] * 3
+    lors = P.synth_phantom_lors(2, 1_000_000, 200.0)
+    tof = (42.47, 3.0)
+    fov = P.FOV(full, n)
+    _say("tof120: GPU")
+    g = _run_gpu(fov, lors, 10, tof=P.Tof(*tof))
+    _say("tof120: oracle")
+    o = _run_oracle(full, n, lors, 10, tof=tof)
+    e = rel_l2(g[-1], o[-1])
+    _say(f"tof120: relL2 {e:.2e}, max {max_rel(g[-1], o[-1]):.2e}")
+    assert e <= 1e-4
+
+
+def test_osem_subsets(gpu):
+    full, n = [180.0] * 3, [30] * 3
+    lors = P.synth_phantom_lors(5, 100_003, 0.0)  # remainder LORs are dropped
+    fov = P.FOV(full, n)
+    it = P.mlem(fov, lors, n_subsets=4)
+    ids = []
+    g = []
+    for _ in range(8):
+        img, osem = next(it)
+        ids.append((osem.iteration, osem.subset))
+        g.append(img.data.copy())
+    assert ids == [(1, 1), (1, 2), (1, 3), (1, 4), (2, 1), (2, 2), (2, 3), (2, 4)]
+    o = _run_oracle(full, n, lors, 2, subsets=4)
+    for k in range(8):
+        assert rel_l2(g[k], o[k]) <= 1e-5
+
+
+def test_sensitivity_matches_oracle(gpu):
+    att = P.FOV([180.0] * 3, [30] * 3)
+    rho = P.synth_density(att)
+    mu = P.density_to_attenuation(rho, 0.096)
+    n, seed, L, D = 200_000, 99, 300.0, 700.0
+    gen = P.sensitivity_lors(att, n, seed, L, D)
+    img, st = P.make_sensitivity_image(mu, n, L, D, seed=seed, return_stats=True)
+    img2 = P.make_sensitivity_image(mu, n, lors=gen)
+    ref, ost = O.sensitivity(att.full_size, att.n, mu.data, gen, threads=16)
+    ref64, _ = O.sensitivity(att.full_size, att.n, mu.data, gen, threads=16, f64=True)
+    assert st["n_steps"] == ost["n_steps"] and st["n_nz"] == ost["n_nz"]
+    e1, e2, e64 = rel_l2(img.data, ref), rel_l2(img2.data, ref), rel_l2(img.data, ref64)
+    _say(f"sensitivity: relL2 gen {e1:.2e} list {e2:.2e} vs f64 {e64:.2e}")
+    assert e1 <= 1e-5 and e2 <= 1e-5 and e64 <= 1e-5
+
+
+def test_sensitivity_two_fovs(gpu):
+    att = P.FOV([180.0] * 3, [30] * 3)
+    out = P.FOV([150.0, 150.0, 120.0], [25, 25, 20])
+    mu = P.Image(att, np.full(att.nvox, 0.0096, np.float32))
+    n, seed, L, D = 50_000, 5, 300.0, 700.0
+    gen = P.sensitivity_lors(att, n, seed, L, D)
+    img = P.make_sensitivity_image(mu, n, L, D, output_fov=out, seed=seed)
+    ref, _ = O.sensitivity(att.full_size, att.n, mu.data, gen, out_full=out.full_size, out_n=out.n, threads=16)
+    assert rel_l2(img.data, ref) <= 1e-5
+
+
+def test_walk_stats_match_oracle(gpu):
+    full, n = [180.0] * 3, [120] * 3
+    lors = P.synth_phantom_lors(8, 200_000, 200.0)
+    tof = (42.47, 3.0)
+    ctx = P.Mlem(P.FOV(full, n), lors, tof=P.Tof(*tof))
+    st = ctx.walk_stats()
+    ost = O.walk_stats(full, n, lors, tof=tof, threads=16)
+    _say(f"walk stats gpu {st} oracle {ost}")
+    assert st["n_steps"] == ost["n_steps"] and st["n_missed"] == ost["n_missed"]
+    assert abs(st["n_nz"] - ost["n_nz"]) <= 1e-4 * ost["n_nz"]  # TOF cut edge under expf ulps
+
+
+def test_edge_cases(gpu):
+    """Empty rows, degenerate LORs, out-of-range skips, zero projections,
+    s <= 0 voxels, more subsets than LORs (projector.rs:103-118, mlem.rs:45-52,90-95)."""
+    full, n = [30.0, 30.0, 30.0], [3, 3, 3]
+    fov = P.FOV(full, n)
+    lors = P.make_lors(
+        [[-100, 50, 0], [5, 5, 5], [-100, 0, 0], [15.0, -100, 1.0], [-100, -100, -100], [-100, 2, 3],
+         [15.0, 0.0, 0.0]],
+        [[100, 50, 0], [5, 5, 5], [100, 0, 0], [15.0, 100, 1.0], [100, 100, 100], [100, -2, -3],
+         [100.0, 0.0, 0.0]])
+    idx, w, cnt, steps = P.system_matrix_rows(fov, lors)
+    for i, l in enumerate(lors):
+        oi, ow, os_ = O.row(full, n, l)
+        m = int(cnt[i])
+        assert m == len(oi) and np.array_equal(idx[i, :m], oi) and np.array_equal(w[i, :m], ow)
+        assert (steps[i] == os_) or (os_ < 0 and steps[i] == 0xffffffff)
+    # MLEM with a sensitivity image containing zeros and negatives
+    sens = np.ones(fov.nvox, np.float32)
+    sens[0], sens[5] = 0.0, -1.0
+    g = _run_gpu(fov, lors, 2, sens=sens)
+    o = _run_oracle(full, n, lors, 2, sens=sens)
+    for a, b in zip(g, o):
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        fin = np.isfinite(b)
+        assert np.array_equal(np.isfinite(a), fin)
+        assert np.allclose(a[fin], b[fin], rtol=1e-6, atol=0)
+    # more subsets than LORs: empty subsets zero the image (x * 0 * s)
+    g = _run_gpu(fov, lors, 1, subsets=10)
+    assert all(np.all(x == 0) for x in g)
