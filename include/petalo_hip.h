@@ -200,6 +200,9 @@ int petalo_parse_quantity(const char* text, char kind, float* value);
  * TOF dt with Gaussian timing noise of the given FWHM (0 = exact dt).
  * Deterministic for a given (seed, n) regardless of thread count. */
 int petalo_synth_phantom_lors(uint64_t seed, size_t n_lors, float tof_fwhm_ps, petalo_lor_t* out);
+/* The counter-based generator behind the synthetic and sensitivity LOR
+ * streams: Philox4x32-10 (Salmon et al., SC'11) of counter ctr under key. */
+int petalo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* Density map [kg/m^3] for the attenuation workload: water cylinder
  * (1000) with a bone insert (1791.67) of r 15 mm at (-40, 0), |z| <= 75. */
 int petalo_synth_density(const petalo_fov_t* fov, float* out);

@@ -24,7 +24,7 @@ import numpy as np
 __all__ = [
     "FOV", "Image", "Osem", "Tof", "Stats", "LOR_DTYPE", "PetaloError", "lib", "mlem", "Mlem",
     "make_sensitivity_image", "sensitivity_lors", "system_matrix_rows", "density_to_attenuation",
-    "synth_phantom_lors", "synth_density", "read_config", "parse_config", "parse_quantity",
+    "synth_phantom_lors", "synth_density", "philox4x32_10", "read_config", "parse_config", "parse_quantity",
     "write_image3d", "read_image3d", "device_count", "make_lors",
     "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_SLAB",
 ]
@@ -116,6 +116,7 @@ _SIGS = {
     "petalo_parse_quantity": (C.c_int, [C.c_char_p, C.c_char, _P]),
     "petalo_synth_phantom_lors": (C.c_int, [C.c_uint64, C.c_size_t, C.c_float, _P]),
     "petalo_synth_density": (C.c_int, [_P, _P]),
+    "petalo_philox4x32_10": (C.c_int, [_P, _P, _P]),
 }
 
 _lib = None
@@ -391,6 +392,14 @@ def system_matrix_rows(fov, lors, tof=None, cap=None):
 def synth_phantom_lors(seed, n, tof_fwhm_ps=0.0):
     out = np.zeros(int(n), LOR_DTYPE)
     _check(lib().petalo_synth_phantom_lors(int(seed), int(n), float(tof_fwhm_ps), _ptr(out)))
+    return out
+
+
+def philox4x32_10(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    _check(lib().petalo_philox4x32_10(_ptr(c), _ptr(k), _ptr(out)))
     return out
 
 

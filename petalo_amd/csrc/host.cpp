@@ -704,6 +704,16 @@ int petalo_synth_phantom_lors(uint64_t seed, size_t n, float fwhm_ps, petalo_lor
   return PETALO_OK;
 }
 
+int petalo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  if (!ctr || !key || !out) return fail(PETALO_ERR_ARG, "NULL argument");
+  const U32x4 r = philox4x32_10(U32x4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+  out[3] = r.w;
+  return PETALO_OK;
+}
+
 int petalo_synth_density(const petalo_fov_t* fov, float* out) {
   if (!fov || !out) return fail(PETALO_ERR_ARG, "NULL argument");
   for (uint32_t z = 0; z < fov->n[2]; ++z)

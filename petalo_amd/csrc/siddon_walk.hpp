@@ -190,21 +190,21 @@ __host__ __device__ __forceinline__ uint32_t step_cap(const DevFov& f) {
 }
 
 // The boundary crossed next (Vector::argmin, geometry/src/vector.rs:150-158:
-// ties go to x, then y) and its position.
+// ties go to x, then y) and its position, as one-hot axis flags so that the
+// update below is branch-free select arithmetic (a switch on the axis makes
+// the compiler spill the per-axis state to a scratch array).
 struct Step {
   float bnd;
-  int32_t axis;  // 0, 1, 2
+  bool ax, ay, az;
 };
 
 __device__ __forceinline__ Step next_boundary(const Walk& w) {
   Step s;
-  if (w.nb_x <= w.nb_y) {
-    if (w.nb_x <= w.nb_z) { s.axis = 0; s.bnd = w.nb_x; }
-    else                  { s.axis = 2; s.bnd = w.nb_z; }
-  } else {
-    if (w.nb_y <= w.nb_z) { s.axis = 1; s.bnd = w.nb_y; }
-    else                  { s.axis = 2; s.bnd = w.nb_z; }
-  }
+  const bool xy = w.nb_x <= w.nb_y;
+  s.ax = xy && (w.nb_x <= w.nb_z);
+  s.ay = !xy && (w.nb_y <= w.nb_z);
+  s.az = !s.ax && !s.ay;
+  s.bnd = s.ax ? w.nb_x : (s.ay ? w.nb_y : w.nb_z);
   return s;
 }
 
@@ -220,25 +220,19 @@ __device__ __forceinline__ float tof_factor(const DevTof& t, float here, float t
   return 0.0f;
 }
 
-// Advance across the boundary of step s (siddon.rs:118-128).  Returns true
-// when the walk has left the FOV.
+// Advance across the boundary of step s (siddon.rs:118-128): only the
+// crossed axis moves.  Returns true when the walk has left the FOV (the
+// crossed axis has no voxels left).
 __device__ __forceinline__ bool advance(Walk& w, const Step& s) {
   w.here = s.bnd;
-  bool done;
-  if (s.axis == 0) {
-    w.nb_x += w.vl_x;
-    w.index += w.d_x;
-    done = (--w.r_x) == 0;
-  } else if (s.axis == 1) {
-    w.nb_y += w.vl_y;
-    w.index += w.d_y;
-    done = (--w.r_y) == 0;
-  } else {
-    w.nb_z += w.vl_z;
-    w.index += w.d_z;
-    done = (--w.r_z) == 0;
-  }
-  return done;
+  w.nb_x = s.ax ? w.nb_x + w.vl_x : w.nb_x;
+  w.nb_y = s.ay ? w.nb_y + w.vl_y : w.nb_y;
+  w.nb_z = s.az ? w.nb_z + w.vl_z : w.nb_z;
+  w.index += s.ax ? w.d_x : (s.ay ? w.d_y : w.d_z);
+  w.r_x -= (int32_t)s.ax;
+  w.r_y -= (int32_t)s.ay;
+  w.r_z -= (int32_t)s.az;
+  return (s.ax && w.r_x == 0) || (s.ay && w.r_y == 0) || (s.az && w.r_z == 0);
 }
 
 }  // namespace petalo

@@ -64,11 +64,12 @@ bool ulps1(float a, float b) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 3 || !std::strcmp(argv[1], "-h") || !std::strcmp(argv[1], "--help")) {
+  const bool help = argc >= 2 && (!std::strcmp(argv[1], "-h") || !std::strcmp(argv[1], "--help"));
+  if (argc < 3 || help) {
     std::printf("Maximum Likelyhood Expectation Maximization\n\n"
                 "usage: mlem <config.toml> <output-directory> [--lors FILE | --synthetic N [--seed S] "
                 "[--tof-fwhm PS]]\n");
-    return argc < 3 ? 1 : 0;
+    return help ? 0 : 1;
   }
   const std::string cfg_path = argv[1], outdir = argv[2];
   std::string lor_file;
