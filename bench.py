@@ -76,7 +76,9 @@ def main():
         sens_ms = (time.perf_counter() - t0) * 1e3
         log(f"[bench] sensitivity image from {n_sens} LORs: {sens_ms:.1f} ms")
 
+    t0 = time.perf_counter()
     ctx = P.Mlem(fov, lors, tof=tof_obj, sensitivity=sens_img, n_subsets=subsets, scheme=a.scheme)
+    log(f"[bench] context ready in {time.perf_counter() - t0:.2f} s")
     ws = ctx.walk_stats()
     log(f"[bench] walk stats {ws}")
 
@@ -84,17 +86,24 @@ def main():
         ctx.step(copy_image=False)
     P.lib().petalo_device_synchronize(0)
 
+    info = ctx.info()
+    log(f"[bench] scheme {info}")
     ctx.set_timing(True)
     proj_ms = []
     upd_ms = []
+    fwd_ms = []
+    sct_ms = []
     P.lib().petalo_device_synchronize(0)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         for _ in range(subsets):
             ctx.step(copy_image=False)
             pm, um = ctx.last_timing()
+            fm, sm, _ = ctx.last_phase_timing()
             proj_ms.append(pm)
             upd_ms.append(um)
+            fwd_ms.append(fm)
+            sct_ms.append(sm)
     P.lib().petalo_device_synchronize(0)
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
@@ -104,7 +113,8 @@ def main():
     proj_avg = float(np.mean(proj_ms))
     alg_bytes = 32 * ws["n_lors"] + 8 * ws["n_nz"]
     achieved = alg_bytes / (proj_avg * 1e-3) / 1e9
-    log(f"[bench] {it_per_s:.2f} iter/s; projection {proj_avg:.3f} ms, update {np.mean(upd_ms):.3f} ms")
+    log(f"[bench] {it_per_s:.2f} iter/s; projection {proj_avg:.3f} ms (forward {np.mean(fwd_ms):.3f}, "
+        f"scatter {np.mean(sct_ms):.3f}), update {np.mean(upd_ms):.3f} ms")
 
     cpu = None
     parity = None
@@ -155,6 +165,9 @@ def main():
         "voxel_steps_per_s": steps_per_s,
         "walk": {"n_steps": ws["n_steps"], "n_nz": ws["n_nz"], "n_skipped": ws["n_skipped"]},
         "projection_ms": proj_avg,
+        "forward_ms": float(np.mean(fwd_ms)),
+        "scatter_ms": float(np.mean(sct_ms)),
+        "scheme": info,
         "update_ms": float(np.mean(upd_ms)),
         "sensitivity_ms": sens_ms,
         "parity_rel_l2_sample": parity,

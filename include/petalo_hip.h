@@ -82,7 +82,7 @@ typedef struct {
 /* Backprojection schemes for petalo_mlem_options_t.scheme. */
 #define PETALO_SCHEME_AUTO 0     /* autotuned choice                        */
 #define PETALO_SCHEME_ATOMIC 1   /* plain: one lane per LOR, f32 global atomics */
-#define PETALO_SCHEME_SLAB 2     /* LDS z-slab accumulation, FP64 flush     */
+#define PETALO_SCHEME_TILE 2     /* image tiles in LDS, binary64 accumulation */
 
 typedef struct {
   int32_t scheme;        /* PETALO_SCHEME_*                                   */
@@ -131,6 +131,12 @@ int petalo_mlem_last_skipped(petalo_mlem_t* ctx, uint64_t* skipped);
  * on the context's stream when timing is on. */
 int petalo_mlem_set_timing(petalo_mlem_t* ctx, int on);
 int petalo_mlem_last_timing(petalo_mlem_t* ctx, double* project_ms, double* update_ms);
+/* Phase split of the last update: forward projection, backprojection
+ * scatter (0 for the atomic scheme, which does both in one kernel) and the
+ * image update. */
+int petalo_mlem_last_phase_timing(petalo_mlem_t* ctx, double* forward_ms, double* scatter_ms, double* update_ms);
+/* Scheme in use and, for the tiled scheme, the number of tile runs. */
+int petalo_mlem_info(petalo_mlem_t* ctx, int32_t* scheme, uint64_t* n_runs_total);
 int petalo_mlem_destroy(petalo_mlem_t* ctx);
 
 /* ---- Sensitivity image (continuous.rs:3-47, main.rs:77) ---------------

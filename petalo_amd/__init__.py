@@ -26,7 +26,7 @@ __all__ = [
     "make_sensitivity_image", "sensitivity_lors", "system_matrix_rows", "density_to_attenuation",
     "synth_phantom_lors", "synth_density", "philox4x32_10", "read_config", "parse_config", "parse_quantity",
     "write_image3d", "read_image3d", "device_count", "make_lors",
-    "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_SLAB",
+    "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -35,7 +35,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libpetalo_hip.so")
 LOR_DTYPE = np.dtype([("p1", "<f4", 3), ("p2", "<f4", 3), ("dt", "<f4"), ("ac", "<f4")])
 assert LOR_DTYPE.itemsize == 32
 
-SCHEME_AUTO, SCHEME_ATOMIC, SCHEME_SLAB = 0, 1, 2
+SCHEME_AUTO, SCHEME_ATOMIC, SCHEME_TILE = 0, 1, 2
 
 
 class PetaloError(RuntimeError):
@@ -102,6 +102,8 @@ _SIGS = {
     "petalo_mlem_last_skipped": (C.c_int, [_P, _P]),
     "petalo_mlem_set_timing": (C.c_int, [_P, C.c_int]),
     "petalo_mlem_last_timing": (C.c_int, [_P, _P, _P]),
+    "petalo_mlem_last_phase_timing": (C.c_int, [_P, _P, _P, _P]),
+    "petalo_mlem_info": (C.c_int, [_P, _P, _P]),
     "petalo_mlem_destroy": (C.c_int, [_P]),
     "petalo_make_sensitivity_image": (C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_uint64, C.c_float,
                                                 C.c_float, _P, _P]),
@@ -314,6 +316,16 @@ class Mlem:
         a, b = C.c_double(0), C.c_double(0)
         _check(lib().petalo_mlem_last_timing(self._ctx, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def last_phase_timing(self):
+        a, b, d = C.c_double(0), C.c_double(0), C.c_double(0)
+        _check(lib().petalo_mlem_last_phase_timing(self._ctx, C.byref(a), C.byref(b), C.byref(d)))
+        return a.value, b.value, d.value
+
+    def info(self):
+        sch, nr = C.c_int32(0), C.c_uint64(0)
+        _check(lib().petalo_mlem_info(self._ctx, C.byref(sch), C.byref(nr)))
+        return {"scheme": int(sch.value), "n_runs": int(nr.value)}
 
     def close(self):
         if self._ctx:

@@ -19,6 +19,9 @@
 #include "../../include/petalo_hip.h"
 #include "capi_internal.hpp"
 #include "kernels.hpp"
+#include "tiles.hpp"
+
+#include <hipcub/hipcub.hpp>
 
 using namespace petalo;
 
@@ -105,6 +108,20 @@ template <class T> struct DevBuf {
   ~DevBuf() { release(); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
 };
 
 void require_device() {
@@ -172,9 +189,15 @@ struct petalo_mlem {
   DevBuf<unsigned long long> counters;
   petalo_osem_t osem{};
   bool timing = false;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  double t_project = 0, t_update = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool have_timing = false;
+  // tiled scheme
+  TileGeom tiles{};
+  std::vector<DevBuf<Run>> runs;     // per subset, sorted by tile
+  std::vector<DevBuf<int4>> work;    // per subset: (tile, first run, end run, 0)
+  std::vector<uint64_t> n_runs;      // per subset
+  DevBuf<float> rbuf;                // 1/(p*ac) per LOR of the current subset
+  DevBuf<double> bp64;               // binary64 backprojection
 };
 
 namespace {
@@ -190,6 +213,140 @@ void launch_project(petalo_mlem* c, size_t lo, size_t count) {
     k_mlem_project<false><<<g, kBlock, 0, c->stream>>>(L, (uint32_t)count, c->dfov, c->dtof, c->image.p, c->bp.p,
                                                         c->counters.p);
   HIP_CHECK(hipGetLastError());
+}
+
+// Tile shape for an image: 16384 voxels (128 KiB of binary64 in LDS).
+TileGeom make_tiles(const petalo_fov_t& fov) {
+  TileGeom g;
+  g.tx = 32;
+  g.ty = 32;
+  g.tz = 16;
+  g.ntx = (int32_t)((fov.n[0] + g.tx - 1) / g.tx);
+  g.nty = (int32_t)((fov.n[1] + g.ty - 1) / g.ty);
+  g.ntz = (int32_t)((fov.n[2] + g.tz - 1) / g.tz);
+  g.ntiles = g.ntx * g.nty * g.ntz;
+  g.tvox = g.tx * g.ty * g.tz;
+  return g;
+}
+
+__global__ void k_tile_starts(const uint32_t* __restrict__ keys, uint64_t n, uint64_t* __restrict__ start) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  if (j == 0 || keys[j] != keys[j - 1]) start[keys[j]] = j;
+}
+
+__global__ void k_gather_runs(const Run* __restrict__ in, const uint32_t* __restrict__ perm, uint64_t n,
+                              Run* __restrict__ out) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = in[perm[j]];
+}
+
+__global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) x[j] = (uint32_t)j;
+}
+
+inline unsigned grid64(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// Cuts the subset's LOR walks into per-tile runs, sorted by tile, and the
+// per-workgroup work items over them.
+void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
+  const TileGeom& g = c->tiles;
+  hipStream_t st = c->stream;
+  uint64_t total = 0;
+  if (count) {
+    DevBuf<uint32_t> nr(count);
+    const Lor* L = c->lors.p + lo;
+    if (c->tof_on)
+      k_bin_runs<true, false><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
+                                                                  nullptr, nullptr, nullptr);
+    else
+      k_bin_runs<false, false><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
+                                                                   nullptr, nullptr, nullptr);
+    HIP_CHECK(hipGetLastError());
+    std::vector<uint32_t> h(count);
+    HIP_CHECK(hipMemcpyAsync(h.data(), nr.p, count * 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<uint64_t> off(count);
+    for (size_t i = 0; i < count; ++i) {
+      off[i] = total;
+      total += h[i];
+    }
+    if (total >= (1ull << 32)) throw std::runtime_error("more than 2^32 tile runs in one subset");
+    DevBuf<uint64_t> doff(count);
+    HIP_CHECK(hipMemcpyAsync(doff.p, off.data(), count * 8, hipMemcpyHostToDevice, st));
+    DevBuf<Run> unsorted(std::max<uint64_t>(total, 1));
+    DevBuf<uint32_t> keys(std::max<uint64_t>(total, 1)), keys2(std::max<uint64_t>(total, 1));
+    DevBuf<uint32_t> perm(std::max<uint64_t>(total, 1)), perm2(std::max<uint64_t>(total, 1));
+    if (c->tof_on)
+      k_bin_runs<true, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
+                                                                 doff.p, unsorted.p, keys.p);
+    else
+      k_bin_runs<false, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
+                                                                  doff.p, unsorted.p, keys.p);
+    HIP_CHECK(hipGetLastError());
+    c->runs[sub].alloc(std::max<uint64_t>(total, 1));
+    std::vector<uint64_t> starts(g.ntiles + 1, UINT64_MAX);
+    if (total) {
+      k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
+      int end_bit = 1;
+      while ((1 << end_bit) < g.ntiles) ++end_bit;
+      size_t tmp_bytes = 0;
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p,
+                                                   (int)total, 0, end_bit, st));
+      DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total,
+                                                   0, end_bit, st));
+      k_gather_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, perm2.p, total, c->runs[sub].p);
+      DevBuf<uint64_t> dstart(g.ntiles);
+      HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, g.ntiles * 8, st));
+      k_tile_starts<<<grid64(total), kBlock, 0, st>>>(keys2.p, total, dstart.p);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, g.ntiles * 8, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+    }
+    starts[g.ntiles] = total;
+    for (int k = g.ntiles - 1; k >= 0; --k)
+      if (starts[k] == UINT64_MAX) starts[k] = starts[k + 1];
+    // work items: chunks of one tile's runs, about 4 per CU in total
+    const uint64_t chunk = std::max<uint64_t>(256, total / 1024 + 1);
+    std::vector<int4> items;
+    for (int k = 0; k < g.ntiles; ++k)
+      for (uint64_t b = starts[k]; b < starts[k + 1]; b += chunk)
+        items.push_back(make_int4(k, (int)b, (int)std::min<uint64_t>(starts[k + 1], b + chunk), 0));
+    c->work[sub].alloc(std::max<size_t>(items.size(), 1));
+    if (!items.empty())
+      HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    c->work[sub].n = items.size();
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+  c->n_runs[sub] = total;
+}
+
+void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
+  if (count == 0) return;
+  const Lor* L = c->lors.p + lo;
+  const unsigned g = grid_for(count);
+  hipStream_t st = c->stream;
+  if (c->tof_on)
+    k_fwd<true><<<g, kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, c->image.p, c->rbuf.p, c->bp64.p,
+                                      c->counters.p);
+  else
+    k_fwd<false><<<g, kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, c->image.p, c->rbuf.p, c->bp64.p,
+                                       c->counters.p);
+  HIP_CHECK(hipGetLastError());
+  if (c->timing) HIP_CHECK(hipEventRecord(c->ev[1], st));
+  const size_t nw = c->work[sub].n;
+  if (nw) {
+    const size_t lds = (size_t)c->tiles.tvox * sizeof(double);
+    if (c->tof_on)
+      k_scatter<true><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->rbuf.p, c->dfov, c->dtof,
+                                                       c->tiles, c->bp64.p);
+    else
+      k_scatter<false><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->rbuf.p, c->dfov,
+                                                        c->dtof, c->tiles, c->bp64.p);
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 void fill_f32(float* p, float v, size_t n, hipStream_t s) {
@@ -248,7 +405,7 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
     if (tof && tof->enabled && !(tof->sigma_ps > 0.0f)) return fail(PETALO_ERR_ARG, "TOF sigma must be positive");
     petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, 0};
     if (options) opt = *options;
-    if (opt.scheme != PETALO_SCHEME_AUTO && opt.scheme != PETALO_SCHEME_ATOMIC)
+    if (opt.scheme != PETALO_SCHEME_AUTO && opt.scheme != PETALO_SCHEME_ATOMIC && opt.scheme != PETALO_SCHEME_TILE)
       return fail(PETALO_ERR_ARG, "unsupported backprojection scheme");
     require_device();
     HIP_CHECK(hipSetDevice(opt.device));
@@ -263,7 +420,7 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
       c->n_lors = n_lors;
       c->n_subsets = n_subsets;
       c->set_size = n_lors / n_subsets;  // remainder LORs ignored (mlem.rs:91)
-      c->scheme = PETALO_SCHEME_ATOMIC;
+      c->scheme = opt.scheme == PETALO_SCHEME_ATOMIC ? PETALO_SCHEME_ATOMIC : PETALO_SCHEME_TILE;
       c->osem = petalo_osem_t{n_subsets, 1, 1};
       HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       const size_t nv = nvox(*fov);
@@ -286,6 +443,21 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
         HIP_CHECK(hipMemcpy(c->sens.p, sensitivity, nv * sizeof(float), hipMemcpyHostToDevice));
       }
       for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
+      if (c->scheme == PETALO_SCHEME_TILE) {
+        c->tiles = make_tiles(*fov);
+        const size_t lds = (size_t)c->tiles.tvox * sizeof(double);
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+        c->runs.resize(n_subsets);
+        c->work.resize(n_subsets);
+        c->n_runs.assign(n_subsets, 0);
+        c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
+        c->bp64.alloc(nv);
+        fill_f64(c->bp64.p, 0.0, nv, c->stream);
+        for (size_t sub = 0; sub < n_subsets; ++sub) build_runs(c, sub, sub * c->set_size, c->set_size);
+      }
       HIP_CHECK(hipStreamSynchronize(c->stream));
     } catch (...) {
       petalo_mlem_destroy(c);
@@ -304,12 +476,21 @@ int petalo_mlem_next(petalo_mlem_t* c, float* image_out, petalo_osem_t* id_out) 
     const size_t lo = (c->osem.subset - 1) * c->set_size;  // Osem::subset (mlem.rs:90-95)
     zero_counters(c->counters.p, c->stream);
     if (c->timing) HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
-    launch_project(c, lo, c->set_size);
-    if (c->timing) HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
-    k_update<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp.p, c->sens.p, (uint32_t)nv);
+    if (c->scheme == PETALO_SCHEME_TILE) {
+      launch_tiled(c, c->osem.subset - 1, lo, c->set_size);
+      if (c->timing) HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
+      k_update_f64<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp64.p, c->sens.p, (uint32_t)nv);
+    } else {
+      launch_project(c, lo, c->set_size);
+      if (c->timing) {
+        HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
+        HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
+      }
+      k_update<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp.p, c->sens.p, (uint32_t)nv);
+    }
     HIP_CHECK(hipGetLastError());
     if (c->timing) {
-      HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
+      HIP_CHECK(hipEventRecord(c->ev[3], c->stream));
       c->have_timing = true;
     }
     if (id_out) *id_out = c->osem;
@@ -397,14 +578,42 @@ int petalo_mlem_last_timing(petalo_mlem_t* c, double* project_ms, double* update
     if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
     if (!c->have_timing) return fail(PETALO_ERR_ARG, "timing is off or no update has run");
     HIP_CHECK(hipSetDevice(c->device));
-    HIP_CHECK(hipEventSynchronize(c->ev[2]));
+    HIP_CHECK(hipEventSynchronize(c->ev[3]));
     float a = 0, b = 0;
-    HIP_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-    HIP_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
     if (project_ms) *project_ms = a;
     if (update_ms) *update_ms = b;
     return PETALO_OK;
   });
+}
+
+int petalo_mlem_last_phase_timing(petalo_mlem_t* c, double* forward_ms, double* scatter_ms, double* update_ms) {
+  return guarded([&] {
+    if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
+    if (!c->have_timing) return fail(PETALO_ERR_ARG, "timing is off or no update has run");
+    HIP_CHECK(hipSetDevice(c->device));
+    HIP_CHECK(hipEventSynchronize(c->ev[3]));
+    float a = 0, b = 0, d = 0;
+    HIP_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+    if (forward_ms) *forward_ms = a;
+    if (scatter_ms) *scatter_ms = b;
+    if (update_ms) *update_ms = d;
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_info(petalo_mlem_t* c, int32_t* scheme, uint64_t* n_runs_total) {
+  if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
+  if (scheme) *scheme = c->scheme;
+  if (n_runs_total) {
+    uint64_t t = 0;
+    for (uint64_t v : c->n_runs) t += v;
+    *n_runs_total = t;
+  }
+  return PETALO_OK;
 }
 
 int petalo_mlem_destroy(petalo_mlem_t* c) {
@@ -418,6 +627,10 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->bp.release();
   c->sens.release();
   c->counters.release();
+  c->runs.clear();
+  c->work.clear();
+  c->rbuf.release();
+  c->bp64.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PETALO_OK;

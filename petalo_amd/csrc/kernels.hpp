@@ -17,7 +17,9 @@
 //             element and replays exactly the steps of walk #1, adding
 //             w * (1/(p*ac)) to the backprojection (projector.rs:144-149).
 // A LOR with a pushed index outside the image is dropped whole
-// (projector.rs:103-118).
+// (projector.rs:103-118).  With TOF, walk #1 first crosses every boundary
+// before the Gaussian window without computing weights (fast_forward:
+// exact, a few operations per boundary).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,9 +51,9 @@ __device__ __forceinline__ Lor load_lor(const Lor* __restrict__ lors, uint32_t i
 
 // Weight of the step about to be taken (siddon.rs:99-111).
 template <bool TOF>
-__device__ __forceinline__ float step_weight(const Walk& w, const Step& s, const DevTof& t) {
+__host__ __device__ __forceinline__ float step_weight(const WalkS& w, const WalkC& c, const Step& s, const DevTof& t) {
   float wt = s.bnd - w.here;
-  if (TOF) wt *= tof_factor(t, w.here, w.tof_peak);
+  if (TOF) wt *= tof_factor(t, w.here, c.tof_peak);
   return wt;
 }
 
@@ -66,39 +68,44 @@ __global__ __launch_bounds__(256) void k_mlem_project(const Lor* __restrict__ lo
   uint32_t skipped = 0;
   if (i < n) {
     const Lor l = load_lor(lors, i);
-    Walk w;
-    if (walk_setup(l, f, w)) {
+    WalkC c;
+    WalkS w;
+    if (walk_setup(l, f, c, w)) {
       // With every remaining count positive the walk provably ends within
       // nx+ny+nz-2 steps; otherwise it is walked in full against the cap.
-      const bool bounded = w.r_x > 0 && w.r_y > 0 && w.r_z > 0;
-      const uint32_t cap = step_cap(f);
-      Walk start = w;
-      uint32_t steps = 0, start_step = 0;
-      bool started = false, bad = false;
-      float p = 0.0f;
-      for (;;) {
-        if (!bounded && steps >= cap) { bad = true; break; }
-        if (TOF && bounded && (w.here - w.tof_peak) >= t.cutoff) break;
-        const Step s = next_boundary(w);
-        const float wt = step_weight<TOF>(w, s, t);
-        if (wt > 0.0f) {
-          if (!started) { start = w; start_step = steps; started = true; }
-          if ((uint32_t)w.index >= (uint32_t)f.nvox) { bad = true; break; }
-          p += wt * img[w.index];
-        }
-        ++steps;
-        if (advance(w, s)) break;
-      }
-      if (bad) {
-        skipped = 1;
-      } else if (started) {
-        const float r = 1.0f / (p * l.ac);
-        w = start;
-        for (uint32_t k = start_step; k < steps; ++k) {
+      const bool bnd = bounded(w);
+      // With TOF, skip exactly to the Gaussian window: no weight before it.
+      const bool live = !(TOF && bnd) || fast_forward(w, c, tof_window_start(t, c.tof_peak));
+      if (live) {
+        const uint32_t cap = step_cap(f);
+        WalkS start = w;
+        uint32_t steps = 0, start_step = 0;
+        bool started = false, bad = false;
+        float p = 0.0f;
+        for (;;) {
+          if (!bnd && steps >= cap) { bad = true; break; }
+          if (TOF && bnd && (w.here - c.tof_peak) >= t.cutoff) break;
           const Step s = next_boundary(w);
-          const float wt = step_weight<TOF>(w, s, t);
-          if (wt > 0.0f) unsafeAtomicAdd(bp + w.index, wt * r);
-          advance(w, s);
+          const float wt = step_weight<TOF>(w, c, s, t);
+          if (wt > 0.0f) {
+            if (!started) { start = w; start_step = steps; started = true; }
+            if ((uint32_t)w.index >= (uint32_t)f.nvox) { bad = true; break; }
+            p += wt * img[w.index];
+          }
+          ++steps;
+          if (advance(w, c, s)) break;
+        }
+        if (bad) {
+          skipped = 1;
+        } else if (started) {
+          const float r = 1.0f / (p * l.ac);
+          w = start;
+          for (uint32_t k = start_step; k < steps; ++k) {
+            const Step s = next_boundary(w);
+            const float wt = step_weight<TOF>(w, c, s, t);
+            if (wt > 0.0f) unsafeAtomicAdd(bp + w.index, wt * r);
+            advance(w, c, s);
+          }
         }
       }
     }
@@ -143,8 +150,9 @@ __global__ __launch_bounds__(256) void k_walk_stats(const Lor* __restrict__ lors
   if (i < n) {
     c_lor = 1;
     const Lor l = load_lor(lors, i);
-    Walk w;
-    if (!walk_setup(l, f, w)) {
+    WalkC c;
+    WalkS w;
+    if (!walk_setup(l, f, c, w)) {
       c_miss = 1;
     } else {
       const uint32_t cap = step_cap(f);
@@ -153,12 +161,12 @@ __global__ __launch_bounds__(256) void k_walk_stats(const Lor* __restrict__ lors
         if (c_steps >= cap) { c_run = 1; bad = true; break; }
         ++c_steps;
         const Step s = next_boundary(w);
-        const float wt = step_weight<TOF>(w, s, t);
+        const float wt = step_weight<TOF>(w, c, s, t);
         if (wt > 0.0f) {
           ++c_nz;
           if ((uint32_t)w.index >= (uint32_t)f.nvox) bad = true;
         }
-        if (advance(w, s)) break;
+        if (advance(w, c, s)) break;
       }
       c_skip = bad ? 1 : 0;
     }
@@ -180,16 +188,17 @@ __global__ __launch_bounds__(256) void k_rows(const Lor* __restrict__ lors, uint
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Lor l = load_lor(lors, i);
-  Walk w;
+  WalkC c;
+  WalkS w;
   uint32_t m = 0, steps = 0;
-  if (walk_setup(l, f, w)) {
+  if (walk_setup(l, f, c, w)) {
     const uint32_t scap = step_cap(f);
     const size_t base = (size_t)i * cap;
     for (;;) {
       if (steps >= scap) { steps = 0xffffffffu; break; }
       ++steps;
       const Step s = next_boundary(w);
-      const float wt = step_weight<TOF>(w, s, t);
+      const float wt = step_weight<TOF>(w, c, s, t);
       if (wt > 0.0f) {
         if (m < cap) {
           idx[base + m] = w.index;
@@ -197,7 +206,7 @@ __global__ __launch_bounds__(256) void k_rows(const Lor* __restrict__ lors, uint
         }
         ++m;
       }
-      if (advance(w, s)) break;
+      if (advance(w, c, s)) break;
     }
   }
   count[i] = m;
@@ -264,11 +273,12 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
     } else {
       l = load_lor(lors, i);
     }
-    Walk w;
-    const bool hit = walk_setup(l, fa, w);
+    WalkC c;
+    WalkS w;
+    const bool hit = walk_setup(l, fa, c, w);
     bool bad = false;
     float p = 0.0f;
-    Walk start = w;
+    WalkS start = w;
     uint32_t start_step = 0;
     bool started = false;
     if (!hit) {
@@ -278,7 +288,7 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
       for (;;) {
         if (c_steps >= cap) { c_run = 1; bad = true; break; }
         const Step s = next_boundary(w);
-        const float wt = step_weight<false>(w, s, notof);
+        const float wt = step_weight<false>(w, c, s, notof);
         if (wt > 0.0f) {
           if (!started) { start = w; start_step = c_steps; started = true; }
           ++c_nz;
@@ -286,7 +296,7 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
           p += wt * mu[w.index];
         }
         ++c_steps;
-        if (advance(w, s)) break;
+        if (advance(w, c, s)) break;
       }
     }
     const float r = 1.0f / expf(-p);
@@ -297,33 +307,34 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
         w = start;
         for (uint32_t k = start_step; k < c_steps; ++k) {
           const Step s = next_boundary(w);
-          const float wt = step_weight<false>(w, s, notof);
+          const float wt = step_weight<false>(w, c, s, notof);
           if (wt > 0.0f) unsafeAtomicAdd(acc + w.index, (double)(wt * r));
-          advance(w, s);
+          advance(w, c, s);
         }
       }
     } else {
-      Walk wb;
-      if (!bad && walk_setup(l, fo, wb)) {
+      WalkC cb;
+      WalkS wb;
+      if (!bad && walk_setup(l, fo, cb, wb)) {
         // check pass over the backward row, then the scatter pass
         const uint32_t cap = step_cap(fo);
-        const Walk b0 = wb;
+        const WalkS b0 = wb;
         uint32_t nb = 0;
         for (;;) {
           if (nb >= cap) { bad = true; break; }
           const Step s = next_boundary(wb);
-          const float wt = step_weight<false>(wb, s, notof);
+          const float wt = step_weight<false>(wb, cb, s, notof);
           ++nb;
           if (wt > 0.0f && (uint32_t)wb.index >= (uint32_t)fo.nvox) { bad = true; break; }
-          if (advance(wb, s)) break;
+          if (advance(wb, cb, s)) break;
         }
         if (!bad) {
           wb = b0;
           for (uint32_t k = 0; k < nb; ++k) {
             const Step s = next_boundary(wb);
-            const float wt = step_weight<false>(wb, s, notof);
+            const float wt = step_weight<false>(wb, cb, s, notof);
             if (wt > 0.0f) unsafeAtomicAdd(acc + wb.index, (double)(wt * r));
-            advance(wb, s);
+            advance(wb, cb, s);
           }
         }
       }

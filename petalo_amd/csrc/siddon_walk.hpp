@@ -43,15 +43,21 @@ struct Lor {
   float x1, y1, z1, x2, y2, z2, dt, ac;
 };
 
-// Walk state.
-struct Walk {
-  float nb_x, nb_y, nb_z;  // distance along the LOR to the next boundary
+// Walk of one LOR, split into the part fixed at setup (WalkC) and the part
+// that moves (WalkS).  Kernels hold WalkC in locals that are only read, so
+// per-axis selects stay register selects (selecting between fields of a
+// struct that lives in memory makes the compiler index it as an array).
+struct WalkC {
   float vl_x, vl_y, vl_z;  // voxel size in LOR-distance units
-  int32_t index;           // current 1-D voxel index
   int32_t d_x, d_y, d_z;   // index change per axis
-  int32_t r_x, r_y, r_z;   // voxels remaining per axis
-  float here;              // distance travelled since the FOV entry
   float tof_peak;          // distance from the entry to the TOF peak
+};
+
+struct WalkS {
+  float nb_x, nb_y, nb_z;  // distance along the LOR to the next boundary
+  float here;              // distance travelled since the FOV entry
+  int32_t index;           // current 1-D voxel index
+  int32_t r_x, r_y, r_z;   // voxels remaining per axis
 };
 
 // nalgebra 0.30.1 Vector3::norm = sqrt((x*x + y*y) + z*z).
@@ -122,7 +128,7 @@ __host__ __device__ __forceinline__ bool fov_entry(const DevFov& f, float x1, fl
 
 // Rust `x.floor() as usize as i32`: saturating to [0, 2^64-1], then the low
 // 32 bits.
-__device__ __forceinline__ int32_t floor_index(float e) {
+__host__ __device__ __forceinline__ int32_t floor_index(float e) {
   const float fl = floorf(e);
   if (!(fl > 0.0f)) return 0;
   if (fl >= 18446744073709551616.0f) return -1;
@@ -134,7 +140,7 @@ __device__ __forceinline__ int32_t floor_index(float e) {
 // find_tof_peak (:156-163), find_entry_point (:138-153),
 // index_trackers (:233-273), voxel_size (:179-185), first_boundaries
 // (:166-174).  Returns false when the LOR misses the FOV.
-__device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, Walk& w) {
+__host__ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, WalkC& c, WalkS& w) {
   float x1 = l.x1, y1 = l.y1, z1 = l.z1, x2 = l.x2, y2 = l.y2, z2 = l.z2;
   const bool fx = (x2 - x1) < 0.0f, fy = (y2 - y1) < 0.0f, fz = (z2 - z1) < 0.0f;
   if (fx) { x1 = -x1; x2 = -x2; }
@@ -148,7 +154,7 @@ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, Walk& 
   const float tof_shift = kC * l.dt / 2.0f;
   const float p1_to_peak = half_len - tof_shift;
   const float p1_to_entry = norm3(ex - x1, ey - y1, ez - z1);
-  w.tof_peak = p1_to_peak - p1_to_entry;
+  c.tof_peak = p1_to_peak - p1_to_entry;
 
   float vx = (ex + f.hw_x) / f.vs_x;
   float vy = (ey + f.hw_y) / f.vs_y;
@@ -158,9 +164,9 @@ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, Walk& 
   if (fabsf(vz) < kEntryEps) vz = 0.0f;
 
   const int32_t ix = floor_index(vx), iy = floor_index(vy), iz = floor_index(vz);
-  w.d_x = fx ? -1 : 1;
-  w.d_y = fy ? -f.nx : f.nx;
-  w.d_z = fz ? -(f.nx * f.ny) : f.nx * f.ny;
+  c.d_x = fx ? -1 : 1;
+  c.d_y = fy ? -f.nx : f.nx;
+  c.d_z = fz ? -(f.nx * f.ny) : f.nx * f.ny;
   w.r_x = f.nx - ix;
   w.r_y = f.ny - iy;
   w.r_z = f.nz - iz;
@@ -171,13 +177,13 @@ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, Walk& 
 
   const float dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
   const float len = norm3(dx, dy, dz);
-  w.vl_x = f.vs_x / (dx / len);
-  w.vl_y = f.vs_y / (dy / len);
-  w.vl_z = f.vs_z / (dz / len);
+  c.vl_x = f.vs_x / (dx / len);
+  c.vl_y = f.vs_y / (dy / len);
+  c.vl_z = f.vs_z / (dz / len);
 
-  w.nb_x = (1.0f - (vx - floorf(vx))) * w.vl_x;
-  w.nb_y = (1.0f - (vy - floorf(vy))) * w.vl_y;
-  w.nb_z = (1.0f - (vz - floorf(vz))) * w.vl_z;
+  w.nb_x = (1.0f - (vx - floorf(vx))) * c.vl_x;
+  w.nb_y = (1.0f - (vy - floorf(vy))) * c.vl_y;
+  w.nb_z = (1.0f - (vz - floorf(vz))) * c.vl_z;
   w.here = 0.0f;
   return true;
 }
@@ -198,19 +204,19 @@ struct Step {
   bool ax, ay, az;
 };
 
-__device__ __forceinline__ Step next_boundary(const Walk& w) {
+__host__ __device__ __forceinline__ Step next_boundary(const WalkS& w) {
   Step s;
   const bool xy = w.nb_x <= w.nb_y;
   s.ax = xy && (w.nb_x <= w.nb_z);
   s.ay = !xy && (w.nb_y <= w.nb_z);
   s.az = !s.ax && !s.ay;
-  s.bnd = s.ax ? w.nb_x : (s.ay ? w.nb_y : w.nb_z);
+  s.bnd = s.ax ? +w.nb_x : (s.ay ? +w.nb_y : +w.nb_z);
   return s;
 }
 
 // TOF Gaussian (src/gauss.rs:47-55) times the 666 mm factor (siddon.rs:106-108),
 // evaluated at the voxel entry distance `here`.
-__device__ __forceinline__ float tof_factor(const DevTof& t, float here, float tof_peak) {
+__host__ __device__ __forceinline__ float tof_factor(const DevTof& t, float here, float tof_peak) {
   const float dx = here - tof_peak;
   if (fabsf(dx) < t.cutoff) {
     const float y = dx / t.sigma;
@@ -223,16 +229,58 @@ __device__ __forceinline__ float tof_factor(const DevTof& t, float here, float t
 // Advance across the boundary of step s (siddon.rs:118-128): only the
 // crossed axis moves.  Returns true when the walk has left the FOV (the
 // crossed axis has no voxels left).
-__device__ __forceinline__ bool advance(Walk& w, const Step& s) {
+__host__ __device__ __forceinline__ bool advance(WalkS& w, const WalkC c, const Step s) {
   w.here = s.bnd;
-  w.nb_x = s.ax ? w.nb_x + w.vl_x : w.nb_x;
-  w.nb_y = s.ay ? w.nb_y + w.vl_y : w.nb_y;
-  w.nb_z = s.az ? w.nb_z + w.vl_z : w.nb_z;
-  w.index += s.ax ? w.d_x : (s.ay ? w.d_y : w.d_z);
+  w.nb_x = s.ax ? w.nb_x + c.vl_x : w.nb_x;
+  w.nb_y = s.ay ? w.nb_y + c.vl_y : w.nb_y;
+  w.nb_z = s.az ? w.nb_z + c.vl_z : w.nb_z;
+  w.index += s.ax ? +c.d_x : (s.ay ? +c.d_y : +c.d_z);
   w.r_x -= (int32_t)s.ax;
   w.r_y -= (int32_t)s.ay;
   w.r_z -= (int32_t)s.az;
   return (s.ax && w.r_x == 0) || (s.ay && w.r_y == 0) || (s.az && w.r_z == 0);
+}
+
+// True when every remaining count is positive: then each step decrements
+// one of them and the walk ends within r_x + r_y + r_z - 2 steps, below the
+// step cap.
+__host__ __device__ __forceinline__ bool bounded(const WalkS& w) { return w.r_x > 0 && w.r_y > 0 && w.r_z > 0; }
+
+// Start of the range of entry distances that may carry TOF weight: any
+// step entering its voxel before this distance has |here - peak| >= cutoff
+// after binary32 rounding (the margin covers the rounding of here - peak
+// for distances below ~8 m), so its weight is exactly 0.
+__host__ __device__ __forceinline__ float tof_window_start(const DevTof& t, float tof_peak) {
+  return (tof_peak - t.cutoff) - (1e-3f + 2e-6f * (fabsf(tof_peak) + t.cutoff));
+}
+
+// Cross, without computing weights, every boundary strictly before T.
+// Each axis's boundary positions are the same repeated binary32 additions
+// the walk performs, so the state reached is exactly the walk's state after
+// its last step whose boundary lies before T (boundaries are crossed in
+// position order, so all of them come first).  Returns false when the walk
+// leaves the FOV before T (crossing the last boundary of an axis), in which
+// case it pushes no element at or after T.  Requires bounded(w).
+__host__ __device__ __forceinline__ bool fast_forward(WalkS& w, const WalkC& c, float T) {
+  float nx = w.nb_x, ny = w.nb_y, nz = w.nb_z, here = w.here;
+  int32_t kx = 0, ky = 0, kz = 0;
+  for (;;) {
+    const bool bx = nx < T, by = ny < T, bz = nz < T;
+    if (!(bx || by || bz)) break;
+    if ((bx && kx + 1 == w.r_x) || (by && ky + 1 == w.r_y) || (bz && kz + 1 == w.r_z)) return false;
+    if (bx) { here = fmaxf(here, nx); nx += c.vl_x; ++kx; }
+    if (by) { here = fmaxf(here, ny); ny += c.vl_y; ++ky; }
+    if (bz) { here = fmaxf(here, nz); nz += c.vl_z; ++kz; }
+  }
+  w.nb_x = nx;
+  w.nb_y = ny;
+  w.nb_z = nz;
+  w.here = here;
+  w.r_x -= kx;
+  w.r_y -= ky;
+  w.r_z -= kz;
+  w.index += kx * c.d_x + ky * c.d_y + kz * c.d_z;
+  return true;
 }
 
 }  // namespace petalo

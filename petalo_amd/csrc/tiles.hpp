@@ -1,0 +1,301 @@
+// tiles.hpp -- tiled backprojection for gfx950.
+//
+// Global float atomics cost one L2 request per distinct 128-byte line per
+// wavefront instruction (about 2e10 requests/s on MI355X), so scattering one
+// lane per LOR into the image is bound by them.  binary64 adds into LDS run
+// at about 1.1e12/s.  The backprojection is therefore split:
+//
+//   once per context   k_bin_runs walks every LOR exactly (fast-forwarded to
+//                      the TOF window) and cuts its pushed steps into runs,
+//                      one per image tile the walk crosses (a 6-connected walk
+//                      is monotone in every flipped axis, so it visits each
+//                      box in one contiguous run).  A run records the walk
+//                      state at its first pushed step and its length.  Runs
+//                      are sorted by tile.
+//   every update       k_fwd computes r = 1/(p*ac) per LOR (walk #1 with the
+//                      gathers).  k_scatter gives each workgroup a chunk of
+//                      one tile's runs: it replays every run from its recorded
+//                      state, computing each weight anew, adds w*r into a
+//                      binary64 LDS copy of the tile, and flushes the tile with
+//                      lane-coalesced binary64 global atomics.
+//
+// LORs whose walk is not provably bounded (an axis with no voxels left at
+// the entry) take the exact one-lane path with global atomics in k_fwd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+#include "siddon_walk.hpp"
+
+namespace petalo {
+
+// Tile shape: nominal dimensions in voxels (edge tiles are cut by the FOV).
+struct TileGeom {
+  int32_t tx, ty, tz;     // nominal tile dimensions
+  int32_t ntx, nty, ntz;  // tiles per axis
+  int32_t ntiles;
+  int32_t tvox;           // tx*ty*tz: LDS entries per tile
+};
+
+// One run: the walk state at the first pushed step of a LOR inside a tile.
+struct __align__(16) Run {
+  float nb_x, nb_y, nb_z, here;
+  float vl_x, vl_y, vl_z, tof_peak;
+  uint32_t local;  // tile-local index of the first voxel (x fastest, nominal strides)
+  uint32_t meta;   // bits 0-15: steps in the run; bits 16-18: axis flips (x, y, z)
+  uint32_t lor;    // index into the context's LOR array
+  uint32_t tile;
+};
+
+constexpr uint32_t kRunMaxSteps = 0xffffu;
+
+// Per-step voxel coordinates (original axes) tracked by the binning walk.
+struct Coord {
+  int32_t x, y, z;
+  int32_t sx, sy, sz;  // +1 or -1 per axis
+};
+
+__host__ __device__ __forceinline__ int32_t tile_of(const TileGeom& g, const Coord& q) {
+  return q.x / g.tx + g.ntx * (q.y / g.ty + g.nty * (q.z / g.tz));
+}
+
+__host__ __device__ __forceinline__ uint32_t local_of(const TileGeom& g, const Coord& q) {
+  return (uint32_t)((q.x % g.tx) + g.tx * ((q.y % g.ty) + g.ty * (q.z % g.tz)));
+}
+
+// Walk #1 with TOF window fast-forward, as in k_mlem_project.  Calls
+// on_step(w, c, s, weight, coord) for every walked step; bounded walks only.
+template <bool TOF, class F>
+__host__ __device__ __forceinline__ void walk_pushes(const Lor& l, const DevFov& f, const DevTof& t, F&& on_step) {
+  WalkC c;
+  WalkS w;
+  if (!walk_setup(l, f, c, w)) return;
+  if (!bounded(w)) return;
+  if (TOF && !fast_forward(w, c, tof_window_start(t, c.tof_peak))) return;
+  Coord q;
+  {
+    const int32_t nxy = f.nx * f.ny;
+    q.z = w.index / nxy;
+    const int32_t rem = w.index - q.z * nxy;
+    q.y = rem / f.nx;
+    q.x = rem - q.y * f.nx;
+    q.sx = c.d_x < 0 ? -1 : 1;
+    q.sy = c.d_y < 0 ? -1 : 1;
+    q.sz = c.d_z < 0 ? -1 : 1;
+  }
+  for (;;) {
+    if (TOF && (w.here - c.tof_peak) >= t.cutoff) break;
+    const Step s = next_boundary(w);
+    const float wt = step_weight<TOF>(w, c, s, t);
+    on_step(w, c, wt, q);
+    q.x += s.ax ? q.sx : 0;
+    q.y += s.ay ? q.sy : 0;
+    q.z += s.az ? q.sz : 0;
+    if (advance(w, c, s)) break;
+  }
+}
+
+// Cuts the pushed steps of LOR i into runs, one per tile crossed; calls
+// emit(run, tile) for each and returns their number.
+template <bool TOF, class Emit>
+__host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, const DevFov& f, const DevTof& t,
+                                                     const TileGeom& g, Emit&& emit) {
+  uint32_t count = 0;
+  int32_t cur = -1;        // tile of the open run
+  uint32_t steps_in = 0;   // steps since the open run's first push
+  uint32_t last_push = 0;  // steps_in at the last push of the open run
+  Run open{};
+  auto close = [&]() {
+    if (cur < 0) return;
+    open.meta = (open.meta & ~0xffffu) | (last_push + 1);
+    emit(open, (uint32_t)cur);
+    ++count;
+    cur = -1;
+  };
+  walk_pushes<TOF>(l, f, t, [&](const WalkS& w, const WalkC& c, float wt, const Coord& q) {
+    const int32_t tl = tile_of(g, q);
+    if (cur >= 0) {
+      ++steps_in;
+      // a run ends where the walk leaves its tile, or when its length field is full
+      if (tl != cur || steps_in >= kRunMaxSteps) close();
+    }
+    if (wt > 0.0f) {
+      if (cur < 0) {
+        cur = tl;
+        steps_in = 0;
+        open.nb_x = w.nb_x;
+        open.nb_y = w.nb_y;
+        open.nb_z = w.nb_z;
+        open.here = w.here;
+        open.vl_x = c.vl_x;
+        open.vl_y = c.vl_y;
+        open.vl_z = c.vl_z;
+        open.tof_peak = c.tof_peak;
+        open.local = local_of(g, q);
+        open.meta = ((q.sx < 0 ? 1u : 0u) | (q.sy < 0 ? 2u : 0u) | (q.sz < 0 ? 4u : 0u)) << 16;
+        open.lor = i;
+        open.tile = (uint32_t)tl;
+      }
+      last_push = steps_in;
+    }
+  });
+  close();
+  return count;
+}
+
+// Binning walk: FILL = false counts runs per LOR into n_runs[i]; FILL = true
+// writes them at runs[offset[i] ...] with their tile ids in keys[].
+template <bool TOF, bool FILL>
+__global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
+                                                  TileGeom g, uint32_t* __restrict__ n_runs,
+                                                  const uint64_t* __restrict__ offset, Run* __restrict__ runs,
+                                                  uint32_t* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Lor l = load_lor(lors, i);
+  const uint64_t base = FILL ? offset[i] : 0;
+  uint32_t k = 0;
+  const uint32_t count = bin_lor<TOF>(l, i, f, t, g, [&](const Run& ru, uint32_t tile) {
+    if (FILL) {
+      runs[base + k] = ru;
+      keys[base + k] = tile;
+    }
+    ++k;
+  });
+  if (!FILL) n_runs[i] = count;
+}
+
+// Replays a run from its recorded state, calling add(local_index, w * r)
+// for every pushed step.  The weights are recomputed with the same binary32
+// operations as walk #1.
+template <bool TOF, class Add>
+__host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, const DevTof& t, const TileGeom& g,
+                                                    Add&& add) {
+  const int32_t dlx = 1, dly = g.tx, dlz = g.tx * g.ty;
+  WalkS w;
+  w.nb_x = ru.nb_x;
+  w.nb_y = ru.nb_y;
+  w.nb_z = ru.nb_z;
+  w.here = ru.here;
+  WalkC c;
+  c.vl_x = ru.vl_x;
+  c.vl_y = ru.vl_y;
+  c.vl_z = ru.vl_z;
+  c.tof_peak = ru.tof_peak;
+  const uint32_t flips = ru.meta >> 16;
+  c.d_x = (flips & 1u) ? -dlx : dlx;
+  c.d_y = (flips & 2u) ? -dly : dly;
+  c.d_z = (flips & 4u) ? -dlz : dlz;
+  w.index = (int32_t)ru.local;
+  w.r_x = w.r_y = w.r_z = 0x7fffffff;  // the run length bounds the replay
+  const uint32_t m = ru.meta & 0xffffu;
+  for (uint32_t k = 0; k < m; ++k) {
+    const Step s = next_boundary(w);
+    const float wt = step_weight<TOF>(w, c, s, t);
+    if (wt > 0.0f) add((uint32_t)w.index, wt * r);
+    advance(w, c, s);
+  }
+}
+
+// Phase 1: forward projection per LOR.  Bounded walks write r = 1/(p*ac) for
+// the tiled scatter; unbounded walks are projected and backprojected here
+// with global binary64 atomics (dropped whole on an out-of-range push, as in
+// projector.rs:103-118).
+template <bool TOF>
+__global__ __launch_bounds__(256) void k_fwd(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
+                                             const float* __restrict__ img, float* __restrict__ r_out,
+                                             double* __restrict__ bp, unsigned long long* __restrict__ counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t skipped = 0;
+  if (i < n) {
+    const Lor l = load_lor(lors, i);
+    WalkC c;
+    WalkS w;
+    float r = 0.0f;
+    if (walk_setup(l, f, c, w)) {
+      const bool bnd = bounded(w);
+      const bool live = !(TOF && bnd) || fast_forward(w, c, tof_window_start(t, c.tof_peak));
+      if (live) {
+        const uint32_t cap = step_cap(f);
+        WalkS start = w;
+        uint32_t steps = 0, start_step = 0;
+        bool started = false, bad = false;
+        float p = 0.0f;
+        for (;;) {
+          if (!bnd && steps >= cap) { bad = true; break; }
+          if (TOF && bnd && (w.here - c.tof_peak) >= t.cutoff) break;
+          const Step s = next_boundary(w);
+          const float wt = step_weight<TOF>(w, c, s, t);
+          if (wt > 0.0f) {
+            if (!started) { start = w; start_step = steps; started = true; }
+            if ((uint32_t)w.index >= (uint32_t)f.nvox) { bad = true; break; }
+            p += wt * img[w.index];
+          }
+          ++steps;
+          if (advance(w, c, s)) break;
+        }
+        r = 1.0f / (p * l.ac);
+        if (bad) {
+          skipped = 1;
+        } else if (!bnd && started) {
+          w = start;
+          for (uint32_t k = start_step; k < steps; ++k) {
+            const Step s = next_boundary(w);
+            const float wt = step_weight<TOF>(w, c, s, t);
+            if (wt > 0.0f) unsafeAtomicAdd(bp + w.index, (double)(wt * r));
+            advance(w, c, s);
+          }
+        }
+      }
+    }
+    r_out[i] = r;
+  }
+  wave_add_u64(counters + CNT_SKIPPED, skipped);
+}
+
+// Phase 2: one workgroup per work item (tile, first run, end run).
+template <bool TOF>
+__global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, const int4* __restrict__ work,
+                                                  const float* __restrict__ r_in, DevFov f, DevTof t, TileGeom g,
+                                                  double* __restrict__ bp) {
+  extern __shared__ double tile[];
+  const int4 wi = work[blockIdx.x];
+  const int32_t tl = wi.x;
+  for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) tile[k] = 0.0;
+  __syncthreads();
+  for (int32_t j = wi.y + (int32_t)threadIdx.x; j < wi.z; j += blockDim.x) {
+    const Run ru = runs[j];
+    const float r = r_in[ru.lor];
+    replay_run<TOF>(ru, r, t, g, [&](uint32_t li, float v) { atomicAdd(tile + li, (double)v); });
+  }
+  __syncthreads();
+  // flush the tile: consecutive lanes take consecutive x, so each wavefront
+  // atomic touches few cache lines
+  const int32_t tzi = tl / (g.ntx * g.nty);
+  const int32_t tyi = (tl / g.ntx) % g.nty;
+  const int32_t txi = tl % g.ntx;
+  const int32_t x0 = txi * g.tx, y0 = tyi * g.ty, z0 = tzi * g.tz;
+  for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) {
+    const double v = tile[k];
+    if (v == 0.0) continue;
+    const int32_t lx = k % g.tx, ly = (k / g.tx) % g.ty, lz = k / (g.tx * g.ty);
+    const int32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    if (x < f.nx && y < f.ny && z < f.nz) unsafeAtomicAdd(bp + x + f.nx * (y + f.ny * z), v);
+  }
+}
+
+// MLEM update with the binary64 backprojection, which is reset:
+// x = s > 0 ? x * (b * s) : 0 with b rounded to binary32 (mlem.rs:45-52).
+__global__ __launch_bounds__(256) void k_update_f64(float* __restrict__ x, double* __restrict__ bp,
+                                                    const float* __restrict__ s, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float b = (float)bp[i];
+  bp[i] = 0.0;
+  const float sj = s ? s[i] : 1.0f;
+  x[i] = sj > 0.0f ? x[i] * (b * sj) : 0.0f;
+}
+
+}  // namespace petalo

@@ -108,13 +108,17 @@ def _run_oracle(full, n, lors, iters, tof=None, sens=None, subsets=1, threads=16
     return out
 
 
+SCHEMES = [P.SCHEME_ATOMIC, P.SCHEME_TILE]
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
 @pytest.mark.parametrize("tof", [None, (42.47, 3.0)])
-def test_mlem_small(gpu, tof):
+def test_mlem_small(gpu, tof, scheme):
     full, n = [48.0, 48.0, 40.0], [12, 12, 10]
     rng = np.random.default_rng(3)
     lors = random_lors(rng, 20000, dt_sigma_ps=84.93 if tof else 0.0)
     fov = P.FOV(full, n)
-    g = _run_gpu(fov, lors, 5, tof=P.Tof(*tof) if tof else None)
+    g = _run_gpu(fov, lors, 5, tof=P.Tof(*tof) if tof else None, scheme=scheme)
     o = _run_oracle(full, n, lors, 5, tof=tof)
     for k in range(5):
         e = rel_l2(g[k], o[k])
@@ -122,13 +126,14 @@ def test_mlem_small(gpu, tof):
         assert e <= 1e-5
 
 
-def test_mlem_config1_full(gpu):
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_mlem_config1_full(gpu, scheme):
     """60^3 x 3 mm, 1e6 phantom LORs, non-TOF, 10 iterations: <= 1e-4 relative L2."""
     full, n = [180.0] * 3, [60] * 3
     lors = P.synth_phantom_lors(1, 1_000_000, 0.0)
     fov = P.FOV(full, n)
-    _say("config1: GPU 10 iterations")
-    g = _run_gpu(fov, lors, 10)
+    _say(f"config1 scheme {scheme}: GPU 10 iterations")
+    g = _run_gpu(fov, lors, 10, scheme=scheme)
     _say("config1: oracle 10 iterations")
     o = _run_oracle(full, n, lors, 10)
     o64 = _run_oracle(full, n, lors, 10, f64=True)
@@ -137,14 +142,15 @@ def test_mlem_config1_full(gpu):
     assert e <= 1e-4
 
 
-def test_mlem_tof_120(gpu):
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_mlem_tof_120(gpu, scheme):
     """120^3 x 1.5 mm TOF (200 ps FWHM data, sigma 42.47 ps, cutoff 3) with 1e6 LORs, 10 iterations."""
     full, n = [180.0] * 3, [120] * 3
     lors = P.synth_phantom_lors(2, 1_000_000, 200.0)
     tof = (42.47, 3.0)
     fov = P.FOV(full, n)
-    _say("tof120: GPU")
-    g = _run_gpu(fov, lors, 10, tof=P.Tof(*tof))
+    _say(f"tof120 scheme {scheme}: GPU")
+    g = _run_gpu(fov, lors, 10, tof=P.Tof(*tof), scheme=scheme)
     _say("tof120: oracle")
     o = _run_oracle(full, n, lors, 10, tof=tof)
     e = rel_l2(g[-1], o[-1])
@@ -152,11 +158,12 @@ def test_mlem_tof_120(gpu):
     assert e <= 1e-4
 
 
-def test_osem_subsets(gpu):
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_osem_subsets(gpu, scheme):
     full, n = [180.0] * 3, [30] * 3
     lors = P.synth_phantom_lors(5, 100_003, 0.0)  # remainder LORs are dropped
     fov = P.FOV(full, n)
-    it = P.mlem(fov, lors, n_subsets=4)
+    it = P.mlem(fov, lors, n_subsets=4, scheme=scheme)
     ids = []
     g = []
     for _ in range(8):
@@ -208,7 +215,8 @@ def test_walk_stats_match_oracle(gpu):
     assert abs(st["n_nz"] - ost["n_nz"]) <= 1e-4 * ost["n_nz"]  # TOF cut edge under expf ulps
 
 
-def test_edge_cases(gpu):
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_edge_cases(gpu, scheme):
     """Empty rows, degenerate LORs, out-of-range skips, zero projections,
     s <= 0 voxels, more subsets than LORs (projector.rs:103-118, mlem.rs:45-52,90-95)."""
     full, n = [30.0, 30.0, 30.0], [3, 3, 3]
@@ -227,7 +235,7 @@ def test_edge_cases(gpu):
     # MLEM with a sensitivity image containing zeros and negatives
     sens = np.ones(fov.nvox, np.float32)
     sens[0], sens[5] = 0.0, -1.0
-    g = _run_gpu(fov, lors, 2, sens=sens)
+    g = _run_gpu(fov, lors, 2, sens=sens, scheme=scheme)
     o = _run_oracle(full, n, lors, 2, sens=sens)
     for a, b in zip(g, o):
         assert np.array_equal(np.isnan(a), np.isnan(b))
@@ -235,5 +243,25 @@ def test_edge_cases(gpu):
         assert np.array_equal(np.isfinite(a), fin)
         assert np.allclose(a[fin], b[fin], rtol=1e-6, atol=0)
     # more subsets than LORs: empty subsets zero the image (x * 0 * s)
-    g = _run_gpu(fov, lors, 1, subsets=10)
+    g = _run_gpu(fov, lors, 1, subsets=10, scheme=scheme)
     assert all(np.all(x == 0) for x in g)
+
+
+@pytest.mark.parametrize("tof", [None, (31.85, 3.0)])
+def test_tile_scheme_odd_shapes(gpu, tof):
+    """Images whose dimensions are not multiples of the tile shape, with
+    oblique and axis-parallel LORs: tiled and atomic schemes agree with the oracle."""
+    full, n = [100.0, 130.0, 170.0], [37, 51, 29]
+    rng = np.random.default_rng(17)
+    lors = random_lors(rng, 30000, radius=200.0, zhalf=120.0, dt_sigma_ps=63.7 if tof else 0.0)
+    lors["p2"][::11, 0] = lors["p1"][::11, 0]
+    lors["p2"][::7, 2] = lors["p1"][::7, 2]
+    fov = P.FOV(full, n)
+    t = P.Tof(*tof) if tof else None
+    o = _run_oracle(full, n, lors, 3, tof=tof)
+    for scheme in SCHEMES:
+        g = _run_gpu(fov, lors, 3, tof=t, scheme=scheme)
+        for k in range(3):
+            e = rel_l2(g[k], o[k])
+            _say(f"odd shapes tof={tof} scheme={scheme} iter {k + 1}: relL2 {e:.2e}")
+            assert e <= 1e-5
