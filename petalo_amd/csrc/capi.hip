@@ -272,7 +272,7 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
       off[i] = total;
       total += h[i];
     }
-    if (total >= (1ull << 32)) throw std::runtime_error("more than 2^32 tile runs in one subset");
+    if (total >= (1ull << 31)) throw std::runtime_error("2^31 or more tile runs in one subset");
     DevBuf<uint64_t> doff(count);
     HIP_CHECK(hipMemcpyAsync(doff.p, off.data(), count * 8, hipMemcpyHostToDevice, st));
     DevBuf<Run> unsorted(std::max<uint64_t>(total, 1));
