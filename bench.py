@@ -8,9 +8,11 @@ One step = one MLEM iteration (1 subset) over all LORs.
 
 Prints ONE JSON line:
   value      MLEM iterations/s (inputs resident in HBM before timing)
-  roofline   the projection kernel's algorithmic bytes per launch
-             (32 B per LOR + 8 B per pushed element) / its average duration
-             from HIP events on the context's stream, against 8 TB/s
+  roofline   the dominant kernel (k_fwd: forward projection) -- its
+             algorithmic bytes per launch (32 B per LOR + 4 B per pushed
+             element) / its average duration from HIP events on the context's
+             stream, against 8 TB/s; traffic from the committed rocprofv3
+             FETCH_SIZE/WRITE_SIZE profile of the same workload
   cpu_baseline  the CPU oracle (binary32 restatement of the reference, all
              host threads, 10 000-LOR jobs with private buffers) on a
              bounded sample of the same workload, scaled to iterations/s
@@ -45,10 +47,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tof_mlem_120", choices=sorted(WORKLOADS))
-    ap.add_argument("--scheme", type=int, default=0, help="0 auto, 1 atomic, 2 slab")
+    ap.add_argument("--scheme", type=int, default=0, help="0 auto, 1 atomic, 2 tiled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-lors", type=int, default=1_000_000)
     ap.add_argument("--no-sensitivity", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run a plain device copy of this size first (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported in this build "
@@ -57,6 +61,11 @@ def main():
         return 2
 
     import petalo_amd as P
+
+    if a.copy_probe_mb:
+        ms = P.copy_probe(a.copy_probe_mb << 20, 3)
+        log(f"[bench] copy probe {a.copy_probe_mb} MiB: {ms:.3f} ms per copy "
+            f"({2 * (a.copy_probe_mb << 20) / ms / 1e6:.0f} GB/s read+write)")
 
     full, nvx, n_lors, fwhm, tof, n_sens, subsets = WORKLOADS[a.workload]
     fov = P.FOV([full] * 3, [nvx] * 3)
@@ -111,8 +120,24 @@ def main():
     it_per_s = a.steps / elapsed
     steps_per_s = ws["n_steps"] * a.steps / elapsed
     proj_avg = float(np.mean(proj_ms))
-    alg_bytes = 32 * ws["n_lors"] + 8 * ws["n_nz"]
-    achieved = alg_bytes / (proj_avg * 1e-3) / 1e9
+    fwd_avg = float(np.mean(fwd_ms))
+    sct_avg = float(np.mean(sct_ms))
+    # Algorithmic bytes: per LOR its 32-byte record, per pushed element one
+    # 4-byte image gather (forward) and one 4-byte backprojection add (scatter).
+    fwd_bytes = 32 * ws["n_lors"] + 4 * ws["n_nz"]
+    sct_bytes = 4 * ws["n_nz"]
+    tiled = info["scheme"] == P.SCHEME_TILE
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{a.workload}.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        if tj.get("scheme") == info["scheme"]:
+            traffic = {k: v["read_bytes"] + v["write_bytes"] for k, v in tj["kernels"].items()}
+    if tiled:
+        dom, dom_bytes, dom_ms = "k_fwd", fwd_bytes, fwd_avg
+    else:
+        dom, dom_bytes, dom_ms = "k_mlem_project", fwd_bytes + sct_bytes, proj_avg
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     log(f"[bench] {it_per_s:.2f} iter/s; projection {proj_avg:.3f} ms (forward {np.mean(fwd_ms):.3f}, "
         f"scatter {np.mean(sct_ms):.3f}), update {np.mean(upd_ms):.3f} ms")
 
@@ -172,7 +197,13 @@ def main():
         "sensitivity_ms": sens_ms,
         "parity_rel_l2_sample": parity,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get(dom), "kernel": dom,
+                     "algorithmic_bytes": dom_bytes, "kernel_ms": dom_ms},
+        "roofline_scatter": ({"achieved": sct_bytes / (sct_avg * 1e-3) / 1e9, "unit": "GB/s",
+                              "algorithmic_bytes": sct_bytes, "kernel_ms": sct_avg,
+                              "traffic": traffic.get("k_scatter")} if tiled else None),
+        "roofline_projection": {"achieved": (fwd_bytes + sct_bytes) / (proj_avg * 1e-3) / 1e9, "unit": "GB/s",
+                                "frac": (fwd_bytes + sct_bytes) / (proj_avg * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -98,6 +98,10 @@ int petalo_version(void);
 int petalo_device_count(int* count);
 /* Blocks until all work on the given device has finished. */
 int petalo_device_synchronize(int device);
+/* Calibration probe for hardware counters: copies `bytes` between two device
+ * buffers with a plain streaming kernel (k_copy_probe) iters+1 times and
+ * returns the average time of the last iters copies. */
+int petalo_copy_probe(size_t bytes, int iters, double* ms_per_copy);
 
 /* ---- FOV (src/fov.rs:22-48,75-91) ------------------------------------- */
 int petalo_fov_derive(const petalo_fov_t* fov, float half_width[3], float voxel_size[3]);

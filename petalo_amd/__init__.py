@@ -91,6 +91,7 @@ _SIGS = {
     "petalo_version": (C.c_int, []),
     "petalo_device_count": (C.c_int, [_P]),
     "petalo_device_synchronize": (C.c_int, [C.c_int]),
+    "petalo_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _P]),
     "petalo_fov_derive": (C.c_int, [_P, _P, _P]),
     "petalo_fov_parse": (C.c_int, [C.c_char_p, _P]),
     "petalo_fov_voxel_centre": (C.c_int, [_P, _P, _P]),
@@ -147,6 +148,13 @@ def _check(rc):
 
 def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def copy_probe(nbytes, iters=3):
+    """Plain device copy of nbytes (calibration for traffic counters); ms per copy."""
+    ms = C.c_double(0)
+    _check(lib().petalo_copy_probe(int(nbytes), int(iters), C.byref(ms)))
+    return ms.value
 
 
 def device_count():

@@ -637,6 +637,41 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
 }
 
 // ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+}  // namespace
+
+extern "C" {
+
+int petalo_copy_probe(size_t bytes, int iters, double* ms_per_copy) {
+  return guarded([&] {
+    if (bytes < 16 || iters < 1) return fail(PETALO_ERR_ARG, "need at least 16 bytes and 1 iteration");
+    require_device();
+    const size_t n = bytes / 16;
+    DevBuf<float4> a(n), b(n);
+    HIP_CHECK(hipMemset(a.p, 0, n * 16));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    k_copy_probe<<<2048, 256>>>(a.p, b.p, n);
+    HIP_CHECK(hipEventRecord(e0));
+    for (int k = 0; k < iters; ++k) k_copy_probe<<<2048, 256>>>(a.p, b.p, n);
+    HIP_CHECK(hipEventRecord(e1));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_per_copy) *ms_per_copy = ms / iters;
+    return PETALO_OK;
+  });
+}
+
 int petalo_system_matrix_rows(const petalo_fov_t* fov, const petalo_tof_t* tof, const petalo_lor_t* lors,
                               size_t n_lors, size_t cap, int32_t* idx, float* w, uint32_t* count,
                               uint32_t* steps) {
