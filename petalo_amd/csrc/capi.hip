@@ -83,7 +83,7 @@ DevFov to_dev(const petalo_fov_t& f) { return make_fov(f.full_size_mm, f.n); }
 uint64_t nvox(const petalo_fov_t& f) { return (uint64_t)f.n[0] * f.n[1] * f.n[2]; }
 
 DevTof to_dev(const petalo_tof_t* t) {
-  if (!t || !t->enabled) return DevTof{1.0f, INFINITY, 0.0f, 0};
+  if (!t || !t->enabled) return no_tof();
   return make_tof(true, t->sigma_ps, t->cutoff);
 }
 
@@ -129,11 +129,11 @@ void require_device() {
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw HipError("no HIP device available");
 }
 
-// Ordering key of a LOR: azimuth of its direction (folded to [0, pi)), then
-// signed transaxial distance from the axis, then axial midpoint.  LORs close
-// in this order are near-parallel neighbours: their walks cross the same
-// voxels, so a wavefront's gathers and atomics share cache lines and its
-// lanes take similar numbers of steps.
+// Ordering key of a LOR inside its group: azimuth of its direction (folded
+// to [0, pi)), then signed transaxial distance from the axis, then axial
+// midpoint.  LORs close in this order are near-parallel neighbours: their
+// walks cross the same voxels, so a wavefront's gathers and LDS adds share
+// cache lines and its lanes take similar numbers of steps.
 uint64_t lor_order_key(const petalo_lor_t& l) {
   const double dx = (double)l.p2[0] - l.p1[0], dy = (double)l.p2[1] - l.p1[1];
   double phi = std::atan2(dy, dx);
@@ -152,22 +152,54 @@ uint64_t lor_order_key(const petalo_lor_t& l) {
   return (q(phi, 0.0, M_PI, 10) << 40) | (q(s, -400.0, 400.0, 20) << 20) | q(zm, -400.0, 400.0, 20);
 }
 
-void order_lors(std::vector<petalo_lor_t>& lors, size_t lo, size_t hi) {
+// Axial position of the LOR's TOF peak (its midpoint without TOF): where its
+// weighted voxels cluster.
+double lor_centre_z(const petalo_lor_t& l, bool tof) {
+  if (!tof) return 0.5 * ((double)l.p1[2] + l.p2[2]);
+  const double d[3] = {(double)l.p2[0] - l.p1[0], (double)l.p2[1] - l.p1[1], (double)l.p2[2] - l.p1[2]};
+  const double L = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (!(L > 0)) return l.p1[2];
+  double t = 0.5 - 0.299792458 * (double)l.dt_ps / (2.0 * L);
+  t = std::min(std::max(t, 0.0), 1.0);
+  return (double)l.p1[2] + t * d[2];
+}
+
+template <class F> void parallel_for(size_t n, F&& f) {
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (size_t k = t; k < n; k += nt) f(k);
+    });
+  for (auto& x : th) x.join();
+}
+
+// Number of XCDs a launch's workgroups are spread over, round-robin by
+// workgroup index (MI355X: 8).
+constexpr int kGroups = 8;
+
+// Reorders lors[lo, hi) (one subset) into kGroups equal-size groups by the
+// axial position of their TOF peak, each group in lor_order_key order.  The
+// forward kernel gives group g to the workgroups that land on XCD g, so each
+// XCD gathers from about a quarter of the image, which its L2 holds.
+void order_lors(std::vector<petalo_lor_t>& lors, size_t lo, size_t hi, bool tof) {
   const size_t m = hi - lo;
   if (m < 2) return;
+  std::vector<std::pair<double, uint32_t>> zc(m);
+  parallel_for(m, [&](size_t k) { zc[k] = {lor_centre_z(lors[lo + k], tof), (uint32_t)k}; });
+  std::sort(zc.begin(), zc.end());
   std::vector<std::pair<uint64_t, uint32_t>> key(m);
-  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  {
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; ++t)
-      th.emplace_back([&, t] {
-        for (size_t k = t; k < m; k += nt) key[k] = {lor_order_key(lors[lo + k]), (uint32_t)k};
-      });
-    for (auto& x : th) x.join();
-  }
-  std::sort(key.begin(), key.end());
+  const size_t gs = (m + kGroups - 1) / kGroups;
+  parallel_for(m, [&](size_t r) {
+    const uint32_t k = zc[r].second;
+    key[r] = {lor_order_key(lors[lo + k]), k};
+  });
+  parallel_for(kGroups, [&](size_t g) {
+    const size_t b = std::min(m, g * gs), e = std::min(m, (g + 1) * gs);
+    std::sort(key.begin() + b, key.begin() + e);
+  });
   std::vector<petalo_lor_t> tmp(lors.begin() + lo, lors.begin() + hi);
-  for (size_t k = 0; k < m; ++k) lors[lo + k] = tmp[key[k].second];
+  parallel_for(m, [&](size_t r) { lors[lo + r] = tmp[key[r].second]; });
 }
 
 }  // namespace
@@ -226,13 +258,17 @@ TileGeom make_tiles(const petalo_fov_t& fov) {
   g.ntz = (int32_t)((fov.n[2] + g.tz - 1) / g.tz);
   g.ntiles = g.ntx * g.nty * g.ntz;
   g.tvox = g.tx * g.ty * g.tz;
+  if (g.ntiles >= 65536) throw std::runtime_error("image too large for the tiled scheme (65536 tiles or more)");
   return g;
 }
 
+// Sorted keys are (tile << 16) | (0xffff - length): the first position of
+// each tile is where the high half changes.
 __global__ void k_tile_starts(const uint32_t* __restrict__ keys, uint64_t n, uint64_t* __restrict__ start) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  if (j == 0 || keys[j] != keys[j - 1]) start[keys[j]] = j;
+  const uint32_t t = keys[j] >> 16;
+  if (j == 0 || (keys[j - 1] >> 16) != t) start[t] = j;
 }
 
 __global__ void k_gather_runs(const Run* __restrict__ in, const uint32_t* __restrict__ perm, uint64_t n,
@@ -289,8 +325,8 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     std::vector<uint64_t> starts(g.ntiles + 1, UINT64_MAX);
     if (total) {
       k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
-      int end_bit = 1;
-      while ((1 << end_bit) < g.ntiles) ++end_bit;
+      int end_bit = 17;
+      while ((1 << (end_bit - 16)) < g.ntiles) ++end_bit;
       size_t tmp_bytes = 0;
       HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p,
                                                    (int)total, 0, end_bit, st));
@@ -326,13 +362,15 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
 void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   if (count == 0) return;
   const Lor* L = c->lors.p + lo;
-  const unsigned g = grid_for(count);
+  // workgroup b -> group b % kGroups (its XCD), chunk b / kGroups of that group
+  const uint32_t gs = (uint32_t)((count + kGroups - 1) / kGroups);
+  const unsigned g = (unsigned)(kGroups * ((gs + kBlock - 1) / kBlock));
   hipStream_t st = c->stream;
   if (c->tof_on)
-    k_fwd<true><<<g, kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, c->image.p, c->rbuf.p, c->bp64.p,
+    k_fwd<true><<<g, kBlock, 0, st>>>(L, (uint32_t)count, gs, c->dfov, c->dtof, c->image.p, c->rbuf.p, c->bp64.p,
                                       c->counters.p);
   else
-    k_fwd<false><<<g, kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, c->image.p, c->rbuf.p, c->bp64.p,
+    k_fwd<false><<<g, kBlock, 0, st>>>(L, (uint32_t)count, gs, c->dfov, c->dtof, c->image.p, c->rbuf.p, c->bp64.p,
                                        c->counters.p);
   HIP_CHECK(hipGetLastError());
   if (c->timing) HIP_CHECK(hipEventRecord(c->ev[1], st));
@@ -428,7 +466,7 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
       // LORs: reorder inside each subset (subset membership is unchanged).
       std::vector<petalo_lor_t> h(lors, lors + n_lors);
       if (opt.sort_lors)
-        for (size_t s = 0; s < n_subsets; ++s) order_lors(h, s * c->set_size, (s + 1) * c->set_size);
+        for (size_t s = 0; s < n_subsets; ++s) order_lors(h, s * c->set_size, (s + 1) * c->set_size, c->tof_on);
       c->lors.alloc(std::max<size_t>(n_lors, 1));
       if (n_lors) HIP_CHECK(hipMemcpy(c->lors.p, h.data(), n_lors * sizeof(Lor), hipMemcpyHostToDevice));
 

@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
                                               double* __restrict__ acc, unsigned long long* __restrict__ counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t c_lor = 0, c_steps = 0, c_nz = 0, c_skip = 0, c_miss = 0, c_run = 0;
-  const DevTof notof{};
+  const DevTof notof = no_tof();
   if (i < n) {
     c_lor = 1;
     Lor l;

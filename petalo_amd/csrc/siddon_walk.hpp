@@ -33,11 +33,15 @@ struct DevFov {
 };
 
 struct DevTof {
-  float sigma;   // mm (sigma_t * C)
-  float cutoff;  // mm (cutoff * sigma)
-  float peak;    // 1 / (sigma * sqrt(TAU))
+  float sigma;      // mm (sigma_t * C)
+  float cutoff;     // mm (cutoff * sigma)
+  float peak;       // 1 / (sigma * sqrt(TAU))
+  float rcp_sigma;  // 1 / sigma, correctly rounded
   int32_t on;
 };
+
+// No TOF: the window test never applies.
+__host__ __device__ inline DevTof no_tof() { return DevTof{1.0f, INFINITY, 0.0f, 1.0f, 0}; }
 
 struct Lor {
   float x1, y1, z1, x2, y2, z2, dt, ac;
@@ -89,7 +93,20 @@ inline DevTof make_tof(bool on, float sigma_ps, float cutoff) {
   const float root_two_pi = sqrtf(kTau);
   t.peak = 1.0f / (t.sigma * root_two_pi);
   t.cutoff = cutoff * t.sigma;
+  t.rcp_sigma = 1.0f / t.sigma;
   return t;
+}
+
+// a / b correctly rounded, from the correctly rounded reciprocal of b:
+// q0 = a*rcp is within one ulp of a/b, the residual a - q0*b is exact in one
+// FMA, and q0 + residual*rcp rounds to the correctly rounded quotient
+// (Markstein, IBM J. Res. Dev. 34 (1990) 111).  Checked exhaustively against
+// binary32 division for every |a| in [2^-149, 2^12) with the TOF widths used
+// here; three instructions instead of the general division sequence.
+__host__ __device__ __forceinline__ float div_rn(float a, float b, float rcp) {
+  const float q0 = a * rcp;
+  const float r = fmaf(-q0, b, a);
+  return fmaf(r, rcp, q0);
 }
 
 // One slab of the ray/box cast (ncollide3d 0.33.0 AABB cast, called from
@@ -204,14 +221,29 @@ struct Step {
   bool ax, ay, az;
 };
 
+// The minimum of the three and equality tests against it pick the same
+// axis as the comparison chain x<=y ? (x<=z ? x : z) : (y<=z ? y : z)
+// whenever no NaN is present, and z when all three are NaN (p1 == p2).
+// Mixed NaN boundaries do not arise from finite endpoints.
 __host__ __device__ __forceinline__ Step next_boundary(const WalkS& w) {
   Step s;
-  const bool xy = w.nb_x <= w.nb_y;
-  s.ax = xy && (w.nb_x <= w.nb_z);
-  s.ay = !xy && (w.nb_y <= w.nb_z);
+  s.bnd = fminf(fminf(w.nb_x, w.nb_y), w.nb_z);
+  s.ax = w.nb_x == s.bnd;
+  s.ay = !s.ax && (w.nb_y == s.bnd);
   s.az = !s.ax && !s.ay;
-  s.bnd = s.ax ? +w.nb_x : (s.ay ? +w.nb_y : +w.nb_z);
   return s;
+}
+
+// exp(x) for the Gaussian, x <= 0.  On the device the hardware exp2 of
+// x*log2(e) (a few ulp from the correctly rounded value; the reference's
+// f32::exp is the host libm's, so TOF weights are compared within a
+// tolerance in any case).
+__host__ __device__ __forceinline__ float tof_exp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __expf(x);
+#else
+  return expf(x);
+#endif
 }
 
 // TOF Gaussian (src/gauss.rs:47-55) times the 666 mm factor (siddon.rs:106-108),
@@ -219,9 +251,9 @@ __host__ __device__ __forceinline__ Step next_boundary(const WalkS& w) {
 __host__ __device__ __forceinline__ float tof_factor(const DevTof& t, float here, float tof_peak) {
   const float dx = here - tof_peak;
   if (fabsf(dx) < t.cutoff) {
-    const float y = dx / t.sigma;
+    const float y = div_rn(dx, t.sigma, t.rcp_sigma);
     const float z = y * y;
-    return kTofFactor * (t.peak * expf(-0.5f * z));
+    return kTofFactor * (t.peak * tof_exp(-0.5f * z));
   }
   return 0.0f;
 }
@@ -262,20 +294,21 @@ __host__ __device__ __forceinline__ float tof_window_start(const DevTof& t, floa
 // leaves the FOV before T (crossing the last boundary of an axis), in which
 // case it pushes no element at or after T.  Requires bounded(w).
 __host__ __device__ __forceinline__ bool fast_forward(WalkS& w, const WalkC& c, float T) {
-  float nx = w.nb_x, ny = w.nb_y, nz = w.nb_z, here = w.here;
+  // Each axis on its own: count its boundaries before T (bounded by its
+  // remaining voxels, so an axis at -inf cannot loop forever) and keep the
+  // last one crossed.
+  float nx = w.nb_x, ny = w.nb_y, nz = w.nb_z;
+  float hx = w.here, hy = w.here, hz = w.here;
   int32_t kx = 0, ky = 0, kz = 0;
-  for (;;) {
-    const bool bx = nx < T, by = ny < T, bz = nz < T;
-    if (!(bx || by || bz)) break;
-    if ((bx && kx + 1 == w.r_x) || (by && ky + 1 == w.r_y) || (bz && kz + 1 == w.r_z)) return false;
-    if (bx) { here = fmaxf(here, nx); nx += c.vl_x; ++kx; }
-    if (by) { here = fmaxf(here, ny); ny += c.vl_y; ++ky; }
-    if (bz) { here = fmaxf(here, nz); nz += c.vl_z; ++kz; }
-  }
+  while (nx < T && kx < w.r_x) { hx = nx; nx += c.vl_x; ++kx; }
+  while (ny < T && ky < w.r_y) { hy = ny; ny += c.vl_y; ++ky; }
+  while (nz < T && kz < w.r_z) { hz = nz; nz += c.vl_z; ++kz; }
+  // Crossing the last boundary of an axis ends the walk.
+  if (kx == w.r_x || ky == w.r_y || kz == w.r_z) return false;
   w.nb_x = nx;
   w.nb_y = ny;
   w.nb_z = nz;
-  w.here = here;
+  w.here = fmaxf(fmaxf(hx, hy), hz);
   w.r_x -= kx;
   w.r_y -= ky;
   w.r_z -= kz;

@@ -160,7 +160,9 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
   const uint32_t count = bin_lor<TOF>(l, i, f, t, g, [&](const Run& ru, uint32_t tile) {
     if (FILL) {
       runs[base + k] = ru;
-      keys[base + k] = tile;
+      // sort key: tile, then longest run first (lanes of a wavefront get
+      // runs of similar length)
+      keys[base + k] = (tile << 16) | (0xffffu - (ru.meta & 0xffffu));
     }
     ++k;
   });
@@ -199,15 +201,20 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
   }
 }
 
-// Phase 1: forward projection per LOR.  Bounded walks write r = 1/(p*ac) for
+// Phase 1: forward projection per LOR, LORs in eight axial groups (see the
+// LOR ordering in capi.hip).  Bounded walks write r = 1/(p*ac) for
 // the tiled scatter; unbounded walks are projected and backprojected here
 // with global binary64 atomics (dropped whole on an out-of-range push, as in
 // projector.rs:103-118).
 template <bool TOF>
-__global__ __launch_bounds__(256) void k_fwd(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
-                                             const float* __restrict__ img, float* __restrict__ r_out,
-                                             double* __restrict__ bp, unsigned long long* __restrict__ counters) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_fwd(const Lor* __restrict__ lors, uint32_t n, uint32_t group_size,
+                                             DevFov f, DevTof t, const float* __restrict__ img,
+                                             float* __restrict__ r_out, double* __restrict__ bp,
+                                             unsigned long long* __restrict__ counters) {
+  // Workgroups land on the XCDs round-robin: workgroup b works on LOR group
+  // b % 8 (chunk b / 8 of it), so each XCD's L2 sees one axial band.
+  const uint32_t off = (blockIdx.x >> 3) * blockDim.x + threadIdx.x;
+  const uint32_t i = off < group_size ? (blockIdx.x & 7u) * group_size + off : n;
   uint32_t skipped = 0;
   if (i < n) {
     const Lor l = load_lor(lors, i);
@@ -215,17 +222,36 @@ __global__ __launch_bounds__(256) void k_fwd(const Lor* __restrict__ lors, uint3
     WalkS w;
     float r = 0.0f;
     if (walk_setup(l, f, c, w)) {
-      const bool bnd = bounded(w);
-      const bool live = !(TOF && bnd) || fast_forward(w, c, tof_window_start(t, c.tof_peak));
-      if (live) {
+      if (bounded(w)) {
+        // Every visited voxel lies inside the image: no index check, no cap.
+        if (!TOF || fast_forward(w, c, tof_window_start(t, c.tof_peak))) {
+          // p = sum w*x in walk order; each product is added one step late
+          // so that the gather's latency overlaps the next step.
+          float p = 0.0f, pw = 0.0f, px = 0.0f;
+          for (;;) {
+            if (TOF && (w.here - c.tof_peak) >= t.cutoff) break;
+            const Step s = next_boundary(w);
+            const float wt = step_weight<TOF>(w, c, s, t);
+            const bool push = wt > 0.0f;
+            const float xv = push ? img[w.index] : 0.0f;
+            p += pw * px;
+            pw = push ? wt : 0.0f;
+            px = xv;
+            if (advance(w, c, s)) break;
+          }
+          p += pw * px;
+          r = 1.0f / (p * l.ac);
+        } else {
+          r = 1.0f / (0.0f * l.ac);
+        }
+      } else {
         const uint32_t cap = step_cap(f);
         WalkS start = w;
         uint32_t steps = 0, start_step = 0;
         bool started = false, bad = false;
         float p = 0.0f;
         for (;;) {
-          if (!bnd && steps >= cap) { bad = true; break; }
-          if (TOF && bnd && (w.here - c.tof_peak) >= t.cutoff) break;
+          if (steps >= cap) { bad = true; break; }
           const Step s = next_boundary(w);
           const float wt = step_weight<TOF>(w, c, s, t);
           if (wt > 0.0f) {
@@ -239,7 +265,7 @@ __global__ __launch_bounds__(256) void k_fwd(const Lor* __restrict__ lors, uint3
         r = 1.0f / (p * l.ac);
         if (bad) {
           skipped = 1;
-        } else if (!bnd && started) {
+        } else if (started) {
           w = start;
           for (uint32_t k = start_step; k < steps; ++k) {
             const Step s = next_boundary(w);
