@@ -228,6 +228,9 @@ struct petalo_mlem {
   std::vector<DevBuf<Run>> runs;     // per subset, sorted by tile
   std::vector<DevBuf<int4>> work;    // per subset: (tile, first run, end run, 0)
   std::vector<uint64_t> n_runs;      // per subset
+  std::vector<DevBuf<uint32_t>> run_off;    // per subset: LOR-major run offsets (n + 1)
+  std::vector<DevBuf<uint32_t>> irregular;  // per subset: LORs with unbounded walks
+  DevBuf<float> partial;             // per-run partial projections (largest subset)
   DevBuf<float> rbuf;                // 1/(p*ac) per LOR of the current subset
   DevBuf<double> bp64;               // binary64 backprojection
 };
@@ -285,11 +288,13 @@ __global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
 inline unsigned grid64(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // Cuts the subset's LOR walks into per-tile runs, sorted by tile, and the
-// per-workgroup work items over them.
+// per-workgroup work items over them; lists the LORs with unbounded walks.
 void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   const TileGeom& g = c->tiles;
   hipStream_t st = c->stream;
   uint64_t total = 0;
+  std::vector<uint32_t> irr;
+  std::vector<uint32_t> off(count + 1, 0);
   if (count) {
     DevBuf<uint32_t> nr(count);
     const Lor* L = c->lors.p + lo;
@@ -303,78 +308,98 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     std::vector<uint32_t> h(count);
     HIP_CHECK(hipMemcpyAsync(h.data(), nr.p, count * 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    std::vector<uint64_t> off(count);
     for (size_t i = 0; i < count; ++i) {
-      off[i] = total;
-      total += h[i];
+      off[i] = (uint32_t)total;
+      if (h[i] == 0xffffffffu) {
+        irr.push_back((uint32_t)i);
+      } else {
+        total += h[i];
+        if (total >= (1ull << 31)) throw std::runtime_error("2^31 or more tile runs in one subset");
+      }
     }
-    if (total >= (1ull << 31)) throw std::runtime_error("2^31 or more tile runs in one subset");
-    DevBuf<uint64_t> doff(count);
-    HIP_CHECK(hipMemcpyAsync(doff.p, off.data(), count * 8, hipMemcpyHostToDevice, st));
-    DevBuf<Run> unsorted(std::max<uint64_t>(total, 1));
-    DevBuf<uint32_t> keys(std::max<uint64_t>(total, 1)), keys2(std::max<uint64_t>(total, 1));
-    DevBuf<uint32_t> perm(std::max<uint64_t>(total, 1)), perm2(std::max<uint64_t>(total, 1));
+    off[count] = (uint32_t)total;
+  }
+  c->run_off[sub].alloc(count + 1);
+  HIP_CHECK(hipMemcpyAsync(c->run_off[sub].p, off.data(), (count + 1) * 4, hipMemcpyHostToDevice, st));
+  c->irregular[sub].alloc(std::max<size_t>(irr.size(), 1));
+  if (!irr.empty())
+    HIP_CHECK(hipMemcpyAsync(c->irregular[sub].p, irr.data(), irr.size() * 4, hipMemcpyHostToDevice, st));
+  c->irregular[sub].n = irr.size();
+  c->runs[sub].alloc(std::max<uint64_t>(total, 1));
+  std::vector<uint64_t> starts(g.ntiles + 1, UINT64_MAX);
+  if (total) {
+    const Lor* L = c->lors.p + lo;
+    DevBuf<Run> unsorted(total);
+    DevBuf<uint32_t> keys(total), keys2(total), perm(total), perm2(total);
     if (c->tof_on)
       k_bin_runs<true, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
-                                                                 doff.p, unsorted.p, keys.p);
+                                                                 c->run_off[sub].p, unsorted.p, keys.p);
     else
       k_bin_runs<false, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
-                                                                  doff.p, unsorted.p, keys.p);
+                                                                  c->run_off[sub].p, unsorted.p, keys.p);
     HIP_CHECK(hipGetLastError());
-    c->runs[sub].alloc(std::max<uint64_t>(total, 1));
-    std::vector<uint64_t> starts(g.ntiles + 1, UINT64_MAX);
-    if (total) {
-      k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
-      int end_bit = 17;
-      while ((1 << (end_bit - 16)) < g.ntiles) ++end_bit;
-      size_t tmp_bytes = 0;
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p,
-                                                   (int)total, 0, end_bit, st));
-      DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total,
-                                                   0, end_bit, st));
-      k_gather_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, perm2.p, total, c->runs[sub].p);
-      DevBuf<uint64_t> dstart(g.ntiles);
-      HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, g.ntiles * 8, st));
-      k_tile_starts<<<grid64(total), kBlock, 0, st>>>(keys2.p, total, dstart.p);
-      HIP_CHECK(hipGetLastError());
-      HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, g.ntiles * 8, hipMemcpyDeviceToHost, st));
-      HIP_CHECK(hipStreamSynchronize(st));
-    }
-    starts[g.ntiles] = total;
-    for (int k = g.ntiles - 1; k >= 0; --k)
-      if (starts[k] == UINT64_MAX) starts[k] = starts[k + 1];
-    // work items: chunks of one tile's runs, about 4 per CU in total
-    const uint64_t chunk = std::max<uint64_t>(256, total / 1024 + 1);
-    std::vector<int4> items;
-    for (int k = 0; k < g.ntiles; ++k)
-      for (uint64_t b = starts[k]; b < starts[k + 1]; b += chunk)
-        items.push_back(make_int4(k, (int)b, (int)std::min<uint64_t>(starts[k + 1], b + chunk), 0));
-    c->work[sub].alloc(std::max<size_t>(items.size(), 1));
-    if (!items.empty())
-      HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
-    c->work[sub].n = items.size();
+    k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
+    int end_bit = 17;
+    while ((1 << (end_bit - 16)) < g.ntiles) ++end_bit;
+    size_t tmp_bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total, 0,
+                                                 end_bit, st));
+    DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total, 0,
+                                                 end_bit, st));
+    k_gather_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, perm2.p, total, c->runs[sub].p);
+    DevBuf<uint64_t> dstart(g.ntiles);
+    HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, g.ntiles * 8, st));
+    k_tile_starts<<<grid64(total), kBlock, 0, st>>>(keys2.p, total, dstart.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, g.ntiles * 8, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
   }
+  starts[g.ntiles] = total;
+  for (int k = g.ntiles - 1; k >= 0; --k)
+    if (starts[k] == UINT64_MAX) starts[k] = starts[k + 1];
+  // work items: chunks of one tile's runs, about 4 per CU in total
+  const uint64_t chunk = std::max<uint64_t>(256, total / 1024 + 1);
+  std::vector<int4> items;
+  for (int k = 0; k < g.ntiles; ++k)
+    for (uint64_t b = starts[k]; b < starts[k + 1]; b += chunk)
+      items.push_back(make_int4(k, (int)b, (int)std::min<uint64_t>(starts[k + 1], b + chunk), 0));
+  c->work[sub].alloc(std::max<size_t>(items.size(), 1));
+  if (!items.empty())
+    HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+  c->work[sub].n = items.size();
+  HIP_CHECK(hipStreamSynchronize(st));
   c->n_runs[sub] = total;
 }
 
 void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   if (count == 0) return;
   const Lor* L = c->lors.p + lo;
-  // workgroup b -> group b % kGroups (its XCD), chunk b / kGroups of that group
-  const uint32_t gs = (uint32_t)((count + kGroups - 1) / kGroups);
-  const unsigned g = (unsigned)(kGroups * ((gs + kBlock - 1) / kBlock));
   hipStream_t st = c->stream;
-  if (c->tof_on)
-    k_fwd<true><<<g, kBlock, 0, st>>>(L, (uint32_t)count, gs, c->dfov, c->dtof, c->image.p, c->rbuf.p, c->bp64.p,
-                                      c->counters.p);
-  else
-    k_fwd<false><<<g, kBlock, 0, st>>>(L, (uint32_t)count, gs, c->dfov, c->dtof, c->image.p, c->rbuf.p, c->bp64.p,
-                                       c->counters.p);
-  HIP_CHECK(hipGetLastError());
-  if (c->timing) HIP_CHECK(hipEventRecord(c->ev[1], st));
   const size_t nw = c->work[sub].n;
+  const size_t n_irr = c->irregular[sub].n;
+  if (nw) {
+    const size_t lds = (size_t)c->tiles.tvox * sizeof(float);
+    if (c->tof_on)
+      k_fwd_tile<true><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->image.p, c->dfov,
+                                                        c->dtof, c->tiles, c->partial.p);
+    else
+      k_fwd_tile<false><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->image.p, c->dfov,
+                                                         c->dtof, c->tiles, c->partial.p);
+    HIP_CHECK(hipGetLastError());
+  }
+  k_lor_reduce<<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->run_off[sub].p, c->partial.p, c->rbuf.p);
+  HIP_CHECK(hipGetLastError());
+  if (n_irr) {
+    if (c->tof_on)
+      k_fwd_irregular<true><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
+                                                                c->dtof, c->image.p, c->bp64.p, c->counters.p);
+    else
+      k_fwd_irregular<false><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
+                                                                 c->dtof, c->image.p, c->bp64.p, c->counters.p);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (c->timing) HIP_CHECK(hipEventRecord(c->ev[1], st));
   if (nw) {
     const size_t lds = (size_t)c->tiles.tvox * sizeof(double);
     if (c->tof_on)
@@ -488,13 +513,24 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
                                       (int)lds));
         HIP_CHECK(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_fwd_tile<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(lds / 2)));
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_fwd_tile<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(lds / 2)));
         c->runs.resize(n_subsets);
         c->work.resize(n_subsets);
+        c->run_off.resize(n_subsets);
+        c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
         c->bp64.alloc(nv);
         fill_f64(c->bp64.p, 0.0, nv, c->stream);
-        for (size_t sub = 0; sub < n_subsets; ++sub) build_runs(c, sub, sub * c->set_size, c->set_size);
+        uint64_t max_runs = 1;
+        for (size_t sub = 0; sub < n_subsets; ++sub) {
+          build_runs(c, sub, sub * c->set_size, c->set_size);
+          max_runs = std::max<uint64_t>(max_runs, c->n_runs[sub]);
+        }
+        c->partial.alloc(max_runs);
       }
       HIP_CHECK(hipStreamSynchronize(c->stream));
     } catch (...) {
@@ -667,6 +703,9 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->counters.release();
   c->runs.clear();
   c->work.clear();
+  c->run_off.clear();
+  c->irregular.clear();
+  c->partial.release();
   c->rbuf.release();
   c->bp64.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -101,8 +101,11 @@ inline DevTof make_tof(bool on, float sigma_ps, float cutoff) {
 // q0 = a*rcp is within one ulp of a/b, the residual a - q0*b is exact in one
 // FMA, and q0 + residual*rcp rounds to the correctly rounded quotient
 // (Markstein, IBM J. Res. Dev. 34 (1990) 111).  Checked exhaustively against
-// binary32 division for every |a| in [2^-149, 2^12) with the TOF widths used
-// here; three instructions instead of the general division sequence.
+// binary32 division for every |a| in [2^-40, 2^12) with ten TOF widths.
+// Below 2^-40 the quotient may differ in its last bit (the residual
+// underflows), but there (a/b)^2 < 2^-76 and the Gaussian evaluates to
+// exactly its peak either way.  Three instructions instead of the general
+// division sequence.
 __host__ __device__ __forceinline__ float div_rn(float a, float b, float rcp) {
   const float q0 = a * rcp;
   const float r = fmaf(-q0, b, a);
@@ -271,6 +274,36 @@ __host__ __device__ __forceinline__ bool advance(WalkS& w, const WalkC c, const 
   w.r_y -= (int32_t)s.ay;
   w.r_z -= (int32_t)s.az;
   return (s.ax && w.r_x == 0) || (s.ay && w.r_y == 0) || (s.az && w.r_z == 0);
+}
+
+// Crossing without the per-axis remaining counts (for walks bounded by a
+// step count, or by the packed counter below).
+__host__ __device__ __forceinline__ void move(WalkS& w, const WalkC c, const Step s) {
+  w.here = s.bnd;
+  w.nb_x = s.ax ? w.nb_x + c.vl_x : w.nb_x;
+  w.nb_y = s.ay ? w.nb_y + c.vl_y : w.nb_y;
+  w.nb_z = s.az ? w.nb_z + c.vl_z : w.nb_z;
+  w.index += s.ax ? +c.d_x : (s.ay ? +c.d_y : +c.d_z);
+}
+
+// The three remaining counts of a bounded walk packed in one word: field d
+// (bits 10d .. 10d+8) holds r_d - 1, bit 10d+9 is a guard that is set.
+// Crossing axis d subtracts 1 << 10d; the crossing that exhausts the axis
+// (field 0) borrows the guard, and the walk has left the FOV exactly when a
+// guard is clear.  Valid while every r_d <= 512.
+constexpr uint32_t kPackGuards = 0x200u | (0x200u << 10) | (0x200u << 20);
+constexpr int32_t kPackMax = 512;
+
+__host__ __device__ __forceinline__ uint32_t pack_remaining(const WalkS& w) {
+  return ((uint32_t)(w.r_x - 1) | 0x200u) | (((uint32_t)(w.r_y - 1) | 0x200u) << 10) |
+         (((uint32_t)(w.r_z - 1) | 0x200u) << 20);
+}
+
+// Crosses step s; returns true when the walk has left the FOV.
+__host__ __device__ __forceinline__ bool advance_packed(WalkS& w, const WalkC c, const Step s, uint32_t& rem) {
+  move(w, c, s);
+  rem -= s.ax ? 1u : (s.ay ? (1u << 10) : (1u << 20));
+  return (rem & kPackGuards) != kPackGuards;
 }
 
 // True when every remaining count is positive: then each step decrements

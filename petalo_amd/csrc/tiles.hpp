@@ -12,15 +12,18 @@
 //                      box in one contiguous run).  A run records the walk
 //                      state at its first pushed step and its length.  Runs
 //                      are sorted by tile.
-//   every update       k_fwd computes r = 1/(p*ac) per LOR (walk #1 with the
-//                      gathers).  k_scatter gives each workgroup a chunk of
+//   every update       k_fwd_tile replays the runs against an LDS copy of
+//                      each tile of the image and writes per-run partial
+//                      projections; k_lor_reduce sums them per LOR into
+//                      r = 1/(p*ac).  k_scatter gives each workgroup a chunk of
 //                      one tile's runs: it replays every run from its recorded
 //                      state, computing each weight anew, adds w*r into a
 //                      binary64 LDS copy of the tile, and flushes the tile with
 //                      lane-coalesced binary64 global atomics.
 //
 // LORs whose walk is not provably bounded (an axis with no voxels left at
-// the entry) take the exact one-lane path with global atomics in k_fwd.
+// the entry) take the exact one-lane path with global atomics
+// (k_fwd_irregular).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,7 +49,7 @@ struct __align__(16) Run {
   uint32_t local;  // tile-local index of the first voxel (x fastest, nominal strides)
   uint32_t meta;   // bits 0-15: steps in the run; bits 16-18: axis flips (x, y, z)
   uint32_t lor;    // index into the context's LOR array
-  uint32_t tile;
+  uint32_t orig;   // position in LOR-major order (LOR by LOR, runs in walk order)
 };
 
 constexpr uint32_t kRunMaxSteps = 0xffffu;
@@ -136,7 +139,7 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
         open.local = local_of(g, q);
         open.meta = ((q.sx < 0 ? 1u : 0u) | (q.sy < 0 ? 2u : 0u) | (q.sz < 0 ? 4u : 0u)) << 16;
         open.lor = i;
-        open.tile = (uint32_t)tl;
+        open.orig = 0;
       }
       last_push = steps_in;
     }
@@ -150,23 +153,30 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
 template <bool TOF, bool FILL>
 __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
                                                   TileGeom g, uint32_t* __restrict__ n_runs,
-                                                  const uint64_t* __restrict__ offset, Run* __restrict__ runs,
+                                                  const uint32_t* __restrict__ offset, Run* __restrict__ runs,
                                                   uint32_t* __restrict__ keys) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Lor l = load_lor(lors, i);
-  const uint64_t base = FILL ? offset[i] : 0;
+  const uint32_t base = FILL ? offset[i] : 0;
   uint32_t k = 0;
   const uint32_t count = bin_lor<TOF>(l, i, f, t, g, [&](const Run& ru, uint32_t tile) {
     if (FILL) {
-      runs[base + k] = ru;
+      Run q = ru;
+      q.orig = base + k;
+      runs[base + k] = q;
       // sort key: tile, then longest run first (lanes of a wavefront get
       // runs of similar length)
       keys[base + k] = (tile << 16) | (0xffffu - (ru.meta & 0xffffu));
     }
     ++k;
   });
-  if (!FILL) n_runs[i] = count;
+  if (!FILL) {
+    // unbounded walks are projected by the one-lane kernel instead
+    WalkC c;
+    WalkS w;
+    n_runs[i] = (walk_setup(l, f, c, w) && !bounded(w)) ? 0xffffffffu : count;
+  }
 }
 
 // Replays a run from its recorded state, calling add(local_index, w * r)
@@ -191,92 +201,101 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
   c.d_y = (flips & 2u) ? -dly : dly;
   c.d_z = (flips & 4u) ? -dlz : dlz;
   w.index = (int32_t)ru.local;
-  w.r_x = w.r_y = w.r_z = 0x7fffffff;  // the run length bounds the replay
-  const uint32_t m = ru.meta & 0xffffu;
+  const uint32_t m = ru.meta & 0xffffu;  // the run length bounds the replay
   for (uint32_t k = 0; k < m; ++k) {
     const Step s = next_boundary(w);
     const float wt = step_weight<TOF>(w, c, s, t);
     if (wt > 0.0f) add((uint32_t)w.index, wt * r);
-    advance(w, c, s);
+    move(w, c, s);
   }
 }
 
-// Phase 1: forward projection per LOR, LORs in eight axial groups (see the
-// LOR ordering in capi.hip).  Bounded walks write r = 1/(p*ac) for
-// the tiled scatter; unbounded walks are projected and backprojected here
-// with global binary64 atomics (dropped whole on an out-of-range push, as in
-// projector.rs:103-118).
+// Forward, tiled: one workgroup per work item copies its tile of the image
+// into LDS and replays its runs, writing each run's partial projection
+// (sum of w*x over the run's pushed steps, in walk order) at its LOR-major
+// position.
 template <bool TOF>
-__global__ __launch_bounds__(256) void k_fwd(const Lor* __restrict__ lors, uint32_t n, uint32_t group_size,
-                                             DevFov f, DevTof t, const float* __restrict__ img,
-                                             float* __restrict__ r_out, double* __restrict__ bp,
-                                             unsigned long long* __restrict__ counters) {
-  // Workgroups land on the XCDs round-robin: workgroup b works on LOR group
-  // b % 8 (chunk b / 8 of it), so each XCD's L2 sees one axial band.
-  const uint32_t off = (blockIdx.x >> 3) * blockDim.x + threadIdx.x;
-  const uint32_t i = off < group_size ? (blockIdx.x & 7u) * group_size + off : n;
+__global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs, const int4* __restrict__ work,
+                                                   const float* __restrict__ img, DevFov f, DevTof t, TileGeom g,
+                                                   float* __restrict__ partial) {
+  extern __shared__ float timg[];
+  const int4 wi = work[blockIdx.x];
+  const int32_t tl = wi.x;
+  const int32_t tzi = tl / (g.ntx * g.nty);
+  const int32_t tyi = (tl / g.ntx) % g.nty;
+  const int32_t txi = tl % g.ntx;
+  const int32_t x0 = txi * g.tx, y0 = tyi * g.ty, z0 = tzi * g.tz;
+  for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) {
+    const int32_t lx = k % g.tx, ly = (k / g.tx) % g.ty, lz = k / (g.tx * g.ty);
+    const int32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    timg[k] = (x < f.nx && y < f.ny && z < f.nz) ? img[x + f.nx * (y + f.ny * z)] : 0.0f;
+  }
+  __syncthreads();
+  for (int32_t j = wi.y + (int32_t)threadIdx.x; j < wi.z; j += blockDim.x) {
+    const Run ru = runs[j];
+    float p = 0.0f;
+    replay_run<TOF>(ru, 1.0f, t, g, [&](uint32_t li, float wt) { p += wt * timg[li]; });
+    partial[ru.orig] = p;
+  }
+}
+
+// r = 1/(p*ac) per LOR with p the sum of its runs' partials in walk order
+// (LORs without runs get 1/(0*ac), as a row without elements does).
+__global__ __launch_bounds__(256) void k_lor_reduce(const Lor* __restrict__ lors, uint32_t n,
+                                                    const uint32_t* __restrict__ run_off,
+                                                    const float* __restrict__ partial, float* __restrict__ r_out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = run_off[i], e = run_off[i + 1];
+  float p = 0.0f;
+  for (uint32_t k = b; k < e; ++k) p += partial[k];
+  r_out[i] = 1.0f / (p * lors[i].ac);
+}
+
+// LORs whose walk is not provably bounded: exact one-lane projection and
+// backprojection with global binary64 atomics; dropped whole on a pushed
+// index outside the image (projector.rs:103-118).
+template <bool TOF>
+__global__ __launch_bounds__(256) void k_fwd_irregular(const Lor* __restrict__ lors, const uint32_t* __restrict__ list,
+                                                       uint32_t n, DevFov f, DevTof t, const float* __restrict__ img,
+                                                       double* __restrict__ bp, unsigned long long* __restrict__ counters) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t skipped = 0;
-  if (i < n) {
-    const Lor l = load_lor(lors, i);
+  if (j < n) {
+    const Lor l = load_lor(lors, list[j]);
     WalkC c;
     WalkS w;
-    float r = 0.0f;
     if (walk_setup(l, f, c, w)) {
-      if (bounded(w)) {
-        // Every visited voxel lies inside the image: no index check, no cap.
-        if (!TOF || fast_forward(w, c, tof_window_start(t, c.tof_peak))) {
-          // p = sum w*x in walk order; each product is added one step late
-          // so that the gather's latency overlaps the next step.
-          float p = 0.0f, pw = 0.0f, px = 0.0f;
-          for (;;) {
-            if (TOF && (w.here - c.tof_peak) >= t.cutoff) break;
-            const Step s = next_boundary(w);
-            const float wt = step_weight<TOF>(w, c, s, t);
-            const bool push = wt > 0.0f;
-            const float xv = push ? img[w.index] : 0.0f;
-            p += pw * px;
-            pw = push ? wt : 0.0f;
-            px = xv;
-            if (advance(w, c, s)) break;
-          }
-          p += pw * px;
-          r = 1.0f / (p * l.ac);
-        } else {
-          r = 1.0f / (0.0f * l.ac);
+      const uint32_t cap = step_cap(f);
+      WalkS start = w;
+      uint32_t steps = 0, start_step = 0;
+      bool started = false, bad = false;
+      float p = 0.0f;
+      for (;;) {
+        if (steps >= cap) { bad = true; break; }
+        const Step s = next_boundary(w);
+        const float wt = step_weight<TOF>(w, c, s, t);
+        if (wt > 0.0f) {
+          if (!started) { start = w; start_step = steps; started = true; }
+          if ((uint32_t)w.index >= (uint32_t)f.nvox) { bad = true; break; }
+          p += wt * img[w.index];
         }
-      } else {
-        const uint32_t cap = step_cap(f);
-        WalkS start = w;
-        uint32_t steps = 0, start_step = 0;
-        bool started = false, bad = false;
-        float p = 0.0f;
-        for (;;) {
-          if (steps >= cap) { bad = true; break; }
+        ++steps;
+        if (advance(w, c, s)) break;
+      }
+      if (bad) {
+        skipped = 1;
+      } else if (started) {
+        const float r = 1.0f / (p * l.ac);
+        w = start;
+        for (uint32_t k = start_step; k < steps; ++k) {
           const Step s = next_boundary(w);
           const float wt = step_weight<TOF>(w, c, s, t);
-          if (wt > 0.0f) {
-            if (!started) { start = w; start_step = steps; started = true; }
-            if ((uint32_t)w.index >= (uint32_t)f.nvox) { bad = true; break; }
-            p += wt * img[w.index];
-          }
-          ++steps;
-          if (advance(w, c, s)) break;
-        }
-        r = 1.0f / (p * l.ac);
-        if (bad) {
-          skipped = 1;
-        } else if (started) {
-          w = start;
-          for (uint32_t k = start_step; k < steps; ++k) {
-            const Step s = next_boundary(w);
-            const float wt = step_weight<TOF>(w, c, s, t);
-            if (wt > 0.0f) unsafeAtomicAdd(bp + w.index, (double)(wt * r));
-            advance(w, c, s);
-          }
+          if (wt > 0.0f) unsafeAtomicAdd(bp + w.index, (double)(wt * r));
+          advance(w, c, s);
         }
       }
     }
-    r_out[i] = r;
   }
   wave_add_u64(counters + CNT_SKIPPED, skipped);
 }
