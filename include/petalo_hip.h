@@ -139,6 +139,21 @@ int petalo_mlem_last_timing(petalo_mlem_t* ctx, double* project_ms, double* upda
  * scatter (0 for the atomic scheme, which does both in one kernel) and the
  * image update. */
 int petalo_mlem_last_phase_timing(petalo_mlem_t* ctx, double* forward_ms, double* scatter_ms, double* update_ms);
+/* Kernel-level split of the last update, written to ms[0 .. min(n, 4)):
+ *   FORWARD_TILES   tiled scheme: the per-tile forward kernel;
+ *                   atomic scheme: the fused projection kernel
+ *   FORWARD_REDUCE  per-LOR reduction to 1/(p*ac) plus the one-lane path
+ *                   for unbounded walks (tiled scheme; 0 otherwise)
+ *   SCATTER         tiled backprojection (0 for the atomic scheme)
+ *   UPDATE          image update */
+enum {
+  PETALO_TIME_FORWARD_TILES = 0,
+  PETALO_TIME_FORWARD_REDUCE = 1,
+  PETALO_TIME_SCATTER = 2,
+  PETALO_TIME_UPDATE = 3,
+  PETALO_N_TIMES = 4
+};
+int petalo_mlem_last_kernel_timing(petalo_mlem_t* ctx, double* ms, size_t n);
 /* Scheme in use and, for the tiled scheme, the number of tile runs. */
 int petalo_mlem_info(petalo_mlem_t* ctx, int32_t* scheme, uint64_t* n_runs_total);
 int petalo_mlem_destroy(petalo_mlem_t* ctx);

@@ -104,6 +104,7 @@ _SIGS = {
     "petalo_mlem_set_timing": (C.c_int, [_P, C.c_int]),
     "petalo_mlem_last_timing": (C.c_int, [_P, _P, _P]),
     "petalo_mlem_last_phase_timing": (C.c_int, [_P, _P, _P, _P]),
+    "petalo_mlem_last_kernel_timing": (C.c_int, [_P, _P, C.c_size_t]),
     "petalo_mlem_info": (C.c_int, [_P, _P, _P]),
     "petalo_mlem_destroy": (C.c_int, [_P]),
     "petalo_make_sensitivity_image": (C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_uint64, C.c_float,
@@ -329,6 +330,12 @@ class Mlem:
         a, b, d = C.c_double(0), C.c_double(0), C.c_double(0)
         _check(lib().petalo_mlem_last_phase_timing(self._ctx, C.byref(a), C.byref(b), C.byref(d)))
         return a.value, b.value, d.value
+
+    def last_kernel_timing(self):
+        """{'forward_tiles', 'forward_reduce', 'scatter', 'update'} in ms."""
+        buf = (C.c_double * 4)()
+        _check(lib().petalo_mlem_last_kernel_timing(self._ctx, buf, 4))
+        return dict(zip(("forward_tiles", "forward_reduce", "scatter", "update"), list(buf)))
 
     def info(self):
         sch, nr = C.c_int32(0), C.c_uint64(0)

@@ -152,16 +152,14 @@ uint64_t lor_order_key(const petalo_lor_t& l) {
   return (q(phi, 0.0, M_PI, 10) << 40) | (q(s, -400.0, 400.0, 20) << 20) | q(zm, -400.0, 400.0, 20);
 }
 
-// Axial position of the LOR's TOF peak (its midpoint without TOF): where its
-// weighted voxels cluster.
-double lor_centre_z(const petalo_lor_t& l, bool tof) {
-  if (!tof) return 0.5 * ((double)l.p1[2] + l.p2[2]);
+// The LOR's TOF peak (its midpoint without TOF): where its weighted voxels
+// cluster.
+void lor_centre(const petalo_lor_t& l, bool tof, double out[3]) {
   const double d[3] = {(double)l.p2[0] - l.p1[0], (double)l.p2[1] - l.p1[1], (double)l.p2[2] - l.p1[2]};
+  double t = 0.5;
   const double L = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-  if (!(L > 0)) return l.p1[2];
-  double t = 0.5 - 0.299792458 * (double)l.dt_ps / (2.0 * L);
-  t = std::min(std::max(t, 0.0), 1.0);
-  return (double)l.p1[2] + t * d[2];
+  if (tof && L > 0) t = std::min(std::max(0.5 - 0.299792458 * (double)l.dt_ps / (2.0 * L), 0.0), 1.0);
+  for (int k = 0; k < 3; ++k) out[k] = (double)l.p1[k] + t * d[k];
 }
 
 template <class F> void parallel_for(size_t n, F&& f) {
@@ -174,30 +172,35 @@ template <class F> void parallel_for(size_t n, F&& f) {
   for (auto& x : th) x.join();
 }
 
-// Number of XCDs a launch's workgroups are spread over, round-robin by
-// workgroup index (MI355X: 8).
-constexpr int kGroups = 8;
+// Tiled scheme: number of work items per subset update (several rounds of
+// resident workgroups) and the least work per item, in replayed steps.
+constexpr uint64_t kWorkItems = 2048;
+constexpr uint64_t kMinItemSteps = 8192;
 
-// Reorders lors[lo, hi) (one subset) into kGroups equal-size groups by the
-// axial position of their TOF peak, each group in lor_order_key order.  The
-// forward kernel gives group g to the workgroups that land on XCD g, so each
-// XCD gathers from about a quarter of the image, which its L2 holds.
-void order_lors(std::vector<petalo_lor_t>& lors, size_t lo, size_t hi, bool tof) {
+// Reorders lors[lo, hi) (one subset) by the image tile that holds their TOF
+// peak, and within a tile by lor_order_key (direction, then offset, then
+// axial position).  LORs with nearby consecutive indices then cross the same
+// few tiles, so the per-LOR values that the tiled kernels read through the
+// runs of one tile come from a small, cache-resident index range.
+void order_lors(std::vector<petalo_lor_t>& lors, size_t lo, size_t hi, bool tof, const petalo_fov_t& fov,
+                const TileGeom& g) {
   const size_t m = hi - lo;
   if (m < 2) return;
-  std::vector<std::pair<double, uint32_t>> zc(m);
-  parallel_for(m, [&](size_t k) { zc[k] = {lor_centre_z(lors[lo + k], tof), (uint32_t)k}; });
-  std::sort(zc.begin(), zc.end());
-  std::vector<std::pair<uint64_t, uint32_t>> key(m);
-  const size_t gs = (m + kGroups - 1) / kGroups;
-  parallel_for(m, [&](size_t r) {
-    const uint32_t k = zc[r].second;
-    key[r] = {lor_order_key(lors[lo + k]), k};
+  std::vector<std::pair<std::pair<uint32_t, uint64_t>, uint32_t>> key(m);
+  parallel_for(m, [&](size_t k) {
+    const petalo_lor_t& l = lors[lo + k];
+    double c[3];
+    lor_centre(l, tof, c);
+    int32_t v[3];
+    for (int a = 0; a < 3; ++a) {
+      const double vs = (double)fov.full_size_mm[a] / fov.n[a];
+      const double x = std::floor((c[a] + 0.5 * fov.full_size_mm[a]) / vs);
+      v[a] = (int32_t)std::min(std::max(x, 0.0), (double)fov.n[a] - 1);
+    }
+    const uint32_t tile = (uint32_t)(v[0] / g.tx + g.ntx * (v[1] / g.ty + g.nty * (v[2] / g.tz)));
+    key[k] = {{tile, lor_order_key(l)}, (uint32_t)k};
   });
-  parallel_for(kGroups, [&](size_t g) {
-    const size_t b = std::min(m, g * gs), e = std::min(m, (g + 1) * gs);
-    std::sort(key.begin() + b, key.begin() + e);
-  });
+  std::sort(key.begin(), key.end());
   std::vector<petalo_lor_t> tmp(lors.begin() + lo, lors.begin() + hi);
   parallel_for(m, [&](size_t r) { lors[lo + r] = tmp[key[r].second]; });
 }
@@ -221,7 +224,8 @@ struct petalo_mlem {
   DevBuf<unsigned long long> counters;
   petalo_osem_t osem{};
   bool timing = false;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // timing: start, forward end, scatter end, update end, forward tile kernel end
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool have_timing = false;
   // tiled scheme
   TileGeom tiles{};
@@ -229,6 +233,7 @@ struct petalo_mlem {
   std::vector<DevBuf<int4>> work;    // per subset: (tile, first run, end run, 0)
   std::vector<uint64_t> n_runs;      // per subset
   std::vector<DevBuf<uint32_t>> run_off;    // per subset: LOR-major run offsets (n + 1)
+  std::vector<DevBuf<uint32_t>> run_pos;    // per subset: sorted position of each LOR-major run
   std::vector<DevBuf<uint32_t>> irregular;  // per subset: LORs with unbounded walks
   DevBuf<float> partial;             // per-run partial projections (largest subset)
   DevBuf<float> rbuf;                // 1/(p*ac) per LOR of the current subset
@@ -251,7 +256,7 @@ void launch_project(petalo_mlem* c, size_t lo, size_t count) {
 }
 
 // Tile shape for an image: 16384 voxels (128 KiB of binary64 in LDS).
-TileGeom make_tiles(const petalo_fov_t& fov) {
+TileGeom tile_shape(const petalo_fov_t& fov) {
   TileGeom g;
   g.tx = 32;
   g.ty = 32;
@@ -261,23 +266,28 @@ TileGeom make_tiles(const petalo_fov_t& fov) {
   g.ntz = (int32_t)((fov.n[2] + g.tz - 1) / g.tz);
   g.ntiles = g.ntx * g.nty * g.ntz;
   g.tvox = g.tx * g.ty * g.tz;
-  if (g.ntiles >= 65536) throw std::runtime_error("image too large for the tiled scheme (65536 tiles or more)");
   return g;
 }
 
-// Sorted keys are (tile << 16) | (0xffff - length): the first position of
-// each tile is where the high half changes.
-__global__ void k_tile_starts(const uint32_t* __restrict__ keys, uint64_t n, uint64_t* __restrict__ start) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t t = keys[j] >> 16;
-  if (j == 0 || (keys[j - 1] >> 16) != t) start[t] = j;
+TileGeom make_tiles(const petalo_fov_t& fov) {
+  const TileGeom g = tile_shape(fov);
+  if (g.ntiles >= 65536) throw std::runtime_error("image too large for the tiled scheme (65536 tiles or more)");
+  if ((uint32_t)g.tvox > kRunMaxLocal + 1) throw std::runtime_error("tile too large for the run record");
+  return g;
 }
 
-__global__ void k_gather_runs(const Run* __restrict__ in, const uint32_t* __restrict__ perm, uint64_t n,
-                              Run* __restrict__ out) {
+// Puts the runs in sorted order, records where each LOR-major run went and
+// a (tile << 12) | length summary of the sorted runs for the host.
+__global__ void k_place_runs(const Run* __restrict__ in, const uint64_t* __restrict__ sorted_keys,
+                             const uint32_t* __restrict__ perm, uint64_t n, Run* __restrict__ out,
+                             uint32_t* __restrict__ pos, uint32_t* __restrict__ summary) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) out[j] = in[perm[j]];
+  if (j >= n) return;
+  const uint32_t src = perm[j];
+  const Run ru = in[src];
+  out[j] = ru;
+  pos[src] = (uint32_t)j;
+  summary[j] = ((uint32_t)(sorted_keys[j] >> 32) << 12) | run_len(ru.code);
 }
 
 __global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
@@ -326,11 +336,14 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     HIP_CHECK(hipMemcpyAsync(c->irregular[sub].p, irr.data(), irr.size() * 4, hipMemcpyHostToDevice, st));
   c->irregular[sub].n = irr.size();
   c->runs[sub].alloc(std::max<uint64_t>(total, 1));
+  c->run_pos[sub].alloc(std::max<uint64_t>(total, 1));
   std::vector<uint64_t> starts(g.ntiles + 1, UINT64_MAX);
+  std::vector<uint32_t> summary(total);  // per sorted run: (tile << 12) | length
   if (total) {
     const Lor* L = c->lors.p + lo;
     DevBuf<Run> unsorted(total);
-    DevBuf<uint32_t> keys(total), keys2(total), perm(total), perm2(total);
+    DevBuf<uint64_t> keys(total), keys2(total);
+    DevBuf<uint32_t> perm(total), perm2(total);
     if (c->tof_on)
       k_bin_runs<true, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
                                                                  c->run_off[sub].p, unsorted.p, keys.p);
@@ -339,31 +352,56 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
                                                                   c->run_off[sub].p, unsorted.p, keys.p);
     HIP_CHECK(hipGetLastError());
     k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
-    int end_bit = 17;
-    while ((1 << (end_bit - 16)) < g.ntiles) ++end_bit;
+    int end_bit = 33;
+    while ((1 << (end_bit - 32)) < g.ntiles) ++end_bit;
     size_t tmp_bytes = 0;
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total, 0,
                                                  end_bit, st));
     DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total, 0,
                                                  end_bit, st));
-    k_gather_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, perm2.p, total, c->runs[sub].p);
-    DevBuf<uint64_t> dstart(g.ntiles);
-    HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, g.ntiles * 8, st));
-    k_tile_starts<<<grid64(total), kBlock, 0, st>>>(keys2.p, total, dstart.p);
+    DevBuf<uint32_t> dsum(total);
+    k_place_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, keys2.p, perm2.p, total, c->runs[sub].p,
+                                                   c->run_pos[sub].p, dsum.p);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, g.ntiles * 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(summary.data(), dsum.p, total * 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
+    for (uint64_t j = total; j-- > 0;) starts[summary[j] >> 12] = j;
   }
   starts[g.ntiles] = total;
   for (int k = g.ntiles - 1; k >= 0; --k)
     if (starts[k] == UINT64_MAX) starts[k] = starts[k + 1];
-  // work items: chunks of one tile's runs, about 4 per CU in total
-  const uint64_t chunk = std::max<uint64_t>(256, total / 1024 + 1);
+  // Work items: chunks of one tile's runs holding about equal numbers of
+  // replayed steps (kWorkItems in total, at least kMinItemSteps each),
+  // largest first so that the last workgroups to start are the short ones.
+  uint64_t total_steps = 0;
+  for (uint32_t v : summary) total_steps += v & kRunMaxSteps;
+  const uint64_t target = std::max<uint64_t>(kMinItemSteps, total_steps / kWorkItems + 1);
+  std::vector<std::pair<uint64_t, int4>> sized;
+  for (int k = 0; k < g.ntiles; ++k) {
+    const uint64_t a = starts[k], b = starts[k + 1];
+    if (a == b) continue;
+    uint64_t steps = 0;
+    for (uint64_t j = a; j < b; ++j) steps += summary[j] & kRunMaxSteps;
+    const uint64_t n_items = std::max<uint64_t>(1, (steps + target - 1) / target);
+    uint64_t acc = 0, begin = a, m = 1;
+    for (uint64_t j = a; j < b; ++j) {
+      acc += summary[j] & kRunMaxSteps;
+      if (j + 1 == b || acc * n_items >= m * steps) {
+        sized.push_back({acc, make_int4(k, (int)begin, (int)(j + 1), 0)});
+        begin = j + 1;
+        ++m;
+      }
+    }
+  }
+  // per-item step counts (acc above is cumulative within the tile)
+  for (size_t q = sized.size(); q-- > 0;)
+    if (q > 0 && sized[q - 1].second.x == sized[q].second.x) sized[q].first -= sized[q - 1].first;
+  std::stable_sort(sized.begin(), sized.end(),
+                   [](const std::pair<uint64_t, int4>& u, const std::pair<uint64_t, int4>& v) { return u.first > v.first; });
   std::vector<int4> items;
-  for (int k = 0; k < g.ntiles; ++k)
-    for (uint64_t b = starts[k]; b < starts[k + 1]; b += chunk)
-      items.push_back(make_int4(k, (int)b, (int)std::min<uint64_t>(starts[k + 1], b + chunk), 0));
+  items.reserve(sized.size());
+  for (auto& it : sized) items.push_back(it.second);
   c->work[sub].alloc(std::max<size_t>(items.size(), 1));
   if (!items.empty())
     HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
@@ -388,7 +426,9 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
                                                          c->dtof, c->tiles, c->partial.p);
     HIP_CHECK(hipGetLastError());
   }
-  k_lor_reduce<<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->run_off[sub].p, c->partial.p, c->rbuf.p);
+  if (c->timing) HIP_CHECK(hipEventRecord(c->ev[4], st));
+  k_lor_reduce<<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->run_off[sub].p, c->run_pos[sub].p,
+                                                   c->partial.p, c->rbuf.p);
   HIP_CHECK(hipGetLastError());
   if (n_irr) {
     if (c->tof_on)
@@ -491,7 +531,8 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
       // LORs: reorder inside each subset (subset membership is unchanged).
       std::vector<petalo_lor_t> h(lors, lors + n_lors);
       if (opt.sort_lors)
-        for (size_t s = 0; s < n_subsets; ++s) order_lors(h, s * c->set_size, (s + 1) * c->set_size, c->tof_on);
+        for (size_t s = 0; s < n_subsets; ++s)
+          order_lors(h, s * c->set_size, (s + 1) * c->set_size, c->tof_on, *fov, tile_shape(*fov));
       c->lors.alloc(std::max<size_t>(n_lors, 1));
       if (n_lors) HIP_CHECK(hipMemcpy(c->lors.p, h.data(), n_lors * sizeof(Lor), hipMemcpyHostToDevice));
 
@@ -520,6 +561,7 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
         c->runs.resize(n_subsets);
         c->work.resize(n_subsets);
         c->run_off.resize(n_subsets);
+        c->run_pos.resize(n_subsets);
         c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
@@ -557,6 +599,7 @@ int petalo_mlem_next(petalo_mlem_t* c, float* image_out, petalo_osem_t* id_out) 
     } else {
       launch_project(c, lo, c->set_size);
       if (c->timing) {
+        HIP_CHECK(hipEventRecord(c->ev[4], c->stream));
         HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
         HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
       }
@@ -679,6 +722,22 @@ int petalo_mlem_last_phase_timing(petalo_mlem_t* c, double* forward_ms, double* 
   });
 }
 
+int petalo_mlem_last_kernel_timing(petalo_mlem_t* c, double* ms, size_t n) {
+  return guarded([&] {
+    if (!c || (!ms && n)) return fail(PETALO_ERR_ARG, "NULL argument");
+    if (!c->have_timing) return fail(PETALO_ERR_ARG, "timing is off or no update has run");
+    HIP_CHECK(hipSetDevice(c->device));
+    HIP_CHECK(hipEventSynchronize(c->ev[3]));
+    float t[PETALO_N_TIMES] = {0, 0, 0, 0};
+    HIP_CHECK(hipEventElapsedTime(&t[PETALO_TIME_FORWARD_TILES], c->ev[0], c->ev[4]));
+    HIP_CHECK(hipEventElapsedTime(&t[PETALO_TIME_FORWARD_REDUCE], c->ev[4], c->ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&t[PETALO_TIME_SCATTER], c->ev[1], c->ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&t[PETALO_TIME_UPDATE], c->ev[2], c->ev[3]));
+    for (size_t k = 0; k < n && k < PETALO_N_TIMES; ++k) ms[k] = t[k];
+    return PETALO_OK;
+  });
+}
+
 int petalo_mlem_info(petalo_mlem_t* c, int32_t* scheme, uint64_t* n_runs_total) {
   if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
   if (scheme) *scheme = c->scheme;
@@ -704,6 +763,7 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->runs.clear();
   c->work.clear();
   c->run_off.clear();
+  c->run_pos.clear();
   c->irregular.clear();
   c->partial.release();
   c->rbuf.release();

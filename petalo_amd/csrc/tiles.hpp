@@ -42,17 +42,21 @@ struct TileGeom {
   int32_t tvox;           // tx*ty*tz: LDS entries per tile
 };
 
-// One run: the walk state at the first pushed step of a LOR inside a tile.
-struct __align__(16) Run {
+// One run: the walk state at the first pushed step of a LOR inside a tile
+// (40 bytes).
+struct __align__(8) Run {
   float nb_x, nb_y, nb_z, here;
   float vl_x, vl_y, vl_z, tof_peak;
-  uint32_t local;  // tile-local index of the first voxel (x fastest, nominal strides)
-  uint32_t meta;   // bits 0-15: steps in the run; bits 16-18: axis flips (x, y, z)
-  uint32_t lor;    // index into the context's LOR array
-  uint32_t orig;   // position in LOR-major order (LOR by LOR, runs in walk order)
+  uint32_t code;  // bits 0-11: steps in the run; bits 12-14: axis flips (x, y, z);
+                  // bits 15-31: tile-local index of the first voxel (x fastest)
+  uint32_t lor;   // index of the LOR within its subset
 };
 
-constexpr uint32_t kRunMaxSteps = 0xffffu;
+constexpr uint32_t kRunMaxSteps = 0xfffu;
+constexpr uint32_t kRunMaxLocal = 0x1ffffu;
+__host__ __device__ __forceinline__ uint32_t run_len(uint32_t code) { return code & 0xfffu; }
+__host__ __device__ __forceinline__ uint32_t run_flips(uint32_t code) { return (code >> 12) & 7u; }
+__host__ __device__ __forceinline__ uint32_t run_local(uint32_t code) { return code >> 15; }
 
 // Per-step voxel coordinates (original axes) tracked by the binning walk.
 struct Coord {
@@ -112,7 +116,7 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
   Run open{};
   auto close = [&]() {
     if (cur < 0) return;
-    open.meta = (open.meta & ~0xffffu) | (last_push + 1);
+    open.code = (open.code & ~kRunMaxSteps) | (last_push + 1);
     emit(open, (uint32_t)cur);
     ++count;
     cur = -1;
@@ -136,10 +140,9 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
         open.vl_y = c.vl_y;
         open.vl_z = c.vl_z;
         open.tof_peak = c.tof_peak;
-        open.local = local_of(g, q);
-        open.meta = ((q.sx < 0 ? 1u : 0u) | (q.sy < 0 ? 2u : 0u) | (q.sz < 0 ? 4u : 0u)) << 16;
+        const uint32_t flips = (q.sx < 0 ? 1u : 0u) | (q.sy < 0 ? 2u : 0u) | (q.sz < 0 ? 4u : 0u);
+        open.code = (local_of(g, q) << 15) | (flips << 12);
         open.lor = i;
-        open.orig = 0;
       }
       last_push = steps_in;
     }
@@ -149,12 +152,16 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
 }
 
 // Binning walk: FILL = false counts runs per LOR into n_runs[i]; FILL = true
-// writes them at runs[offset[i] ...] with their tile ids in keys[].
+// writes them at runs[offset[i] ...] (LOR-major) with their sort keys:
+// tile, then block of 65536 consecutive LORs (LORs are ordered spatially, so
+// a block's 1/(p*ac) values stay cache-resident while its runs in a tile are
+// replayed), then longest run first (lanes of a wavefront get runs of similar
+// length).
 template <bool TOF, bool FILL>
 __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
                                                   TileGeom g, uint32_t* __restrict__ n_runs,
                                                   const uint32_t* __restrict__ offset, Run* __restrict__ runs,
-                                                  uint32_t* __restrict__ keys) {
+                                                  uint64_t* __restrict__ keys) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Lor l = load_lor(lors, i);
@@ -162,12 +169,8 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
   uint32_t k = 0;
   const uint32_t count = bin_lor<TOF>(l, i, f, t, g, [&](const Run& ru, uint32_t tile) {
     if (FILL) {
-      Run q = ru;
-      q.orig = base + k;
-      runs[base + k] = q;
-      // sort key: tile, then longest run first (lanes of a wavefront get
-      // runs of similar length)
-      keys[base + k] = (tile << 16) | (0xffffu - (ru.meta & 0xffffu));
+      runs[base + k] = ru;
+      keys[base + k] = ((uint64_t)tile << 32) | ((uint64_t)(i >> 16) << 12) | (kRunMaxSteps - run_len(ru.code));
     }
     ++k;
   });
@@ -196,12 +199,12 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
   c.vl_y = ru.vl_y;
   c.vl_z = ru.vl_z;
   c.tof_peak = ru.tof_peak;
-  const uint32_t flips = ru.meta >> 16;
+  const uint32_t flips = run_flips(ru.code);
   c.d_x = (flips & 1u) ? -dlx : dlx;
   c.d_y = (flips & 2u) ? -dly : dly;
   c.d_z = (flips & 4u) ? -dlz : dlz;
-  w.index = (int32_t)ru.local;
-  const uint32_t m = ru.meta & 0xffffu;  // the run length bounds the replay
+  w.index = (int32_t)run_local(ru.code);
+  const uint32_t m = run_len(ru.code);  // the run length bounds the replay
   for (uint32_t k = 0; k < m; ++k) {
     const Step s = next_boundary(w);
     const float wt = step_weight<TOF>(w, c, s, t);
@@ -212,8 +215,10 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
 
 // Forward, tiled: one workgroup per work item copies its tile of the image
 // into LDS and replays its runs, writing each run's partial projection
-// (sum of w*x over the run's pushed steps, in walk order) at its LOR-major
-// position.
+// (sum of w*x over the run's pushed steps, in walk order) at the run's
+// sorted position.  Each lane loads its next run record before replaying the
+// current one, so the record's latency hides behind the replay (the replay
+// itself touches only LDS).
 template <bool TOF>
 __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs, const int4* __restrict__ work,
                                                    const float* __restrict__ img, DevFov f, DevTof t, TileGeom g,
@@ -231,24 +236,35 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs,
     timg[k] = (x < f.nx && y < f.ny && z < f.nz) ? img[x + f.nx * (y + f.ny * z)] : 0.0f;
   }
   __syncthreads();
-  for (int32_t j = wi.y + (int32_t)threadIdx.x; j < wi.z; j += blockDim.x) {
-    const Run ru = runs[j];
+  int32_t j = wi.y + (int32_t)threadIdx.x;
+  if (j >= wi.z) return;
+  const int32_t last = wi.z - 1;
+  const int32_t stride = (int32_t)blockDim.x;
+  Run cur = runs[j];
+  for (;;) {
+    const int32_t jn = j + stride;
+    const Run nxt = runs[jn < last ? jn : last];  // in bounds; unused past the end
     float p = 0.0f;
-    replay_run<TOF>(ru, 1.0f, t, g, [&](uint32_t li, float wt) { p += wt * timg[li]; });
-    partial[ru.orig] = p;
+    replay_run<TOF>(cur, 1.0f, t, g, [&](uint32_t li, float wt) { p += wt * timg[li]; });
+    partial[j] = p;
+    if (jn > last) break;
+    cur = nxt;
+    j = jn;
   }
 }
 
-// r = 1/(p*ac) per LOR with p the sum of its runs' partials in walk order
-// (LORs without runs get 1/(0*ac), as a row without elements does).
+// r = 1/(p*ac) per LOR with p the sum of its runs' partials in walk order;
+// run_pos[run_off[i] ...] are the sorted positions of LOR i's runs (LORs
+// without runs get 1/(0*ac), as a row without elements does).
 __global__ __launch_bounds__(256) void k_lor_reduce(const Lor* __restrict__ lors, uint32_t n,
                                                     const uint32_t* __restrict__ run_off,
+                                                    const uint32_t* __restrict__ run_pos,
                                                     const float* __restrict__ partial, float* __restrict__ r_out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t b = run_off[i], e = run_off[i + 1];
   float p = 0.0f;
-  for (uint32_t k = b; k < e; ++k) p += partial[k];
+  for (uint32_t k = b; k < e; ++k) p += partial[run_pos[k]];
   r_out[i] = 1.0f / (p * lors[i].ac);
 }
 
@@ -300,7 +316,9 @@ __global__ __launch_bounds__(256) void k_fwd_irregular(const Lor* __restrict__ l
   wave_add_u64(counters + CNT_SKIPPED, skipped);
 }
 
-// Phase 2: one workgroup per work item (tile, first run, end run).
+// Backprojection, tiled: one workgroup per work item (tile, first run, end
+// run).  Lanes keep two runs in flight: the record of the run after next and
+// the 1/(p*ac) of the next run load while the current run is replayed.
 template <bool TOF>
 __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, const int4* __restrict__ work,
                                                   const float* __restrict__ r_in, DevFov f, DevTof t, TileGeom g,
@@ -310,10 +328,26 @@ __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, 
   const int32_t tl = wi.x;
   for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) tile[k] = 0.0;
   __syncthreads();
-  for (int32_t j = wi.y + (int32_t)threadIdx.x; j < wi.z; j += blockDim.x) {
-    const Run ru = runs[j];
-    const float r = r_in[ru.lor];
-    replay_run<TOF>(ru, r, t, g, [&](uint32_t li, float v) { atomicAdd(tile + li, (double)v); });
+  int32_t j = wi.y + (int32_t)threadIdx.x;
+  if (j < wi.z) {
+    const int32_t last = wi.z - 1;
+    const int32_t stride = (int32_t)blockDim.x;
+    const int32_t j1 = j + stride;
+    Run cur = runs[j];
+    Run nxt = runs[j1 < last ? j1 : last];
+    float r = r_in[cur.lor];
+    for (;;) {
+      const int32_t jn = j + stride;
+      const int32_t jnn = jn + stride;
+      const float rn = r_in[nxt.lor];
+      const Run nn = runs[jnn < last ? jnn : last];
+      replay_run<TOF>(cur, r, t, g, [&](uint32_t li, float v) { atomicAdd(tile + li, (double)v); });
+      if (jn > last) break;
+      cur = nxt;
+      nxt = nn;
+      r = rn;
+      j = jn;
+    }
   }
   __syncthreads();
   // flush the tile: consecutive lanes take consecutive x, so each wavefront
