@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tof_mlem_120", choices=sorted(WORKLOADS))
     ap.add_argument("--scheme", type=int, default=0, help="0 auto, 1 atomic, 2 tiled")
+    ap.add_argument("--tile", default=None, help="tiled scheme tile shape x,y,z (default 32,32,16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-lors", type=int, default=1_000_000)
     ap.add_argument("--no-sensitivity", action="store_true")
@@ -88,7 +89,8 @@ def main():
         log(f"[bench] sensitivity image from {n_sens} LORs: {sens_ms:.1f} ms")
 
     t0 = time.perf_counter()
-    ctx = P.Mlem(fov, lors, tof=tof_obj, sensitivity=sens_img, n_subsets=subsets, scheme=a.scheme)
+    tile = tuple(int(v) for v in a.tile.split(",")) if a.tile else None
+    ctx = P.Mlem(fov, lors, tof=tof_obj, sensitivity=sens_img, n_subsets=subsets, scheme=a.scheme, tile=tile)
     log(f"[bench] context ready in {time.perf_counter() - t0:.2f} s")
     ws = ctx.walk_stats()
     log(f"[bench] walk stats {ws}")
@@ -189,7 +191,7 @@ def main():
             "fov_mm": full, "voxels": [nvx] * 3, "n_lors": n_lors, "subsets": subsets,
             "tof_fwhm_ps": fwhm, "tof_sigma_ps": tof[0] if tof else None,
             "tof_cutoff": tof[1] if tof else None, "sensitivity_lors": n_sens if sens_img is not None else 0,
-            "scheme": a.scheme,
+            "scheme": a.scheme, "tile": list(tile) if tile else None,
         },
         "voxel_steps_per_s": steps_per_s,
         "walk": {"n_steps": ws["n_steps"], "n_nz": ws["n_nz"], "n_skipped": ws["n_skipped"]},

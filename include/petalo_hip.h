@@ -88,7 +88,9 @@ typedef struct {
   int32_t scheme;        /* PETALO_SCHEME_*                                   */
   int32_t sort_lors;     /* 1: reorder LORs inside each subset for locality   */
   int32_t device;        /* HIP device ordinal                                */
-  int32_t reserved;
+  int32_t tile[3];       /* tiled scheme: tile shape in voxels (x, y, z); zeros
+                            choose the default 32 x 32 x 16.  At most 16384
+                            voxels per tile.                                  */
 } petalo_mlem_options_t;
 
 /* ---- library ----------------------------------------------------------- */

@@ -65,7 +65,7 @@ class _Stats(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("sort_lors", C.c_int32), ("device", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("tile", C.c_int32 * 3)]
 
 
 class _Config(C.Structure):
@@ -268,7 +268,7 @@ class Mlem:
     subset update, like petalo::mlem::mlem (src/mlem.rs:8-31)."""
 
     def __init__(self, fov, lors, tof=None, sensitivity=None, n_subsets=1, scheme=SCHEME_AUTO,
-                 sort_lors=True, device=0):
+                 sort_lors=True, device=0, tile=None):
         self.fov = fov
         lors = np.ascontiguousarray(lors, LOR_DTYPE)
         sens = None
@@ -278,7 +278,7 @@ class Mlem:
             if sens.size != fov.nvox:
                 raise PetaloError(6, "sensitivity image size does not match the FOV")
         self._ctx = C.c_void_p()
-        opt = _Options(int(scheme), 1 if sort_lors else 0, int(device), 0)
+        opt = _Options(int(scheme), 1 if sort_lors else 0, int(device), (C.c_int32 * 3)(*(tile or (0, 0, 0))))
         _check(lib().petalo_mlem_create(C.byref(self._ctx), C.byref(fov._c()), _tof_ptr(tof),
                                         _ptr(lors), len(lors), _ptr(sens), int(n_subsets),
                                         C.byref(opt)))

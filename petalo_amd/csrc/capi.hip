@@ -176,6 +176,14 @@ template <class F> void parallel_for(size_t n, F&& f) {
 // resident workgroups) and the least work per item, in replayed steps.
 constexpr uint64_t kWorkItems = 2048;
 constexpr uint64_t kMinItemSteps = 8192;
+// Runs of a tile are sorted by blocks of 2^kLorBlockShift consecutive LORs
+// before length (see k_bin_runs).
+constexpr uint32_t kLorBlockShift = 14;
+// Largest tile (LDS entries, row padding included): its binary64 copy fills
+// the 160 KiB of LDS of a CU.
+constexpr int32_t kMaxTileVoxels = 20480;
+// Default tile budget: two workgroups' binary64 tiles per CU.
+constexpr int32_t kScatterEntries = 10240;
 
 // Reorders lors[lo, hi) (one subset) by the image tile that holds their TOF
 // peak, and within a tile by lor_order_key (direction, then offset, then
@@ -237,6 +245,7 @@ struct petalo_mlem {
   std::vector<DevBuf<uint32_t>> irregular;  // per subset: LORs with unbounded walks
   DevBuf<float> partial;             // per-run partial projections (largest subset)
   DevBuf<float> rbuf;                // 1/(p*ac) per LOR of the current subset
+  DevBuf<float> ac;                  // additive corrections, in context LOR order
   DevBuf<double> bp64;               // binary64 backprojection
 };
 
@@ -255,29 +264,42 @@ void launch_project(petalo_mlem* c, size_t lo, size_t count) {
   HIP_CHECK(hipGetLastError());
 }
 
-// Tile shape for an image: 16384 voxels (128 KiB of binary64 in LDS).
-TileGeom tile_shape(const petalo_fov_t& fov) {
+// Splits n voxels into equal tiles of at most `target` voxels.
+int32_t even_split(uint32_t n, int32_t target) {
+  const int32_t parts = (int32_t)((n + target - 1) / target);
+  return (int32_t)((n + parts - 1) / parts);
+}
+
+// Tile shape for an image.  The LDS row pitch is odd (tx or tx + 1): with an
+// even pitch every voxel of a column falls on the same LDS bank, and the
+// binary64 atomics of lanes working on nearby parallel LORs collide.  By
+// default the tile is about 40 x 40 voxels transaxially, cut evenly from the
+// image, and as deep axially as keeps its binary64 copy within
+// kScatterEntries, so two 1024-lane workgroups fit a CU's LDS.
+TileGeom tile_shape(const petalo_fov_t& fov, const int32_t shape[3]) {
   TileGeom g;
-  g.tx = 32;
-  g.ty = 32;
-  g.tz = 16;
+  g.tx = shape[0] > 0 ? shape[0] : even_split(fov.n[0], 40);
+  g.ty = shape[1] > 0 ? shape[1] : even_split(fov.n[1], 40);
+  g.px = g.tx | 1;
+  g.tz = shape[2] > 0 ? shape[2] : even_split(fov.n[2], std::max(1, kScatterEntries / (g.px * g.ty)));
   g.ntx = (int32_t)((fov.n[0] + g.tx - 1) / g.tx);
   g.nty = (int32_t)((fov.n[1] + g.ty - 1) / g.ty);
   g.ntz = (int32_t)((fov.n[2] + g.tz - 1) / g.tz);
   g.ntiles = g.ntx * g.nty * g.ntz;
-  g.tvox = g.tx * g.ty * g.tz;
+  g.tvox = g.px * g.ty * g.tz;
   return g;
 }
 
-TileGeom make_tiles(const petalo_fov_t& fov) {
-  const TileGeom g = tile_shape(fov);
+TileGeom make_tiles(const petalo_fov_t& fov, const int32_t shape[3]) {
+  const TileGeom g = tile_shape(fov, shape);
   if (g.ntiles >= 65536) throw std::runtime_error("image too large for the tiled scheme (65536 tiles or more)");
-  if ((uint32_t)g.tvox > kRunMaxLocal + 1) throw std::runtime_error("tile too large for the run record");
+  if (g.tvox > kMaxTileVoxels) throw std::runtime_error("tile over 20480 LDS entries (row pitch included)");
+  if (g.tx + g.ty + g.tz > (int32_t)kRunMaxSteps) throw std::runtime_error("tile too long for the run record");
   return g;
 }
 
 // Puts the runs in sorted order, records where each LOR-major run went and
-// a (tile << 12) | length summary of the sorted runs for the host.
+// writes a (tile << 12) | length summary of the sorted runs for the host.
 __global__ void k_place_runs(const Run* __restrict__ in, const uint64_t* __restrict__ sorted_keys,
                              const uint32_t* __restrict__ perm, uint64_t n, Run* __restrict__ out,
                              uint32_t* __restrict__ pos, uint32_t* __restrict__ summary) {
@@ -310,10 +332,10 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     const Lor* L = c->lors.p + lo;
     if (c->tof_on)
       k_bin_runs<true, false><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
-                                                                  nullptr, nullptr, nullptr);
+                                                                  nullptr, nullptr, nullptr, 0u);
     else
       k_bin_runs<false, false><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
-                                                                   nullptr, nullptr, nullptr);
+                                                                   nullptr, nullptr, nullptr, 0u);
     HIP_CHECK(hipGetLastError());
     std::vector<uint32_t> h(count);
     HIP_CHECK(hipMemcpyAsync(h.data(), nr.p, count * 4, hipMemcpyDeviceToHost, st));
@@ -346,10 +368,10 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     DevBuf<uint32_t> perm(total), perm2(total);
     if (c->tof_on)
       k_bin_runs<true, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
-                                                                 c->run_off[sub].p, unsorted.p, keys.p);
+                                                                 c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift);
     else
       k_bin_runs<false, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
-                                                                  c->run_off[sub].p, unsorted.p, keys.p);
+                                                                  c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift);
     HIP_CHECK(hipGetLastError());
     k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
     int end_bit = 33;
@@ -427,8 +449,8 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     HIP_CHECK(hipGetLastError());
   }
   if (c->timing) HIP_CHECK(hipEventRecord(c->ev[4], st));
-  k_lor_reduce<<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->run_off[sub].p, c->run_pos[sub].p,
-                                                   c->partial.p, c->rbuf.p);
+  k_lor_reduce<<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
+                                                     c->run_pos[sub].p, c->partial.p, c->rbuf.p);
   HIP_CHECK(hipGetLastError());
   if (n_irr) {
     if (c->tof_on)
@@ -506,10 +528,12 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
     if (n_lors && !lors) return fail(PETALO_ERR_ARG, "lors is NULL");
     if (n_lors >= (1ull << 32)) return fail(PETALO_ERR_ARG, "more than 2^32-1 LORs");
     if (tof && tof->enabled && !(tof->sigma_ps > 0.0f)) return fail(PETALO_ERR_ARG, "TOF sigma must be positive");
-    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, 0};
+    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}};
     if (options) opt = *options;
     if (opt.scheme != PETALO_SCHEME_AUTO && opt.scheme != PETALO_SCHEME_ATOMIC && opt.scheme != PETALO_SCHEME_TILE)
       return fail(PETALO_ERR_ARG, "unsupported backprojection scheme");
+    for (int a = 0; a < 3; ++a)
+      if (opt.tile[a] < 0) return fail(PETALO_ERR_ARG, "negative tile dimension");
     require_device();
     HIP_CHECK(hipSetDevice(opt.device));
 
@@ -532,7 +556,7 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
       std::vector<petalo_lor_t> h(lors, lors + n_lors);
       if (opt.sort_lors)
         for (size_t s = 0; s < n_subsets; ++s)
-          order_lors(h, s * c->set_size, (s + 1) * c->set_size, c->tof_on, *fov, tile_shape(*fov));
+          order_lors(h, s * c->set_size, (s + 1) * c->set_size, c->tof_on, *fov, tile_shape(*fov, opt.tile));
       c->lors.alloc(std::max<size_t>(n_lors, 1));
       if (n_lors) HIP_CHECK(hipMemcpy(c->lors.p, h.data(), n_lors * sizeof(Lor), hipMemcpyHostToDevice));
 
@@ -548,7 +572,7 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
       }
       for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
       if (c->scheme == PETALO_SCHEME_TILE) {
-        c->tiles = make_tiles(*fov);
+        c->tiles = make_tiles(*fov, opt.tile);
         const size_t lds = (size_t)c->tiles.tvox * sizeof(double);
         HIP_CHECK(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
@@ -565,6 +589,12 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
         c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
+        {
+          std::vector<float> hac(n_lors);
+          for (size_t i = 0; i < n_lors; ++i) hac[i] = h[i].additive_correction;
+          c->ac.alloc(std::max<size_t>(n_lors, 1));
+          if (n_lors) HIP_CHECK(hipMemcpy(c->ac.p, hac.data(), n_lors * sizeof(float), hipMemcpyHostToDevice));
+        }
         c->bp64.alloc(nv);
         fill_f64(c->bp64.p, 0.0, nv, c->stream);
         uint64_t max_runs = 1;
@@ -766,6 +796,7 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->run_pos.clear();
   c->irregular.clear();
   c->partial.release();
+  c->ac.release();
   c->rbuf.release();
   c->bp64.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -241,24 +241,34 @@ __host__ __device__ __forceinline__ Step next_boundary(const WalkS& w) {
 // x*log2(e) (a few ulp from the correctly rounded value; the reference's
 // f32::exp is the host libm's, so TOF weights are compared within a
 // tolerance in any case).
-__host__ __device__ __forceinline__ float tof_exp(float x) {
+// exp(-z/2) for z = (dx/sigma)^2 >= 0.  On the device this is __expf(-0.5f*z)
+// (v_exp_f32 of the argument times log2(e)) with the two constant factors
+// merged: -0.5f*z is exact in binary32 wherever z is normal, so
+// (-0.5f*z)*log2e and z*(-0.5f*log2e) round the same real number; where z is
+// subnormal both arguments are tiny and the result is 1.
+__host__ __device__ __forceinline__ float tof_gauss_exp(float z) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return __expf(x);
+  constexpr float kNegHalfLog2e = -0.5f * 1.44269504088896341f;
+  return __builtin_amdgcn_exp2f(z * kNegHalfLog2e);
 #else
-  return expf(x);
+  return expf(-0.5f * z);
 #endif
 }
 
 // TOF Gaussian (src/gauss.rs:47-55) times the 666 mm factor (siddon.rs:106-108),
-// evaluated at the voxel entry distance `here`.
+// for a voxel entry distance `here` known to lie inside the window
+// |here - tof_peak| < cutoff.
+__host__ __device__ __forceinline__ float tof_factor_in_window(const DevTof& t, float here, float tof_peak) {
+  const float dx = here - tof_peak;
+  const float y = div_rn(dx, t.sigma, t.rcp_sigma);
+  const float z = y * y;
+  return kTofFactor * (t.peak * tof_gauss_exp(z));
+}
+
+// The same, zero outside the window (strict cutoff, gauss.rs:47-55).
 __host__ __device__ __forceinline__ float tof_factor(const DevTof& t, float here, float tof_peak) {
   const float dx = here - tof_peak;
-  if (fabsf(dx) < t.cutoff) {
-    const float y = div_rn(dx, t.sigma, t.rcp_sigma);
-    const float z = y * y;
-    return kTofFactor * (t.peak * tof_exp(-0.5f * z));
-  }
-  return 0.0f;
+  return fabsf(dx) < t.cutoff ? tof_factor_in_window(t, here, tof_peak) : 0.0f;
 }
 
 // Advance across the boundary of step s (siddon.rs:118-128): only the

@@ -39,7 +39,9 @@ struct TileGeom {
   int32_t tx, ty, tz;     // nominal tile dimensions
   int32_t ntx, nty, ntz;  // tiles per axis
   int32_t ntiles;
-  int32_t tvox;           // tx*ty*tz: LDS entries per tile
+  int32_t px;             // LDS row pitch in entries (>= tx; an odd pitch spreads a
+                          // column of voxels over the LDS banks)
+  int32_t tvox;           // px*ty*tz: LDS entries per tile
 };
 
 // One run: the walk state at the first pushed step of a LOR inside a tile
@@ -48,7 +50,7 @@ struct __align__(8) Run {
   float nb_x, nb_y, nb_z, here;
   float vl_x, vl_y, vl_z, tof_peak;
   uint32_t code;  // bits 0-11: steps in the run; bits 12-14: axis flips (x, y, z);
-                  // bits 15-31: tile-local index of the first voxel (x fastest)
+                  // bits 15-31: tile-local LDS index of the first voxel (x fastest)
   uint32_t lor;   // index of the LOR within its subset
 };
 
@@ -69,7 +71,7 @@ __host__ __device__ __forceinline__ int32_t tile_of(const TileGeom& g, const Coo
 }
 
 __host__ __device__ __forceinline__ uint32_t local_of(const TileGeom& g, const Coord& q) {
-  return (uint32_t)((q.x % g.tx) + g.tx * ((q.y % g.ty) + g.ty * (q.z % g.tz)));
+  return (uint32_t)((q.x % g.tx) + g.px * ((q.y % g.ty) + g.ty * (q.z % g.tz)));
 }
 
 // Walk #1 with TOF window fast-forward, as in k_mlem_project.  Calls
@@ -153,15 +155,15 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
 
 // Binning walk: FILL = false counts runs per LOR into n_runs[i]; FILL = true
 // writes them at runs[offset[i] ...] (LOR-major) with their sort keys:
-// tile, then block of 65536 consecutive LORs (LORs are ordered spatially, so
-// a block's 1/(p*ac) values stay cache-resident while its runs in a tile are
-// replayed), then longest run first (lanes of a wavefront get runs of similar
-// length).
+// tile, then block of 2^lor_shift consecutive LORs (LORs are ordered
+// spatially, so a block's 1/(p*ac) values stay cache-resident while its runs
+// in a tile are replayed), then longest run first (lanes of a wavefront get
+// runs of similar length).
 template <bool TOF, bool FILL>
 __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
                                                   TileGeom g, uint32_t* __restrict__ n_runs,
                                                   const uint32_t* __restrict__ offset, Run* __restrict__ runs,
-                                                  uint64_t* __restrict__ keys) {
+                                                  uint64_t* __restrict__ keys, uint32_t lor_shift) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Lor l = load_lor(lors, i);
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
   const uint32_t count = bin_lor<TOF>(l, i, f, t, g, [&](const Run& ru, uint32_t tile) {
     if (FILL) {
       runs[base + k] = ru;
-      keys[base + k] = ((uint64_t)tile << 32) | ((uint64_t)(i >> 16) << 12) | (kRunMaxSteps - run_len(ru.code));
+      keys[base + k] = ((uint64_t)tile << 32) | ((uint64_t)(i >> lor_shift) << 12) | (kRunMaxSteps - run_len(ru.code));
     }
     ++k;
   });
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
 template <bool TOF, class Add>
 __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, const DevTof& t, const TileGeom& g,
                                                     Add&& add) {
-  const int32_t dlx = 1, dly = g.tx, dlz = g.tx * g.ty;
+  const int32_t dlx = 1, dly = g.px, dlz = g.px * g.ty;
   WalkS w;
   w.nb_x = ru.nb_x;
   w.nb_y = ru.nb_y;
@@ -207,7 +209,12 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
   const uint32_t m = run_len(ru.code);  // the run length bounds the replay
   for (uint32_t k = 0; k < m; ++k) {
     const Step s = next_boundary(w);
-    const float wt = step_weight<TOF>(w, c, s, t);
+    // Every step of a run lies inside the TOF window, so the window test of
+    // step_weight is skipped: the run spans the walk's first to last pushed
+    // step in the tile, pushed steps lie inside the window, `here` does not
+    // decrease along the walk and the window is an interval of here - peak.
+    float wt = s.bnd - w.here;
+    if (TOF) wt *= tof_factor_in_window(t, w.here, c.tof_peak);
     if (wt > 0.0f) add((uint32_t)w.index, wt * r);
     move(w, c, s);
   }
@@ -216,7 +223,9 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
 // Forward, tiled: one workgroup per work item copies its tile of the image
 // into LDS and replays its runs, writing each run's partial projection
 // (sum of w*x over the run's pushed steps, in walk order) at the run's
-// sorted position.  Each lane loads its next run record before replaying the
+// sorted position (coalesced; writing them LOR-major instead scatters 4-byte
+// stores over the whole array and costs more than the gather in
+// k_lor_reduce).  Each lane loads its next run record before replaying the
 // current one, so the record's latency hides behind the replay (the replay
 // itself touches only LDS).
 template <bool TOF>
@@ -231,9 +240,9 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs,
   const int32_t txi = tl % g.ntx;
   const int32_t x0 = txi * g.tx, y0 = tyi * g.ty, z0 = tzi * g.tz;
   for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) {
-    const int32_t lx = k % g.tx, ly = (k / g.tx) % g.ty, lz = k / (g.tx * g.ty);
+    const int32_t lx = k % g.px, ly = (k / g.px) % g.ty, lz = k / (g.px * g.ty);
     const int32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
-    timg[k] = (x < f.nx && y < f.ny && z < f.nz) ? img[x + f.nx * (y + f.ny * z)] : 0.0f;
+    timg[k] = (lx < g.tx && x < f.nx && y < f.ny && z < f.nz) ? img[x + f.nx * (y + f.ny * z)] : 0.0f;
   }
   __syncthreads();
   int32_t j = wi.y + (int32_t)threadIdx.x;
@@ -256,16 +265,57 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs,
 // r = 1/(p*ac) per LOR with p the sum of its runs' partials in walk order;
 // run_pos[run_off[i] ...] are the sorted positions of LOR i's runs (LORs
 // without runs get 1/(0*ac), as a row without elements does).
-__global__ __launch_bounds__(256) void k_lor_reduce(const Lor* __restrict__ lors, uint32_t n,
+//
+// Each lane takes kReduceLors LORs, 64 apart (loads stay coalesced across
+// the wavefront), and issues the loads of all of them together: the first
+// four runs of every LOR in one batch, any further runs in a loop.  Each sum
+// keeps its order.  Workgroups land on the eight XCDs round-robin by index;
+// XCD x takes the x-th eighth of the LORs, so each L2 caches the partials of
+// one eighth.  Launch with reduce_grid(n) workgroups of 256.
+constexpr uint32_t kReduceLors = 4;
+inline unsigned reduce_grid(size_t n) {
+  const size_t share = (n + 7) / 8;
+  const size_t per_group = 256 * kReduceLors;
+  return (unsigned)(8 * ((share + per_group - 1) / per_group));
+}
+__global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac, uint32_t n,
                                                     const uint32_t* __restrict__ run_off,
                                                     const uint32_t* __restrict__ run_pos,
                                                     const float* __restrict__ partial, float* __restrict__ r_out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t b = run_off[i], e = run_off[i + 1];
-  float p = 0.0f;
-  for (uint32_t k = b; k < e; ++k) p += partial[run_pos[k]];
-  r_out[i] = 1.0f / (p * lors[i].ac);
+  const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+  const uint32_t share = (n + 7u) / 8u;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t first = slot * blockDim.x * kReduceLors + wave * 64u * kReduceLors + lane;
+  uint32_t idx[kReduceLors], b[kReduceLors], e[kReduceLors];
+  bool on[kReduceLors];
+#pragma unroll
+  for (uint32_t u = 0; u < kReduceLors; ++u) {
+    const uint32_t local = first + 64u * u;
+    idx[u] = xcd * share + local;
+    on[u] = local < share && idx[u] < n;
+    b[u] = on[u] ? run_off[idx[u]] : 0u;
+    e[u] = on[u] ? run_off[idx[u] + 1] : 0u;
+  }
+  uint32_t q[kReduceLors][4];
+#pragma unroll
+  for (uint32_t u = 0; u < kReduceLors; ++u)
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) q[u][k] = b[u] + k < e[u] ? run_pos[b[u] + k] : 0u;
+  float v[kReduceLors][4];
+#pragma unroll
+  for (uint32_t u = 0; u < kReduceLors; ++u)
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) v[u][k] = b[u] + k < e[u] ? partial[q[u][k]] : 0.0f;
+#pragma unroll
+  for (uint32_t u = 0; u < kReduceLors; ++u) {
+    if (!on[u]) continue;
+    float p = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      if (b[u] + k < e[u]) p += v[u][k];
+    for (uint32_t k = b[u] + 4; k < e[u]; ++k) p += partial[run_pos[k]];
+    r_out[idx[u]] = 1.0f / (p * ac[idx[u]]);
+  }
 }
 
 // LORs whose walk is not provably bounded: exact one-lane projection and
@@ -359,9 +409,9 @@ __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, 
   for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) {
     const double v = tile[k];
     if (v == 0.0) continue;
-    const int32_t lx = k % g.tx, ly = (k / g.tx) % g.ty, lz = k / (g.tx * g.ty);
+    const int32_t lx = k % g.px, ly = (k / g.px) % g.ty, lz = k / (g.px * g.ty);
     const int32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
-    if (x < f.nx && y < f.ny && z < f.nz) unsafeAtomicAdd(bp + x + f.nx * (y + f.ny * z), v);
+    if (lx < g.tx && x < f.nx && y < f.ny && z < f.nz) unsafeAtomicAdd(bp + x + f.nx * (y + f.ny * z), v);
   }
 }
 

@@ -269,3 +269,22 @@ def test_tile_scheme_odd_shapes(gpu, tof):
             e = rel_l2(g[k], o[k])
             _say(f"odd shapes tof={tof} scheme={scheme} iter {k + 1}: relL2 {e:.2e}")
             assert e <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [(8, 16, 4), (5, 7, 3), (64, 64, 4), (40, 40, 6)])
+def test_tile_shapes(gpu, tile):
+    """Explicit tile shapes (even and odd row pitch, tiles wider than the
+    image, tiles that cut it unevenly) give the oracle's TOF images."""
+    full, n = [100.0, 130.0, 170.0], [37, 51, 29]
+    tof = (42.47, 3.0)
+    rng = np.random.default_rng(23)
+    lors = random_lors(rng, 20000, radius=200.0, zhalf=120.0, dt_sigma_ps=63.7)
+    lors["p2"][::13, 1] = lors["p1"][::13, 1]
+    fov = P.FOV(full, n)
+    o = _run_oracle(full, n, lors, 2, tof=tof)
+    g = _run_gpu(fov, lors, 2, tof=P.Tof(*tof), scheme=P.SCHEME_TILE, tile=tile)
+    for k in range(2):
+        e = rel_l2(g[k], o[k])
+        _say(f"tile {tile} iter {k + 1}: relL2 {e:.2e}")
+        assert e <= 1e-5

@@ -212,6 +212,18 @@ def test_no_gpu_reports_error_not_crash():
     assert e.value.code == 2
 
 
+def test_mlem_option_checks_come_first():
+    """Argument errors are reported before any device work (no GPU needed)."""
+    lors = P.make_lors([[-100, 0, 0]], [[100, 0, 0]])
+    fov = P.FOV([10.0] * 3, [2] * 3)
+    with pytest.raises(P.PetaloError) as e:
+        P.Mlem(fov, lors, tile=(8, -1, 4))
+    assert e.value.code == 1 and "tile" in str(e.value)
+    with pytest.raises(P.PetaloError) as e:
+        P.Mlem(fov, lors, scheme=7)
+    assert e.value.code == 1
+
+
 def test_tools_help():
     for tool in ("mlem", "make_sensitivity_image"):
         exe = os.path.join(ROOT, "petalo_amd", "bin", tool)
