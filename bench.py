@@ -202,6 +202,10 @@ def main():
         "scheme": info,
         "update_ms": float(np.mean(upd_ms)),
         "sensitivity_ms": sens_ms,
+        "sensitivity": ({"lors": n_sens, "ms": sens_ms, "n_steps": sens_stats["n_steps"], "n_nz": sens_stats["n_nz"],
+                         "algorithmic_bytes": 8 * sens_stats["n_nz"] + 4 * fov.nvox,
+                         "achieved_gbs": (8 * sens_stats["n_nz"] + 4 * fov.nvox) / (sens_ms * 1e-3) / 1e9}
+                        if sens_stats else None),
         "parity_rel_l2_sample": parity,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get(dom), "kernel": dom,

@@ -449,7 +449,7 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     HIP_CHECK(hipGetLastError());
   }
   if (c->timing) HIP_CHECK(hipEventRecord(c->ev[4], st));
-  k_lor_reduce<<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
+  k_lor_reduce<false><<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
                                                      c->run_pos[sub].p, c->partial.p, c->rbuf.p);
   HIP_CHECK(hipGetLastError());
   if (n_irr) {
@@ -885,12 +885,157 @@ int petalo_sensitivity_lors(const petalo_fov_t* fov, size_t n_lors, uint64_t see
     if (n_lors == 0) return PETALO_OK;
     require_device();
     DevBuf<Lor> d(n_lors);
-    k_sens_lors<<<grid_for(n_lors), kBlock>>>((uint32_t)n_lors, seed, to_dev(*fov), L, D / 2.0f, d.p);
+    k_sens_lors<<<grid_for(n_lors), kBlock>>>((uint32_t)n_lors, 0u, seed, to_dev(*fov), L, D / 2.0f, d.p);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(out, d.p, n_lors * sizeof(Lor), hipMemcpyDeviceToHost));
     return PETALO_OK;
   });
 }
+
+}  // extern "C"
+
+namespace {
+
+// Sensitivity image on the tiled scheme, LORs in batches of kSensBatch.
+constexpr uint32_t kSensBatch = 1u << 24;
+// Work item size of the sensitivity kernels, in runs.
+constexpr uint32_t kSensItemRuns = 8192;
+
+// Unbounded walks (count 0xffffffff): copied out for the one-lane kernel,
+// their count set to 0.  counts[n] is set to 0 for the exclusive scan.
+__global__ void k_take_irregular(uint32_t* __restrict__ counts, uint32_t n, const Lor* __restrict__ lors,
+                                 Lor* __restrict__ irr, uint32_t* __restrict__ n_irr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == n) counts[n] = 0;
+  if (i >= n || counts[i] != 0xffffffffu) return;
+  counts[i] = 0;
+  irr[atomicAdd(n_irr, 1u)] = lors[i];
+}
+
+// pos[perm[j]] = j.
+__global__ void k_invert_perm(const uint32_t* __restrict__ perm, uint64_t n, uint32_t* __restrict__ pos) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) pos[perm[j]] = (uint32_t)j;
+}
+
+// First sorted position of every tile; keys hold the tile from bit tshift.
+__global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uint32_t tshift,
+                              uint64_t* __restrict__ start) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t t = (uint32_t)(keys[j] >> tshift);
+  if (j == 0 || (uint32_t)(keys[j - 1] >> tshift) != t) start[t] = j;
+}
+
+// Sensitivity image (one FOV) with the tiled kernels: per batch, bin the
+// LORs into runs, sort their indices by tile (the runs are read through the
+// permutation: each is used once or twice), then (attenuation present) forward
+// project mu over the runs, reduce to 1/exp(-p) per LOR, and backproject;
+// with mu == 0 everywhere every LOR has p = 0 exactly and r = 1, so the
+// forward half is skipped.  Unbounded walks go to the one-lane kernel.
+void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const float* mu_host, const float* dmu,
+                const Lor* given, size_t n_lors, uint64_t seed, float L, float R, double* acc,
+                unsigned long long* cnt) {
+  const int32_t no_shape[3] = {0, 0, 0};
+  const TileGeom g = make_tiles(fov, no_shape);
+  const size_t na = nvox(fov);
+  bool mu_zero = true;
+  for (size_t k = 0; k < na && mu_zero; ++k) mu_zero = mu_host[k] == 0.0f;
+  const uint32_t shift = mu_zero ? 32u : kLorBlockShift;
+  const uint32_t tshift = mu_zero ? 12u : 32u;
+  int tile_bits = 1;
+  while ((1 << tile_bits) < g.ntiles) ++tile_bits;
+  const int end_bit = (int)tshift + tile_bits;
+  const DevTof notof = no_tof();
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)((size_t)g.tvox * sizeof(double))));
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_fwd_tile<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)((size_t)g.tvox * sizeof(float))));
+  const size_t bmax = std::min<size_t>(n_lors, kSensBatch);
+  DevBuf<Lor> lb(bmax), irr(bmax);
+  DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1);
+  DevBuf<float> rbuf;
+  if (!mu_zero) rbuf.alloc(bmax);
+  for (size_t first = 0; first < n_lors; first += kSensBatch) {
+    const uint32_t b = (uint32_t)std::min<size_t>(kSensBatch, n_lors - first);
+    if (given)
+      HIP_CHECK(hipMemcpyAsync(lb.p, given + first, (size_t)b * sizeof(Lor), hipMemcpyDeviceToDevice, s));
+    else
+      k_sens_lors<<<grid_for(b), kBlock, 0, s>>>(b, (uint32_t)first, seed, f, L, R, lb.p);
+    k_bin_runs<false, false><<<grid_for(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr,
+                                                             nullptr, shift, cnt);
+    HIP_CHECK(hipMemsetAsync(n_irr.p, 0, 4, s));
+    k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts.p, b, lb.p, irr.p, n_irr.p);
+    HIP_CHECK(hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts.p, offs.p, (int)b + 1, s));
+    DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, counts.p, offs.p, (int)b + 1, s));
+    uint32_t total = 0, h_irr = 0;
+    HIP_CHECK(hipMemcpyAsync(&total, offs.p + b, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&h_irr, n_irr.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (total >= (1u << 31)) throw std::runtime_error("2^31 or more tile runs in one sensitivity batch");
+    if (total) {
+      DevBuf<Run> unsorted(total);
+      DevBuf<uint64_t> keys(total), keys2(total);
+      DevBuf<uint32_t> perm(total), perm2(total), pos;
+      k_bin_runs<false, true><<<grid_for(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, unsorted.p,
+                                                              keys.p, shift);
+      k_iota<<<grid64(total), kBlock, 0, s>>>(perm.p, total);
+      HIP_CHECK(hipGetLastError());
+      size_t sort_bytes = 0;
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys.p, keys2.p, perm.p, perm2.p,
+                                                   (int)total, 0, end_bit, s));
+      DevBuf<unsigned char> stmp(std::max<size_t>(sort_bytes, 1));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp.p, sort_bytes, keys.p, keys2.p, perm.p, perm2.p,
+                                                   (int)total, 0, end_bit, s));
+      if (!mu_zero) {
+        pos.alloc(total);
+        k_invert_perm<<<grid64(total), kBlock, 0, s>>>(perm2.p, total, pos.p);
+      }
+      DevBuf<uint64_t> dstart(g.ntiles);
+      HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, (size_t)g.ntiles * 8, s));
+      k_tile_starts<<<grid64(total), kBlock, 0, s>>>(keys2.p, total, tshift, dstart.p);
+      HIP_CHECK(hipGetLastError());
+      std::vector<uint64_t> starts(g.ntiles + 1);
+      HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, (size_t)g.ntiles * 8, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      starts[g.ntiles] = total;
+      for (int k = g.ntiles - 1; k >= 0; --k)
+        if (starts[k] == UINT64_MAX) starts[k] = starts[k + 1];
+      std::vector<int4> items;
+      for (int k = 0; k < g.ntiles; ++k)
+        for (uint64_t a = starts[k]; a < starts[k + 1]; a += kSensItemRuns)
+          items.push_back(make_int4(k, (int)a, (int)std::min<uint64_t>(starts[k + 1], a + kSensItemRuns), 0));
+      std::stable_sort(items.begin(), items.end(),
+                       [](const int4& u, const int4& v) { return u.z - u.y > v.z - v.y; });
+      DevBuf<int4> work(items.size());
+      HIP_CHECK(hipMemcpyAsync(work.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+      const unsigned nw = (unsigned)items.size();
+      if (!mu_zero) {
+        DevBuf<float> partial(total);
+        k_fwd_tile<false><<<nw, 1024, (size_t)g.tvox * sizeof(float), s>>>(unsorted.p, work.p, dmu, f, notof, g,
+                                                                           partial.p, perm2.p);
+        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, pos.p, partial.p, rbuf.p);
+        HIP_CHECK(hipGetLastError());
+      }
+      k_scatter<false><<<nw, 1024, (size_t)g.tvox * sizeof(double), s>>>(unsorted.p, work.p,
+                                                                         mu_zero ? nullptr : rbuf.p, f, notof,
+                                                                         g, acc, perm2.p);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipStreamSynchronize(s));  // batch buffers are released at scope end
+    }
+    if (h_irr) {
+      k_sens<false, false><<<grid_for(h_irr), kBlock, 0, s>>>(irr.p, h_irr, seed, L, R, f, dmu, f, acc, cnt);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
 
 int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, const petalo_fov_t* out_fov,
                                   const petalo_lor_t* lors, size_t n_lors, uint64_t seed, float L, float D,
@@ -929,12 +1074,12 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
       const unsigned g = grid_for(n_lors);
       const uint32_t n = (uint32_t)n_lors;
       const float R = D / 2.0f;
-      if (lors) {
-        if (two) k_sens<false, true><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
-        else     k_sens<false, false><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
+      if (!two) {
+        sens_tiled(s, *att_fov, fa, mu, dmu.p, lors ? dl.p : nullptr, n_lors, seed, L, R, acc.p, cnt.p);
       } else {
-        if (two) k_sens<true, true><<<g, kBlock, 0, s>>>(nullptr, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
-        else     k_sens<true, false><<<g, kBlock, 0, s>>>(nullptr, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
+        // two FOVs: one lane per LOR, binary64 global atomics
+        if (lors) k_sens<false, true><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
+        else      k_sens<true, true><<<g, kBlock, 0, s>>>(nullptr, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
       }
       HIP_CHECK(hipGetLastError());
     }
@@ -945,7 +1090,8 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
     HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (stats)
-      *stats = petalo_stats_t{h[CNT_LORS], h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED], h[CNT_RUNAWAY]};
+      *stats = petalo_stats_t{(uint64_t)n_lors, h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED],
+                              h[CNT_RUNAWAY]};
     return PETALO_OK;
   });
 }

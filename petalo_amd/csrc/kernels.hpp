@@ -242,12 +242,13 @@ __device__ __forceinline__ Lor sens_lor(uint64_t seed, uint32_t i, const DevFov&
   return Lor{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
 }
 
-__global__ __launch_bounds__(256) void k_sens_lors(uint32_t n, uint64_t seed, DevFov f, float L, float R,
-                                                   Lor* __restrict__ out) {
+// LORs first .. first + n - 1 of the stream.
+__global__ __launch_bounds__(256) void k_sens_lors(uint32_t n, uint32_t first, uint64_t seed, DevFov f, float L,
+                                                   float R, Lor* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   bool found;
-  const Lor l = sens_lor(seed, i, f, L, R, found);
+  const Lor l = sens_lor(seed, first + i, f, L, R, found);
   float4* p = reinterpret_cast<float4*>(out + i);
   p[0] = make_float4(l.x1, l.y1, l.z1, l.x2);
   p[1] = make_float4(l.y2, l.z2, l.dt, l.ac);

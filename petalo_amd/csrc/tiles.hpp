@@ -60,24 +60,40 @@ __host__ __device__ __forceinline__ uint32_t run_len(uint32_t code) { return cod
 __host__ __device__ __forceinline__ uint32_t run_flips(uint32_t code) { return (code >> 12) & 7u; }
 __host__ __device__ __forceinline__ uint32_t run_local(uint32_t code) { return code >> 15; }
 
-// Per-step voxel coordinates (original axes) tracked by the binning walk.
+// Tile-local voxel coordinates (original axes) and tile id, tracked
+// incrementally by the binning walk: no division per step.
 struct Coord {
-  int32_t x, y, z;
+  int32_t lx, ly, lz;  // position inside the tile
+  int32_t tile;
   int32_t sx, sy, sz;  // +1 or -1 per axis
 };
 
-__host__ __device__ __forceinline__ int32_t tile_of(const TileGeom& g, const Coord& q) {
-  return q.x / g.tx + g.ntx * (q.y / g.ty + g.nty * (q.z / g.tz));
+__host__ __device__ __forceinline__ uint32_t local_of(const TileGeom& g, const Coord& q) {
+  return (uint32_t)(q.lx + g.px * (q.ly + g.ty * q.lz));
 }
 
-__host__ __device__ __forceinline__ uint32_t local_of(const TileGeom& g, const Coord& q) {
-  return (uint32_t)((q.x % g.tx) + g.px * ((q.y % g.ty) + g.ty * (q.z % g.tz)));
+// Moves q one voxel along the crossed axis of s (exactly one of ax, ay, az).
+__host__ __device__ __forceinline__ void coord_step(Coord& q, const TileGeom& g, const Step& s) {
+  if (s.ax) {
+    q.lx += q.sx;
+    if (q.lx == g.tx) { q.lx = 0; q.tile += 1; }
+    else if (q.lx < 0) { q.lx = g.tx - 1; q.tile -= 1; }
+  } else if (s.ay) {
+    q.ly += q.sy;
+    if (q.ly == g.ty) { q.ly = 0; q.tile += g.ntx; }
+    else if (q.ly < 0) { q.ly = g.ty - 1; q.tile -= g.ntx; }
+  } else {
+    q.lz += q.sz;
+    if (q.lz == g.tz) { q.lz = 0; q.tile += g.ntx * g.nty; }
+    else if (q.lz < 0) { q.lz = g.tz - 1; q.tile -= g.ntx * g.nty; }
+  }
 }
 
 // Walk #1 with TOF window fast-forward, as in k_mlem_project.  Calls
-// on_step(w, c, s, weight, coord) for every walked step; bounded walks only.
+// on_step(w, c, weight, coord) for every walked step; bounded walks only.
 template <bool TOF, class F>
-__host__ __device__ __forceinline__ void walk_pushes(const Lor& l, const DevFov& f, const DevTof& t, F&& on_step) {
+__host__ __device__ __forceinline__ void walk_pushes(const Lor& l, const DevFov& f, const DevTof& t,
+                                                     const TileGeom& g, F&& on_step) {
   WalkC c;
   WalkS w;
   if (!walk_setup(l, f, c, w)) return;
@@ -86,10 +102,14 @@ __host__ __device__ __forceinline__ void walk_pushes(const Lor& l, const DevFov&
   Coord q;
   {
     const int32_t nxy = f.nx * f.ny;
-    q.z = w.index / nxy;
-    const int32_t rem = w.index - q.z * nxy;
-    q.y = rem / f.nx;
-    q.x = rem - q.y * f.nx;
+    const int32_t z = w.index / nxy;
+    const int32_t rem = w.index - z * nxy;
+    const int32_t y = rem / f.nx;
+    const int32_t x = rem - y * f.nx;
+    q.lx = x % g.tx;
+    q.ly = y % g.ty;
+    q.lz = z % g.tz;
+    q.tile = x / g.tx + g.ntx * (y / g.ty + g.nty * (z / g.tz));
     q.sx = c.d_x < 0 ? -1 : 1;
     q.sy = c.d_y < 0 ? -1 : 1;
     q.sz = c.d_z < 0 ? -1 : 1;
@@ -99,9 +119,7 @@ __host__ __device__ __forceinline__ void walk_pushes(const Lor& l, const DevFov&
     const Step s = next_boundary(w);
     const float wt = step_weight<TOF>(w, c, s, t);
     on_step(w, c, wt, q);
-    q.x += s.ax ? q.sx : 0;
-    q.y += s.ay ? q.sy : 0;
-    q.z += s.az ? q.sz : 0;
+    coord_step(q, g, s);
     if (advance(w, c, s)) break;
   }
 }
@@ -110,7 +128,8 @@ __host__ __device__ __forceinline__ void walk_pushes(const Lor& l, const DevFov&
 // emit(run, tile) for each and returns their number.
 template <bool TOF, class Emit>
 __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, const DevFov& f, const DevTof& t,
-                                                     const TileGeom& g, Emit&& emit) {
+                                                     const TileGeom& g, Emit&& emit, uint32_t* n_steps = nullptr,
+                                                     uint32_t* n_nz = nullptr) {
   uint32_t count = 0;
   int32_t cur = -1;        // tile of the open run
   uint32_t steps_in = 0;   // steps since the open run's first push
@@ -123,8 +142,10 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
     ++count;
     cur = -1;
   };
-  walk_pushes<TOF>(l, f, t, [&](const WalkS& w, const WalkC& c, float wt, const Coord& q) {
-    const int32_t tl = tile_of(g, q);
+  walk_pushes<TOF>(l, f, t, g, [&](const WalkS& w, const WalkC& c, float wt, const Coord& q) {
+    if (n_steps) ++*n_steps;
+    if (n_nz && wt > 0.0f) ++*n_nz;
+    const int32_t tl = q.tile;
     if (cur >= 0) {
       ++steps_in;
       // a run ends where the walk leaves its tile, or when its length field is full
@@ -155,32 +176,51 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
 
 // Binning walk: FILL = false counts runs per LOR into n_runs[i]; FILL = true
 // writes them at runs[offset[i] ...] (LOR-major) with their sort keys:
-// tile, then block of 2^lor_shift consecutive LORs (LORs are ordered
-// spatially, so a block's 1/(p*ac) values stay cache-resident while its runs
-// in a tile are replayed), then longest run first (lanes of a wavefront get
-// runs of similar length).
+// tile (bits 32 and up), then block of 2^lor_shift consecutive LORs (bits
+// 12-31; LORs are ordered spatially, so a block's 1/(p*ac) values stay
+// cache-resident while its runs in a tile are replayed), then longest run
+// first (bits 0-11; lanes of a wavefront get runs of similar length).  With
+// lor_shift >= 32 the key is (tile << 12) | length field only.
 template <bool TOF, bool FILL>
 __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
                                                   TileGeom g, uint32_t* __restrict__ n_runs,
                                                   const uint32_t* __restrict__ offset, Run* __restrict__ runs,
-                                                  uint64_t* __restrict__ keys, uint32_t lor_shift) {
+                                                  uint64_t* __restrict__ keys, uint32_t lor_shift,
+                                                  unsigned long long* __restrict__ counters = nullptr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const Lor l = load_lor(lors, i);
-  const uint32_t base = FILL ? offset[i] : 0;
-  uint32_t k = 0;
-  const uint32_t count = bin_lor<TOF>(l, i, f, t, g, [&](const Run& ru, uint32_t tile) {
-    if (FILL) {
-      runs[base + k] = ru;
-      keys[base + k] = ((uint64_t)tile << 32) | ((uint64_t)(i >> lor_shift) << 12) | (kRunMaxSteps - run_len(ru.code));
+  uint32_t steps = 0, nz = 0, missed = 0;
+  if (i < n) {
+    const Lor l = load_lor(lors, i);
+    const uint32_t base = FILL ? offset[i] : 0;
+    uint32_t k = 0;
+    const uint32_t count = bin_lor<TOF>(
+        l, i, f, t, g,
+        [&](const Run& ru, uint32_t tile) {
+          if (FILL) {
+            runs[base + k] = ru;
+            // lor_shift >= 32: no LOR-block field, key = (tile << 12) | ...
+            const uint64_t blk = lor_shift >= 32 ? 0u : (uint64_t)(i >> lor_shift);
+            const uint32_t tshift = lor_shift >= 32 ? 12u : 32u;
+            keys[base + k] = ((uint64_t)tile << tshift) | (blk << 12) | (kRunMaxSteps - run_len(ru.code));
+          }
+          ++k;
+        },
+        FILL ? nullptr : &steps, FILL ? nullptr : &nz);
+    if (!FILL) {
+      // unbounded walks are projected by the one-lane kernels instead
+      WalkC c;
+      WalkS w;
+      const bool hit = walk_setup(l, f, c, w);
+      missed = hit ? 0u : 1u;
+      n_runs[i] = (hit && !bounded(w)) ? 0xffffffffu : count;
     }
-    ++k;
-  });
-  if (!FILL) {
-    // unbounded walks are projected by the one-lane kernel instead
-    WalkC c;
-    WalkS w;
-    n_runs[i] = (walk_setup(l, f, c, w) && !bounded(w)) ? 0xffffffffu : count;
+  }
+  if (!FILL && counters) {
+    // exact work counters of the bounded walks (unbounded ones count where
+    // they are projected)
+    wave_add_u64(counters + CNT_STEPS, steps);
+    wave_add_u64(counters + CNT_NZ, nz);
+    wave_add_u64(counters + CNT_MISSED, missed);
   }
 }
 
@@ -231,7 +271,8 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
 template <bool TOF>
 __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs, const int4* __restrict__ work,
                                                    const float* __restrict__ img, DevFov f, DevTof t, TileGeom g,
-                                                   float* __restrict__ partial) {
+                                                   float* __restrict__ partial,
+                                                   const uint32_t* __restrict__ perm = nullptr) {
   extern __shared__ float timg[];
   const int4 wi = work[blockIdx.x];
   const int32_t tl = wi.x;
@@ -249,10 +290,11 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs,
   if (j >= wi.z) return;
   const int32_t last = wi.z - 1;
   const int32_t stride = (int32_t)blockDim.x;
-  Run cur = runs[j];
+  Run cur = runs[perm ? perm[j] : j];
   for (;;) {
     const int32_t jn = j + stride;
-    const Run nxt = runs[jn < last ? jn : last];  // in bounds; unused past the end
+    const int32_t jc = jn < last ? jn : last;  // in bounds; unused past the end
+    const Run nxt = runs[perm ? perm[jc] : jc];
     float p = 0.0f;
     replay_run<TOF>(cur, 1.0f, t, g, [&](uint32_t li, float wt) { p += wt * timg[li]; });
     partial[j] = p;
@@ -262,7 +304,8 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs,
   }
 }
 
-// r = 1/(p*ac) per LOR with p the sum of its runs' partials in walk order;
+// r = 1/(p*ac) per LOR (MLEM; 1/exp(-p) for the sensitivity image) with p the
+// sum of its runs' partials in walk order;
 // run_pos[run_off[i] ...] are the sorted positions of LOR i's runs (LORs
 // without runs get 1/(0*ac), as a row without elements does).
 //
@@ -278,6 +321,7 @@ inline unsigned reduce_grid(size_t n) {
   const size_t per_group = 256 * kReduceLors;
   return (unsigned)(8 * ((share + per_group - 1) / per_group));
 }
+template <bool SENS>
 __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac, uint32_t n,
                                                     const uint32_t* __restrict__ run_off,
                                                     const uint32_t* __restrict__ run_pos,
@@ -314,7 +358,9 @@ __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac
     for (uint32_t k = 0; k < 4; ++k)
       if (b[u] + k < e[u]) p += v[u][k];
     for (uint32_t k = b[u] + 4; k < e[u]; ++k) p += partial[run_pos[k]];
-    r_out[idx[u]] = 1.0f / (p * ac[idx[u]]);
+    // MLEM: r = 1/(p*ac); sensitivity: p adapted to exp(-p), r = 1/exp(-p)
+    // (projector.rs:67-74,144-149)
+    r_out[idx[u]] = SENS ? 1.0f / expf(-p) : 1.0f / (p * ac[idx[u]]);
   }
 }
 
@@ -369,10 +415,13 @@ __global__ __launch_bounds__(256) void k_fwd_irregular(const Lor* __restrict__ l
 // Backprojection, tiled: one workgroup per work item (tile, first run, end
 // run).  Lanes keep two runs in flight: the record of the run after next and
 // the 1/(p*ac) of the next run load while the current run is replayed.
+// r_in == nullptr backprojects the weights themselves (r = 1).  With perm,
+// sorted run j is runs[perm[j]] (runs left in LOR-major order).
 template <bool TOF>
 __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, const int4* __restrict__ work,
                                                   const float* __restrict__ r_in, DevFov f, DevTof t, TileGeom g,
-                                                  double* __restrict__ bp) {
+                                                  double* __restrict__ bp,
+                                                  const uint32_t* __restrict__ perm = nullptr) {
   extern __shared__ double tile[];
   const int4 wi = work[blockIdx.x];
   const int32_t tl = wi.x;
@@ -383,14 +432,16 @@ __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, 
     const int32_t last = wi.z - 1;
     const int32_t stride = (int32_t)blockDim.x;
     const int32_t j1 = j + stride;
-    Run cur = runs[j];
-    Run nxt = runs[j1 < last ? j1 : last];
-    float r = r_in[cur.lor];
+    const int32_t j1c = j1 < last ? j1 : last;
+    Run cur = runs[perm ? perm[j] : j];
+    Run nxt = runs[perm ? perm[j1c] : j1c];
+    float r = r_in ? r_in[cur.lor] : 1.0f;
     for (;;) {
       const int32_t jn = j + stride;
       const int32_t jnn = jn + stride;
-      const float rn = r_in[nxt.lor];
-      const Run nn = runs[jnn < last ? jnn : last];
+      const float rn = r_in ? r_in[nxt.lor] : 1.0f;
+      const int32_t jnc = jnn < last ? jnn : last;
+      const Run nn = runs[perm ? perm[jnc] : jnc];
       replay_run<TOF>(cur, r, t, g, [&](uint32_t li, float v) { atomicAdd(tile + li, (double)v); });
       if (jn > last) break;
       cur = nxt;

@@ -207,6 +207,56 @@ def test_sensitivity_two_fovs(gpu):
     assert rel_l2(img.data, ref) <= 1e-5
 
 
+def test_sensitivity_no_attenuation(gpu):
+    """mu = 0 everywhere (the tiled path skips the forward half: p = 0 exactly)."""
+    att = P.FOV([180.0] * 3, [60] * 3)
+    mu = P.Image(att, np.zeros(att.nvox, np.float32))
+    n, seed, L, D = 300_000, 3, 300.0, 700.0
+    gen = P.sensitivity_lors(att, n, seed, L, D)
+    img, st = P.make_sensitivity_image(mu, n, L, D, seed=seed, return_stats=True)
+    ref, ost = O.sensitivity(att.full_size, att.n, mu.data, gen, threads=16)
+    assert st["n_steps"] == ost["n_steps"] and st["n_nz"] == ost["n_nz"] and st["n_lors"] == n
+    e = rel_l2(img.data, ref)
+    _say(f"sensitivity mu=0: relL2 {e:.2e}")
+    assert e <= 1e-5
+
+
+def test_sensitivity_irregular_lors(gpu):
+    """Given LORs mixing ordinary ones with face-grazing, axis-parallel,
+    degenerate and missing ones: image and counters (skips, misses) match the oracle."""
+    att = P.FOV([30.0] * 3, [6] * 3)
+    mu = P.Image(att, np.random.default_rng(4).uniform(0.0, 0.02, att.nvox).astype(np.float32))
+    rng = np.random.default_rng(12)
+    normal = random_lors(rng, 5000, radius=60.0, zhalf=20.0)
+    odd = P.make_lors(
+        [[-100, 5, 0], [5, 5, 5], [-100, 0, 0], [15.0, -100, 1.0], [-100, -100, -100], [-100, 2, 3],
+         [15.0, 0.0, 0.0], [-15.0, -100.0, -15.0], [0.0, 0.0, -100.0], [-100, 50, 0]],
+        [[100, 5, 0], [5, 5, 5], [100, 0, 0], [15.0, 100, 1.0], [100, 100, 100], [100, -2, -3],
+         [100.0, 0.0, 0.0], [-15.0, 100.0, -15.0], [0.0, 0.0, 100.0], [100, 50, 0]])
+    lors = np.concatenate([normal[:2000], odd, normal[2000:]])
+    img, st = P.make_sensitivity_image(mu, len(lors), lors=lors, return_stats=True)
+    ref, ost = O.sensitivity(att.full_size, att.n, mu.data, lors, threads=16)
+    _say(f"irregular sensitivity: gpu {st} oracle {ost}")
+    for k in ("n_steps", "n_nz", "n_skipped", "n_missed", "n_runaway"):
+        assert st[k] == ost[k], k
+    assert rel_l2(img.data, ref) <= 1e-5
+
+
+def test_sensitivity_batches_agree_with_one_lane_path(gpu):
+    """More LORs than one batch of the tiled path: it agrees with the
+    one-lane kernel (selected by passing the output FOV explicitly)."""
+    att = P.FOV([180.0] * 3, [30] * 3)
+    rho = P.synth_density(att)
+    mu = P.density_to_attenuation(rho, 0.096)
+    n, seed, L, D = (1 << 24) + 4099, 21, 300.0, 700.0
+    a, sa = P.make_sensitivity_image(mu, n, L, D, seed=seed, return_stats=True)
+    b, sb = P.make_sensitivity_image(mu, n, L, D, output_fov=att, seed=seed, return_stats=True)
+    e = rel_l2(a.data, b.data)
+    _say(f"sensitivity {n} LORs: tiled vs one-lane relL2 {e:.2e}; steps {sa['n_steps']} {sb['n_steps']}")
+    assert sa["n_steps"] == sb["n_steps"] and sa["n_nz"] == sb["n_nz"]
+    assert e <= 1e-6
+
+
 def test_walk_stats_match_oracle(gpu):
     full, n = [180.0] * 3, [120] * 3
     lors = P.synth_phantom_lors(8, 200_000, 200.0)
