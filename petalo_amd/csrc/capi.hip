@@ -331,10 +331,10 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     DevBuf<uint32_t> nr(count);
     const Lor* L = c->lors.p + lo;
     if (c->tof_on)
-      k_bin_runs<true, false><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
+      k_bin_runs<true, false><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
                                                                   nullptr, nullptr, nullptr, 0u);
     else
-      k_bin_runs<false, false><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
+      k_bin_runs<false, false><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
                                                                    nullptr, nullptr, nullptr, 0u);
     HIP_CHECK(hipGetLastError());
     std::vector<uint32_t> h(count);
@@ -367,10 +367,10 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     DevBuf<uint64_t> keys(total), keys2(total);
     DevBuf<uint32_t> perm(total), perm2(total);
     if (c->tof_on)
-      k_bin_runs<true, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
+      k_bin_runs<true, true><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
                                                                  c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift);
     else
-      k_bin_runs<false, true><<<grid_for(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
+      k_bin_runs<false, true><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
                                                                   c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift);
     HIP_CHECK(hipGetLastError());
     k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
@@ -962,7 +962,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       HIP_CHECK(hipMemcpyAsync(lb.p, given + first, (size_t)b * sizeof(Lor), hipMemcpyDeviceToDevice, s));
     else
       k_sens_lors<<<grid_for(b), kBlock, 0, s>>>(b, (uint32_t)first, seed, f, L, R, lb.p);
-    k_bin_runs<false, false><<<grid_for(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr,
+    k_bin_runs<false, false><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr,
                                                              nullptr, shift, cnt);
     HIP_CHECK(hipMemsetAsync(n_irr.p, 0, 4, s));
     k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts.p, b, lb.p, irr.p, n_irr.p);
@@ -980,7 +980,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       DevBuf<Run> unsorted(total);
       DevBuf<uint64_t> keys(total), keys2(total);
       DevBuf<uint32_t> perm(total), perm2(total), pos;
-      k_bin_runs<false, true><<<grid_for(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, unsorted.p,
+      k_bin_runs<false, true><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, unsorted.p,
                                                               keys.p, shift);
       k_iota<<<grid64(total), kBlock, 0, s>>>(perm.p, total);
       HIP_CHECK(hipGetLastError());

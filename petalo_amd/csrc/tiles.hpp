@@ -60,33 +60,35 @@ __host__ __device__ __forceinline__ uint32_t run_len(uint32_t code) { return cod
 __host__ __device__ __forceinline__ uint32_t run_flips(uint32_t code) { return (code >> 12) & 7u; }
 __host__ __device__ __forceinline__ uint32_t run_local(uint32_t code) { return code >> 15; }
 
-// Tile-local voxel coordinates (original axes) and tile id, tracked
-// incrementally by the binning walk: no division per step.
+// Position of the binning walk relative to the tiles, tracked per step
+// without division: per axis the voxel steps left in the current tile in the
+// walk's direction (cx, cy, cz), and the tile id.
 struct Coord {
-  int32_t lx, ly, lz;  // position inside the tile
+  int32_t cx, cy, cz;     // steps along the axis until the walk leaves the tile
   int32_t tile;
-  int32_t sx, sy, sz;  // +1 or -1 per axis
+  int32_t sx, sy, sz;     // +1 or -1 per axis (walk direction, original axes)
 };
 
+// Tile-local LDS index of the current voxel.
 __host__ __device__ __forceinline__ uint32_t local_of(const TileGeom& g, const Coord& q) {
-  return (uint32_t)(q.lx + g.px * (q.ly + g.ty * q.lz));
+  const int32_t lx = q.sx > 0 ? g.tx - q.cx : q.cx - 1;
+  const int32_t ly = q.sy > 0 ? g.ty - q.cy : q.cy - 1;
+  const int32_t lz = q.sz > 0 ? g.tz - q.cz : q.cz - 1;
+  return (uint32_t)(lx + g.px * (ly + g.ty * lz));
 }
 
-// Moves q one voxel along the crossed axis of s (exactly one of ax, ay, az).
+// Moves q one voxel along the crossed axis of s (exactly one of ax, ay, az),
+// with selects only: the walk's lanes take different axes.
 __host__ __device__ __forceinline__ void coord_step(Coord& q, const TileGeom& g, const Step& s) {
-  if (s.ax) {
-    q.lx += q.sx;
-    if (q.lx == g.tx) { q.lx = 0; q.tile += 1; }
-    else if (q.lx < 0) { q.lx = g.tx - 1; q.tile -= 1; }
-  } else if (s.ay) {
-    q.ly += q.sy;
-    if (q.ly == g.ty) { q.ly = 0; q.tile += g.ntx; }
-    else if (q.ly < 0) { q.ly = g.ty - 1; q.tile -= g.ntx; }
-  } else {
-    q.lz += q.sz;
-    if (q.lz == g.tz) { q.lz = 0; q.tile += g.ntx * g.nty; }
-    else if (q.lz < 0) { q.lz = g.tz - 1; q.tile -= g.ntx * g.nty; }
-  }
+  const int32_t left = (s.ax ? q.cx : (s.ay ? q.cy : q.cz)) - 1;
+  const bool cross = left == 0;
+  const int32_t reload = s.ax ? g.tx : (s.ay ? g.ty : g.tz);
+  const int32_t next = cross ? reload : left;
+  q.cx = s.ax ? next : q.cx;
+  q.cy = s.ay ? next : q.cy;
+  q.cz = s.az ? next : q.cz;
+  const int32_t dt = s.ax ? q.sx : (s.ay ? q.sy * g.ntx : q.sz * g.ntx * g.nty);
+  q.tile += cross ? dt : 0;
 }
 
 // Walk #1 with TOF window fast-forward, as in k_mlem_project.  Calls
@@ -106,13 +108,13 @@ __host__ __device__ __forceinline__ void walk_pushes(const Lor& l, const DevFov&
     const int32_t rem = w.index - z * nxy;
     const int32_t y = rem / f.nx;
     const int32_t x = rem - y * f.nx;
-    q.lx = x % g.tx;
-    q.ly = y % g.ty;
-    q.lz = z % g.tz;
-    q.tile = x / g.tx + g.ntx * (y / g.ty + g.nty * (z / g.tz));
     q.sx = c.d_x < 0 ? -1 : 1;
     q.sy = c.d_y < 0 ? -1 : 1;
     q.sz = c.d_z < 0 ? -1 : 1;
+    q.cx = q.sx > 0 ? g.tx - x % g.tx : x % g.tx + 1;
+    q.cy = q.sy > 0 ? g.ty - y % g.ty : y % g.ty + 1;
+    q.cz = q.sz > 0 ? g.tz - z % g.tz : z % g.tz + 1;
+    q.tile = x / g.tx + g.ntx * (y / g.ty + g.nty * (z / g.tz));
   }
   for (;;) {
     if (TOF && (w.here - c.tof_peak) >= t.cutoff) break;
@@ -181,15 +183,23 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
 // cache-resident while its runs in a tile are replayed), then longest run
 // first (bits 0-11; lanes of a wavefront get runs of similar length).  With
 // lor_shift >= 32 the key is (tile << 12) | length field only.
+// Each lane bins kBinLors LORs, 256 apart, one after the other: walks differ
+// in length, and a lane's total over several LORs varies less than one
+// walk's, so the lanes of a wavefront finish closer together.
+constexpr uint32_t kBinLors = 8;
+inline unsigned bin_grid(size_t n) { return (unsigned)((n + 256 * kBinLors - 1) / (256 * kBinLors)); }
+
 template <bool TOF, bool FILL>
 __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
                                                   TileGeom g, uint32_t* __restrict__ n_runs,
                                                   const uint32_t* __restrict__ offset, Run* __restrict__ runs,
                                                   uint64_t* __restrict__ keys, uint32_t lor_shift,
                                                   unsigned long long* __restrict__ counters = nullptr) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t steps = 0, nz = 0, missed = 0;
-  if (i < n) {
+  const uint32_t first = blockIdx.x * blockDim.x * kBinLors + threadIdx.x;
+  for (uint32_t u = 0; u < kBinLors; ++u) {
+    const uint32_t i = first + u * blockDim.x;
+    if (i >= n) break;
     const Lor l = load_lor(lors, i);
     const uint32_t base = FILL ? offset[i] : 0;
     uint32_t k = 0;
@@ -211,7 +221,7 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
       WalkC c;
       WalkS w;
       const bool hit = walk_setup(l, f, c, w);
-      missed = hit ? 0u : 1u;
+      missed += hit ? 0u : 1u;
       n_runs[i] = (hit && !bounded(w)) ? 0xffffffffu : count;
     }
   }
