@@ -270,6 +270,47 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
   }
 }
 
+// Forward replay of a run against the LDS tile: p = sum of w * x over the
+// run's pushed steps in walk order.  Each step's LDS read is consumed one
+// step later, so its latency overlaps the next step's arithmetic; every step
+// of a run lies inside the tile, so the read is always in bounds, and a step
+// that is not pushed adds nothing (the select keeps x out of it, as the
+// reference never reads it).
+template <bool TOF>
+__device__ __forceinline__ float replay_project(const Run& ru, const DevTof& t, const TileGeom& g,
+                                                const float* __restrict__ timg) {
+  const int32_t dlx = 1, dly = g.px, dlz = g.px * g.ty;
+  WalkS w;
+  w.nb_x = ru.nb_x;
+  w.nb_y = ru.nb_y;
+  w.nb_z = ru.nb_z;
+  w.here = ru.here;
+  WalkC c;
+  c.vl_x = ru.vl_x;
+  c.vl_y = ru.vl_y;
+  c.vl_z = ru.vl_z;
+  c.tof_peak = ru.tof_peak;
+  const uint32_t flips = run_flips(ru.code);
+  c.d_x = (flips & 1u) ? -dlx : dlx;
+  c.d_y = (flips & 2u) ? -dly : dly;
+  c.d_z = (flips & 4u) ? -dlz : dlz;
+  w.index = (int32_t)run_local(ru.code);
+  const uint32_t m = run_len(ru.code);
+  float p = 0.0f, w_prev = 0.0f, x_prev = 0.0f;
+  for (uint32_t k = 0; k < m; ++k) {
+    const Step s = next_boundary(w);
+    float wt = s.bnd - w.here;  // in-window: see replay_run
+    if (TOF) wt *= tof_factor_in_window(t, w.here, c.tof_peak);
+    const float x = timg[w.index];
+    p += w_prev > 0.0f ? w_prev * x_prev : 0.0f;
+    w_prev = wt;
+    x_prev = x;
+    move(w, c, s);
+  }
+  p += w_prev > 0.0f ? w_prev * x_prev : 0.0f;
+  return p;
+}
+
 // Forward, tiled: one workgroup per work item copies its tile of the image
 // into LDS and replays its runs, writing each run's partial projection
 // (sum of w*x over the run's pushed steps, in walk order) at the run's
@@ -305,9 +346,7 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs,
     const int32_t jn = j + stride;
     const int32_t jc = jn < last ? jn : last;  // in bounds; unused past the end
     const Run nxt = runs[perm ? perm[jc] : jc];
-    float p = 0.0f;
-    replay_run<TOF>(cur, 1.0f, t, g, [&](uint32_t li, float wt) { p += wt * timg[li]; });
-    partial[j] = p;
+    partial[j] = replay_project<TOF>(cur, t, g, timg);
     if (jn > last) break;
     cur = nxt;
     j = jn;
