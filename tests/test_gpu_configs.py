@@ -1,0 +1,99 @@
+"""Parity at the shapes of the other benchmark configurations, with LOR counts
+reduced so that the CPU oracle finishes in seconds.
+
+* precision study: 200^3 x 1 mm, TOF (150 ps FWHM data, sigma 31.85 ps,
+  cutoff 3), 20 iterations; relative L2 and max error of every iteration
+  against the binary32 and the binary64 oracle are printed.  Against the
+  binary64 oracle the difference is that of the binary32 walk itself, which
+  the reference shares: about 15% of the 200^3 rows differ from the exact
+  (binary64) walk in at least one corner voxel, and with 2e5 LORs (0.025 per
+  voxel) such a voxel is zero in one image and not in the other;
+* attenuation: density map -> mu on 256 x 256 x 128 x 1.2 mm, sensitivity
+  image with attenuation (2e6 LORs), then TOF-OSEM with 16 subsets over 4
+  iterations on the configuration's full 2e7 LORs, using (the normalised
+  inverse of) that sensitivity image.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+import petalo_amd as P
+from helpers import max_rel, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_precision_study_200(gpu):
+    full, n = [200.0] * 3, [200] * 3
+    tof = (31.85, 3.0)
+    lors = P.synth_phantom_lors(31, 200_000, 150.0)
+    fov = P.FOV(full, n)
+    it = P.mlem(fov, lors, tof=P.Tof(*tof))
+    x32 = np.ones(fov.nvox, np.float32)
+    x64 = np.ones(fov.nvox, np.float64)
+    worst32 = 0.0
+    for k in range(20):
+        g, _ = next(it)
+        O.mlem_update(full, n, lors, x32, tof=tof, threads=16)
+        O.mlem_update(full, n, lors, x64, tof=tof, threads=16, f64=True)
+        e32, m32 = rel_l2(g.data, x32), max_rel(g.data, x32)
+        e64, m64 = rel_l2(g.data, x64), max_rel(g.data, x64)
+        o32 = rel_l2(x32, x64)
+        _say(f"precision 200^3 iter {k + 1:2d}: GPU vs f32 oracle relL2 {e32:.2e} max {m32:.2e}; "
+             f"vs f64 oracle relL2 {e64:.2e} max {m64:.2e} (f32 oracle vs f64 oracle {o32:.2e})")
+        worst32 = max(worst32, e32)
+    it.close()
+    assert worst32 <= 1e-4
+
+
+def test_attenuation_sensitivity_and_tof_osem_256(gpu):
+    full, n = [307.2, 307.2, 153.6], [256, 256, 128]
+    fov = P.FOV(full, n)
+    rho = P.synth_density(fov)
+    mu = P.density_to_attenuation(rho, 0.096)
+    n_sens, seed, L, D = 2_000_000, 77, 300.0, 700.0
+    gen = P.sensitivity_lors(fov, n_sens, seed, L, D)
+    sens, st = P.make_sensitivity_image(mu, n_sens, L, D, seed=seed, return_stats=True)
+    ref, ost = O.sensitivity(full, n, mu.data, gen, threads=16)
+    ref64, _ = O.sensitivity(full, n, mu.data, gen, threads=16, f64=True)
+    assert st["n_steps"] == ost["n_steps"] and st["n_nz"] == ost["n_nz"]
+    e, e64 = rel_l2(sens.data, ref), rel_l2(sens.data, ref64)
+    _say(f"attenuated sensitivity 256x256x128: relL2 vs f32 oracle {e:.2e}, vs f64 oracle {e64:.2e}, "
+         f"max {max_rel(sens.data, ref64):.2e}")
+    assert e <= 1e-5
+
+    # The reference multiplies each update by the sensitivity image it is
+    # given (mlem.rs:45-52); it is passed subsets * median(s) / s, the
+    # normalised inverse of the attenuation-corrected image, so that the
+    # image keeps its scale over the 64 updates (with s itself every update
+    # scales the image by ~1e-5 and it underflows to 0 / NaN).
+    tof = (42.47, 3.0)
+    lors = P.synth_phantom_lors(41, 20_000_000, 200.0)  # 1.25e6 LORs per subset
+    subsets = 16
+    raw = sens.data
+    s = np.where(raw > 0, subsets * np.median(raw[raw > 0]) / np.where(raw > 0, raw, 1), 0).astype(np.float32)
+    it = P.mlem(fov, lors, tof=P.Tof(*tof), sensitivity=P.Image(fov, s), n_subsets=subsets)
+    x = np.ones(fov.nvox, np.float32)
+    size = len(lors) // subsets
+    worst = 0.0
+    for k in range(4 * subsets):
+        g, osem = next(it)
+        sub = k % subsets
+        O.mlem_update(full, n, lors[sub * size:(sub + 1) * size], x, sens=s, tof=tof, threads=16)
+        # voxels no LOR of the subsets so far has reached are 0 and stay 0;
+        # a LOR whose projection is 0 makes its voxels 0 * inf = NaN, in the
+        # reference as here: non-finite voxels must coincide
+        fin = np.isfinite(x)
+        assert np.array_equal(fin, np.isfinite(g.data)), f"update {k + 1}: non-finite voxels differ"
+        assert np.array_equal(np.isnan(x), np.isnan(g.data))
+        e = rel_l2(g.data[fin], x[fin])
+        worst = max(worst, e)
+        if sub == subsets - 1:
+            _say(f"TOF-OSEM 16x4 iteration {osem.iteration}: relL2 {e:.2e} max {max_rel(g.data[fin], x[fin]):.2e}"
+                 f" ({fin.size - fin.sum()} non-finite voxels, {(x[fin] > 0).sum()} positive)")
+    it.close()
+    assert np.isfinite(worst) and worst <= 1e-4
