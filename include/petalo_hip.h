@@ -136,6 +136,9 @@ int petalo_mlem_last_skipped(petalo_mlem_t* ctx, uint64_t* skipped);
 /* Per-kernel device time of the last update in ms, measured with HIP events
  * on the context's stream when timing is on. */
 int petalo_mlem_set_timing(petalo_mlem_t* ctx, int on);
+/* Without timing, each subset update is captured once as a HIP graph and
+ * replayed (default on); off launches the kernels one by one. */
+int petalo_mlem_set_graphs(petalo_mlem_t* ctx, int on);
 int petalo_mlem_last_timing(petalo_mlem_t* ctx, double* project_ms, double* update_ms);
 /* Phase split of the last update: forward projection, backprojection
  * scatter (0 for the atomic scheme, which does both in one kernel) and the
@@ -159,6 +162,30 @@ int petalo_mlem_last_kernel_timing(petalo_mlem_t* ctx, double* ms, size_t n);
 /* Scheme in use and, for the tiled scheme, the number of tile runs. */
 int petalo_mlem_info(petalo_mlem_t* ctx, int32_t* scheme, uint64_t* n_runs_total);
 int petalo_mlem_destroy(petalo_mlem_t* ctx);
+
+/* ---- Batched reconstructions -------------------------------------------
+ * n independent MLEM/OSEM reconstructions (each as petalo_mlem_create: its
+ * own FOV, TOF, LORs, sensitivity image), all with n_subsets subsets, advanced
+ * together: each petalo_batch_next performs one subset update of every
+ * reconstruction with one forward and one backprojection launch for all of
+ * them (work items of all their image tiles, largest first), replayed as a
+ * HIP graph per subset.  tofs_or_null: n entries or NULL (no TOF);
+ * sensitivities_or_null: n pointers (entries may be NULL) or NULL.  Tiled
+ * scheme only.  No counterpart in the reference, whose mlem runs one
+ * reconstruction per call (src/mlem.rs:8-31). */
+typedef struct petalo_batch petalo_batch_t;
+int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs, const petalo_tof_t* tofs_or_null,
+                        const petalo_lor_t* const* lors, const size_t* n_lors,
+                        const float* const* sensitivities_or_null, size_t n_subsets,
+                        const petalo_mlem_options_t* options);
+/* One subset update of every reconstruction; id_out: the Osem before advance. */
+int petalo_batch_next(petalo_batch_t* batch, petalo_osem_t* id_out_or_null);
+int petalo_batch_get_image(petalo_batch_t* batch, size_t r, float* image_out);
+/* timing: HIP events around each petalo_batch_next (petalo_batch_last_ms);
+ * use_graphs: replay captured graphs (default 1) or launch directly. */
+int petalo_batch_set_options(petalo_batch_t* batch, int timing, int use_graphs);
+int petalo_batch_last_ms(petalo_batch_t* batch, double* ms);
+int petalo_batch_destroy(petalo_batch_t* batch);
 
 /* ---- Sensitivity image (continuous.rs:3-47, main.rs:77) ---------------
  * Backprojects n_lors LORs through the attenuation image mu [1/mm] on

@@ -26,7 +26,7 @@ __all__ = [
     "make_sensitivity_image", "sensitivity_lors", "system_matrix_rows", "density_to_attenuation",
     "synth_phantom_lors", "synth_density", "philox4x32_10", "read_config", "parse_config", "parse_quantity",
     "write_image3d", "read_image3d", "device_count", "make_lors",
-    "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE",
+    "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE", "Batch",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -107,6 +107,13 @@ _SIGS = {
     "petalo_mlem_last_kernel_timing": (C.c_int, [_P, _P, C.c_size_t]),
     "petalo_mlem_info": (C.c_int, [_P, _P, _P]),
     "petalo_mlem_destroy": (C.c_int, [_P]),
+    "petalo_mlem_set_graphs": (C.c_int, [_P, C.c_int]),
+    "petalo_batch_create": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P, _P, C.c_size_t, _P]),
+    "petalo_batch_next": (C.c_int, [_P, _P]),
+    "petalo_batch_get_image": (C.c_int, [_P, C.c_size_t, _P]),
+    "petalo_batch_set_options": (C.c_int, [_P, C.c_int, C.c_int]),
+    "petalo_batch_last_ms": (C.c_int, [_P, _P]),
+    "petalo_batch_destroy": (C.c_int, [_P]),
     "petalo_make_sensitivity_image": (C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_uint64, C.c_float,
                                                 C.c_float, _P, _P]),
     "petalo_sensitivity_lors": (C.c_int, [_P, C.c_size_t, C.c_uint64, C.c_float, C.c_float, _P]),
@@ -321,6 +328,10 @@ class Mlem:
     def set_timing(self, on=True):
         _check(lib().petalo_mlem_set_timing(self._ctx, 1 if on else 0))
 
+    def set_graphs(self, on=True):
+        """Replay each subset update as a captured HIP graph (default) or launch directly."""
+        _check(lib().petalo_mlem_set_graphs(self._ctx, 1 if on else 0))
+
     def last_timing(self):
         a, b = C.c_double(0), C.c_double(0)
         _check(lib().petalo_mlem_last_timing(self._ctx, C.byref(a), C.byref(b)))
@@ -346,6 +357,76 @@ class Mlem:
         if self._ctx:
             lib().petalo_mlem_destroy(self._ctx)
             self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent reconstructions advanced together, one subset update of
+    each per step(), with one forward and one backprojection launch for all."""
+
+    def __init__(self, fovs, lors_list, tofs=None, sensitivities=None, n_subsets=1, sort_lors=True, device=0,
+                 tile=None):
+        n = len(fovs)
+        if len(lors_list) != n:
+            raise PetaloError(1, "one LOR array per FOV")
+        self.fovs = list(fovs)
+        self._lors = [np.ascontiguousarray(l, LOR_DTYPE) for l in lors_list]
+        cfov = (_Fov * n)(*[f._c() for f in fovs])
+        ctof = None
+        if tofs is not None:
+            ctof = (_Tof * n)()
+            for k, t in enumerate(tofs):
+                if t is not None:
+                    ctof[k] = t._c()
+        ptrs = (C.c_void_p * n)(*[l.ctypes.data if len(l) else None for l in self._lors])
+        counts = (C.c_size_t * n)(*[len(l) for l in self._lors])
+        csens = None
+        self._sens = []
+        if sensitivities is not None:
+            csens = (C.c_void_p * n)()
+            for k, sv in enumerate(sensitivities):
+                if sv is None:
+                    continue
+                a = np.ascontiguousarray(sv.data if isinstance(sv, Image) else sv, np.float32).reshape(-1)
+                if a.size != fovs[k].nvox:
+                    raise PetaloError(6, "sensitivity image size does not match the FOV")
+                self._sens.append(a)
+                csens[k] = a.ctypes.data
+        opt = _Options(SCHEME_TILE, 1 if sort_lors else 0, int(device), (C.c_int32 * 3)(*(tile or (0, 0, 0))))
+        self._b = C.c_void_p()
+        _check(lib().petalo_batch_create(C.byref(self._b), n, cfov, ctof, ptrs, counts, csens, int(n_subsets),
+                                         C.byref(opt)))
+
+    def __len__(self):
+        return len(self.fovs)
+
+    def step(self):
+        osem = _Osem()
+        _check(lib().petalo_batch_next(self._b, C.byref(osem)))
+        return Osem(osem.n_subsets, osem.iteration, osem.subset)
+
+    def image(self, r):
+        out = np.empty(self.fovs[r].nvox, np.float32)
+        _check(lib().petalo_batch_get_image(self._b, int(r), _ptr(out)))
+        return Image(self.fovs[r], out)
+
+    def set_options(self, timing=False, graphs=True):
+        _check(lib().petalo_batch_set_options(self._b, 1 if timing else 0, 1 if graphs else 0))
+
+    def last_ms(self):
+        v = C.c_double(0)
+        _check(lib().petalo_batch_last_ms(self._b, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._b:
+            lib().petalo_batch_destroy(self._b)
+            self._b = C.c_void_p()
 
     def __del__(self):
         try:

@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <stdexcept>
@@ -247,6 +249,11 @@ struct petalo_mlem {
   DevBuf<float> rbuf;                // 1/(p*ac) per LOR of the current subset
   DevBuf<float> ac;                  // additive corrections, in context LOR order
   DevBuf<double> bp64;               // binary64 backprojection
+  std::vector<std::vector<uint64_t>> item_steps;  // per subset: replayed steps of each work item
+  // execution
+  bool own_stream = true;            // false inside a batch (the batch's stream)
+  bool use_graphs = true;            // replay each subset update as a captured HIP graph
+  std::vector<hipGraphExec_t> graphs;  // per subset, captured on first use
 };
 
 namespace {
@@ -423,7 +430,11 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
                    [](const std::pair<uint64_t, int4>& u, const std::pair<uint64_t, int4>& v) { return u.first > v.first; });
   std::vector<int4> items;
   items.reserve(sized.size());
-  for (auto& it : sized) items.push_back(it.second);
+  c->item_steps[sub].clear();
+  for (auto& it : sized) {
+    items.push_back(it.second);
+    c->item_steps[sub].push_back(it.first);
+  }
   c->work[sub].alloc(std::max<size_t>(items.size(), 1));
   if (!items.empty())
     HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
@@ -432,12 +443,31 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   c->n_runs[sub] = total;
 }
 
-void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
+// 1/(p*ac) per LOR from the partials of the forward kernel, then the
+// one-lane path for the subset's unbounded walks.
+void launch_reduce(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   if (count == 0) return;
   const Lor* L = c->lors.p + lo;
   hipStream_t st = c->stream;
-  const size_t nw = c->work[sub].n;
+  k_lor_reduce<false><<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
+                                                            c->run_pos[sub].p, c->partial.p, c->rbuf.p);
+  HIP_CHECK(hipGetLastError());
   const size_t n_irr = c->irregular[sub].n;
+  if (n_irr) {
+    if (c->tof_on)
+      k_fwd_irregular<true><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
+                                                                c->dtof, c->image.p, c->bp64.p, c->counters.p);
+    else
+      k_fwd_irregular<false><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
+                                                                 c->dtof, c->image.p, c->bp64.p, c->counters.p);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count, bool timing) {
+  if (count == 0) return;
+  hipStream_t st = c->stream;
+  const size_t nw = c->work[sub].n;
   if (nw) {
     const size_t lds = (size_t)c->tiles.tvox * sizeof(float);
     if (c->tof_on)
@@ -448,20 +478,9 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
                                                          c->dtof, c->tiles, c->partial.p);
     HIP_CHECK(hipGetLastError());
   }
-  if (c->timing) HIP_CHECK(hipEventRecord(c->ev[4], st));
-  k_lor_reduce<false><<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
-                                                     c->run_pos[sub].p, c->partial.p, c->rbuf.p);
-  HIP_CHECK(hipGetLastError());
-  if (n_irr) {
-    if (c->tof_on)
-      k_fwd_irregular<true><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
-                                                                c->dtof, c->image.p, c->bp64.p, c->counters.p);
-    else
-      k_fwd_irregular<false><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
-                                                                 c->dtof, c->image.p, c->bp64.p, c->counters.p);
-    HIP_CHECK(hipGetLastError());
-  }
-  if (c->timing) HIP_CHECK(hipEventRecord(c->ev[1], st));
+  if (timing) HIP_CHECK(hipEventRecord(c->ev[4], st));
+  launch_reduce(c, sub, lo, count);
+  if (timing) HIP_CHECK(hipEventRecord(c->ev[1], st));
   if (nw) {
     const size_t lds = (size_t)c->tiles.tvox * sizeof(double);
     if (c->tof_on)
@@ -491,6 +510,77 @@ void zero_counters(unsigned long long* p, hipStream_t s) {
   HIP_CHECK(hipGetLastError());
 }
 
+// The tile kernels take their LDS tile as dynamic shared memory; allow up to
+// the largest tile on the current device, for every context (a lower limit
+// set for one context would break another with larger tiles).
+void ensure_lds_attributes() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::vector<bool> done;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((size_t)dev < done.size() && done[dev]) return;
+  const int lds = kMaxTileVoxels * (int)sizeof(double);
+  const void* fns[] = {(const void*)k_scatter<true>,        (const void*)k_scatter<false>,
+                       (const void*)k_fwd_tile<true>,       (const void*)k_fwd_tile<false>,
+                       (const void*)k_scatter_batch<true>,  (const void*)k_scatter_batch<false>,
+                       (const void*)k_fwd_tile_batch<true>, (const void*)k_fwd_tile_batch<false>};
+  for (const void* f : fns) HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  if (done.size() <= (size_t)dev) done.resize(dev + 1, false);
+  done[dev] = true;
+}
+
+// Everything one subset update launches, on the context's stream.  With
+// timing, HIP events bracket the phases (not while a graph is captured).
+void enqueue_update(petalo_mlem* c, bool timing) {
+  const size_t nv = nvox(c->fov);
+  const size_t sub = c->osem.subset - 1;
+  const size_t lo = sub * c->set_size;  // Osem::subset (mlem.rs:90-95)
+  zero_counters(c->counters.p, c->stream);
+  if (timing) HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
+  if (c->scheme == PETALO_SCHEME_TILE) {
+    launch_tiled(c, sub, lo, c->set_size, timing);
+    if (timing) HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
+    k_update_f64<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp64.p, c->sens.p, (uint32_t)nv);
+  } else {
+    launch_project(c, lo, c->set_size);
+    if (timing) {
+      HIP_CHECK(hipEventRecord(c->ev[4], c->stream));
+      HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
+      HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
+    }
+    k_update<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp.p, c->sens.p, (uint32_t)nv);
+  }
+  HIP_CHECK(hipGetLastError());
+  if (timing) HIP_CHECK(hipEventRecord(c->ev[3], c->stream));
+}
+
+// The subset update as a HIP graph: captured from the stream on first use
+// (kernel nodes only), then launched whole.
+void launch_update_graph(petalo_mlem* c, hipStream_t st, std::vector<hipGraphExec_t>& graphs, size_t sub,
+                         const std::function<void()>& enqueue) {
+  if (graphs.size() <= sub) graphs.resize(sub + 1, nullptr);
+  if (!graphs[sub]) {
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(st, &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_CHECK(err);
+    graphs[sub] = e;
+  }
+  (void)c;
+  HIP_CHECK(hipGraphLaunch(graphs[sub], st));
+}
+
 }  // namespace
 
 extern "C" {
@@ -516,27 +606,31 @@ int petalo_device_synchronize(int device) {
   });
 }
 
-int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petalo_tof_t* tof,
-                       const petalo_lor_t* lors, size_t n_lors, const float* sensitivity, size_t n_subsets,
-                       const petalo_mlem_options_t* options) {
-  return guarded([&] {
-    if (!out) return fail(PETALO_ERR_ARG, "out is NULL");
-    *out = nullptr;
-    std::string why;
-    if (!fov_ok(fov, why)) return fail(PETALO_ERR_ARG, why);
-    if (n_subsets == 0) return fail(PETALO_ERR_ARG, "n_subsets must be at least 1");
-    if (n_lors && !lors) return fail(PETALO_ERR_ARG, "lors is NULL");
-    if (n_lors >= (1ull << 32)) return fail(PETALO_ERR_ARG, "more than 2^32-1 LORs");
-    if (tof && tof->enabled && !(tof->sigma_ps > 0.0f)) return fail(PETALO_ERR_ARG, "TOF sigma must be positive");
-    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}};
-    if (options) opt = *options;
-    if (opt.scheme != PETALO_SCHEME_AUTO && opt.scheme != PETALO_SCHEME_ATOMIC && opt.scheme != PETALO_SCHEME_TILE)
-      return fail(PETALO_ERR_ARG, "unsupported backprojection scheme");
-    for (int a = 0; a < 3; ++a)
-      if (opt.tile[a] < 0) return fail(PETALO_ERR_ARG, "negative tile dimension");
-    require_device();
-    HIP_CHECK(hipSetDevice(opt.device));
+}  // extern "C"
 
+namespace {
+
+// Argument checks of petalo_mlem_create; "" when the arguments are usable.
+std::string check_mlem_args(const petalo_fov_t* fov, const petalo_tof_t* tof, const petalo_lor_t* lors,
+                            size_t n_lors, size_t n_subsets, const petalo_mlem_options_t& opt) {
+  std::string why;
+  if (!fov_ok(fov, why)) return why;
+  if (n_subsets == 0) return "n_subsets must be at least 1";
+  if (n_lors && !lors) return "lors is NULL";
+  if (n_lors >= (1ull << 32)) return "more than 2^32-1 LORs";
+  if (tof && tof->enabled && !(tof->sigma_ps > 0.0f)) return "TOF sigma must be positive";
+  if (opt.scheme != PETALO_SCHEME_AUTO && opt.scheme != PETALO_SCHEME_ATOMIC && opt.scheme != PETALO_SCHEME_TILE)
+    return "unsupported backprojection scheme";
+  for (int a = 0; a < 3; ++a)
+    if (opt.tile[a] < 0) return "negative tile dimension";
+  return "";
+}
+
+// Builds a context on the current device; its stream is `shared` when given
+// (a batch's), else its own.  Throws on failure.
+petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, const petalo_lor_t* lors,
+                            size_t n_lors, const float* sensitivity, size_t n_subsets,
+                            const petalo_mlem_options_t& opt, hipStream_t shared) {
     auto* c = new petalo_mlem();
     try {
       c->device = opt.device;
@@ -549,7 +643,12 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
       c->set_size = n_lors / n_subsets;  // remainder LORs ignored (mlem.rs:91)
       c->scheme = opt.scheme == PETALO_SCHEME_ATOMIC ? PETALO_SCHEME_ATOMIC : PETALO_SCHEME_TILE;
       c->osem = petalo_osem_t{n_subsets, 1, 1};
-      HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      if (shared) {
+        c->stream = shared;
+        c->own_stream = false;
+      } else {
+        HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      }
       const size_t nv = nvox(*fov);
 
       // LORs: reorder inside each subset (subset membership is unchanged).
@@ -573,16 +672,9 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
       for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
       if (c->scheme == PETALO_SCHEME_TILE) {
         c->tiles = make_tiles(*fov, opt.tile);
-        const size_t lds = (size_t)c->tiles.tvox * sizeof(double);
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_fwd_tile<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(lds / 2)));
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_fwd_tile<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(lds / 2)));
+        ensure_lds_attributes();
         c->runs.resize(n_subsets);
+        c->item_steps.resize(n_subsets);
         c->work.resize(n_subsets);
         c->run_off.resize(n_subsets);
         c->run_pos.resize(n_subsets);
@@ -609,7 +701,26 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
       petalo_mlem_destroy(c);
       throw;
     }
-    *out = c;
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petalo_tof_t* tof,
+                       const petalo_lor_t* lors, size_t n_lors, const float* sensitivity, size_t n_subsets,
+                       const petalo_mlem_options_t* options) {
+  return guarded([&] {
+    if (!out) return fail(PETALO_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}};
+    if (options) opt = *options;
+    const std::string why = check_mlem_args(fov, tof, lors, n_lors, n_subsets, opt);
+    if (!why.empty()) return fail(PETALO_ERR_ARG, why);
+    require_device();
+    HIP_CHECK(hipSetDevice(opt.device));
+    *out = create_context(fov, tof, lors, n_lors, sensitivity, n_subsets, opt, nullptr);
     return PETALO_OK;
   });
 }
@@ -619,26 +730,12 @@ int petalo_mlem_next(petalo_mlem_t* c, float* image_out, petalo_osem_t* id_out) 
     if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
     HIP_CHECK(hipSetDevice(c->device));
     const size_t nv = nvox(c->fov);
-    const size_t lo = (c->osem.subset - 1) * c->set_size;  // Osem::subset (mlem.rs:90-95)
-    zero_counters(c->counters.p, c->stream);
-    if (c->timing) HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
-    if (c->scheme == PETALO_SCHEME_TILE) {
-      launch_tiled(c, c->osem.subset - 1, lo, c->set_size);
-      if (c->timing) HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
-      k_update_f64<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp64.p, c->sens.p, (uint32_t)nv);
+    if (c->timing || !c->use_graphs) {
+      enqueue_update(c, c->timing);
+      c->have_timing = c->timing;
     } else {
-      launch_project(c, lo, c->set_size);
-      if (c->timing) {
-        HIP_CHECK(hipEventRecord(c->ev[4], c->stream));
-        HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
-        HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
-      }
-      k_update<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp.p, c->sens.p, (uint32_t)nv);
-    }
-    HIP_CHECK(hipGetLastError());
-    if (c->timing) {
-      HIP_CHECK(hipEventRecord(c->ev[3], c->stream));
-      c->have_timing = true;
+      launch_update_graph(c, c->stream, c->graphs, c->osem.subset - 1, [&] { enqueue_update(c, false); });
+      c->have_timing = false;
     }
     if (id_out) *id_out = c->osem;
     // Osem::advance (mlem.rs:82-88)
@@ -720,6 +817,12 @@ int petalo_mlem_set_timing(petalo_mlem_t* c, int on) {
   return PETALO_OK;
 }
 
+int petalo_mlem_set_graphs(petalo_mlem_t* c, int on) {
+  if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
+  c->use_graphs = on != 0;
+  return PETALO_OK;
+}
+
 int petalo_mlem_last_timing(petalo_mlem_t* c, double* project_ms, double* update_ms) {
   return guarded([&] {
     if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
@@ -785,6 +888,9 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& g : c->graphs)
+    if (g) (void)hipGraphExecDestroy(g);
+  c->graphs.clear();
   c->lors.release();
   c->image.release();
   c->bp.release();
@@ -799,8 +905,209 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->ac.release();
   c->rbuf.release();
   c->bp64.release();
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return PETALO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched reconstructions
+}  // extern "C"
+
+struct petalo_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<petalo_mlem*> ctx;
+  size_t n_subsets = 1;
+  petalo_osem_t osem{};
+  std::vector<DevBuf<ReconDesc>> descs;   // per subset: one entry per reconstruction
+  std::vector<DevBuf<int4>> items[2];     // [TOF][subset]: (tile, first, end, reconstruction)
+  size_t lds_f32 = 0, lds_f64 = 0;        // largest tile over the reconstructions
+  bool use_graphs = true;
+  std::vector<hipGraphExec_t> graphs;     // per subset
+  bool timing = false, have_timing = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+void batch_enqueue(petalo_batch* b, size_t sub) {
+  hipStream_t st = b->stream;
+  for (petalo_mlem* c : b->ctx) zero_counters(c->counters.p, st);
+  const ReconDesc* rd = b->descs[sub].p;
+  if (b->items[1][sub].n)
+    k_fwd_tile_batch<true><<<(unsigned)b->items[1][sub].n, 1024, b->lds_f32, st>>>(rd, b->items[1][sub].p);
+  if (b->items[0][sub].n)
+    k_fwd_tile_batch<false><<<(unsigned)b->items[0][sub].n, 1024, b->lds_f32, st>>>(rd, b->items[0][sub].p);
+  HIP_CHECK(hipGetLastError());
+  for (petalo_mlem* c : b->ctx) launch_reduce(c, sub, sub * c->set_size, c->set_size);
+  if (b->items[1][sub].n)
+    k_scatter_batch<true><<<(unsigned)b->items[1][sub].n, 1024, b->lds_f64, st>>>(rd, b->items[1][sub].p);
+  if (b->items[0][sub].n)
+    k_scatter_batch<false><<<(unsigned)b->items[0][sub].n, 1024, b->lds_f64, st>>>(rd, b->items[0][sub].p);
+  HIP_CHECK(hipGetLastError());
+  for (petalo_mlem* c : b->ctx) {
+    const size_t nv = nvox(c->fov);
+    k_update_f64<<<grid_for(nv), kBlock, 0, st>>>(c->image.p, c->bp64.p, c->sens.p, (uint32_t)nv);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int petalo_batch_destroy(petalo_batch_t* b);
+
+int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs, const petalo_tof_t* tofs,
+                        const petalo_lor_t* const* lors, const size_t* n_lors, const float* const* sens,
+                        size_t n_subsets, const petalo_mlem_options_t* options) {
+  return guarded([&] {
+    if (!out) return fail(PETALO_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n == 0) return fail(PETALO_ERR_ARG, "a batch needs at least one reconstruction");
+    if (!fovs || !lors || !n_lors) return fail(PETALO_ERR_ARG, "NULL argument");
+    petalo_mlem_options_t opt{PETALO_SCHEME_TILE, 1, 0, {0, 0, 0}};
+    if (options) opt = *options;
+    if (opt.scheme == PETALO_SCHEME_ATOMIC) return fail(PETALO_ERR_ARG, "batches run the tiled scheme");
+    opt.scheme = PETALO_SCHEME_TILE;
+    for (size_t r = 0; r < n; ++r) {
+      const std::string why = check_mlem_args(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],
+                                              n_subsets, opt);
+      if (!why.empty()) return fail(PETALO_ERR_ARG, "reconstruction " + std::to_string(r) + ": " + why);
+    }
+    require_device();
+    HIP_CHECK(hipSetDevice(opt.device));
+    auto* b = new petalo_batch();
+    try {
+      b->device = opt.device;
+      b->n_subsets = n_subsets;
+      b->osem = petalo_osem_t{n_subsets, 1, 1};
+      HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+      for (auto& e : b->ev) HIP_CHECK(hipEventCreate(&e));
+      for (size_t r = 0; r < n; ++r)
+        b->ctx.push_back(create_context(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],
+                                        sens ? sens[r] : nullptr, n_subsets, opt, b->stream));
+      for (petalo_mlem* c : b->ctx) {
+        b->lds_f32 = std::max(b->lds_f32, (size_t)c->tiles.tvox * sizeof(float));
+        b->lds_f64 = std::max(b->lds_f64, (size_t)c->tiles.tvox * sizeof(double));
+      }
+      b->descs.resize(n_subsets);
+      for (int t = 0; t < 2; ++t) b->items[t].resize(n_subsets);
+      for (size_t sub = 0; sub < n_subsets; ++sub) {
+        std::vector<ReconDesc> d(n);
+        std::vector<std::pair<uint64_t, int4>> it[2];
+        for (size_t r = 0; r < n; ++r) {
+          petalo_mlem* c = b->ctx[r];
+          d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->dfov, c->dtof,
+                           c->tiles};
+          std::vector<int4> h(c->work[sub].n);
+          if (!h.empty())
+            HIP_CHECK(hipMemcpy(h.data(), c->work[sub].p, h.size() * sizeof(int4), hipMemcpyDeviceToHost));
+          for (size_t k = 0; k < h.size(); ++k) {
+            int4 w = h[k];
+            w.w = (int)r;
+            it[c->tof_on ? 1 : 0].push_back({c->item_steps[sub][k], w});
+          }
+        }
+        b->descs[sub].alloc(n);
+        HIP_CHECK(hipMemcpy(b->descs[sub].p, d.data(), n * sizeof(ReconDesc), hipMemcpyHostToDevice));
+        for (int t = 0; t < 2; ++t) {
+          // largest first over the whole batch
+          std::stable_sort(it[t].begin(), it[t].end(),
+                           [](const std::pair<uint64_t, int4>& u, const std::pair<uint64_t, int4>& v) {
+                             return u.first > v.first;
+                           });
+          std::vector<int4> w;
+          for (auto& x : it[t]) w.push_back(x.second);
+          b->items[t][sub].alloc(std::max<size_t>(w.size(), 1));
+          if (!w.empty())
+            HIP_CHECK(hipMemcpy(b->items[t][sub].p, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice));
+          b->items[t][sub].n = w.size();
+        }
+      }
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+    } catch (...) {
+      petalo_batch_destroy(b);
+      throw;
+    }
+    *out = b;
+    return PETALO_OK;
+  });
+}
+
+int petalo_batch_next(petalo_batch_t* b, petalo_osem_t* id_out) {
+  return guarded([&] {
+    if (!b) return fail(PETALO_ERR_ARG, "batch is NULL");
+    HIP_CHECK(hipSetDevice(b->device));
+    const size_t sub = b->osem.subset - 1;
+    if (b->timing) HIP_CHECK(hipEventRecord(b->ev[0], b->stream));
+    if (b->use_graphs)
+      launch_update_graph(nullptr, b->stream, b->graphs, sub, [&] { batch_enqueue(b, sub); });
+    else
+      batch_enqueue(b, sub);
+    if (b->timing) HIP_CHECK(hipEventRecord(b->ev[1], b->stream));
+    b->have_timing = b->timing;
+    if (id_out) *id_out = b->osem;
+    auto advance = [](petalo_osem_t& o) {  // Osem::advance (mlem.rs:82-88)
+      o.subset += 1;
+      if (o.subset > o.n_subsets) {
+        o.subset = 1;
+        o.iteration += 1;
+      }
+    };
+    advance(b->osem);
+    for (petalo_mlem* c : b->ctx) advance(c->osem);
+    return PETALO_OK;
+  });
+}
+
+int petalo_batch_get_image(petalo_batch_t* b, size_t r, float* out) {
+  return guarded([&] {
+    if (!b || !out) return fail(PETALO_ERR_ARG, "NULL argument");
+    if (r >= b->ctx.size()) return fail(PETALO_ERR_ARG, "reconstruction index out of range");
+    HIP_CHECK(hipSetDevice(b->device));
+    petalo_mlem* c = b->ctx[r];
+    HIP_CHECK(hipMemcpyAsync(out, c->image.p, nvox(c->fov) * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    return PETALO_OK;
+  });
+}
+
+int petalo_batch_set_options(petalo_batch_t* b, int timing, int use_graphs) {
+  if (!b) return fail(PETALO_ERR_ARG, "batch is NULL");
+  b->timing = timing != 0;
+  b->use_graphs = use_graphs != 0;
+  b->have_timing = false;
+  return PETALO_OK;
+}
+
+int petalo_batch_last_ms(petalo_batch_t* b, double* ms) {
+  return guarded([&] {
+    if (!b || !ms) return fail(PETALO_ERR_ARG, "NULL argument");
+    if (!b->have_timing) return fail(PETALO_ERR_ARG, "timing is off or no update has run");
+    HIP_CHECK(hipSetDevice(b->device));
+    HIP_CHECK(hipEventSynchronize(b->ev[1]));
+    float t = 0;
+    HIP_CHECK(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
+    *ms = t;
+    return PETALO_OK;
+  });
+}
+
+int petalo_batch_destroy(petalo_batch_t* b) {
+  if (!b) return PETALO_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& g : b->graphs)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (petalo_mlem* c : b->ctx) petalo_mlem_destroy(c);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  b->descs.clear();
+  for (auto& v : b->items) v.clear();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
   return PETALO_OK;
 }
 
@@ -947,10 +1254,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   while ((1 << tile_bits) < g.ntiles) ++tile_bits;
   const int end_bit = (int)tshift + tile_bits;
   const DevTof notof = no_tof();
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)((size_t)g.tvox * sizeof(double))));
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_fwd_tile<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)((size_t)g.tvox * sizeof(float))));
+  ensure_lds_attributes();
   const size_t bmax = std::min<size_t>(n_lors, kSensBatch);
   DevBuf<Lor> lb(bmax), irr(bmax);
   DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1);

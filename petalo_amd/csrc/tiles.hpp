@@ -320,12 +320,11 @@ __device__ __forceinline__ float replay_project(const Run& ru, const DevTof& t, 
 // current one, so the record's latency hides behind the replay (the replay
 // itself touches only LDS).
 template <bool TOF>
-__global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs, const int4* __restrict__ work,
-                                                   const float* __restrict__ img, DevFov f, DevTof t, TileGeom g,
-                                                   float* __restrict__ partial,
-                                                   const uint32_t* __restrict__ perm = nullptr) {
+__device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, const int4 wi,
+                                              const float* __restrict__ img, const DevFov& f, const DevTof& t,
+                                              const TileGeom& g, float* __restrict__ partial,
+                                              const uint32_t* __restrict__ perm) {
   extern __shared__ float timg[];
-  const int4 wi = work[blockIdx.x];
   const int32_t tl = wi.x;
   const int32_t tzi = tl / (g.ntx * g.nty);
   const int32_t tyi = (tl / g.ntx) % g.nty;
@@ -351,6 +350,36 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs,
     cur = nxt;
     j = jn;
   }
+}
+
+template <bool TOF>
+__global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs, const int4* __restrict__ work,
+                                                   const float* __restrict__ img, DevFov f, DevTof t, TileGeom g,
+                                                   float* __restrict__ partial,
+                                                   const uint32_t* __restrict__ perm = nullptr) {
+  fwd_tile_body<TOF>(runs, work[blockIdx.x], img, f, t, g, partial, perm);
+}
+
+// One reconstruction of a batch, as the batched tile kernels see it: the runs
+// and buffers of its current subset.
+struct ReconDesc {
+  const Run* runs;
+  const float* image;
+  float* partial;
+  const float* r;
+  double* bp;
+  DevFov f;
+  DevTof t;
+  TileGeom g;
+};
+
+// Batched forward: work item w = (tile, first run, end run, reconstruction).
+template <bool TOF>
+__global__ __launch_bounds__(1024) void k_fwd_tile_batch(const ReconDesc* __restrict__ rd,
+                                                         const int4* __restrict__ work) {
+  const int4 wi = work[blockIdx.x];
+  const ReconDesc& r = rd[wi.w];
+  fwd_tile_body<TOF>(r.runs, wi, r.image, r.f, r.t, r.g, r.partial, nullptr);
 }
 
 // r = 1/(p*ac) per LOR (MLEM; 1/exp(-p) for the sensitivity image) with p the
@@ -467,12 +496,11 @@ __global__ __launch_bounds__(256) void k_fwd_irregular(const Lor* __restrict__ l
 // r_in == nullptr backprojects the weights themselves (r = 1).  With perm,
 // sorted run j is runs[perm[j]] (runs left in LOR-major order).
 template <bool TOF>
-__global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, const int4* __restrict__ work,
-                                                  const float* __restrict__ r_in, DevFov f, DevTof t, TileGeom g,
-                                                  double* __restrict__ bp,
-                                                  const uint32_t* __restrict__ perm = nullptr) {
+__device__ __forceinline__ void scatter_body(const Run* __restrict__ runs, const int4 wi,
+                                             const float* __restrict__ r_in, const DevFov& f, const DevTof& t,
+                                             const TileGeom& g, double* __restrict__ bp,
+                                             const uint32_t* __restrict__ perm) {
   extern __shared__ double tile[];
-  const int4 wi = work[blockIdx.x];
   const int32_t tl = wi.x;
   for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) tile[k] = 0.0;
   __syncthreads();
@@ -513,6 +541,23 @@ __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, 
     const int32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
     if (lx < g.tx && x < f.nx && y < f.ny && z < f.nz) unsafeAtomicAdd(bp + x + f.nx * (y + f.ny * z), v);
   }
+}
+
+template <bool TOF>
+__global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, const int4* __restrict__ work,
+                                                  const float* __restrict__ r_in, DevFov f, DevTof t, TileGeom g,
+                                                  double* __restrict__ bp,
+                                                  const uint32_t* __restrict__ perm = nullptr) {
+  scatter_body<TOF>(runs, work[blockIdx.x], r_in, f, t, g, bp, perm);
+}
+
+// Batched backprojection: work item w = (tile, first run, end run, reconstruction).
+template <bool TOF>
+__global__ __launch_bounds__(1024) void k_scatter_batch(const ReconDesc* __restrict__ rd,
+                                                        const int4* __restrict__ work) {
+  const int4 wi = work[blockIdx.x];
+  const ReconDesc& r = rd[wi.w];
+  scatter_body<TOF>(r.runs, wi, r.r, r.f, r.t, r.g, r.bp, nullptr);
 }
 
 // MLEM update with the binary64 backprojection, which is reset:

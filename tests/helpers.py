@@ -54,3 +54,17 @@ def ulps(a, b):
     a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
     b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
     return np.abs(a - b)
+
+
+def rel_l2_finite(x, ref):
+    """Relative L2 over the voxels finite in both, after checking that the
+    non-finite voxels (NaN from 0 * inf when a LOR projects to 0, inf where a
+    projection of 0 meets a non-zero voxel, as in the reference) coincide."""
+    x = np.asarray(x)
+    ref = np.asarray(ref)
+    assert np.array_equal(np.isnan(x), np.isnan(ref)), "NaN voxels differ"
+    assert np.array_equal(np.isinf(x), np.isinf(ref)), "infinite voxels differ"
+    fin = np.isfinite(ref)
+    if not np.any(ref[fin]):
+        return float(np.abs(x[fin]).max(initial=0.0))
+    return rel_l2(x[fin], ref[fin])

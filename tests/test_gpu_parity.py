@@ -13,7 +13,7 @@ import pytest
 
 import oracle_ffi as O
 import petalo_amd as P
-from helpers import max_rel, proptest_lors, random_lors, rel_l2, ulps
+from helpers import max_rel, proptest_lors, random_lors, rel_l2, rel_l2_finite, ulps
 
 pytestmark = pytest.mark.gpu
 
@@ -338,3 +338,58 @@ def test_tile_shapes(gpu, tile):
         e = rel_l2(g[k], o[k])
         _say(f"tile {tile} iter {k + 1}: relL2 {e:.2e}")
         assert e <= 1e-5
+
+
+def test_graph_replays_match_direct_launches(gpu):
+    """Subset updates replayed as captured graphs (the default) give the
+    images of direct launches, replay after replay, with host reads between."""
+    full, n = [60.0, 60.0, 48.0], [20, 20, 16]
+    rng = np.random.default_rng(31)
+    lors = random_lors(rng, 40000, radius=120.0, zhalf=60.0, dt_sigma_ps=84.93)
+    fov = P.FOV(full, n)
+    tof = P.Tof(42.47, 3.0)
+    a = P.Mlem(fov, lors, tof=tof, n_subsets=3)
+    b = P.Mlem(fov, lors, tof=tof, n_subsets=3)
+    b.set_graphs(False)
+    o = _run_oracle(full, n, lors, 3, tof=(42.47, 3.0), subsets=3)
+    for k in range(9):
+        ga, ia = a.step()
+        gb, ib = b.step()
+        assert (ia.iteration, ia.subset) == (ib.iteration, ib.subset)
+        e, eo = rel_l2_finite(ga.data, gb.data), rel_l2_finite(ga.data, o[k])
+        _say(f"graph vs direct update {k + 1}: relL2 {e:.2e}; vs oracle {eo:.2e}; "
+             f"{int(np.sum(~np.isfinite(ga.data)))} non-finite voxels")
+        assert e <= 1e-6 and eo <= 1e-5
+    a.close()
+    b.close()
+
+
+def test_batch_matches_single_reconstructions(gpu):
+    """A batch of reconstructions of different shapes, TOF settings, LOR
+    counts (one empty) and sensitivity images gives each one's own images."""
+    rng = np.random.default_rng(32)
+    specs = [([48.0, 48.0, 40.0], [12, 12, 10], (42.47, 3.0), 20000),
+             ([60.0, 60.0, 60.0], [30, 30, 30], None, 30000),
+             ([40.0, 56.0, 32.0], [17, 23, 11], (31.85, 3.0), 15000),
+             ([30.0, 30.0, 30.0], [6, 6, 6], None, 0),
+             ([90.0, 90.0, 60.0], [45, 45, 30], (42.47, 3.0), 40000)]
+    fovs, lors, tofs, sens = [], [], [], []
+    for k, (full, n, tof, m) in enumerate(specs):
+        fovs.append(P.FOV(full, n))
+        lors.append(random_lors(rng, m, radius=150.0, zhalf=80.0, dt_sigma_ps=84.93 if tof else 0.0))
+        tofs.append(P.Tof(*tof) if tof else None)
+        nv = int(np.prod(n))
+        sens.append(rng.uniform(0.5, 2.0, nv).astype(np.float32) if k % 2 == 0 else None)
+    b = P.Batch(fovs, lors, tofs=tofs, sensitivities=sens, n_subsets=2)
+    singles = [P.Mlem(fovs[k], lors[k], tof=tofs[k], sensitivity=sens[k], n_subsets=2) for k in range(len(specs))]
+    for step in range(6):
+        osem = b.step()
+        assert (osem.iteration, osem.subset) == (step // 2 + 1, step % 2 + 1)
+        for k, s in enumerate(singles):
+            g, _ = s.step()
+            e = rel_l2_finite(b.image(k).data, g.data)
+            _say(f"batch step {step + 1} reconstruction {k}: relL2 vs single {e:.2e}")
+            assert e <= 1e-6
+    b.close()
+    for s in singles:
+        s.close()
