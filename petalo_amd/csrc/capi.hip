@@ -922,6 +922,10 @@ struct petalo_batch {
   petalo_osem_t osem{};
   std::vector<DevBuf<ReconDesc>> descs;   // per subset: one entry per reconstruction
   std::vector<DevBuf<int4>> items[2];     // [TOF][subset]: (tile, first, end, reconstruction)
+  std::vector<DevBuf<ReduceDesc>> reduce; // per subset
+  std::vector<unsigned> reduce_blocks;    // per subset
+  DevBuf<UpdateDesc> update;
+  unsigned update_blocks = 0;
   size_t lds_f32 = 0, lds_f64 = 0;        // largest tile over the reconstructions
   bool use_graphs = true;
   std::vector<hipGraphExec_t> graphs;     // per subset
@@ -931,25 +935,39 @@ struct petalo_batch {
 
 namespace {
 
+// Work items of a whole batch per subset update.
+constexpr uint64_t kBatchItems = 3072;
+
 void batch_enqueue(petalo_batch* b, size_t sub) {
   hipStream_t st = b->stream;
-  for (petalo_mlem* c : b->ctx) zero_counters(c->counters.p, st);
+  const uint32_t n = (uint32_t)b->ctx.size();
+  k_zero_counters_batch<<<1, 256, 0, st>>>(b->update.p, n, CNT_N);
   const ReconDesc* rd = b->descs[sub].p;
   if (b->items[1][sub].n)
     k_fwd_tile_batch<true><<<(unsigned)b->items[1][sub].n, 1024, b->lds_f32, st>>>(rd, b->items[1][sub].p);
   if (b->items[0][sub].n)
     k_fwd_tile_batch<false><<<(unsigned)b->items[0][sub].n, 1024, b->lds_f32, st>>>(rd, b->items[0][sub].p);
   HIP_CHECK(hipGetLastError());
-  for (petalo_mlem* c : b->ctx) launch_reduce(c, sub, sub * c->set_size, c->set_size);
+  if (b->reduce_blocks[sub]) k_lor_reduce_batch<<<b->reduce_blocks[sub], kBlock, 0, st>>>(b->reduce[sub].p, n);
+  HIP_CHECK(hipGetLastError());
+  for (petalo_mlem* c : b->ctx) {
+    // the rare unbounded walks, one lane each
+    const size_t n_irr = c->irregular[sub].n;
+    if (!n_irr) continue;
+    const Lor* L = c->lors.p + sub * c->set_size;
+    if (c->tof_on)
+      k_fwd_irregular<true><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
+                                                                c->dtof, c->image.p, c->bp64.p, c->counters.p);
+    else
+      k_fwd_irregular<false><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
+                                                                 c->dtof, c->image.p, c->bp64.p, c->counters.p);
+  }
   if (b->items[1][sub].n)
     k_scatter_batch<true><<<(unsigned)b->items[1][sub].n, 1024, b->lds_f64, st>>>(rd, b->items[1][sub].p);
   if (b->items[0][sub].n)
     k_scatter_batch<false><<<(unsigned)b->items[0][sub].n, 1024, b->lds_f64, st>>>(rd, b->items[0][sub].p);
   HIP_CHECK(hipGetLastError());
-  for (petalo_mlem* c : b->ctx) {
-    const size_t nv = nvox(c->fov);
-    k_update_f64<<<grid_for(nv), kBlock, 0, st>>>(c->image.p, c->bp64.p, c->sens.p, (uint32_t)nv);
-  }
+  k_update_batch<<<b->update_blocks, kBlock, 0, st>>>(b->update.p, n);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -993,33 +1011,75 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
         b->lds_f64 = std::max(b->lds_f64, (size_t)c->tiles.tvox * sizeof(double));
       }
       b->descs.resize(n_subsets);
+      b->reduce.resize(n_subsets);
+      b->reduce_blocks.assign(n_subsets, 0);
       for (int t = 0; t < 2; ++t) b->items[t].resize(n_subsets);
+      {
+        std::vector<UpdateDesc> u(n);
+        uint32_t blocks = 0;
+        for (size_t r = 0; r < n; ++r) {
+          petalo_mlem* c = b->ctx[r];
+          const uint32_t nv = (uint32_t)nvox(c->fov);
+          u[r] = UpdateDesc{c->image.p, c->bp64.p, c->sens.p, c->counters.p, nv, blocks};
+          blocks += grid_for(nv);
+        }
+        b->update.alloc(n);
+        HIP_CHECK(hipMemcpy(b->update.p, u.data(), n * sizeof(UpdateDesc), hipMemcpyHostToDevice));
+        b->update_blocks = blocks;
+      }
       for (size_t sub = 0; sub < n_subsets; ++sub) {
         std::vector<ReconDesc> d(n);
-        std::vector<std::pair<uint64_t, int4>> it[2];
+        std::vector<ReduceDesc> rd(n);
+        uint32_t rblocks = 0;
+        // (reconstruction, tile, first, end) with steps, from each context's items
+        struct Item { int r, tile, a, e; uint64_t steps; bool tof; };
+        std::vector<Item> all;
+        uint64_t total_steps = 0;
         for (size_t r = 0; r < n; ++r) {
           petalo_mlem* c = b->ctx[r];
           d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->dfov, c->dtof,
                            c->tiles};
+          const uint32_t m = (uint32_t)c->set_size;
+          rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->partial.p,
+                             c->rbuf.p, m, rblocks};
+          rblocks += (m + kBlock * kReduceLors - 1) / (kBlock * kReduceLors);
           std::vector<int4> h(c->work[sub].n);
           if (!h.empty())
             HIP_CHECK(hipMemcpy(h.data(), c->work[sub].p, h.size() * sizeof(int4), hipMemcpyDeviceToHost));
           for (size_t k = 0; k < h.size(); ++k) {
-            int4 w = h[k];
-            w.w = (int)r;
-            it[c->tof_on ? 1 : 0].push_back({c->item_steps[sub][k], w});
+            all.push_back(Item{(int)r, h[k].x, h[k].y, h[k].z, c->item_steps[sub][k], c->tof_on});
+            total_steps += c->item_steps[sub][k];
           }
         }
+        // Merge adjacent items of one tile up to the batch's item size, so
+        // that small reconstructions do not pay a tile load and flush per
+        // few runs; then largest first over the whole batch.
+        const uint64_t target = std::max<uint64_t>(kMinItemSteps, total_steps / kBatchItems + 1);
+        std::sort(all.begin(), all.end(), [](const Item& u, const Item& v) {
+          return u.r != v.r ? u.r < v.r : (u.tile != v.tile ? u.tile < v.tile : u.a < v.a);
+        });
+        std::vector<Item> merged;
+        for (const Item& x : all) {
+          if (!merged.empty()) {
+            Item& m = merged.back();
+            if (m.r == x.r && m.tile == x.tile && m.e == x.a && m.steps + x.steps <= target) {
+              m.e = x.e;
+              m.steps += x.steps;
+              continue;
+            }
+          }
+          merged.push_back(x);
+        }
+        std::stable_sort(merged.begin(), merged.end(), [](const Item& u, const Item& v) { return u.steps > v.steps; });
         b->descs[sub].alloc(n);
         HIP_CHECK(hipMemcpy(b->descs[sub].p, d.data(), n * sizeof(ReconDesc), hipMemcpyHostToDevice));
+        b->reduce[sub].alloc(n);
+        HIP_CHECK(hipMemcpy(b->reduce[sub].p, rd.data(), n * sizeof(ReduceDesc), hipMemcpyHostToDevice));
+        b->reduce_blocks[sub] = rblocks;
         for (int t = 0; t < 2; ++t) {
-          // largest first over the whole batch
-          std::stable_sort(it[t].begin(), it[t].end(),
-                           [](const std::pair<uint64_t, int4>& u, const std::pair<uint64_t, int4>& v) {
-                             return u.first > v.first;
-                           });
           std::vector<int4> w;
-          for (auto& x : it[t]) w.push_back(x.second);
+          for (const Item& x : merged)
+            if (x.tof == (t == 1)) w.push_back(make_int4(x.tile, x.a, x.e, x.r));
           b->items[t][sub].alloc(std::max<size_t>(w.size(), 1));
           if (!w.empty())
             HIP_CHECK(hipMemcpy(b->items[t][sub].p, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -1105,6 +1165,8 @@ int petalo_batch_destroy(petalo_batch_t* b) {
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   b->descs.clear();
+  b->reduce.clear();
+  b->update.release();
   for (auto& v : b->items) v.clear();
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;

@@ -399,22 +399,19 @@ inline unsigned reduce_grid(size_t n) {
   const size_t per_group = 256 * kReduceLors;
   return (unsigned)(8 * ((share + per_group - 1) / per_group));
 }
+// Reduces LORs base + first + 64*u (u < kReduceLors, those below base + end).
 template <bool SENS>
-__global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac, uint32_t n,
-                                                    const uint32_t* __restrict__ run_off,
-                                                    const uint32_t* __restrict__ run_pos,
-                                                    const float* __restrict__ partial, float* __restrict__ r_out) {
-  const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
-  const uint32_t share = (n + 7u) / 8u;
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t first = slot * blockDim.x * kReduceLors + wave * 64u * kReduceLors + lane;
+__device__ __forceinline__ void reduce_lors(const float* __restrict__ ac, uint32_t base, uint32_t first,
+                                            uint32_t end, const uint32_t* __restrict__ run_off,
+                                            const uint32_t* __restrict__ run_pos,
+                                            const float* __restrict__ partial, float* __restrict__ r_out) {
   uint32_t idx[kReduceLors], b[kReduceLors], e[kReduceLors];
   bool on[kReduceLors];
 #pragma unroll
   for (uint32_t u = 0; u < kReduceLors; ++u) {
     const uint32_t local = first + 64u * u;
-    idx[u] = xcd * share + local;
-    on[u] = local < share && idx[u] < n;
+    idx[u] = base + local;
+    on[u] = local < end;
     b[u] = on[u] ? run_off[idx[u]] : 0u;
     e[u] = on[u] ? run_off[idx[u] + 1] : 0u;
   }
@@ -440,6 +437,62 @@ __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac
     // (projector.rs:67-74,144-149)
     r_out[idx[u]] = SENS ? 1.0f / expf(-p) : 1.0f / (p * ac[idx[u]]);
   }
+}
+
+template <bool SENS>
+__global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac, uint32_t n,
+                                                    const uint32_t* __restrict__ run_off,
+                                                    const uint32_t* __restrict__ run_pos,
+                                                    const float* __restrict__ partial, float* __restrict__ r_out) {
+  const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+  const uint32_t share = (n + 7u) / 8u;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t first = slot * blockDim.x * kReduceLors + wave * 64u * kReduceLors + lane;
+  const uint32_t base = xcd * share;
+  const uint32_t end = base + share < n ? share : (base < n ? n - base : 0u);
+  reduce_lors<SENS>(ac, base, first, end, run_off, run_pos, partial, r_out);
+}
+
+// Per-reconstruction tables of the batched small kernels; blocks of one
+// reconstruction are consecutive from block0.
+struct ReduceDesc {
+  const float* ac;
+  const uint32_t* run_off;
+  const uint32_t* run_pos;
+  const float* partial;
+  float* r;
+  uint32_t n;
+  uint32_t block0;
+};
+
+struct UpdateDesc {
+  float* x;
+  double* bp;
+  const float* s;
+  unsigned long long* counters;
+  uint32_t nv;
+  uint32_t block0;
+};
+
+// Index of the table entry whose block range holds block b (entries sorted
+// by block0, the first at 0).
+template <class D>
+__device__ __forceinline__ uint32_t find_entry(const D* __restrict__ d, uint32_t n, uint32_t b) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (d[mid].block0 <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Batched per-LOR reduction: 256 * kReduceLors LORs per block.
+__global__ __launch_bounds__(256) void k_lor_reduce_batch(const ReduceDesc* __restrict__ d, uint32_t n_rec) {
+  const uint32_t r = find_entry(d, n_rec, blockIdx.x);
+  const ReduceDesc q = d[r];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t first = (blockIdx.x - q.block0) * blockDim.x * kReduceLors + wave * 64u * kReduceLors + lane;
+  reduce_lors<false>(q.ac, 0u, first, q.n, q.run_off, q.run_pos, q.partial, q.r);
 }
 
 // LORs whose walk is not provably bounded: exact one-lane projection and
@@ -558,6 +611,23 @@ __global__ __launch_bounds__(1024) void k_scatter_batch(const ReconDesc* __restr
   const int4 wi = work[blockIdx.x];
   const ReconDesc& r = rd[wi.w];
   scatter_body<TOF>(r.runs, wi, r.r, r.f, r.t, r.g, r.bp, nullptr);
+}
+
+// Batched MLEM update (as k_update_f64) over every reconstruction's image.
+__global__ __launch_bounds__(256) void k_update_batch(const UpdateDesc* __restrict__ d, uint32_t n_rec) {
+  const uint32_t r = find_entry(d, n_rec, blockIdx.x);
+  const UpdateDesc q = d[r];
+  const uint32_t i = (blockIdx.x - q.block0) * blockDim.x + threadIdx.x;
+  if (i >= q.nv) return;
+  const float b = (float)q.bp[i];
+  q.bp[i] = 0.0;
+  const float sj = q.s ? q.s[i] : 1.0f;
+  q.x[i] = sj > 0.0f ? q.x[i] * (b * sj) : 0.0f;
+}
+
+// Zeroes the work counters of every reconstruction of a batch.
+__global__ void k_zero_counters_batch(const UpdateDesc* __restrict__ d, uint32_t n_rec, uint32_t n_cnt) {
+  for (uint32_t k = threadIdx.x; k < n_rec * n_cnt; k += blockDim.x) d[k / n_cnt].counters[k % n_cnt] = 0ull;
 }
 
 // MLEM update with the binary64 backprojection, which is reset:
