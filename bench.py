@@ -74,6 +74,23 @@ def run_batched(P, a):
         b.step()
         ms_direct.append(b.last_ms())
     b.close()
+    # The plain drop-in for comparison: one reconstruction at a time, one
+    # lane per LOR with global atomics (PETALO_SCHEME_ATOMIC), one full
+    # iteration each after one warm-up subset update.
+    plain_s = 0.0
+    if not a.no_cpu_baseline:
+        for k in range(n_rec):
+            c = P.Mlem(fovs[k], lors[k], tof=tof, n_subsets=subsets, scheme=P.SCHEME_ATOMIC)
+            c.step(copy_image=False)
+            P.lib().petalo_device_synchronize(0)
+            t1 = time.perf_counter()
+            for _ in range(subsets - 1):
+                c.step(copy_image=False)
+            c.step(copy_image=False)
+            P.lib().petalo_device_synchronize(0)
+            plain_s += time.perf_counter() - t1
+            c.close()
+        log(f"[bench] plain one-at-a-time atomic path: {plain_s * 1e3:.1f} ms per full iteration of all 32")
     out = {
         "metric": "batched TOF-OSEM full iterations/s (32 reconstructions, 20 subsets)",
         "value": a.steps / el, "unit": "iter/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -84,6 +101,8 @@ def run_batched(P, a):
                    "total_lors": int(counts.sum())},
         "subset_update_ms_graphs": el / (a.steps * subsets) * 1e3,
         "subset_update_ms_direct_launches": float(np.mean(ms_direct)),
+        "plain_atomic_full_iteration_ms": plain_s * 1e3 if plain_s else None,
+        "speedup_vs_plain_atomic": (plain_s / (el / a.steps)) if plain_s else None,
     }
     print(json.dumps(out), flush=True)
     return 0
