@@ -187,6 +187,60 @@ int petalo_batch_set_options(petalo_batch_t* batch, int timing, int use_graphs);
 int petalo_batch_last_ms(petalo_batch_t* batch, double* ms);
 int petalo_batch_destroy(petalo_batch_t* batch);
 
+/* ---- LOR ingestion (src/io/hdf5.rs:97-228, src/discrete.rs:60-213,
+ * src/lorogram.rs, src/lorogram/build_scattergram.rs) ----------------------
+ * Turns the records of a reco_info/lors dataset, as the caller read them from
+ * the HDF5 file (Hdf5Lor, hdf5.rs:282-297: dt [ns], x1 y1 z1 x2 y2 z2 [mm],
+ * q1 q2, E1 E2 [keV]; 11 floats per record), into MLEM LORs on the device:
+ * records [events_min, events_max) (hdf5.rs:36-66), cut on |z| < z_max, on
+ * charge and on energy before smearing (511 keV * (1 - 1.6 fwhm)), energies
+ * smeared with a Gaussian of FWHM fwhm * E (hdf5.rs:230-238), cut on the energy
+ * bounds, both ends moved inside their detector element (discrete.rs:78-118,
+ * `adjust`), optionally counted into a scattergram of trues (min E >= 510 keV)
+ * and scatters (hdf5.rs:69-92), then converted (dt in ps; additive correction
+ * = (scatters + trues) / trues of the LOR's bin, or FLT_MAX for an empty bin,
+ * lorogram.rs:65-74; 1 without a scattergram).  The smearing draws come from a
+ * Philox4x32-10 stream keyed by `seed` and counted by record index (the
+ * reference uses an unseeded thread RNG). */
+#define PETALO_ADJUST_NONE 0          /* Adjust::No                      */
+#define PETALO_ADJUST_CENTRE 1        /* Adjust::Centre                  */
+#define PETALO_ADJUST_RANDOM 2        /* Adjust::Random                  */
+#define PETALO_ADJUST_RANDOM_Z_PHI 3  /* Adjust::RandomZPhi (the reader's) */
+
+typedef struct {
+  uint64_t events_min, events_max;   /* [input] events: record range [min, max) */
+  int32_t has_events_min, has_events_max;
+  int32_t has_charge_min, has_charge_max;  /* [input] charge: min <= q < max */
+  float charge_min, charge_max;
+  int32_t has_energy_min, has_energy_max;  /* [input] energy (after smearing) */
+  float energy_min, energy_max;
+  float smear_energy_fwhm;           /* [smear_energy] fwhm, a fraction of E */
+  int32_t has_z_max;                 /* [detector_full_axial_length]: z_max = dz / 2 */
+  float z_max_mm;
+  float r_min_mm, dr_mm, dz_mm, da_mm;  /* element discretization (the dataset's
+                                           r_min, dr, dz, da attributes) */
+  int32_t adjust;                    /* PETALO_ADJUST_*                   */
+  uint64_t seed;
+} petalo_ingest_t;
+
+/* Scattergram axes (config [scatter_correction]): bins = 0 and max = 0 leave
+ * an axis out (one bin spanning everything); bins alone take the builder's
+ * default range (r 30 mm, z 200 mm, dz 1000 mm, dt 1700 ps), a range alone 30
+ * bins (build_scattergram.rs:60-108).  All axes out: no scattergram. */
+typedef struct {
+  uint32_t phi_bins;
+  uint32_t z_bins;  float z_length_mm;
+  uint32_t dz_bins; float dz_max_mm;
+  uint32_t r_bins;  float r_max_mm;
+  uint32_t dt_bins; float dt_max_ps;
+} petalo_scattergram_spec_t;
+
+/* out must hold capacity LORs (at most the number of records in the event
+ * range are produced); *n_out = LORs written, *n_read_out = records read. */
+int petalo_ingest_lors(const float* records, size_t n_records, const petalo_ingest_t* params,
+                       const petalo_scattergram_spec_t* scatter_or_null, petalo_lor_t* out, size_t capacity,
+                       size_t* n_out, uint64_t* n_read_out);
+
 /* ---- Sensitivity image (continuous.rs:3-47, main.rs:77) ---------------
  * Backprojects n_lors LORs through the attenuation image mu [1/mm] on
  * att_fov, adapting each projection p to exp(-p) (projector.rs:72-74), into

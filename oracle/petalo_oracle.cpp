@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstring>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -638,4 +639,203 @@ int orc_walk_stats_f32(const float full_mm[3], const uint32_t n[3], const orc_to
   return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// LOR ingestion
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  // Salmon, Moraes, Dror, Shaw (SC'11): 10 rounds of the Philox S-box with
+  // multipliers 0xD2511F53, 0xCD9E8D57 and Weyl key increments 0x9E3779B9,
+  // 0xBB67AE85.
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t a = (uint64_t)0xD2511F53u * c0;
+    const uint64_t b = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
+    const uint32_t n1 = (uint32_t)b;
+    const uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+    const uint32_t n3 = (uint32_t)a;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct Rec { float dt, x1, y1, z1, x2, y2, z2, q1, q2, e1, e2; };
+
+float uni(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+
+void draw(uint64_t seed, uint64_t rec, uint32_t block, uint32_t tag, float u[4]) {
+  const uint32_t ctr[4] = {(uint32_t)rec, (uint32_t)(rec >> 32), block, tag};
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t o[4];
+  orc_philox4x32_10(ctr, key, o);
+  for (int k = 0; k < 4; ++k) u[k] = uni(o[k]);
+}
+
+bool in_bounds(int32_t has_min, float mn, int32_t has_max, float mx, float v) {
+  // config/mlem.rs:141-146
+  return (!has_min || mn <= v) && (!has_max || mx > v);
+}
+
+// discrete.rs:170-187 and 78-118, binary32 in the reference's order
+struct Disc { float dr, dz, n_radial, d_azimuthal; int adjust; };
+
+Disc disc_of(const orc_ingest_t& p) {
+  const float tau = 6.2831855f;
+  Disc d;
+  d.dr = p.dr_mm;
+  d.dz = p.dz_mm;
+  const float r = p.r_min_mm + (p.dr_mm / 2.0f);
+  d.n_radial = r / p.dr_mm;
+  const float mid = tau * r;
+  const float n_az = std::round(mid / p.da_mm);
+  d.d_azimuthal = tau / n_az;
+  d.adjust = p.adjust;
+  return d;
+}
+
+void adjust_end(const Disc& d, float& x, float& y, float& z, float uz, float uphi, float ur) {
+  if (d.adjust == 0) return;
+  const int32_t n_phi = (int32_t)std::round(std::atan2(y, x) / d.d_azimuthal);
+  const int32_t n_z = (int32_t)std::round(z / d.dz);
+  float r, phi;
+  if (d.adjust == 1) {
+    r = d.n_radial * d.dr;
+    z = (float)n_z * d.dz;
+    phi = (float)n_phi * d.d_azimuthal;
+  } else {
+    z = (((float)n_z - 0.5f) + uz) * d.dz;
+    phi = (((float)n_phi - 0.5f) + uphi) * d.d_azimuthal;
+    r = d.adjust == 2 ? ((d.n_radial - 0.5f) + ur) * d.dr : std::hypot(x, y);
+  }
+  x = r * std::cos(phi);
+  y = r * std::sin(phi);
+}
+
+// ndhistogram 0.9.0 axes as restated in the library's header
+struct Ax {
+  uint32_t n; bool cyclic; float low, high;
+  uint64_t stride() const { return cyclic ? n : (uint64_t)n + 2; }
+  int64_t index(float x) const {
+    const float f = (x - low) / (high - low);
+    if (std::isnan(f)) return -1;
+    if (cyclic) {
+      const float g = f - std::floor(f);
+      int64_t i = (int64_t)(g * (float)n);
+      return i >= (int64_t)n ? (int64_t)n - 1 : i;
+    }
+    if (f < 0.0f) return 0;
+    if (f >= 1.0f) return (int64_t)n + 1;
+    int64_t i = (int64_t)(f * (float)n);
+    return (i >= (int64_t)n ? (int64_t)n - 1 : i) + 1;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_ingest_f32(const float* records, size_t n_records, const orc_ingest_t* pp,
+                       const orc_sgram_spec_t* sp, orc_lor_t* out, uint64_t* n_read) {
+  const orc_ingest_t& p = *pp;
+  const uint64_t lo = p.has_events_min ? std::min<uint64_t>(p.events_min, n_records) : 0;
+  const uint64_t hi = p.has_events_max ? std::min<uint64_t>(p.events_max, n_records) : n_records;
+  const uint64_t n = hi > lo ? hi - lo : 0;
+  if (n_read) *n_read = n;
+  const Disc d = disc_of(p);
+  // hdf5.rs:108: 511 * (1 - 1.6 fwhm)
+  const float pre = 511.0f * (1.0f - 1.6f * p.smear_energy_fwhm);
+  std::vector<Rec> kept;
+  for (uint64_t k = 0; k < n; ++k) {
+    const uint64_t i = lo + k;
+    Rec r;
+    std::memcpy(&r, records + 11 * i, sizeof r);
+    if (p.has_z_max && !(std::fabs(r.z1) < p.z_max_mm && std::fabs(r.z2) < p.z_max_mm)) continue;
+    if (!(in_bounds(p.has_charge_min, p.charge_min, p.has_charge_max, p.charge_max, r.q1) &&
+          in_bounds(p.has_charge_min, p.charge_min, p.has_charge_max, p.charge_max, r.q2))) continue;
+    if (!(pre < r.e1 && pre < r.e2)) continue;
+    float ue[4];
+    draw(p.seed, i, 0, 0xe6e7u, ue);
+    // hdf5.rs:230-238: Normal(e, fwhm*e/2.35); standard normal by Box-Muller
+    auto smear_e = [&](float e, float u1, float u2) {
+      const float zn = std::sqrt(-2.0f * std::log(1.0f - u1)) * std::cos(6.2831855f * u2);
+      const float f = p.smear_energy_fwhm * e;
+      const float sigma = f / 2.35f;
+      return e + sigma * zn;
+    };
+    r.e1 = smear_e(r.e1, ue[0], ue[1]);
+    r.e2 = smear_e(r.e2, ue[2], ue[3]);
+    if (!(in_bounds(p.has_energy_min, p.energy_min, p.has_energy_max, p.energy_max, r.e1) &&
+          in_bounds(p.has_energy_min, p.energy_min, p.has_energy_max, p.energy_max, r.e2))) continue;
+    float a[4], b[4];
+    draw(p.seed, i, 1, 0x9051u, a);
+    draw(p.seed, i, 2, 0x9051u, b);
+    adjust_end(d, r.x1, r.y1, r.z1, a[0], a[1], b[0]);
+    adjust_end(d, r.x2, r.y2, r.z2, a[2], a[3], b[1]);
+    kept.push_back(r);
+  }
+  // scattergram (lorogram.rs, build_scattergram.rs:60-158)
+  bool sg = false;
+  Ax ax[5];
+  if (sp) {
+    const bool hz = sp->z_bins || sp->z_length_mm > 0, hdz = sp->dz_bins || sp->dz_max_mm > 0,
+               hr = sp->r_bins || sp->r_max_mm > 0, hdt = sp->dt_bins || sp->dt_max_ps > 0;
+    sg = sp->phi_bins || hz || hdz || hr || hdt;
+    const float l = 66666666.6f, t = 66666666.6f;
+    auto nb = [](uint32_t v, bool on) { return on ? (v ? v : 30u) : 1u; };
+    const float zl = hz ? (sp->z_length_mm > 0 ? sp->z_length_mm : 200.0f) : l;
+    ax[0] = Ax{sp->phi_bins ? sp->phi_bins : 1u, true, 0.0f, 6.2831855f};
+    ax[1] = Ax{nb(sp->z_bins, hz), false, -(zl / 2.0f), zl / 2.0f};
+    ax[2] = Ax{nb(sp->dz_bins, hdz), false, 0.0f, hdz ? (sp->dz_max_mm > 0 ? sp->dz_max_mm : 1000.0f) : l};
+    ax[3] = Ax{nb(sp->r_bins, hr), false, 0.0f, hr ? (sp->r_max_mm > 0 ? sp->r_max_mm : 30.0f) : l};
+    const float tm = hdt ? (sp->dt_max_ps > 0 ? sp->dt_max_ps : 1700.0f) : t;
+    ax[4] = Ax{nb(sp->dt_bins, hdt), false, -tm, tm};
+  }
+  auto bin_of = [&](const Rec& r) -> int64_t {
+    // lorogram.rs:166-221 (the dt axis is indexed by the distance from the axis)
+    const float dx = r.x2 - r.x1, dy = r.y2 - r.y1;
+    const float num = dx * r.y1 - dy * r.x1;
+    const float den = std::sqrt(dx * dx + dy * dy);
+    float phi = std::atan2(dy, dx);
+    if (num / den < 0.0f) phi = phi + 3.1415927f;
+    const float dist = std::fabs(num) / den;
+    const float c[5] = {phi, (r.z1 + r.z2) / 2.0f, std::fabs(r.z1 - r.z2), dist, dist};
+    int64_t b = 0;
+    for (int k = 0; k < 5; ++k) {
+      const int64_t i = ax[k].index(c[k]);
+      if (i < 0) return -1;
+      b = b * (int64_t)ax[k].stride() + i;
+    }
+    return b;
+  };
+  std::unordered_map<int64_t, std::pair<uint64_t, uint64_t>> hist;  // bin -> (trues, scatters)
+  if (sg)
+    for (const Rec& r : kept) {
+      const int64_t b = bin_of(r);
+      if (b < 0) continue;
+      auto& h = hist[b];
+      if (std::fmin(r.e1, r.e2) < 510.0f) ++h.second; else ++h.first;
+    }
+  for (size_t k = 0; k < kept.size(); ++k) {
+    const Rec& r = kept[k];
+    float ac = 1.0f;
+    if (sg) {
+      const int64_t b = bin_of(r);
+      const auto it = b < 0 ? hist.end() : hist.find(b);
+      const uint64_t t = it == hist.end() ? 0 : it->second.first;
+      const uint64_t s = it == hist.end() ? 0 : it->second.second;
+      ac = t > 0 ? ((float)s + (float)t) / (float)t : 3.40282347e+38f;  // lorogram.rs:65-74
+    }
+    // LOR::from(Hdf5Lor) (hdf5.rs:298-320): ns -> ps
+    out[k] = orc_lor_t{{r.x1, r.y1, r.z1}, {r.x2, r.y2, r.z2}, r.dt * 1000.0f, ac};
+  }
+  return (int64_t)kept.size();
+}
 }  // extern "C"

@@ -136,6 +136,48 @@ int orc_walk_stats_f32(const float full_mm[3], const uint32_t n[3],
                        const orc_tof_t* tof, const orc_lor_t* lors, size_t n_lors,
                        int n_threads, orc_stats_t* stats);
 
+/* LOR ingestion (src/io/hdf5.rs:97-228, src/discrete.rs:60-213,
+ * src/lorogram.rs): records of 11 floats (dt ns, x1 y1 z1 x2 y2 z2 mm, q1 q2,
+ * E1 E2 keV) -> cuts, energy smearing, element adjustment, scattergram,
+ * LORs.  The smearing draws restate the library's documented stream:
+ * Philox4x32-10 (Salmon et al., SC'11) under key (seed lo, seed hi), counter
+ * (record lo, record hi, block, tag); uniforms from the top 24 bits; energy
+ * block 0 tag 0xe6e7 (Box-Muller pairs (x, y) for E1, (z, w) for E2);
+ * position blocks 1 and 2 tag 0x9051 (end 1: z = a.x, phi = a.y, r = b.x;
+ * end 2: z = a.z, phi = a.w, r = b.y).  Same field layout as the library's
+ * parameter structs. */
+typedef struct {
+  uint64_t events_min, events_max;
+  int32_t has_events_min, has_events_max;
+  int32_t has_charge_min, has_charge_max;
+  float charge_min, charge_max;
+  int32_t has_energy_min, has_energy_max;
+  float energy_min, energy_max;
+  float smear_energy_fwhm;
+  int32_t has_z_max;
+  float z_max_mm;
+  float r_min_mm, dr_mm, dz_mm, da_mm;
+  int32_t adjust;      /* 0 none, 1 centre, 2 random, 3 random z and phi */
+  uint64_t seed;
+} orc_ingest_t;
+
+typedef struct {
+  uint32_t phi_bins;
+  uint32_t z_bins;  float z_length_mm;
+  uint32_t dz_bins; float dz_max_mm;
+  uint32_t r_bins;  float r_max_mm;
+  uint32_t dt_bins; float dt_max_ps;
+} orc_sgram_spec_t;
+
+/* Returns the number of LORs written to out (capacity >= records in range);
+ * *n_read = records in the event range. */
+int64_t orc_ingest_f32(const float* records, size_t n_records, const orc_ingest_t* p,
+                       const orc_sgram_spec_t* sgram_or_null, orc_lor_t* out, uint64_t* n_read);
+
+/* Philox4x32-10 block (for pinning the restatement against the paper's
+ * known-answer vectors). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,8 @@ __all__ = [
     "make_sensitivity_image", "sensitivity_lors", "system_matrix_rows", "density_to_attenuation",
     "synth_phantom_lors", "synth_density", "philox4x32_10", "read_config", "parse_config", "parse_quantity",
     "write_image3d", "read_image3d", "device_count", "make_lors",
-    "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE", "Batch",
+    "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE", "Batch", "ingest_lors", "ingest_params",
+    "scattergram_spec", "H5LOR_DTYPE",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -66,6 +67,67 @@ class _Stats(C.Structure):
 class _Options(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("sort_lors", C.c_int32), ("device", C.c_int32),
                 ("tile", C.c_int32 * 3)]
+
+
+class _Ingest(C.Structure):
+    _fields_ = [("events_min", C.c_uint64), ("events_max", C.c_uint64),
+                ("has_events_min", C.c_int32), ("has_events_max", C.c_int32),
+                ("has_charge_min", C.c_int32), ("has_charge_max", C.c_int32),
+                ("charge_min", C.c_float), ("charge_max", C.c_float),
+                ("has_energy_min", C.c_int32), ("has_energy_max", C.c_int32),
+                ("energy_min", C.c_float), ("energy_max", C.c_float),
+                ("smear_energy_fwhm", C.c_float), ("has_z_max", C.c_int32), ("z_max_mm", C.c_float),
+                ("r_min_mm", C.c_float), ("dr_mm", C.c_float), ("dz_mm", C.c_float), ("da_mm", C.c_float),
+                ("adjust", C.c_int32), ("seed", C.c_uint64)]
+
+
+class _SgramSpec(C.Structure):
+    _fields_ = [("phi_bins", C.c_uint32), ("z_bins", C.c_uint32), ("z_length_mm", C.c_float),
+                ("dz_bins", C.c_uint32), ("dz_max_mm", C.c_float), ("r_bins", C.c_uint32),
+                ("r_max_mm", C.c_float), ("dt_bins", C.c_uint32), ("dt_max_ps", C.c_float)]
+
+
+ADJUST_NONE, ADJUST_CENTRE, ADJUST_RANDOM, ADJUST_RANDOM_Z_PHI = 0, 1, 2, 3
+H5LOR_DTYPE = np.dtype([(k, "<f4") for k in ("dt", "x1", "y1", "z1", "x2", "y2", "z2", "q1", "q2", "E1", "E2")])
+
+
+def ingest_params(events=(None, None), charge=(None, None), energy=(None, None), smear_energy_fwhm=0.0,
+                  z_max=None, discretize=(0.0, 0.0, 0.0, 0.0), adjust=ADJUST_RANDOM_Z_PHI, seed=0):
+    """Parameters of ingest_lors; bounds are (min, max) pairs with None for open."""
+    p = _Ingest()
+    p.has_events_min, p.events_min = (0, 0) if events[0] is None else (1, int(events[0]))
+    p.has_events_max, p.events_max = (0, 0) if events[1] is None else (1, int(events[1]))
+    p.has_charge_min, p.charge_min = (0, 0.0) if charge[0] is None else (1, float(charge[0]))
+    p.has_charge_max, p.charge_max = (0, 0.0) if charge[1] is None else (1, float(charge[1]))
+    p.has_energy_min, p.energy_min = (0, 0.0) if energy[0] is None else (1, float(energy[0]))
+    p.has_energy_max, p.energy_max = (0, 0.0) if energy[1] is None else (1, float(energy[1]))
+    p.smear_energy_fwhm = float(smear_energy_fwhm)
+    p.has_z_max, p.z_max_mm = (0, 0.0) if z_max is None else (1, float(z_max))
+    p.r_min_mm, p.dr_mm, p.dz_mm, p.da_mm = (float(v) for v in discretize)
+    p.adjust = int(adjust)
+    p.seed = int(seed)
+    return p
+
+
+def scattergram_spec(phi=0, z=(0, 0.0), dz=(0, 0.0), r=(0, 0.0), dt=(0, 0.0)):
+    """[scatter_correction] axes: phi bins; (bins, length or max) for z, dz, r, dt (0 = unset)."""
+    return _SgramSpec(int(phi), int(z[0]), float(z[1]), int(dz[0]), float(dz[1]), int(r[0]), float(r[1]),
+                      int(dt[0]), float(dt[1]))
+
+
+def ingest_lors(records, params, scattergram=None):
+    """reco_info/lors records (H5LOR_DTYPE or n x 11 float32) -> (LORs, records read)
+    (src/io/hdf5.rs:97-228)."""
+    rec = np.ascontiguousarray(records)
+    rec = rec.view(np.float32).reshape(-1, 11) if rec.dtype != np.float32 else rec.reshape(-1, 11)
+    rec = np.ascontiguousarray(rec, np.float32)
+    out = np.zeros(len(rec), LOR_DTYPE)
+    n_out = C.c_size_t(0)
+    n_read = C.c_uint64(0)
+    _check(lib().petalo_ingest_lors(_ptr(rec), len(rec), C.byref(params),
+                                    C.byref(scattergram) if scattergram is not None else None,
+                                    _ptr(out), len(out), C.byref(n_out), C.byref(n_read)))
+    return out[:n_out.value].copy(), int(n_read.value)
 
 
 class _Config(C.Structure):
@@ -128,6 +190,7 @@ _SIGS = {
     "petalo_synth_phantom_lors": (C.c_int, [C.c_uint64, C.c_size_t, C.c_float, _P]),
     "petalo_synth_density": (C.c_int, [_P, _P]),
     "petalo_philox4x32_10": (C.c_int, [_P, _P, _P]),
+    "petalo_ingest_lors": (C.c_int, [_P, C.c_size_t, _P, _P, _P, C.c_size_t, _P, _P]),
 }
 
 _lib = None

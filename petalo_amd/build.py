@@ -32,7 +32,7 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
 
-SOURCES = ["capi.hip", "host.cpp"]
+SOURCES = ["capi.hip", "ingest.hip", "host.cpp"]
 
 
 def _run(cmd):

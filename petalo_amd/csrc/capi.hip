@@ -20,6 +20,7 @@
 
 #include "../../include/petalo_hip.h"
 #include "capi_internal.hpp"
+#include "hip_util.hpp"
 #include "kernels.hpp"
 #include "tiles.hpp"
 
@@ -42,29 +43,8 @@ int fail(int code, const std::string& msg) {
 
 namespace {
 
-struct HipError : std::runtime_error {
-  explicit HipError(const std::string& m) : std::runtime_error(m) {}
-};
 
-#define HIP_CHECK(expr)                                                                              \
-  do {                                                                                               \
-    hipError_t e_ = (expr);                                                                          \
-    if (e_ != hipSuccess) throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));         \
-  } while (0)
 
-template <class F> int guarded(F&& f) {
-  try {
-    return f();
-  } catch (const HipError& e) {
-    return fail(PETALO_ERR_HIP, e.what());
-  } catch (const std::bad_alloc&) {
-    return fail(PETALO_ERR_NOMEM, "host allocation failed");
-  } catch (const std::exception& e) {
-    return fail(PETALO_ERR_ARG, e.what());
-  } catch (...) {
-    return fail(PETALO_ERR_ARG, "unknown error");
-  }
-}
 
 bool fov_ok(const petalo_fov_t* f, std::string& why) {
   if (!f) { why = "fov is NULL"; return false; }
@@ -89,47 +69,8 @@ DevTof to_dev(const petalo_tof_t* t) {
   return make_tof(true, t->sigma_ps, t->cutoff);
 }
 
-constexpr int kBlock = 256;
-inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-template <class T> struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  explicit DevBuf(size_t count) { alloc(count); }
-  void alloc(size_t count) {
-    release();
-    n = count;
-    if (count) HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  ~DevBuf() { release(); }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
-    o.p = nullptr;
-    o.n = 0;
-  }
-  DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) {
-      release();
-      p = o.p;
-      n = o.n;
-      o.p = nullptr;
-      o.n = 0;
-    }
-    return *this;
-  }
-};
 
-void require_device() {
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw HipError("no HIP device available");
-}
 
 // Ordering key of a LOR inside its group: azimuth of its direction (folded
 // to [0, pi)), then signed transaxial distance from the axis, then axial

@@ -202,6 +202,33 @@ def sensitivity(att_full, att_n, mu, lors, out_full=None, out_n=None, threads=1,
     return out, st.as_dict()
 
 
+def ingest(records, params, spec=None):
+    """Oracle LOR ingestion; params / spec are the library's ctypes structs
+    (same layout as orc_ingest_t / orc_sgram_spec_t)."""
+    rec = np.ascontiguousarray(records)
+    rec = rec.view(np.float32).reshape(-1, 11) if rec.dtype != np.float32 else rec.reshape(-1, 11)
+    rec = np.ascontiguousarray(rec, np.float32)
+    out = np.zeros(len(rec), LOR_DTYPE)
+    n_read = C.c_uint64(0)
+    L = lib()
+    L.orc_ingest_f32.restype = C.c_int64
+    L.orc_ingest_f32.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    m = L.orc_ingest_f32(_p(rec), len(rec), C.byref(params), C.byref(spec) if spec is not None else None,
+                         _p(out), C.byref(n_read))
+    return out[:m].copy(), int(n_read.value)
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    L = lib()
+    L.orc_philox4x32_10.restype = None
+    L.orc_philox4x32_10.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_philox4x32_10(_p(c), _p(k), _p(o))
+    return o
+
+
 def rho_to_mu_factor(rho_to_mu):
     return lib().orc_rho_to_mu_factor_f32(rho_to_mu)
 

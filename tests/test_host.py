@@ -224,6 +224,18 @@ def test_mlem_option_checks_come_first():
     assert e.value.code == 1
 
 
+def test_ingest_argument_checks_come_first():
+    rec = np.zeros((3, 11), np.float32)
+    with pytest.raises(P.PetaloError) as e:
+        P.ingest_lors(rec, P.ingest_params(adjust=7))
+    assert e.value.code == 1
+    with pytest.raises(P.PetaloError) as e:
+        P.ingest_lors(rec, P.ingest_params(adjust=P.ADJUST_RANDOM_Z_PHI, discretize=(350.0, 0.0, 3.0, 3.0)))
+    assert e.value.code == 1
+    out, n = P.ingest_lors(rec, P.ingest_params(events=(5, 9), adjust=P.ADJUST_NONE))
+    assert n == 0 and len(out) == 0  # empty event range: nothing to do, no device needed
+
+
 def test_tools_help():
     for tool in ("mlem", "make_sensitivity_image"):
         exe = os.path.join(ROOT, "petalo_amd", "bin", tool)
