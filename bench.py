@@ -149,7 +149,7 @@ def main():
     lors = P.synth_phantom_lors(1, n_lors, fwhm)
     log(f"[bench] LORs ready in {time.time() - t0:.1f} s")
 
-    sens_img, sens_ms, sens_stats = None, None, None
+    sens_img, sens_ms, sens_stats, discrete = None, None, None, None
     if n_sens and not a.no_sensitivity:
         mu = P.Image(fov, np.zeros(fov.nvox, np.float32))
         P.make_sensitivity_image(mu, 1_000_000, 300.0, 700.0, seed=7)  # warm-up
@@ -157,6 +157,15 @@ def main():
         sens_img, sens_stats = P.make_sensitivity_image(mu, n_sens, 300.0, 700.0, seed=7, return_stats=True)
         sens_ms = (time.perf_counter() - t0) * 1e3
         log(f"[bench] sensitivity image from {n_sens} LORs: {sens_ms:.1f} ms")
+        # discrete detector: all pairs of 10 mm elements of a 700 mm ring, 600 mm long
+        att = P.density_to_attenuation(P.synth_density(fov), 0.096)
+        dgeom = dict(r_min=350.0, dr=20.0, dz=10.0, da=10.0)
+        t0 = time.perf_counter()
+        _, dst = P.make_sensitivity_image_discrete(att, 600.0, return_stats=True, **dgeom)
+        dms = (time.perf_counter() - t0) * 1e3
+        discrete = {"detector": dict(dgeom, length=600.0), "pairs": dst["n_lors"], "ms": dms,
+                    "n_steps": dst["n_steps"], "n_nz": dst["n_nz"], "n_missed": dst["n_missed"]}
+        log(f"[bench] discrete sensitivity image, {dst['n_lors']} element pairs: {dms:.1f} ms")
 
     t0 = time.perf_counter()
     tile = tuple(int(v) for v in a.tile.split(",")) if a.tile else None
@@ -300,6 +309,7 @@ def main():
                          "algorithmic_bytes": 8 * sens_stats["n_nz"] + 4 * fov.nvox,
                          "achieved_gbs": (8 * sens_stats["n_nz"] + 4 * fov.nvox) / (sens_ms * 1e-3) / 1e9}
                         if sens_stats else None),
+        "discrete_sensitivity": discrete,
         "parity_rel_l2_sample": parity,
         "ingest": ingest,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -253,6 +253,19 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu,
                                   const petalo_fov_t* out_fov_or_null, const petalo_lor_t* lors_or_null,
                                   size_t n_lors, uint64_t seed, float detector_length_mm,
                                   float detector_diameter_mm, float* out, petalo_stats_t* stats_or_null);
+/* Sensitivity image of a discretized detector (discrete::sensitivity_image,
+ * src/bin/make_sensitivity_image/discrete.rs:3-62): element centres of a
+ * ring of inner radius r_min, thickness dr, element width dz x da (at
+ * r_min + dr/2), axial length detector_length (discrete.rs:150-176); one LOR
+ * per element pair i <= k, self pairs included, each end adjusted inside its
+ * element (PETALO_ADJUST_*, draws from Philox keyed by seed and counted by
+ * pair index); projected as the continuous image; normalised by n (n - 1) / 2
+ * over the n elements (discrete.rs:16-19, which counts neither self pairs nor
+ * the FOV filter).  stats->n_lors = pairs. */
+int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const float* mu,
+                                           const petalo_fov_t* out_fov_or_null, float detector_length_mm,
+                                           float r_min_mm, float dr_mm, float dz_mm, float da_mm, int32_t adjust,
+                                           uint64_t seed, float* out, petalo_stats_t* stats_or_null);
 /* The LORs petalo_make_sensitivity_image generates for (seed, fov, cylinder). */
 int petalo_sensitivity_lors(const petalo_fov_t* att_fov, size_t n_lors, uint64_t seed,
                             float detector_length_mm, float detector_diameter_mm, petalo_lor_t* out);

@@ -26,6 +26,7 @@
 #include "petalo_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -741,6 +742,43 @@ struct Ax {
 }  // namespace
 
 extern "C" {
+
+int64_t orc_discrete_lors(float r_min, float dr, float dz, float da, float length, int adjust, uint64_t seed,
+                          orc_lor_t* out, size_t capacity, uint64_t* n_elements) {
+  orc_ingest_t p{};
+  p.r_min_mm = r_min;
+  p.dr_mm = dr;
+  p.dz_mm = dz;
+  p.da_mm = da;
+  p.adjust = adjust;
+  const Disc d = disc_of(p);
+  // centre_all_elements (discrete.rs:150-176)
+  const float tau = 6.2831855f;
+  const float rr = r_min + (dr / 2.0f);
+  const uint32_t n_az = (uint32_t)std::round((tau * rr) / da);
+  const float r = d.n_radial * dr;
+  const int32_t n_half = (int32_t)(uint32_t)std::round((length / 2.0f) / dz);
+  std::vector<std::array<float, 3>> pts;
+  for (uint32_t a = 0; a < n_az; ++a) {
+    const float phi = d.d_azimuthal * (float)a;
+    for (int32_t nz = -n_half; nz <= n_half; ++nz) pts.push_back({r * std::cos(phi), r * std::sin(phi), (float)nz * dz});
+  }
+  const uint64_t n = pts.size();
+  if (n_elements) *n_elements = n;
+  uint64_t t = 0;
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint64_t k = i; k < n; ++k, ++t) {
+      if (t >= capacity) continue;
+      float pe[3] = {pts[k][0], pts[k][1], pts[k][2]}, qe[3] = {pts[i][0], pts[i][1], pts[i][2]};
+      float a4[4], b4[4];
+      draw(seed, t, 1, 0x9a1bu, a4);
+      draw(seed, t, 2, 0x9a1bu, b4);
+      adjust_end(d, pe[0], pe[1], pe[2], a4[0], a4[1], b4[0]);
+      adjust_end(d, qe[0], qe[1], qe[2], a4[2], a4[3], b4[1]);
+      out[t] = orc_lor_t{{pe[0], pe[1], pe[2]}, {qe[0], qe[1], qe[2]}, 0.0f, 1.0f};
+    }
+  return (int64_t)t;
+}
 
 int64_t orc_ingest_f32(const float* records, size_t n_records, const orc_ingest_t* pp,
                        const orc_sgram_spec_t* sp, orc_lor_t* out, uint64_t* n_read) {

@@ -174,6 +174,17 @@ typedef struct {
 int64_t orc_ingest_f32(const float* records, size_t n_records, const orc_ingest_t* p,
                        const orc_sgram_spec_t* sgram_or_null, orc_lor_t* out, uint64_t* n_read);
 
+/* LORs of a discretized detector (bin/make_sensitivity_image/discrete.rs:
+ * 23-62, src/discrete.rs:150-176): element centres azimuth-major, pairs
+ * (i, k) for i in 0..n, k in i..n, LOR from element k to element i, each end
+ * adjusted (0 none, 1 centre, 2 random, 3 random z and phi) with draws from
+ * Philox key (seed lo, seed hi), counter (pair lo, pair hi, block, 0x9a1b),
+ * blocks 1 and 2 (end k: z = a.x, phi = a.y, r = b.x; end i: a.z, a.w,
+ * b.y).  Writes min(pairs, capacity) LORs; returns the pair count and the
+ * element count in *n_elements. */
+int64_t orc_discrete_lors(float r_min_mm, float dr_mm, float dz_mm, float da_mm, float length_mm, int adjust,
+                          uint64_t seed, orc_lor_t* out, size_t capacity, uint64_t* n_elements);
+
 /* Philox4x32-10 block (for pinning the restatement against the paper's
  * known-answer vectors). */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -27,7 +27,7 @@ __all__ = [
     "synth_phantom_lors", "synth_density", "philox4x32_10", "read_config", "parse_config", "parse_quantity",
     "write_image3d", "read_image3d", "device_count", "make_lors",
     "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE", "Batch", "ingest_lors", "ingest_params",
-    "scattergram_spec", "H5LOR_DTYPE",
+    "scattergram_spec", "H5LOR_DTYPE", "make_sensitivity_image_discrete",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -191,6 +191,8 @@ _SIGS = {
     "petalo_synth_density": (C.c_int, [_P, _P]),
     "petalo_philox4x32_10": (C.c_int, [_P, _P, _P]),
     "petalo_ingest_lors": (C.c_int, [_P, C.c_size_t, _P, _P, _P, C.c_size_t, _P, _P]),
+    "petalo_make_sensitivity_image_discrete": (C.c_int, [_P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float,
+                                                         C.c_float, C.c_int32, C.c_uint64, _P, _P]),
 }
 
 _lib = None
@@ -528,6 +530,25 @@ def make_sensitivity_image(attenuation, n_lors, detector_length=None, detector_d
         lp, int(n_lors), int(seed), L, D, _ptr(out), C.byref(st)))
     img = Image(out_fov, out)
     return (img, _stats(st)) if return_stats else img
+
+
+def make_sensitivity_image_discrete(attenuation, detector_length, r_min, dr, dz, da,
+                                    adjust=ADJUST_RANDOM_Z_PHI, output_fov=None, seed=0, return_stats=False):
+    """Sensitivity image of a discretized detector (discrete::sensitivity_image,
+    src/bin/make_sensitivity_image/discrete.rs:3-62): all element pairs."""
+    att_fov = attenuation.fov
+    ofov = output_fov or att_fov
+    out = np.zeros(ofov.nvox, np.float32)
+    st = _Stats()
+    mu = np.ascontiguousarray(attenuation.data, np.float32)
+    _check(lib().petalo_make_sensitivity_image_discrete(
+        C.byref(att_fov._c()), _ptr(mu), C.byref(output_fov._c()) if output_fov is not None else None,
+        float(detector_length), float(r_min), float(dr), float(dz), float(da), int(adjust), int(seed),
+        _ptr(out), C.byref(st)))
+    img = Image(ofov, out)
+    if return_stats:
+        return img, Stats({f: int(getattr(st, f)) for f, _ in _Stats._fields_})
+    return img
 
 
 def sensitivity_lors(fov, n_lors, seed, detector_length, detector_diameter):

@@ -23,6 +23,7 @@
 #include "hip_util.hpp"
 #include "kernels.hpp"
 #include "tiles.hpp"
+#include "ingest.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -1243,9 +1244,11 @@ __global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uin
 // project mu over the runs, reduce to 1/exp(-p) per LOR, and backproject;
 // with mu == 0 everywhere every LOR has p = 0 exactly and r = 1, so the
 // forward half is skipped.  Unbounded walks go to the one-lane kernel.
+// produce(dst, first, count, stream) writes LORs first .. first + count - 1.
+using LorSource = std::function<void(Lor*, uint64_t, uint32_t, hipStream_t)>;
+
 void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const float* mu_host, const float* dmu,
-                const Lor* given, size_t n_lors, uint64_t seed, float L, float R, double* acc,
-                unsigned long long* cnt) {
+                uint64_t n_lors, const LorSource& produce, double* acc, unsigned long long* cnt) {
   const int32_t no_shape[3] = {0, 0, 0};
   const TileGeom g = make_tiles(fov, no_shape);
   const size_t na = nvox(fov);
@@ -1258,17 +1261,15 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   const int end_bit = (int)tshift + tile_bits;
   const DevTof notof = no_tof();
   ensure_lds_attributes();
-  const size_t bmax = std::min<size_t>(n_lors, kSensBatch);
+  const size_t bmax = (size_t)std::min<uint64_t>(n_lors, kSensBatch);
   DevBuf<Lor> lb(bmax), irr(bmax);
   DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1);
   DevBuf<float> rbuf;
   if (!mu_zero) rbuf.alloc(bmax);
-  for (size_t first = 0; first < n_lors; first += kSensBatch) {
-    const uint32_t b = (uint32_t)std::min<size_t>(kSensBatch, n_lors - first);
-    if (given)
-      HIP_CHECK(hipMemcpyAsync(lb.p, given + first, (size_t)b * sizeof(Lor), hipMemcpyDeviceToDevice, s));
-    else
-      k_sens_lors<<<grid_for(b), kBlock, 0, s>>>(b, (uint32_t)first, seed, f, L, R, lb.p);
+  for (uint64_t first = 0; first < n_lors; first += kSensBatch) {
+    const uint32_t b = (uint32_t)std::min<uint64_t>(kSensBatch, n_lors - first);
+    produce(lb.p, first, b, s);
+    HIP_CHECK(hipGetLastError());
     k_bin_runs<false, false><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr,
                                                              nullptr, shift, cnt);
     HIP_CHECK(hipMemsetAsync(n_irr.p, 0, 4, s));
@@ -1334,7 +1335,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       HIP_CHECK(hipStreamSynchronize(s));  // batch buffers are released at scope end
     }
     if (h_irr) {
-      k_sens<false, false><<<grid_for(h_irr), kBlock, 0, s>>>(irr.p, h_irr, seed, L, R, f, dmu, f, acc, cnt);
+      k_sens<false, false><<<grid_for(h_irr), kBlock, 0, s>>>(irr.p, h_irr, 0, 0.0f, 0.0f, f, dmu, f, acc, cnt);
       HIP_CHECK(hipGetLastError());
     }
   }
@@ -1343,6 +1344,86 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
 }  // namespace
 
 extern "C" {
+
+}  // extern "C"
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_pair_lors(uint64_t first, uint32_t n, DevElements e, uint64_t seed,
+                                                   Lor* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Lor l = pair_lor(e, seed, first + i);
+  float4* p = reinterpret_cast<float4*>(out + i);
+  p[0] = make_float4(l.x1, l.y1, l.z1, l.x2);
+  p[1] = make_float4(l.y2, l.z2, l.dt, l.ac);
+}
+
+}  // namespace
+
+extern "C" {
+
+int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const float* mu,
+                                           const petalo_fov_t* out_fov, float detector_length_mm, float r_min_mm,
+                                           float dr_mm, float dz_mm, float da_mm, int32_t adjust, uint64_t seed,
+                                           float* out, petalo_stats_t* stats) {
+  return guarded([&] {
+    std::string why;
+    if (!fov_ok(att_fov, why)) return fail(PETALO_ERR_ARG, why);
+    if (out_fov && !fov_ok(out_fov, why)) return fail(PETALO_ERR_ARG, why);
+    if (!mu || !out) return fail(PETALO_ERR_ARG, "NULL argument");
+    if (adjust < PETALO_ADJUST_NONE || adjust > PETALO_ADJUST_RANDOM_Z_PHI)
+      return fail(PETALO_ERR_ARG, "unknown element adjustment");
+    if (!(dr_mm > 0.0f && dz_mm > 0.0f && da_mm > 0.0f && r_min_mm >= 0.0f && detector_length_mm >= 0.0f))
+      return fail(PETALO_ERR_ARG, "element discretization needs positive dr, dz, da");
+    const DevElements el = make_elements(r_min_mm, dr_mm, dz_mm, da_mm, detector_length_mm, adjust);
+    const uint64_t npts = el.n_points;
+    if (npts < 2 || npts > (1ull << 31)) return fail(PETALO_ERR_ARG, "element count out of range");
+    const uint64_t n_pairs = npts * (npts + 1) / 2;  // i <= k, self pairs included (discrete.rs:55)
+    require_device();
+    const DevFov fa = to_dev(*att_fov);
+    const DevFov fo = to_dev(out_fov ? *out_fov : *att_fov);
+    const size_t na = nvox(*att_fov), no = nvox(out_fov ? *out_fov : *att_fov);
+    hipStream_t s;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    DevBuf<float> dmu(na), dout(no);
+    DevBuf<double> acc(no);
+    DevBuf<unsigned long long> cnt(CNT_N);
+    HIP_CHECK(hipMemcpyAsync(dmu.p, mu, na * sizeof(float), hipMemcpyHostToDevice, s));
+    fill_f64(acc.p, 0.0, no, s);
+    zero_counters(cnt.p, s);
+    LorSource src = [&](Lor* dst, uint64_t first, uint32_t n, hipStream_t st) {
+      k_pair_lors<<<grid_for(n), kBlock, 0, st>>>(first, n, el, seed, dst);
+    };
+    if (!out_fov) {
+      sens_tiled(s, *att_fov, fa, mu, dmu.p, n_pairs, src, acc.p, cnt.p);
+    } else {
+      // two grids: one lane per LOR, binary64 global atomics
+      DevBuf<Lor> lb((size_t)std::min<uint64_t>(n_pairs, kSensBatch));
+      for (uint64_t first = 0; first < n_pairs; first += kSensBatch) {
+        const uint32_t b = (uint32_t)std::min<uint64_t>(kSensBatch, n_pairs - first);
+        src(lb.p, first, b, s);
+        k_sens<false, true><<<grid_for(b), kBlock, 0, s>>>(lb.p, b, 0, 0.0f, 0.0f, fa, dmu.p, fo, acc.p, cnt.p);
+        HIP_CHECK(hipGetLastError());
+      }
+    }
+    // normalised by n (n - 1) / 2 over the element count (discrete.rs:16-19)
+    const double norm = (double)npts * (double)(npts - 1) / 2.0;
+    k_sens_finalize<<<grid_for(no), kBlock, 0, s>>>(acc.p, dout.p, (float)norm, no);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(out, dout.p, no * sizeof(float), hipMemcpyDeviceToHost, s));
+    unsigned long long h[CNT_N];
+    HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (stats)
+      *stats = petalo_stats_t{n_pairs, h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED], h[CNT_RUNAWAY]};
+    return PETALO_OK;
+  });
+}
 
 int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, const petalo_fov_t* out_fov,
                                   const petalo_lor_t* lors, size_t n_lors, uint64_t seed, float L, float D,
@@ -1382,7 +1463,14 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
       const uint32_t n = (uint32_t)n_lors;
       const float R = D / 2.0f;
       if (!two) {
-        sens_tiled(s, *att_fov, fa, mu, dmu.p, lors ? dl.p : nullptr, n_lors, seed, L, R, acc.p, cnt.p);
+        const float Rr = R;
+        LorSource src = lors ? LorSource([&](Lor* dst, uint64_t first, uint32_t cnt_, hipStream_t st) {
+          HIP_CHECK(hipMemcpyAsync(dst, dl.p + first, (size_t)cnt_ * sizeof(Lor), hipMemcpyDeviceToDevice, st));
+        })
+                             : LorSource([&](Lor* dst, uint64_t first, uint32_t cnt_, hipStream_t st) {
+          k_sens_lors<<<grid_for(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst);
+        });
+        sens_tiled(s, *att_fov, fa, mu, dmu.p, n_lors, src, acc.p, cnt.p);
       } else {
         // two FOVs: one lane per LOR, binary64 global atomics
         if (lors) k_sens<false, true><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);

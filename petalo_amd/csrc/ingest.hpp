@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include "philox.hpp"
+#include "siddon_walk.hpp"
 
 namespace petalo {
 
@@ -122,6 +123,74 @@ __host__ __device__ inline bool ingest_record(const DevIngest& p, uint64_t index
   adjust_point(p.d, l.x1, l.y1, l.z1, u1);
   adjust_point(p.d, l.x2, l.y2, l.z2, u2);
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// Detector elements (Discretize::centre_all_elements, discrete.rs:150-176):
+// n_azimuthal x (2 n_half_axial + 1) element centres, azimuth outer, axial
+// inner, at radius n_radial * dr.
+struct DevElements {
+  DevDiscretize d;
+  float r;
+  int32_t n_half_axial;
+  uint32_t n_axial, n_azimuthal;
+  uint64_t n_points;
+};
+
+__host__ __device__ inline DevElements make_elements(float r_min, float dr, float dz, float da, float length,
+                                                     int32_t adjust) {
+  DevElements e;
+  e.d = make_discretize(r_min, dr, dz, da, adjust);
+  e.r = e.d.n_radial * dr;
+  const float half_length = length / 2.0f;
+  e.n_half_axial = (int32_t)(uint32_t)roundf(half_length / dz);
+  e.n_axial = 2u * (uint32_t)e.n_half_axial + 1u;
+  const float rr = r_min + dr / 2.0f;
+  e.n_azimuthal = (uint32_t)roundf(kTwoPi * rr / da);
+  e.n_points = (uint64_t)e.n_axial * e.n_azimuthal;
+  return e;
+}
+
+__host__ __device__ inline void element_centre(const DevElements& e, uint64_t idx, float& x, float& y, float& z) {
+  const uint32_t az = (uint32_t)(idx / e.n_axial);
+  const int32_t n_z = (int32_t)(idx % e.n_axial) - e.n_half_axial;
+  const float phi = e.d.d_azimuthal * (float)az;
+  x = e.r * cosf(phi);
+  y = e.r * sinf(phi);
+  z = (float)n_z * e.d.dz;
+}
+
+// Pair t of make_lors (discrete.rs make_lors, bin/make_sensitivity_image/
+// discrete.rs:43-62): for i in 0..N, for k in i..N, t counts (i, k); the LOR
+// runs from element k to element i, both ends adjusted.
+__host__ __device__ inline void pair_of(uint64_t n, uint64_t t, uint64_t& i, uint64_t& k) {
+  auto row_start = [n](uint64_t r) { return r * n - (r * (r - 1)) / 2; };  // r*N - r(r-1)/2
+  const double b = 2.0 * (double)n + 1.0;
+  double g = (b - sqrt(b * b - 8.0 * (double)t)) / 2.0;
+  uint64_t r = g < 0.0 ? 0 : (uint64_t)g;
+  if (r >= n) r = n - 1;
+  while (r + 1 < n && row_start(r + 1) <= t) ++r;
+  while (r > 0 && row_start(r) > t) --r;
+  i = r;
+  k = r + (t - row_start(r));
+}
+
+constexpr uint32_t kPairTag = 0x9a1bu;
+
+__host__ __device__ inline Lor pair_lor(const DevElements& e, uint64_t seed, uint64_t t) {
+  uint64_t i, k;
+  pair_of(e.n_points, t, i, k);
+  float px, py, pz, qx, qy, qz;
+  element_centre(e, k, px, py, pz);
+  element_centre(e, i, qx, qy, qz);
+  const uint32_t lo = (uint32_t)t, hi = (uint32_t)(t >> 32);
+  const U32x4 a = philox(seed, lo, hi, 1, kPairTag);
+  const U32x4 b = philox(seed, lo, hi, 2, kPairTag);
+  const float up[3] = {unit_f32(a.x), unit_f32(a.y), unit_f32(b.x)};
+  const float uq[3] = {unit_f32(a.z), unit_f32(a.w), unit_f32(b.y)};
+  adjust_point(e.d, px, py, pz, up);
+  adjust_point(e.d, qx, qy, qz, uq);
+  return Lor{px, py, pz, qx, qy, qz, 0.0f, 1.0f};
 }
 
 // ---------------------------------------------------------------------------

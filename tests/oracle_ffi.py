@@ -218,6 +218,19 @@ def ingest(records, params, spec=None):
     return out[:m].copy(), int(n_read.value)
 
 
+def discrete_lors(r_min, dr, dz, da, length, adjust, seed=0, capacity=None):
+    """(LORs, number of elements) of a discretized detector, all pairs."""
+    L = lib()
+    L.orc_discrete_lors.restype = C.c_int64
+    L.orc_discrete_lors.argtypes = [C.c_float] * 5 + [C.c_int, C.c_uint64, C.c_void_p, C.c_size_t, C.c_void_p]
+    ne = C.c_uint64(0)
+    m = L.orc_discrete_lors(r_min, dr, dz, da, length, adjust, seed, None, 0, C.byref(ne))
+    cap = m if capacity is None else min(m, capacity)
+    out = np.zeros(cap, LOR_DTYPE)
+    L.orc_discrete_lors(r_min, dr, dz, da, length, adjust, seed, _p(out) if cap else None, cap, C.byref(ne))
+    return out, int(ne.value)
+
+
 def philox(ctr, key):
     c = np.asarray(ctr, np.uint32)
     k = np.asarray(key, np.uint32)

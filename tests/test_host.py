@@ -236,6 +236,19 @@ def test_ingest_argument_checks_come_first():
     assert n == 0 and len(out) == 0  # empty event range: nothing to do, no device needed
 
 
+def test_discrete_sensitivity_argument_checks_come_first():
+    fov = P.FOV([60.0] * 3, [6] * 3)
+    mu = P.Image(fov, np.zeros(fov.nvox, np.float32))
+    bad = [dict(r_min=100.0, dr=10.0, dz=10.0, da=10.0, adjust=9),
+           dict(r_min=100.0, dr=0.0, dz=10.0, da=10.0),
+           dict(r_min=100.0, dr=10.0, dz=-1.0, da=10.0),
+           dict(r_min=0.0, dr=1.0, dz=10.0, da=1000.0)]  # one element: no pairs
+    for kw in bad:
+        with pytest.raises(P.PetaloError) as e:
+            P.make_sensitivity_image_discrete(mu, 10.0, **kw)
+        assert e.value.code == 1, kw
+
+
 def test_tools_help():
     for tool in ("mlem", "make_sensitivity_image"):
         exe = os.path.join(ROOT, "petalo_amd", "bin", tool)
