@@ -256,11 +256,11 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu,
 /* Sensitivity image of a discretized detector (discrete::sensitivity_image,
  * src/bin/make_sensitivity_image/discrete.rs:3-62): element centres of a
  * ring of inner radius r_min, thickness dr, element width dz x da (at
- * r_min + dr/2), axial length detector_length (discrete.rs:150-176); one LOR
+ * r_min + dr/2), axial length detector_length (discrete.rs:150-171); one LOR
  * per element pair i <= k, self pairs included, each end adjusted inside its
  * element (PETALO_ADJUST_*, draws from Philox keyed by seed and counted by
  * pair index); projected as the continuous image; normalised by n (n - 1) / 2
- * over the n elements (discrete.rs:16-19, which counts neither self pairs nor
+ * over the n elements (discrete.rs:16-20, which counts neither self pairs nor
  * the FOV filter).  stats->n_lors = pairs. */
 int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const float* mu,
                                            const petalo_fov_t* out_fov_or_null, float detector_length_mm,
@@ -288,10 +288,13 @@ int petalo_image3d_read_header(const char* path, petalo_fov_t* fov_out);
 int petalo_image3d_read(const char* path, petalo_fov_t* fov_out, float* data, size_t capacity);
 
 /* ---- MLEM config TOML (src/config/mlem.rs:73-253) ----------------------
- * Supported tables: [input] (file, dataset, energy/charge/events bounds),
- * [iterations], [fov], [tof], [attenuation_correction]; [scatter_correction],
- * [smear_energy] and [detector_full_axial_length] are parsed and checked but
- * not used on this path.  Unknown keys are rejected (deny_unknown_fields). */
+ * Tables: [input] (file, dataset, energy/charge/events bounds),
+ * [iterations], [fov], [tof], [attenuation_correction], [scatter_correction]
+ * (axes as petalo_scattergram_spec_t; all zero when absent), [smear_energy]
+ * (fwhm as a ratio) and [detector_full_axial_length] (dz in mm).  Unknown
+ * keys are rejected (deny_unknown_fields).  petalo_config_ingest maps a
+ * config onto the ingestion parameters as io::hdf5::read_lors does
+ * (hdf5.rs:97-130, 240-262). */
 typedef struct {
   char input_file[512];
   char input_dataset[256];
@@ -306,10 +309,22 @@ typedef struct {
   float energy_min, energy_max, charge_min, charge_max;
   uint64_t events_min, events_max;
   int32_t has_scatter_correction;
+  petalo_scattergram_spec_t scatter;
+  int32_t has_smear_energy;
+  float smear_energy_fwhm;
+  int32_t has_detector_full_axial_length;
+  float detector_full_axial_length_mm;
 } petalo_config_t;
 
 int petalo_config_parse(const char* toml_text, petalo_config_t* out);
 int petalo_config_read(const char* path, petalo_config_t* out);
+/* Ingestion parameters of read_lors for this config: event range, charge and
+ * energy bounds, smearing FWHM (the reference requires [smear_energy]:
+ * PETALO_ERR_ARG without it), |z| < dz/2 of [detector_full_axial_length],
+ * element discretization (r_min, dr, dz, da — the dataset's attributes) with
+ * the RandomZPhi adjustment (hdf5.rs:247-253), the smearing seed. */
+int petalo_config_ingest(const petalo_config_t* config, float r_min_mm, float dr_mm, float dz_mm, float da_mm,
+                         uint64_t seed, petalo_ingest_t* out);
 /* Parses a quantity string such as "200 ps", "3 cm", "1.5 mm" into base
  * units (mm or ps); kind: 'L' length, 'T' time. */
 int petalo_parse_quantity(const char* text, char kind, float* value);

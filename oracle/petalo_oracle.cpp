@@ -752,7 +752,7 @@ int64_t orc_discrete_lors(float r_min, float dr, float dz, float da, float lengt
   p.da_mm = da;
   p.adjust = adjust;
   const Disc d = disc_of(p);
-  // centre_all_elements (discrete.rs:150-176)
+  // centre_all_elements (discrete.rs:150-171)
   const float tau = 6.2831855f;
   const float rr = r_min + (dr / 2.0f);
   const uint32_t n_az = (uint32_t)std::round((tau * rr) / da);

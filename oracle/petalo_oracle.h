@@ -175,7 +175,7 @@ int64_t orc_ingest_f32(const float* records, size_t n_records, const orc_ingest_
                        const orc_sgram_spec_t* sgram_or_null, orc_lor_t* out, uint64_t* n_read);
 
 /* LORs of a discretized detector (bin/make_sensitivity_image/discrete.rs:
- * 23-62, src/discrete.rs:150-176): element centres azimuth-major, pairs
+ * 23-62, src/discrete.rs:150-171): element centres azimuth-major, pairs
  * (i, k) for i in 0..n, k in i..n, LOR from element k to element i, each end
  * adjusted (0 none, 1 centre, 2 random, 3 random z and phi) with draws from
  * Philox key (seed lo, seed hi), counter (pair lo, pair hi, block, 0x9a1b),

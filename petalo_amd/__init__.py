@@ -142,7 +142,9 @@ class _Config(C.Structure):
         ("energy_min", C.c_float), ("energy_max", C.c_float),
         ("charge_min", C.c_float), ("charge_max", C.c_float),
         ("events_min", C.c_uint64), ("events_max", C.c_uint64),
-        ("has_scatter_correction", C.c_int32),
+        ("has_scatter_correction", C.c_int32), ("scatter", _SgramSpec),
+        ("has_smear_energy", C.c_int32), ("smear_energy_fwhm", C.c_float),
+        ("has_detector_full_axial_length", C.c_int32), ("detector_full_axial_length_mm", C.c_float),
     ]
 
 
@@ -191,6 +193,7 @@ _SIGS = {
     "petalo_synth_density": (C.c_int, [_P, _P]),
     "petalo_philox4x32_10": (C.c_int, [_P, _P, _P]),
     "petalo_ingest_lors": (C.c_int, [_P, C.c_size_t, _P, _P, _P, C.c_size_t, _P, _P]),
+    "petalo_config_ingest": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint64, _P]),
     "petalo_make_sensitivity_image_discrete": (C.c_int, [_P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float,
                                                          C.c_float, C.c_int32, C.c_uint64, _P, _P]),
 }
@@ -634,6 +637,27 @@ class Config:
         self.charge = (b(c.has_charge_min, float(c.charge_min)), b(c.has_charge_max, float(c.charge_max)))
         self.events = (b(c.has_events_min, int(c.events_min)), b(c.has_events_max, int(c.events_max)))
         self.has_scatter_correction = bool(c.has_scatter_correction)
+        g = c.scatter
+        self.scatter_correction = ({
+            "phi_bins": g.phi_bins, "z_bins": g.z_bins, "z_length_mm": g.z_length_mm, "dz_bins": g.dz_bins,
+            "dz_max_mm": g.dz_max_mm, "r_bins": g.r_bins, "r_max_mm": g.r_max_mm, "dt_bins": g.dt_bins,
+            "dt_max_ps": g.dt_max_ps} if c.has_scatter_correction else None)
+        self.smear_energy_fwhm = float(c.smear_energy_fwhm) if c.has_smear_energy else None
+        self.detector_full_axial_length = (float(c.detector_full_axial_length_mm)
+                                           if c.has_detector_full_axial_length else None)
+        self._c = c
+
+    def ingest_params(self, discretize, seed=0):
+        """Ingestion parameters of read_lors for this config (hdf5.rs:97-130,
+        240-262); ``discretize`` = the dataset's (r_min, dr, dz, da) in mm."""
+        p = _Ingest()
+        _check(lib().petalo_config_ingest(C.byref(self._c), *(float(v) for v in discretize), int(seed),
+                                          C.byref(p)))
+        return p
+
+    def scattergram_spec(self):
+        """The [scatter_correction] axes, or None."""
+        return self._c.scatter if self.has_scatter_correction else None
 
 
 def parse_config(text):

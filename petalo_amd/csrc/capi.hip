@@ -1411,7 +1411,7 @@ int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const fl
         HIP_CHECK(hipGetLastError());
       }
     }
-    // normalised by n (n - 1) / 2 over the element count (discrete.rs:16-19)
+    // normalised by n (n - 1) / 2 over the element count (discrete.rs:16-20)
     const double norm = (double)npts * (double)(npts - 1) / 2.0;
     k_sens_finalize<<<grid_for(no), kBlock, 0, s>>>(acc.p, dout.p, (float)norm, no);
     HIP_CHECK(hipGetLastError());

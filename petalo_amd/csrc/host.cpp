@@ -454,20 +454,31 @@ void config_from_table(const Table& root, petalo_config_t& c) {
       deny_unknown(st, s.keys, std::string("scatter_correction.") + s.name);
       for (const auto& k : s.keys)
         if (!get(st, k.c_str())) throw ParseError(std::string("missing field '") + k + "' in scatter_correction");
-      as_uint(get(st, "bins"), "bins");
-      if (s.qty) quantity(as_str(get(st, s.qty), s.qty), s.kind);
+      const uint64_t bins = as_uint(get(st, "bins"), "bins");
+      if (bins >= (1ull << 31)) throw ParseError("scatter_correction: too many bins");
+      const float q = s.qty ? quantity(as_str(get(st, s.qty), s.qty), s.kind) : 0.0f;
+      const std::string n = s.name;
+      petalo_scattergram_spec_t& g = c.scatter;
+      if (n == "phi") g.phi_bins = (uint32_t)bins;
+      else if (n == "r") g.r_bins = (uint32_t)bins, g.r_max_mm = q;
+      else if (n == "dz") g.dz_bins = (uint32_t)bins, g.dz_max_mm = q;
+      else if (n == "dt") g.dt_bins = (uint32_t)bins, g.dt_max_ps = q;
+      else g.z_bins = (uint32_t)bins, g.z_length_mm = q;
     }
     c.has_scatter_correction = 1;
   }
   if (const Value* se = get(root, "smear_energy")) {
     const Table& t = as_table(se, "smear_energy");
     deny_unknown(t, {"fwhm"}, "smear_energy");
-    quantity(as_str(get(t, "fwhm"), "smear_energy.fwhm"), 'R');
+    if (!get(t, "fwhm")) throw ParseError("missing field 'fwhm' in smear_energy");
+    c.smear_energy_fwhm = quantity(as_str(get(t, "fwhm"), "smear_energy.fwhm"), 'R');
+    c.has_smear_energy = 1;
   }
   if (const Value* dl = get(root, "detector_full_axial_length")) {
     const Table& t = as_table(dl, "detector_full_axial_length");
     deny_unknown(t, {"dz"}, "detector_full_axial_length");
-    quantity(as_str(get(t, "dz"), "detector_full_axial_length.dz"), 'L');
+    c.detector_full_axial_length_mm = quantity(as_str(get(t, "dz"), "detector_full_axial_length.dz"), 'L');
+    c.has_detector_full_axial_length = 1;
   }
 }
 
@@ -609,6 +620,40 @@ int petalo_config_parse(const char* text, petalo_config_t* out) {
     config_from_table(root, *out);
     return PETALO_OK;
   });
+}
+
+int petalo_config_ingest(const petalo_config_t* c, float r_min_mm, float dr_mm, float dz_mm, float da_mm,
+                         uint64_t seed, petalo_ingest_t* out) {
+  if (!c || !out) return fail(PETALO_ERR_ARG, "NULL argument");
+  // read_hdf5_lors unwraps config.smear_energy (hdf5.rs:107-108)
+  if (!c->has_smear_energy) return fail(PETALO_ERR_ARG, "the config has no [smear_energy] table");
+  petalo_ingest_t p;
+  std::memset(&p, 0, sizeof p);
+  p.has_events_min = c->has_events_min;
+  p.has_events_max = c->has_events_max;
+  p.events_min = c->events_min;
+  p.events_max = c->events_max;
+  p.has_charge_min = c->has_charge_min;
+  p.has_charge_max = c->has_charge_max;
+  p.charge_min = c->charge_min;
+  p.charge_max = c->charge_max;
+  p.has_energy_min = c->has_energy_min;
+  p.has_energy_max = c->has_energy_max;
+  p.energy_min = c->energy_min;
+  p.energy_max = c->energy_max;
+  p.smear_energy_fwhm = c->smear_energy_fwhm;
+  // z_max = detector_full_axial_length.dz / 2 (hdf5.rs:99)
+  p.has_z_max = c->has_detector_full_axial_length;
+  p.z_max_mm = c->detector_full_axial_length_mm / 2.0f;
+  // smear_positions: the dataset's discretization, Adjust::RandomZPhi (hdf5.rs:240-253)
+  p.r_min_mm = r_min_mm;
+  p.dr_mm = dr_mm;
+  p.dz_mm = dz_mm;
+  p.da_mm = da_mm;
+  p.adjust = PETALO_ADJUST_RANDOM_Z_PHI;
+  p.seed = seed;
+  *out = p;
+  return PETALO_OK;
 }
 
 int petalo_config_read(const char* path, petalo_config_t* out) {

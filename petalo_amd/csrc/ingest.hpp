@@ -126,7 +126,7 @@ __host__ __device__ inline bool ingest_record(const DevIngest& p, uint64_t index
 }
 
 // ---------------------------------------------------------------------------
-// Detector elements (Discretize::centre_all_elements, discrete.rs:150-176):
+// Detector elements (Discretize::centre_all_elements, discrete.rs:150-171):
 // n_azimuthal x (2 n_half_axial + 1) element centres, azimuth outer, axial
 // inner, at radius n_radial * dr.
 struct DevElements {

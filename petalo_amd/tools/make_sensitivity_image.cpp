@@ -4,13 +4,18 @@
 //   make_sensitivity_image -i density.raw [-o out.raw] -l LENGTH -r RADIUS
 //       [--rho-to-mu 0.095] [--output-image-dims '10 cm 20 cm 255 mm 30 30 35']
 //       [--seed S] continuous N_LORS
+//   make_sensitivity_image ... discrete --dr DR --dz DZ --da DA
+//       --adjust no|centre|random|random-z-phi
 //
 // The density image [kg/m^3] becomes mu [1/mm]; N_LORS random LORs with both
 // ends on the detector cylinder (length, inner radius) that enter the FOV
 // are backprojected with weights exp(+integral of mu) and the image is
 // normalised by N_LORS.  The LORs come from a seeded counter-based stream
 // (the reference draws them from an unseeded generator).  The discrete
-// detector model is not part of this library.
+// model backprojects every pair of detector-element centres (elements of
+// DR x DZ x DA at inner radius RADIUS), both ends adjusted inside their
+// element, normalised by n(n-1)/2 over the n elements
+// (src/bin/make_sensitivity_image/discrete.rs:3-62).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -38,7 +43,9 @@ int main(int argc, char** argv) {
   float L = 0, R = 0, rho_to_mu = 0.095f;
   uint64_t seed = 0;
   size_t n_lors = 0;
-  bool have_type = false;
+  bool have_type = false, discrete = false;
+  float dr = 0, dz = 0, da = 0;
+  int32_t adjust = -1;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto val = [&]() -> std::string {
@@ -52,7 +59,8 @@ int main(int argc, char** argv) {
     };
     if (a == "-h" || a == "--help") {
       std::printf("Create sensitivity image from a density image\n\nusage: make_sensitivity_image -i IN [-o OUT] "
-                  "-l LENGTH -r RADIUS [--rho-to-mu X] [--output-image-dims DIMS] [--seed S] continuous N\n");
+                  "-l LENGTH -r RADIUS [--rho-to-mu X] [--output-image-dims DIMS] [--seed S]\n"
+                  "       (continuous N | discrete --dr DR --dz DZ --da DA --adjust no|centre|random|random-z-phi)\n");
       return 0;
     }
     if (a == "-i" || a == "--input") in = val();
@@ -66,10 +74,26 @@ int main(int argc, char** argv) {
     else if (a == "continuous") {
       n_lors = std::strtoull(val().c_str(), nullptr, 10);
       have_type = true;
-    } else if (a == "discrete") die("the discrete detector model is not supported");
-    else die("unknown argument " + a);
+    } else if (a == "discrete") {
+      have_type = discrete = true;
+    } else if (discrete && a == "--dr") dr = len(val());
+    else if (discrete && a == "--dz") dz = len(val());
+    else if (discrete && a == "--da") da = len(val());
+    else if (discrete && a == "--adjust") {
+      // clap's value names of discrete::Adjust (src/discrete.rs:24-41)
+      const std::string v = val();
+      adjust = v == "no" ? PETALO_ADJUST_NONE
+               : v == "centre" ? PETALO_ADJUST_CENTRE
+               : v == "random" ? PETALO_ADJUST_RANDOM
+               : v == "random-z-phi" ? PETALO_ADJUST_RANDOM_Z_PHI
+                                     : -1;
+      if (adjust < 0) die("unknown --adjust " + v + " (no, centre, random, random-z-phi)");
+    } else
+      die("unknown argument " + a);
   }
-  if (in.empty() || !(L > 0) || !(R > 0) || !have_type) die("need -i, -l, -r and 'continuous N'");
+  if (in.empty() || !(L > 0) || !(R > 0) || !have_type) die("need -i, -l, -r and 'continuous N' or 'discrete ...'");
+  if (discrete && (!(dr > 0) || !(dz > 0) || !(da > 0) || adjust < 0))
+    die("discrete needs --dr, --dz, --da and --adjust");
 
   auto t0 = std::chrono::steady_clock::now();
   auto report = [&](const std::string& msg) {
@@ -92,12 +116,19 @@ int main(int argc, char** argv) {
   if (!dims.empty()) check(petalo_fov_parse(dims.c_str(), &ofov), "parsing --output-image-dims");
   const size_t no = (size_t)ofov.n[0] * ofov.n[1] * ofov.n[2];
   std::vector<float> sens(no);
-  std::printf("Creating sensitivity image, using %zu LORs ... ", n_lors);
-  std::fflush(stdout);
   petalo_stats_t st;
-  check(petalo_make_sensitivity_image(&fov, mu.data(), dims.empty() ? nullptr : &ofov, nullptr, n_lors, seed, L,
-                                      R * 2.0f, sens.data(), &st),
-        "sensitivity image");
+  if (discrete) {
+    check(petalo_make_sensitivity_image_discrete(&fov, mu.data(), dims.empty() ? nullptr : &ofov, L, R, dr, dz, da,
+                                                 adjust, seed, sens.data(), &st),
+          "sensitivity image");
+    std::printf("%llu LORs between detector elements\n", (unsigned long long)st.n_lors);
+  } else {
+    std::printf("Creating sensitivity image, using %zu LORs ... ", n_lors);
+    std::fflush(stdout);
+    check(petalo_make_sensitivity_image(&fov, mu.data(), dims.empty() ? nullptr : &ofov, nullptr, n_lors, seed, L,
+                                        R * 2.0f, sens.data(), &st),
+          "sensitivity image");
+  }
   report("done");
   check(petalo_image3d_write(out.c_str(), &ofov, sens.data()), "writing sensitivity image");
   report("Wrote sensitivity image to \"" + out + "\"");

@@ -1,6 +1,7 @@
 // mlem -- MLEM/OSEM reconstruction driver over the C ABI (src/bin/mlem.rs:47-105).
 //
-//   mlem <config.toml> <output-directory> [--lors FILE | --synthetic N [--seed S] [--tof-fwhm PS]]
+//   mlem <config.toml> <output-directory>
+//        [--lors FILE | --records FILE --discretize "R_MIN DR DZ DA" | --synthetic N [--tof-fwhm PS]] [--seed S]
 //
 // Reads the config, copies it to <output-directory>/mlem-config.toml, loads
 // the sensitivity image named in [attenuation_correction] (sizes must match
@@ -8,8 +9,13 @@
 // updates and writes each image as "<output-directory><it:02>-<sub:02>.raw"
 // (Image3D).  As in the reference, the directory string is concatenated
 // without a separator.  LORs come from a file of 32-byte records
-// (petalo_lor_t) or from the seeded synthetic phantom; HDF5 ingestion is not
-// part of this library.
+// (petalo_lor_t), from the seeded synthetic phantom, or from the raw bytes of
+// the reco_info/lors dataset (44-byte Hdf5Lor records, e.g. written by h5py's
+// `f["reco_info/lors"][:].tofile(path)`): these go through the config's cuts,
+// energy smearing, RandomZPhi position smearing with the dataset's
+// discretization (--discretize, its r_min/dr/dz/da attributes in mm) and
+// scatter correction, as io::hdf5::read_lors does (src/io/hdf5.rs:186-228);
+// the HDF5 file read itself is not part of this library.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -67,12 +73,12 @@ int main(int argc, char** argv) {
   const bool help = argc >= 2 && (!std::strcmp(argv[1], "-h") || !std::strcmp(argv[1], "--help"));
   if (argc < 3 || help) {
     std::printf("Maximum Likelyhood Expectation Maximization\n\n"
-                "usage: mlem <config.toml> <output-directory> [--lors FILE | --synthetic N [--seed S] "
-                "[--tof-fwhm PS]]\n");
+                "usage: mlem <config.toml> <output-directory> [--lors FILE | --records FILE --discretize "
+                "\"R_MIN DR DZ DA\" | --synthetic N [--tof-fwhm PS]] [--seed S]\n");
     return help ? 0 : 1;
   }
   const std::string cfg_path = argv[1], outdir = argv[2];
-  std::string lor_file;
+  std::string lor_file, record_file, discretize;
   size_t synth_n = 0;
   uint64_t seed = 1;
   float fwhm = 0.0f;
@@ -83,6 +89,8 @@ int main(int argc, char** argv) {
       return argv[++i];
     };
     if (a == "--lors") lor_file = val();
+    else if (a == "--records") record_file = val();
+    else if (a == "--discretize") discretize = val();
     else if (a == "--synthetic") synth_n = std::strtoull(val().c_str(), nullptr, 10);
     else if (a == "--seed") seed = std::strtoull(val().c_str(), nullptr, 10);
     else if (a == "--tof-fwhm") fwhm = std::strtof(val().c_str(), nullptr);
@@ -133,11 +141,33 @@ int main(int argc, char** argv) {
     lors.resize(bytes / sizeof(petalo_lor_t));
     f.seekg(0);
     f.read(reinterpret_cast<char*>(lors.data()), (std::streamsize)bytes);
+  } else if (!record_file.empty()) {
+    float d[4];
+    if (std::sscanf(discretize.c_str(), "%f %f %f %f", d, d + 1, d + 2, d + 3) != 4)
+      die("--records needs --discretize \"R_MIN DR DZ DA\" (mm)");
+    std::ifstream f(record_file, std::ios::binary | std::ios::ate);
+    if (!f) die("cannot read " + record_file);
+    const size_t bytes = (size_t)f.tellg();
+    if (bytes % 44) die("record file size is not a multiple of 44 bytes");
+    std::vector<float> rec(bytes / 4);
+    f.seekg(0);
+    f.read(reinterpret_cast<char*>(rec.data()), (std::streamsize)bytes);
+    petalo_ingest_t ip;
+    check(petalo_config_ingest(&cfg, d[0], d[1], d[2], d[3], seed, &ip), "ingestion parameters");
+    const size_t n_rec = bytes / 44;
+    lors.resize(n_rec);
+    size_t n_out = 0;
+    uint64_t n_read = 0;
+    check(petalo_ingest_lors(rec.data(), n_rec, &ip, cfg.has_scatter_correction ? &cfg.scatter : nullptr,
+                             lors.data(), lors.size(), &n_out, &n_read),
+          "ingesting LORs");
+    lors.resize(n_out);
+    std::printf("   Reading LORs: %llu read, %zu selected\n", (unsigned long long)n_read, n_out);
   } else if (synth_n) {
     lors.resize(synth_n);
     check(petalo_synth_phantom_lors(seed, synth_n, fwhm, lors.data()), "synthetic LORs");
   } else {
-    die("no LORs: give --lors FILE or --synthetic N");
+    die("no LORs: give --lors FILE, --records FILE or --synthetic N");
   }
   progress.done("Loaded " + std::to_string(lors.size()) + " LORs");
 

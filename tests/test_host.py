@@ -44,6 +44,50 @@ def test_config_file_fixture():
     assert c.sensitivity_image == exp["sensitivity_image"]
     assert c.energy == (434.0, 600.0) and c.charge == (123.0, 456.0) and c.events == (100000, 200000)
     assert c.has_scatter_correction
+    # the fixture's [scatter_correction] table
+    assert c.scatter_correction == {"phi_bins": 30, "r_bins": 31, "r_max_mm": 32.0, "dz_bins": 33, "dz_max_mm": 34.0,
+                                    "dt_bins": 35, "dt_max_ps": 36.0, "z_bins": 37, "z_length_mm": 380.0}
+    assert c.smear_energy_fwhm is None and c.detector_full_axial_length is None
+
+
+def test_config_scattergram_kat():
+    """config_scattergram (src/config/mlem.rs:474-510)."""
+    c = P.parse_config(FULL + """
+[scatter_correction]
+phi.bins = 12
+r  .bins = 34
+dz .bins = 99
+dt .bins = 98
+z  .bins = 97
+
+r  .max  = "56 mm"
+dz .max  = "78 mm"
+dt .max  = "90 ps"
+z.length = "38 cm"
+""")
+    assert c.scatter_correction == {"phi_bins": 12, "r_bins": 34, "r_max_mm": 56.0, "dz_bins": 99, "dz_max_mm": 78.0,
+                                    "dt_bins": 98, "dt_max_ps": 90.0, "z_bins": 97, "z_length_mm": 380.0}
+    part = P.parse_config(FULL + '[scatter_correction]\nphi.bins = 7\n')
+    assert part.scatter_correction["phi_bins"] == 7 and part.scatter_correction["r_bins"] == 0
+    with pytest.raises(P.PetaloError):  # BinsMax needs both fields
+        P.parse_config(FULL + '[scatter_correction]\nr.bins = 7\n')
+
+
+def test_config_to_ingestion_parameters():
+    """read_lors's use of the config (hdf5.rs:97-130, 240-253): bounds pass
+    through, z_max = axial length / 2, RandomZPhi with the dataset's
+    discretization; the smearing table is mandatory."""
+    text = open(os.path.join(GOLDEN, "mlem-config.toml")).read()
+    with pytest.raises(P.PetaloError):
+        P.parse_config(text).ingest_params((350.0, 30.0, 3.0, 3.0))
+    c = P.parse_config(text + '\n[smear_energy]\nfwhm = "15 %"\n[detector_full_axial_length]\ndz = "1 m"\n')
+    assert abs(c.smear_energy_fwhm - 0.15) < 1e-7 and c.detector_full_axial_length == 1000.0
+    p = c.ingest_params((350.0, 30.0, 3.0, 3.0), seed=9)
+    assert (p.has_events_min, p.has_events_max, p.events_min, p.events_max) == (1, 1, 100000, 200000)
+    assert (p.energy_min, p.energy_max, p.charge_min, p.charge_max) == (434.0, 600.0, 123.0, 456.0)
+    assert p.has_z_max == 1 and p.z_max_mm == 500.0 and abs(p.smear_energy_fwhm - 0.15) < 1e-7
+    assert (p.r_min_mm, p.dr_mm, p.dz_mm, p.da_mm) == (350.0, 30.0, 3.0, 3.0)
+    assert p.adjust == P.ADJUST_RANDOM_Z_PHI and p.seed == 9
 
 
 def test_config_fov_units():
