@@ -250,6 +250,29 @@ def main():
                   "records_per_s": m / (ims * 1e-3)}
         log(f"[bench] ingestion of {m} records: {ims:.1f} ms with host transfers ({len(got)} LORs kept)")
 
+    # NEMA7 figures of merit (457 regions) of a noisy 2 mm image, 130 x 130 x 91
+    foms = None
+    if not a.no_sensitivity:
+        nema = P.FOV([260.0, 260.0, 182.0], [130, 130, 91])
+        hw = np.float32(130.0), np.float32(91.0)
+        ax = (np.arange(130, dtype=np.float32) + np.float32(0.5)) * np.float32(2.0) - hw[0]
+        az = (np.arange(91, dtype=np.float32) + np.float32(0.5)) * np.float32(2.0) - hw[1]
+        zz, yy, xx = np.meshgrid(az, ax, ax, indexing="ij")
+        act = np.ones(zz.shape, np.float32)
+        act[xx ** 2 + yy ** 2 < 256.0] = 0.25
+        for d, k in zip([10.0, 13.0, 17.0, 22.0, 28.0, 37.0], [1, 2, 3, 4, 5, 0]):
+            sx, sy = 57.2 * np.cos(np.pi * k / 3), 57.2 * np.sin(np.pi * k / 3)
+            act[(xx - sx) ** 2 + (yy - sy) ** 2 + zz ** 2 < (d / 2) ** 2] = 4.0
+        act *= np.random.default_rng(2).gamma(10.0, 0.1, act.shape).astype(np.float32)
+        img = P.Image(nema, act.ravel())
+        P.phantom_foms(img, P.PHANTOM_NEMA7)
+        t0 = time.perf_counter()
+        fv, aocs = P.phantom_foms(img, P.PHANTOM_NEMA7)
+        fms = (time.perf_counter() - t0) * 1e3
+        foms = {"phantom": "nema7", "voxels": nema.nvox, "regions": 6 * 66 + len(aocs), "ms_with_transfers": fms,
+                "crc": [round(float(c), 2) for c in fv[:, 1]]}
+        log(f"[bench] NEMA7 figures of merit ({foms['regions']} regions, {nema.nvox} voxels): {fms:.2f} ms")
+
     cpu = None
     parity = None
     if not a.no_cpu_baseline:
@@ -312,6 +335,7 @@ def main():
         "discrete_sensitivity": discrete,
         "parity_rel_l2_sample": parity,
         "ingest": ingest,
+        "foms": foms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get(dom), "kernel": dom,
                      "algorithmic_bytes": dom_bytes, "kernel_ms": dom_ms},

@@ -329,6 +329,63 @@ int petalo_config_ingest(const petalo_config_t* config, float r_min_mm, float dr
  * units (mm or ps); kind: 'L' length, 'T' time. */
 int petalo_parse_quantity(const char* text, char kind, float* value);
 
+/* ---- Figures of merit (src/fom.rs:20-466, src/bin/foms.rs) --------------
+ * Regions of interest (fom.rs ROI): a voxel belongs to a region when its
+ * centre (fov.rs:42-53) satisfies, in binary32 as written (fom.rs:34-63):
+ *   SPHERE      (x-c0)^2 + (y-c1)^2 + (z-c2)^2 < r^2
+ *   CYLINDER_X  (y-c0)^2 + (z-c1)^2 < r^2
+ *   CYLINDER_Y  (x-c0)^2 + (z-c1)^2 < r^2
+ *   CYLINDER_Z  (x-c0)^2 + (y-c1)^2 < r^2
+ *   DISC_Z      z == c2 exactly and (x-c0)^2 + (y-c1)^2 < r^2 */
+#define PETALO_ROI_SPHERE     0
+#define PETALO_ROI_CYLINDER_X 1
+#define PETALO_ROI_CYLINDER_Y 2
+#define PETALO_ROI_CYLINDER_Z 3
+#define PETALO_ROI_DISC_Z     4
+typedef struct {
+  int32_t kind;  /* PETALO_ROI_* */
+  float c[3];    /* centre coordinates as listed above, mm (unused ones ignored) */
+  float r_mm;
+} petalo_roi_t;
+/* Per region: voxel count, mean and population standard deviation of the
+ * voxel values (fom::mean, fom::mu_and_sigma, fom.rs:309-321), accumulated
+ * in FP64 (the reference sums binary32 values in voxel order). */
+typedef struct {
+  uint64_t count;
+  double mean;
+  double sigma;
+} petalo_roi_stats_t;
+/* Statistics of n regions of an image (host pointers); one workgroup per
+ * region over the region's bounding box of voxels.  An empty region has
+ * count 0 and mean = sigma = NaN. */
+int petalo_roi_stats(const petalo_fov_t* fov, const float* image, const petalo_roi_t* rois, size_t n_rois,
+                     petalo_roi_stats_t* out);
+/* Image::foms (fom.rs:424-450): background = mean over background regions of
+ * their means; per region crc (fom.rs:452-461, percent, hot when the
+ * region's activity exceeds the background's) and snr = (mean - background) /
+ * sigma.  PETALO_ERR_ARG if a region holds no voxel (the reference panics). */
+int petalo_image_foms(const petalo_fov_t* fov, const float* image, const petalo_roi_t* rois,
+                      const float* activities, size_t n_rois, const petalo_roi_t* background_rois,
+                      size_t n_background, float background_activity, float* crcs_out, float* snrs_out,
+                      float* background_measured_out);
+/* The foms tool's phantom analyses (src/bin/foms.rs): six hot spheres (4:1)
+ * in discs on the slice of the sphere centres, background discs of each
+ * sphere's radius on the phantom's background slices; per sphere (in the
+ * reference's order) radius, crc %, background variability % and snr %.
+ * NEMA7 also gives the lung-insert accuracy of corrections per slice
+ * (foms.rs 7.4.2: mean in a 15 mm disc / the 37 mm sphere's crc, x 100),
+ * written up to aoc_capacity values, *n_aocs_out = how many there are. */
+#define PETALO_PHANTOM_NEMA7    0
+#define PETALO_PHANTOM_JASZCZAK 1
+typedef struct {
+  float r_mm;
+  float crc;
+  float bg_variability;
+  float snr;
+} petalo_fom_t;
+int petalo_phantom_foms(const petalo_fov_t* fov, const float* image, int32_t phantom, petalo_fom_t foms_out[6],
+                        float* aocs_out, size_t aoc_capacity, size_t* n_aocs_out);
+
 /* ---- Synthetic workloads (seeded, Philox4x32-10 counter-based) ---------
  * Water cylinder (r 80 mm, |z| <= 75 mm, activity 1) with six hot spheres
  * (activity 4, radii 5..15 mm on a 50 mm ring), isotropic back-to-back

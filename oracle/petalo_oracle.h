@@ -16,6 +16,7 @@
  *   MLEM update     src/mlem.rs:33-66
  *   sensitivity     src/bin/make_sensitivity_image/continuous.rs:3-17,
  *                   main.rs:77-94
+ *   FOMs            src/fom.rs:34-461, src/bin/foms.rs:40-255 (fom_oracle.cpp)
  * plus an independent classic Siddon (parametric alpha merge, Siddon 1985)
  * and a brute-force per-voxel clipper, both in binary64, used only to pin
  * the restatement against the mathematics.
@@ -184,6 +185,25 @@ int64_t orc_ingest_f32(const float* records, size_t n_records, const orc_ingest_
  * element count in *n_elements. */
 int64_t orc_discrete_lors(float r_min_mm, float dr_mm, float dz_mm, float da_mm, float length_mm, int adjust,
                           uint64_t seed, orc_lor_t* out, size_t capacity, uint64_t* n_elements);
+
+/* ---- Figures of merit (fom_oracle.cpp; src/fom.rs, src/bin/foms.rs) ----
+ * Region kinds: 0 sphere (c = centre), 1 cylinder x (c = y, z), 2 cylinder
+ * y (c = x, z), 3 cylinder z (c = x, y), 4 disc z (c = x, y, z).  Statistics
+ * in binary32 in the reference's order (f64 = 0) or in binary64 (f64 = 1).
+ * Return -1 when a region holds no voxel centre (the reference panics). */
+typedef struct {
+  int32_t kind;
+  float c[3];
+  float r_mm;
+} orc_roi_t;
+int orc_roi_stats(const float full_mm[3], const uint32_t n[3], const float* image, const orc_roi_t* rois,
+                  size_t n_rois, int f64, uint64_t* counts, double* means, double* sigmas);
+int orc_image_foms(const float full_mm[3], const uint32_t n[3], const float* image, const orc_roi_t* rois,
+                   const float* activities, size_t n_rois, const orc_roi_t* bg_rois, size_t n_bg,
+                   float bg_activity, int f64, float* crcs, float* snrs, float* bg_measured);
+/* phantom 0 NEMA7, 1 Jaszczak; foms = 6 x (r, crc, bg variability, snr). */
+int orc_phantom_foms(const float full_mm[3], const uint32_t n[3], const float* image, int phantom, int f64,
+                     float foms[24], float* aocs, size_t aoc_capacity, size_t* n_aocs);
 
 /* Philox4x32-10 block (for pinning the restatement against the paper's
  * known-answer vectors). */

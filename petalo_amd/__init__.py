@@ -28,6 +28,7 @@ __all__ = [
     "write_image3d", "read_image3d", "device_count", "make_lors",
     "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE", "Batch", "ingest_lors", "ingest_params",
     "scattergram_spec", "H5LOR_DTYPE", "make_sensitivity_image_discrete",
+    "roi", "roi_stats", "image_foms", "phantom_foms", "ROI_DTYPE", "ROI_STATS_DTYPE",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -193,6 +194,9 @@ _SIGS = {
     "petalo_synth_density": (C.c_int, [_P, _P]),
     "petalo_philox4x32_10": (C.c_int, [_P, _P, _P]),
     "petalo_ingest_lors": (C.c_int, [_P, C.c_size_t, _P, _P, _P, C.c_size_t, _P, _P]),
+    "petalo_roi_stats": (C.c_int, [_P, _P, _P, C.c_size_t, _P]),
+    "petalo_image_foms": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, C.c_float, _P, _P, _P]),
+    "petalo_phantom_foms": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_size_t, _P]),
     "petalo_config_ingest": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint64, _P]),
     "petalo_make_sensitivity_image_discrete": (C.c_int, [_P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float,
                                                          C.c_float, C.c_int32, C.c_uint64, _P, _P]),
@@ -552,6 +556,57 @@ def make_sensitivity_image_discrete(attenuation, detector_length, r_min, dr, dz,
     if return_stats:
         return img, Stats({f: int(getattr(st, f)) for f, _ in _Stats._fields_})
     return img
+
+
+# ---------------------------------------------------------------------------
+# Figures of merit (src/fom.rs, src/bin/foms.rs)
+ROI_SPHERE, ROI_CYLINDER_X, ROI_CYLINDER_Y, ROI_CYLINDER_Z, ROI_DISC_Z = 0, 1, 2, 3, 4
+PHANTOM_NEMA7, PHANTOM_JASZCZAK = 0, 1
+ROI_DTYPE = np.dtype([("kind", np.int32), ("c", np.float32, 3), ("r", np.float32)])
+ROI_STATS_DTYPE = np.dtype([("count", np.uint64), ("mean", np.float64), ("sigma", np.float64)])
+
+
+def roi(kind, centre, r):
+    """One region: ROI::Sphere((x, y, z), r), CylinderX((y, z), r),
+    CylinderY((x, z), r), CylinderZ((x, y), r) or DiscZ((x, y, z), r)."""
+    c = list(centre) + [0.0] * (3 - len(centre))
+    return np.array([(kind, c, r)], ROI_DTYPE)[0]
+
+
+def roi_stats(image, rois):
+    """Per region: voxel count, mean, population sigma (binary64)."""
+    rois = np.ascontiguousarray(np.atleast_1d(np.asarray(rois, ROI_DTYPE)))
+    out = np.zeros(len(rois), ROI_STATS_DTYPE)
+    data = np.ascontiguousarray(image.data, np.float32)
+    _check(lib().petalo_roi_stats(C.byref(image.fov._c()), _ptr(data), _ptr(rois), len(rois), _ptr(out)))
+    return out
+
+
+def image_foms(image, rois, activities, background_rois, background_activity):
+    """Image::foms (fom.rs:424-450): (crcs %, snrs, background measured)."""
+    rois = np.ascontiguousarray(np.atleast_1d(np.asarray(rois, ROI_DTYPE)))
+    bg = np.ascontiguousarray(np.atleast_1d(np.asarray(background_rois, ROI_DTYPE)))
+    act = np.ascontiguousarray(activities, np.float32)
+    if len(act) != len(rois):
+        raise PetaloError(1, "one activity per region")
+    crcs, snrs, bgm = np.zeros(len(rois), np.float32), np.zeros(len(rois), np.float32), C.c_float(0)
+    data = np.ascontiguousarray(image.data, np.float32)
+    _check(lib().petalo_image_foms(C.byref(image.fov._c()), _ptr(data), _ptr(rois), _ptr(act), len(rois),
+                                   _ptr(bg), len(bg), float(background_activity), _ptr(crcs), _ptr(snrs),
+                                   C.byref(bgm)))
+    return crcs, snrs, bgm.value
+
+
+def phantom_foms(image, phantom, aoc_capacity=256):
+    """The foms tool's NEMA7 / Jaszczak analysis (src/bin/foms.rs):
+    (foms [6 x (r mm, crc %, bg variability %, snr %)], AOCs % (NEMA7))."""
+    foms = np.zeros((6, 4), np.float32)
+    aocs = np.zeros(aoc_capacity, np.float32)
+    na = C.c_size_t(0)
+    data = np.ascontiguousarray(image.data, np.float32)
+    _check(lib().petalo_phantom_foms(C.byref(image.fov._c()), _ptr(data), int(phantom), _ptr(foms), _ptr(aocs),
+                                     aoc_capacity, C.byref(na)))
+    return foms, aocs[:min(na.value, aoc_capacity)].copy()
 
 
 def sensitivity_lors(fov, n_lors, seed, detector_length, detector_diameter):

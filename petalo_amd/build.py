@@ -18,7 +18,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libpetalo_hip.so")
-TOOLS = {"mlem": "mlem.cpp", "make_sensitivity_image": "make_sensitivity_image.cpp"}
+TOOLS = {"mlem": "mlem.cpp", "make_sensitivity_image": "make_sensitivity_image.cpp", "foms": "foms.cpp"}
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -32,7 +32,7 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
 
-SOURCES = ["capi.hip", "ingest.hip", "host.cpp"]
+SOURCES = ["capi.hip", "ingest.hip", "fom.hip", "host.cpp"]
 
 
 def _run(cmd):

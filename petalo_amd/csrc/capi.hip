@@ -44,23 +44,6 @@ int fail(int code, const std::string& msg) {
 
 namespace {
 
-
-
-
-bool fov_ok(const petalo_fov_t* f, std::string& why) {
-  if (!f) { why = "fov is NULL"; return false; }
-  for (int i = 0; i < 3; ++i) {
-    if (f->n[i] == 0) { why = "fov has zero voxels along an axis"; return false; }
-    if (!(f->full_size_mm[i] > 0.0f) || !std::isfinite(f->full_size_mm[i])) {
-      why = "fov size must be positive and finite";
-      return false;
-    }
-  }
-  const uint64_t nv = (uint64_t)f->n[0] * f->n[1] * f->n[2];
-  if (nv >= (1ull << 31)) { why = "fov has 2^31 or more voxels"; return false; }
-  return true;
-}
-
 DevFov to_dev(const petalo_fov_t& f) { return make_fov(f.full_size_mm, f.n); }
 
 uint64_t nvox(const petalo_fov_t& f) { return (uint64_t)f.n[0] * f.n[1] * f.n[2]; }

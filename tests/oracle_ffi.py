@@ -231,6 +231,53 @@ def discrete_lors(r_min, dr, dz, da, length, adjust, seed=0, capacity=None):
     return out, int(ne.value)
 
 
+ROI_DTYPE = np.dtype([("kind", np.int32), ("c", np.float32, 3), ("r", np.float32)])
+
+
+def roi_stats(full, n, image, rois, f64=False):
+    """(counts, means, sigmas) of regions (ROI_DTYPE records) of an image."""
+    L = lib()
+    L.orc_roi_stats.restype = C.c_int
+    f, nn = _fov(full, n)
+    rois = np.ascontiguousarray(rois, ROI_DTYPE)
+    img = np.ascontiguousarray(image, np.float32)
+    k = len(rois)
+    cnt, mean, sig = np.zeros(k, np.uint64), np.zeros(k), np.zeros(k)
+    L.orc_roi_stats(_p(f), _p(nn), _p(img), _p(rois), C.c_size_t(k), C.c_int(int(f64)), _p(cnt), _p(mean), _p(sig))
+    return cnt, mean, sig
+
+
+def image_foms(full, n, image, rois, activities, bg_rois, bg_activity, f64=False):
+    L = lib()
+    f, nn = _fov(full, n)
+    rois = np.ascontiguousarray(rois, ROI_DTYPE)
+    bg = np.ascontiguousarray(bg_rois, ROI_DTYPE)
+    act = np.ascontiguousarray(activities, np.float32)
+    img = np.ascontiguousarray(image, np.float32)
+    crcs, snrs, bgm = np.zeros(len(rois), np.float32), np.zeros(len(rois), np.float32), C.c_float(0)
+    rc = L.orc_image_foms(_p(f), _p(nn), _p(img), _p(rois), _p(act), C.c_size_t(len(rois)), _p(bg),
+                          C.c_size_t(len(bg)), C.c_float(bg_activity), C.c_int(int(f64)), _p(crcs), _p(snrs),
+                          C.byref(bgm))
+    if rc != 0:
+        raise ValueError("a region holds no voxel")
+    return crcs, snrs, bgm.value
+
+
+def phantom_foms(full, n, image, phantom, f64=False, aoc_capacity=256):
+    """(foms [6 x (r, crc, bg variability, snr)], aocs) of the foms tool."""
+    L = lib()
+    f, nn = _fov(full, n)
+    img = np.ascontiguousarray(image, np.float32)
+    foms = np.zeros((6, 4), np.float32)
+    aocs = np.zeros(aoc_capacity, np.float32)
+    na = C.c_size_t(0)
+    rc = L.orc_phantom_foms(_p(f), _p(nn), _p(img), C.c_int(phantom), C.c_int(int(f64)), _p(foms), _p(aocs),
+                            C.c_size_t(aoc_capacity), C.byref(na))
+    if rc != 0:
+        raise ValueError("a region holds no voxel")
+    return foms, aocs[:min(na.value, aoc_capacity)].copy()
+
+
 def philox(ctr, key):
     c = np.asarray(ctr, np.uint32)
     k = np.asarray(key, np.uint32)

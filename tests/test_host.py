@@ -293,8 +293,24 @@ def test_discrete_sensitivity_argument_checks_come_first():
         assert e.value.code == 1, kw
 
 
+def test_fom_argument_checks_come_first():
+    fov = P.FOV([60.0] * 3, [6] * 3)
+    img = P.Image(fov, np.ones(fov.nvox, np.float32))
+    with pytest.raises(P.PetaloError) as e:
+        P.phantom_foms(img, 7)
+    assert e.value.code == 1
+    for bad in (P.roi(9, [0, 0, 0], 1.0), P.roi(P.ROI_SPHERE, [0, np.nan, 0], 1.0),
+                P.roi(P.ROI_SPHERE, [0, 0, 0], np.inf)):
+        with pytest.raises(P.PetaloError) as e:
+            P.roi_stats(img, [bad])
+        assert e.value.code == 1
+    with pytest.raises(P.PetaloError):
+        P.image_foms(img, [P.roi(P.ROI_SPHERE, [0, 0, 0], 5.0)], [2.0], [], 1.0)  # no background
+    assert len(P.roi_stats(img, np.zeros(0, P.ROI_DTYPE))) == 0  # nothing to do, no device needed
+
+
 def test_tools_help():
-    for tool in ("mlem", "make_sensitivity_image"):
+    for tool in ("mlem", "make_sensitivity_image", "foms"):
         exe = os.path.join(ROOT, "petalo_amd", "bin", tool)
         out = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=30)
         assert out.returncode == 0 and "usage" in out.stdout
