@@ -329,7 +329,7 @@ int petalo_config_ingest(const petalo_config_t* config, float r_min_mm, float dr
  * units (mm or ps); kind: 'L' length, 'T' time. */
 int petalo_parse_quantity(const char* text, char kind, float* value);
 
-/* ---- Figures of merit (src/fom.rs:20-466, src/bin/foms.rs) --------------
+/* ---- Figures of merit (src/fom.rs:20-461, src/bin/foms.rs) --------------
  * Regions of interest (fom.rs ROI): a voxel belongs to a region when its
  * centre (fov.rs:42-53) satisfies, in binary32 as written (fom.rs:34-63):
  *   SPHERE      (x-c0)^2 + (y-c1)^2 + (z-c2)^2 < r^2
@@ -348,7 +348,7 @@ typedef struct {
   float r_mm;
 } petalo_roi_t;
 /* Per region: voxel count, mean and population standard deviation of the
- * voxel values (fom::mean, fom::mu_and_sigma, fom.rs:309-321), accumulated
+ * voxel values (fom::mean, fom::mu_and_sigma, fom.rs:276-288), accumulated
  * in FP64 (the reference sums binary32 values in voxel order). */
 typedef struct {
   uint64_t count;
@@ -360,7 +360,7 @@ typedef struct {
  * count 0 and mean = sigma = NaN. */
 int petalo_roi_stats(const petalo_fov_t* fov, const float* image, const petalo_roi_t* rois, size_t n_rois,
                      petalo_roi_stats_t* out);
-/* Image::foms (fom.rs:424-450): background = mean over background regions of
+/* Image::foms (fom.rs:426-450): background = mean over background regions of
  * their means; per region crc (fom.rs:452-461, percent, hot when the
  * region's activity exceeds the background's) and snr = (mean - background) /
  * sigma.  PETALO_ERR_ARG if a region holds no voxel (the reference panics). */

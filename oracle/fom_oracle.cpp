@@ -2,10 +2,10 @@
 // INFRASTRUCTURE ONLY, part of liboracle.so; see petalo_oracle.h).
 //
 //   ROI membership      src/fom.rs:34-63 (binary32 as written)
-//   values_inside_roi   src/fom.rs:77-86 (voxel index order)
-//   mean, mu_and_sigma  src/fom.rs:309-321 (sequential sums)
-//   Image::foms, crc    src/fom.rs:424-461
-//   phantom analyses    src/bin/foms.rs:40-255
+//   values_inside_roi   src/fom.rs:81-88 (voxel index order)
+//   mean, mu_and_sigma  src/fom.rs:276-288 (sequential sums)
+//   Image::foms, crc    src/fom.rs:426-461
+//   phantom analyses    src/bin/foms.rs:42-250
 //
 // Every statistic is computed twice: in binary32 in the reference's order
 // (T = float) and in binary64 (T = double), each voxel value widened first.
@@ -66,7 +66,7 @@ bool roi_contains(const orc_roi_t& roi, const float p[3]) {
   }
 }
 
-// Image::values_inside_roi (fom.rs:77-86).
+// Image::values_inside_roi (fom.rs:81-88).
 std::vector<float> values_inside(const Grid& g, const float* img, const orc_roi_t& roi) {
   std::vector<float> out;
   const size_t nv = (size_t)g.n[0] * g.n[1] * g.n[2];
@@ -78,7 +78,7 @@ std::vector<float> values_inside(const Grid& g, const float* img, const orc_roi_
   return out;
 }
 
-// fom::mean (fom.rs:309-311): reduce(a + b) / len; false when empty.
+// fom::mean (fom.rs:276-278): reduce(a + b) / len; false when empty.
 template <class T, class V> bool mean_of(const std::vector<V>& v, T& m) {
   if (v.empty()) return false;
   T s = (T)v[0];
@@ -87,7 +87,7 @@ template <class T, class V> bool mean_of(const std::vector<V>& v, T& m) {
   return true;
 }
 
-// fom::mu_and_sigma (fom.rs:313-321).
+// fom::mu_and_sigma (fom.rs:280-288).
 template <class T, class V> bool mu_sigma(const std::vector<V>& v, T& mu, T& sigma) {
   if (!mean_of(v, mu)) return false;
   T s = (T)0;
@@ -106,7 +106,7 @@ template <class T> T crc(T roi_measured, T roi_activity, T bgd_measured, T bgd_a
                        : (T)1 - (roi_measured / bgd_measured));
 }
 
-// fom::position_to_index / index_to_position (fom.rs:118-126).
+// fom::position_to_index / index_to_position (fom.rs:115-123).
 uint64_t pos_to_index(float x, float hw, float vs) {
   const float q = (x + hw) / vs;
   if (!(q > 0.0f)) return 0;  // saturating `as usize`
@@ -129,7 +129,7 @@ orc_roi_t disc(float x, float y, float z, float r) {
 template <class T>
 int phantom_foms(const Grid& g, const float* img, int phantom, float foms[24], float* aocs, size_t cap,
                  size_t* n_aocs) {
-  // nema7_foms / jaszczak_foms (foms.rs:89-214)
+  // nema7_foms / jaszczak_foms (foms.rs:89-211)
   struct S { int pos; float d; float a; };
   std::vector<S> spheres;
   std::vector<std::pair<float, float>> xys;
@@ -155,16 +155,16 @@ int phantom_foms(const Grid& g, const float* img, int phantom, float foms[24], f
   }
   const T bg_activity = (T)1;
   const float hw = g.hw[2], vs = g.vs[2];
-  // sphere_foms (foms.rs:40-87)
+  // sphere_foms (foms.rs:42-87)
   for (float& z : zs) z = closest_centre(z, hw, vs);
   const float fz = closest_centre(sphere_z, hw, vs);
   T crc37 = (T)NAN;
   bool have37 = false;
   for (size_t k = 0; k < spheres.size(); ++k) {
-    // sphere() (foms.rs:216-222)
+    // sphere() (foms.rs:214-218)
     const float radians = 6.2831855f * (float)spheres[k].pos / 6.0f;
     const float x = ring_r * std::cos(radians), y = ring_r * std::sin(radians), r = spheres[k].d / 2.0f;
-    // contrast_and_variability (foms.rs:226-255)
+    // contrast_and_variability (foms.rs:222-250)
     T sm, ss;
     if (!mu_sigma<T>(values_inside(g, img, disc(x, y, fz, r)), sm, ss)) return -1;
     std::vector<T> bg_means;
@@ -188,7 +188,7 @@ int phantom_foms(const Grid& g, const float* img, int phantom, float foms[24], f
   }
   size_t n = 0;
   if (phantom == 0) {
-    // 7.4.2 (foms.rs:141-175)
+    // 7.4.2 (foms.rs:136-170)
     const float hi_limit = 70.0f - 30.0f, lo_limit = 70.0f - 180.0f + 30.0f;
     const float hi_c = closest_centre(hi_limit, hw, vs), lo_c = closest_centre(lo_limit, hw, vs);
     const float hi = hi_c <= hi_limit ? hi_c : closest_centre(hi_c - vs, hw, vs);
@@ -244,7 +244,7 @@ int orc_roi_stats(const float full_mm[3], const uint32_t n[3], const float* imag
 int orc_image_foms(const float full_mm[3], const uint32_t n[3], const float* image, const orc_roi_t* rois,
                    const float* activities, size_t n_rois, const orc_roi_t* bg_rois, size_t n_bg,
                    float bg_activity, int f64, float* crcs, float* snrs, float* bg_measured) {
-  // Image::foms (fom.rs:424-450)
+  // Image::foms (fom.rs:426-450)
   const Grid g = grid_of(full_mm, n);
   auto run = [&](auto zero) -> int {
     using T = decltype(zero);

@@ -16,7 +16,7 @@
  *   MLEM update     src/mlem.rs:33-66
  *   sensitivity     src/bin/make_sensitivity_image/continuous.rs:3-17,
  *                   main.rs:77-94
- *   FOMs            src/fom.rs:34-461, src/bin/foms.rs:40-255 (fom_oracle.cpp)
+ *   FOMs            src/fom.rs:34-461, src/bin/foms.rs:42-250 (fom_oracle.cpp)
  * plus an independent classic Siddon (parametric alpha merge, Siddon 1985)
  * and a brute-force per-voxel clipper, both in binary64, used only to pin
  * the restatement against the mathematics.

@@ -583,7 +583,7 @@ def roi_stats(image, rois):
 
 
 def image_foms(image, rois, activities, background_rois, background_activity):
-    """Image::foms (fom.rs:424-450): (crcs %, snrs, background measured)."""
+    """Image::foms (fom.rs:426-450): (crcs %, snrs, background measured)."""
     rois = np.ascontiguousarray(np.atleast_1d(np.asarray(rois, ROI_DTYPE)))
     bg = np.ascontiguousarray(np.atleast_1d(np.asarray(background_rois, ROI_DTYPE)))
     act = np.ascontiguousarray(activities, np.float32)

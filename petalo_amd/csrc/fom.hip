@@ -7,7 +7,7 @@
 // wider than the region on each side; the device tests every voxel of the
 // box, so membership is decided only by the exact test.  One workgroup per
 // region: count and sum, a block reduction, then the sum of squared
-// deviations from the mean (two passes, as fom::mu_and_sigma, fom.rs:314-321),
+// deviations from the mean (two passes, as fom::mu_and_sigma, fom.rs:280-288),
 // in binary64.
 #include <hip/hip_runtime.h>
 
@@ -220,7 +220,7 @@ double crc(double roi_measured, double roi_activity, double bgd_measured, double
                                                : 1.0 - (roi_measured / bgd_measured));
 }
 
-// fom::mu_and_sigma (fom.rs:314-321) of a list of values.
+// fom::mu_and_sigma (fom.rs:280-288) of a list of values.
 void mu_and_sigma(const std::vector<double>& v, double& mu, double& sigma) {
   double s = 0.0;
   for (double x : v) s += x;
@@ -232,7 +232,7 @@ void mu_and_sigma(const std::vector<double>& v, double& mu, double& sigma) {
 
 petalo_roi_t disc_z(float x, float y, float z, float r) { return petalo_roi_t{PETALO_ROI_DISC_Z, {x, y, z}, r}; }
 
-// fom::position_to_index (fom.rs:118-121): binary32 ratio, saturating cast.
+// fom::position_to_index (fom.rs:115-118): binary32 ratio, saturating cast.
 uint64_t position_to_index(float position, float half_width, float voxel_size) {
   const float q = (position + half_width) / voxel_size;
   if (!(q > 0.0f)) return 0;
@@ -240,7 +240,7 @@ uint64_t position_to_index(float position, float half_width, float voxel_size) {
   return (uint64_t)q;
 }
 
-// fom::index_to_position (fom.rs:123-126).
+// fom::index_to_position (fom.rs:120-123).
 float index_to_position(uint64_t index, float half_width, float voxel_size) {
   return ((float)index + 0.5f) * voxel_size - half_width;
 }
@@ -254,7 +254,7 @@ struct PhantomSpec {
   float bg_activity;
 };
 
-// nema7_foms / jaszczak_foms constants (foms.rs:91-135, 182-214).
+// nema7_foms / jaszczak_foms constants (foms.rs:89-134, 174-200).
 PhantomSpec phantom_spec(int32_t phantom) {
   PhantomSpec p;
   p.bg_activity = 1.0f;
@@ -314,7 +314,7 @@ int petalo_image_foms(const petalo_fov_t* fov, const float* image, const petalo_
     std::vector<petalo_roi_t> all(background_rois, background_rois + n_background);
     all.insert(all.end(), rois, rois + n_rois);
     const auto st = region_stats(*fov, image, all);
-    // background_measured = sum of background means / their number (fom.rs:427-430)
+    // background_measured = sum of background means / their number (fom.rs:428-431)
     double bg = 0.0;
     for (size_t k = 0; k < n_background; ++k) {
       require_voxels(st[k], ("background region " + std::to_string(k)).c_str());
@@ -325,7 +325,7 @@ int petalo_image_foms(const petalo_fov_t* fov, const float* image, const petalo_
       const petalo_roi_stats_t& r = st[n_background + k];
       require_voxels(r, ("region " + std::to_string(k)).c_str());
       crcs_out[k] = (float)crc(r.mean, activities[k], bg, background_activity);
-      snrs_out[k] = (float)((r.mean - bg) / r.sigma);  // fom.rs:442
+      snrs_out[k] = (float)((r.mean - bg) / r.sigma);  // fom.rs:440
     }
     if (background_measured_out) *background_measured_out = (float)bg;
     return PETALO_OK;
@@ -343,25 +343,25 @@ int petalo_phantom_foms(const petalo_fov_t* fov, const float* image, int32_t pha
     const PhantomSpec p = phantom_spec(phantom);
     const DevFov f = make_fov(fov->full_size_mm, fov->n);
     const float hw = f.hw_z, vs = f.vs_z;
-    // sphere_foms (foms.rs:40-87): regions z-aligned with slice centres
+    // sphere_foms (foms.rs:42-87): regions z-aligned with slice centres
     const float fz = nearest_centre(p.sphere_z, hw, vs);
     std::vector<float> bg_z;
     for (float z : p.bg_z) bg_z.push_back(nearest_centre(z, hw, vs));
     std::vector<petalo_roi_t> rois;
     float sx[6], sy[6], sr[6];
     for (int k = 0; k < 6; ++k) {
-      // sphere(): the k-th of six angular positions (foms.rs:216-222)
+      // sphere(): the k-th of six angular positions (foms.rs:214-218)
       const float radians = 6.2831855f * (float)p.spheres[k].pos / 6.0f;
       sx[k] = p.ring_r * std::cos(radians);
       sy[k] = p.ring_r * std::sin(radians);
       sr[k] = p.spheres[k].diameter / 2.0f;
       rois.push_back(disc_z(sx[k], sy[k], fz, sr[k]));
-      // background discs of this sphere's radius: xy outer, z inner (foms.rs:241-245)
+      // background discs of this sphere's radius: xy outer, z inner (foms.rs:237-241)
       for (size_t b = 0; b < p.bg_x.size(); ++b)
         for (float z : bg_z) rois.push_back(disc_z(p.bg_x[b], p.bg_y[b], z, sr[k]));
     }
     const size_t per_sphere = 1 + p.bg_x.size() * bg_z.size();
-    // NEMA7 7.4.2: lung insert slices at least 30 mm from the phantom's ends (foms.rs:141-175)
+    // NEMA7 7.4.2: lung insert slices at least 30 mm from the phantom's ends (foms.rs:136-170)
     uint64_t lo_index = 1, hi_index = 0;
     if (phantom == PETALO_PHANTOM_NEMA7) {
       const float hi_limit = 70.0f - 30.0f, lo_limit = 70.0f - 180.0f + 30.0f;
@@ -378,7 +378,7 @@ int petalo_phantom_foms(const petalo_fov_t* fov, const float* image, int32_t pha
     const auto st = region_stats(*fov, image, rois);
     double crc37 = std::numeric_limits<double>::quiet_NaN();
     for (int k = 0; k < 6; ++k) {
-      // contrast_and_variability (foms.rs:226-255)
+      // contrast_and_variability (foms.rs:222-250)
       const petalo_roi_stats_t& sph = st[k * per_sphere];
       require_voxels(sph, "sphere region");
       std::vector<double> bg_means;
