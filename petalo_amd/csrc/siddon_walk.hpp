@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 namespace petalo {
 
 // mm/ps, units/src/lib.rs:156 (binary32 rounds it to 0.29979247).
@@ -38,10 +40,13 @@ struct DevTof {
   float peak;       // 1 / (sigma * sqrt(TAU))
   float rcp_sigma;  // 1 / sigma, correctly rounded
   int32_t on;
+  // Replays of the tiled scheme: gauss(dx) * 666 = amp * exp2(-(dx * g_scale)^2)
+  float g_scale;    // sqrt(log2(e) / 2) / sigma
+  float amp;        // 666 * peak
 };
 
 // No TOF: the window test never applies.
-__host__ __device__ inline DevTof no_tof() { return DevTof{1.0f, INFINITY, 0.0f, 1.0f, 0}; }
+__host__ __device__ inline DevTof no_tof() { return DevTof{1.0f, INFINITY, 0.0f, 1.0f, 0, 0.0f, 1.0f}; }
 
 struct Lor {
   float x1, y1, z1, x2, y2, z2, dt, ac;
@@ -94,6 +99,8 @@ inline DevTof make_tof(bool on, float sigma_ps, float cutoff) {
   t.peak = 1.0f / (t.sigma * root_two_pi);
   t.cutoff = cutoff * t.sigma;
   t.rcp_sigma = 1.0f / t.sigma;
+  t.g_scale = (float)(std::sqrt(0.5 * 1.4426950408889634) / (double)t.sigma);
+  t.amp = kTofFactor * t.peak;
   return t;
 }
 
@@ -263,6 +270,21 @@ __host__ __device__ __forceinline__ float tof_factor_in_window(const DevTof& t, 
   const float y = div_rn(dx, t.sigma, t.rcp_sigma);
   const float z = y * y;
   return kTofFactor * (t.peak * tof_gauss_exp(z));
+}
+
+// The Gaussian's shape alone, exp(-(dx/sigma)^2 / 2) = exp2(-(dx * g_scale)^2),
+// inside the window, for the replays of the tiled scheme: two multiplies and
+// one exp2 instead of a correctly rounded division, a square and three
+// multiplies; the 666 * peak factor is applied once per run.  Differs from
+// tof_factor_in_window / (666 * peak) by a few ulp (the argument is at most
+// 9 log2(e) / 2 inside a 3-sigma window).
+__host__ __device__ __forceinline__ float tof_shape(const DevTof& t, float here, float tof_peak) {
+  const float u = (here - tof_peak) * t.g_scale;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_exp2f(-(u * u));
+#else
+  return std::exp2(-(u * u));
+#endif
 }
 
 // The same, zero outside the window (strict cutoff, gauss.rs:47-55).

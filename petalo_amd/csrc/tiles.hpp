@@ -235,8 +235,9 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
 }
 
 // Replays a run from its recorded state, calling add(local_index, w * r)
-// for every pushed step.  The weights are recomputed with the same binary32
-// operations as walk #1.
+// for every pushed step.  The voxel sequence and step lengths are walk #1's
+// bit for bit; the TOF weight is length * tof_shape * (666 * peak), with the
+// constant folded into r once per run (a few ulp from walk #1's product).
 template <bool TOF, class Add>
 __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, const DevTof& t, const TileGeom& g,
                                                     Add&& add) {
@@ -257,25 +258,29 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
   c.d_z = (flips & 4u) ? -dlz : dlz;
   w.index = (int32_t)run_local(ru.code);
   const uint32_t m = run_len(ru.code);  // the run length bounds the replay
+  const float rr = TOF ? r * t.amp : r;
   for (uint32_t k = 0; k < m; ++k) {
     const Step s = next_boundary(w);
     // Every step of a run lies inside the TOF window, so the window test of
     // step_weight is skipped: the run spans the walk's first to last pushed
     // step in the tile, pushed steps lie inside the window, `here` does not
     // decrease along the walk and the window is an interval of here - peak.
-    float wt = s.bnd - w.here;
-    if (TOF) wt *= tof_factor_in_window(t, w.here, c.tof_peak);
-    if (wt > 0.0f) add((uint32_t)w.index, wt * r);
+    // The Gaussian is at least exp(-4.5) there, so wl > 0 exactly when the
+    // step length is (a step of length 0 is not pushed).
+    float wl = s.bnd - w.here;
+    if (TOF) wl *= tof_shape(t, w.here, c.tof_peak);
+    if (wl > 0.0f) add((uint32_t)w.index, wl * rr);
     move(w, c, s);
   }
 }
 
 // Forward replay of a run against the LDS tile: p = sum of w * x over the
-// run's pushed steps in walk order.  Each step's LDS read is consumed one
-// step later, so its latency overlaps the next step's arithmetic; every step
-// of a run lies inside the tile, so the read is always in bounds, and a step
-// that is not pushed adds nothing (the select keeps x out of it, as the
-// reference never reads it).
+// run's pushed steps in walk order, as (666 * peak) * sum of
+// (length * tof_shape) * x with fused multiply-adds.  Each step's LDS read is
+// consumed one step later, so its latency overlaps the next step's
+// arithmetic; every step of a run lies inside the tile, so the read is always
+// in bounds, and a step that is not pushed adds nothing (the select keeps x
+// out of it, as the reference never reads it).
 template <bool TOF>
 __device__ __forceinline__ float replay_project(const Run& ru, const DevTof& t, const TileGeom& g,
                                                 const float* __restrict__ timg) {
@@ -299,16 +304,16 @@ __device__ __forceinline__ float replay_project(const Run& ru, const DevTof& t, 
   float p = 0.0f, w_prev = 0.0f, x_prev = 0.0f;
   for (uint32_t k = 0; k < m; ++k) {
     const Step s = next_boundary(w);
-    float wt = s.bnd - w.here;  // in-window: see replay_run
-    if (TOF) wt *= tof_factor_in_window(t, w.here, c.tof_peak);
+    float wl = s.bnd - w.here;  // in-window: see replay_run
+    if (TOF) wl *= tof_shape(t, w.here, c.tof_peak);
     const float x = timg[w.index];
-    p += w_prev > 0.0f ? w_prev * x_prev : 0.0f;
-    w_prev = wt;
+    p = fmaf(w_prev, w_prev > 0.0f ? x_prev : 0.0f, p);
+    w_prev = wl;
     x_prev = x;
     move(w, c, s);
   }
-  p += w_prev > 0.0f ? w_prev * x_prev : 0.0f;
-  return p;
+  p = fmaf(w_prev, w_prev > 0.0f ? x_prev : 0.0f, p);
+  return TOF ? t.amp * p : p;
 }
 
 // Forward, tiled: one workgroup per work item copies its tile of the image
