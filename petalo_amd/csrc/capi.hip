@@ -394,7 +394,7 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count, bool timi
   hipStream_t st = c->stream;
   const size_t nw = c->work[sub].n;
   if (nw) {
-    const size_t lds = (size_t)c->tiles.tvox * sizeof(float);
+    const size_t lds = ((size_t)c->tiles.tvox + 1) * sizeof(float);
     if (c->tof_on)
       k_fwd_tile<true><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->image.p, c->dfov,
                                                         c->dtof, c->tiles, c->partial.p);
@@ -932,7 +932,7 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
         b->ctx.push_back(create_context(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],
                                         sens ? sens[r] : nullptr, n_subsets, opt, b->stream));
       for (petalo_mlem* c : b->ctx) {
-        b->lds_f32 = std::max(b->lds_f32, (size_t)c->tiles.tvox * sizeof(float));
+        b->lds_f32 = std::max(b->lds_f32, ((size_t)c->tiles.tvox + 1) * sizeof(float));
         b->lds_f64 = std::max(b->lds_f64, (size_t)c->tiles.tvox * sizeof(double));
       }
       b->descs.resize(n_subsets);
@@ -1306,7 +1306,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       const unsigned nw = (unsigned)items.size();
       if (!mu_zero) {
         DevBuf<float> partial(total);
-        k_fwd_tile<false><<<nw, 1024, (size_t)g.tvox * sizeof(float), s>>>(unsorted.p, work.p, dmu, f, notof, g,
+        k_fwd_tile<false><<<nw, 1024, ((size_t)g.tvox + 1) * sizeof(float), s>>>(unsorted.p, work.p, dmu, f, notof, g,
                                                                            partial.p, perm2.p);
         k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, pos.p, partial.p, rbuf.p);
         HIP_CHECK(hipGetLastError());

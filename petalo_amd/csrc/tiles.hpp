@@ -234,20 +234,16 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
   }
 }
 
-// Replays a run from its recorded state, calling add(local_index, w * r)
-// for every pushed step.  The voxel sequence and step lengths are walk #1's
-// bit for bit; the TOF weight is length * tof_shape * (666 * peak), with the
-// constant folded into r once per run (a few ulp from walk #1's product).
-template <bool TOF, class Add>
-__host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, const DevTof& t, const TileGeom& g,
-                                                    Add&& add) {
-  const int32_t dlx = 1, dly = g.px, dlz = g.px * g.ty;
-  WalkS w;
+// Walk state of a run for a replay against an LDS tile of elements of
+// 2^kShift bytes: the voxel index and its per-axis strides are kept in bytes,
+// so a step's LDS address needs no scaling.
+template <int kShift>
+__host__ __device__ __forceinline__ void run_state(const Run& ru, const TileGeom& g, WalkS& w, WalkC& c) {
+  const int32_t dlx = 1 << kShift, dly = g.px << kShift, dlz = (g.px * g.ty) << kShift;
   w.nb_x = ru.nb_x;
   w.nb_y = ru.nb_y;
   w.nb_z = ru.nb_z;
   w.here = ru.here;
-  WalkC c;
   c.vl_x = ru.vl_x;
   c.vl_y = ru.vl_y;
   c.vl_z = ru.vl_z;
@@ -256,20 +252,34 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
   c.d_x = (flips & 1u) ? -dlx : dlx;
   c.d_y = (flips & 2u) ? -dly : dly;
   c.d_z = (flips & 4u) ? -dlz : dlz;
-  w.index = (int32_t)run_local(ru.code);
+  w.index = (int32_t)(run_local(ru.code) << kShift);
+}
+
+// Replays a run from its recorded state, calling add(byte_offset, w * r)
+// for every pushed step (offsets into a tile of binary64 elements).  The
+// voxel sequence and step lengths are walk #1's bit for bit; the TOF weight
+// is length * tof_shape * (666 * peak), with the constant folded into r once
+// per run (a few ulp from walk #1's product).  Every step of a run lies
+// inside the TOF window, so the window test of step_weight is skipped: the
+// run spans the walk's first to last pushed step in the tile, pushed steps
+// lie inside the window, `here` does not decrease along the walk and the
+// window is an interval of here - peak.  The Gaussian is at least exp(-4.5)
+// there, so wl > 0 exactly when the step length is; a step of length 0 is
+// not pushed.  GUARD false adds it anyway, which adds +0 when r is finite
+// (the caller's condition), saving the test.
+template <bool TOF, bool GUARD, class Add>
+__host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, const DevTof& t, const TileGeom& g,
+                                                    Add&& add) {
+  WalkS w;
+  WalkC c;
+  run_state<3>(ru, g, w, c);
   const uint32_t m = run_len(ru.code);  // the run length bounds the replay
   const float rr = TOF ? r * t.amp : r;
   for (uint32_t k = 0; k < m; ++k) {
     const Step s = next_boundary(w);
-    // Every step of a run lies inside the TOF window, so the window test of
-    // step_weight is skipped: the run spans the walk's first to last pushed
-    // step in the tile, pushed steps lie inside the window, `here` does not
-    // decrease along the walk and the window is an interval of here - peak.
-    // The Gaussian is at least exp(-4.5) there, so wl > 0 exactly when the
-    // step length is (a step of length 0 is not pushed).
     float wl = s.bnd - w.here;
     if (TOF) wl *= tof_shape(t, w.here, c.tof_peak);
-    if (wl > 0.0f) add((uint32_t)w.index, wl * rr);
+    if (!GUARD || wl > 0.0f) add((uint32_t)w.index, wl * rr);
     move(w, c, s);
   }
 }
@@ -279,44 +289,34 @@ __host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, cons
 // (length * tof_shape) * x with fused multiply-adds.  Each step's LDS read is
 // consumed one step later, so its latency overlaps the next step's
 // arithmetic; every step of a run lies inside the tile, so the read is always
-// in bounds, and a step that is not pushed adds nothing (the select keeps x
-// out of it, as the reference never reads it).
-template <bool TOF>
+// in bounds.  A step that is not pushed adds nothing: with GUARD the select
+// keeps x out of it, as the reference never reads it; without GUARD (the
+// caller's tile holds only finite values) it adds 0 * x = +0.
+template <bool TOF, bool GUARD>
 __device__ __forceinline__ float replay_project(const Run& ru, const DevTof& t, const TileGeom& g,
                                                 const float* __restrict__ timg) {
-  const int32_t dlx = 1, dly = g.px, dlz = g.px * g.ty;
   WalkS w;
-  w.nb_x = ru.nb_x;
-  w.nb_y = ru.nb_y;
-  w.nb_z = ru.nb_z;
-  w.here = ru.here;
   WalkC c;
-  c.vl_x = ru.vl_x;
-  c.vl_y = ru.vl_y;
-  c.vl_z = ru.vl_z;
-  c.tof_peak = ru.tof_peak;
-  const uint32_t flips = run_flips(ru.code);
-  c.d_x = (flips & 1u) ? -dlx : dlx;
-  c.d_y = (flips & 2u) ? -dly : dly;
-  c.d_z = (flips & 4u) ? -dlz : dlz;
-  w.index = (int32_t)run_local(ru.code);
+  run_state<2>(ru, g, w, c);
+  const char* base = reinterpret_cast<const char*>(timg);
   const uint32_t m = run_len(ru.code);
   float p = 0.0f, w_prev = 0.0f, x_prev = 0.0f;
   for (uint32_t k = 0; k < m; ++k) {
     const Step s = next_boundary(w);
     float wl = s.bnd - w.here;  // in-window: see replay_run
     if (TOF) wl *= tof_shape(t, w.here, c.tof_peak);
-    const float x = timg[w.index];
-    p = fmaf(w_prev, w_prev > 0.0f ? x_prev : 0.0f, p);
+    const float x = *reinterpret_cast<const float*>(base + w.index);
+    p = fmaf(w_prev, (!GUARD || w_prev > 0.0f) ? x_prev : 0.0f, p);
     w_prev = wl;
     x_prev = x;
     move(w, c, s);
   }
-  p = fmaf(w_prev, w_prev > 0.0f ? x_prev : 0.0f, p);
+  p = fmaf(w_prev, (!GUARD || w_prev > 0.0f) ? x_prev : 0.0f, p);
   return TOF ? t.amp * p : p;
 }
 
 // Forward, tiled: one workgroup per work item copies its tile of the image
+// (dynamic LDS: tvox + 1 words)
 // into LDS and replays its runs, writing each run's partial projection
 // (sum of w*x over the run's pushed steps, in walk order) at the run's
 // sorted position (coalesced; writing them LOR-major instead scatters 4-byte
@@ -335,12 +335,22 @@ __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, cons
   const int32_t tyi = (tl / g.ntx) % g.nty;
   const int32_t txi = tl % g.ntx;
   const int32_t x0 = txi * g.tx, y0 = tyi * g.ty, z0 = tzi * g.tz;
+  // one word after the tile: set when the tile holds a non-finite voxel
+  volatile int32_t* odd_tile = reinterpret_cast<volatile int32_t*>(timg + g.tvox);
+  if (threadIdx.x == 0) *odd_tile = 0;
+  __syncthreads();
+  bool odd = false;
   for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) {
     const int32_t lx = k % g.px, ly = (k / g.px) % g.ty, lz = k / (g.px * g.ty);
     const int32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
-    timg[k] = (lx < g.tx && x < f.nx && y < f.ny && z < f.nz) ? img[x + f.nx * (y + f.ny * z)] : 0.0f;
+    const float v = (lx < g.tx && x < f.nx && y < f.ny && z < f.nz) ? img[x + f.nx * (y + f.ny * z)] : 0.0f;
+    timg[k] = v;
+    odd |= !(fabsf(v) <= 3.402823466e38f);  // inf or NaN
   }
+  if (odd) *odd_tile = 1;
   __syncthreads();
+  // the zero-length-step select is needed only where a voxel is not finite
+  const bool guard = *odd_tile != 0;
   int32_t j = wi.y + (int32_t)threadIdx.x;
   if (j >= wi.z) return;
   const int32_t last = wi.z - 1;
@@ -350,7 +360,7 @@ __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, cons
     const int32_t jn = j + stride;
     const int32_t jc = jn < last ? jn : last;  // in bounds; unused past the end
     const Run nxt = runs[perm ? perm[jc] : jc];
-    partial[j] = replay_project<TOF>(cur, t, g, timg);
+    partial[j] = guard ? replay_project<TOF, true>(cur, t, g, timg) : replay_project<TOF, false>(cur, t, g, timg);
     if (jn > last) break;
     cur = nxt;
     j = jn;
@@ -577,7 +587,13 @@ __device__ __forceinline__ void scatter_body(const Run* __restrict__ runs, const
       const float rn = r_in ? r_in[nxt.lor] : 1.0f;
       const int32_t jnc = jnn < last ? jnn : last;
       const Run nn = runs[perm ? perm[jnc] : jnc];
-      replay_run<TOF>(cur, r, t, g, [&](uint32_t li, float v) { atomicAdd(tile + li, (double)v); });
+      char* base = reinterpret_cast<char*>(tile);
+      auto add = [&](uint32_t off, float v) { atomicAdd(reinterpret_cast<double*>(base + off), (double)v); };
+      // without a non-finite r in the wavefront, zero-length steps may add +0
+      if (__all(fabsf(r) <= 3.402823466e38f))
+        replay_run<TOF, false>(cur, r, t, g, add);
+      else
+        replay_run<TOF, true>(cur, r, t, g, add);
       if (jn > last) break;
       cur = nxt;
       nxt = nn;
