@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--workload", default="tof_mlem_120", choices=sorted(WORKLOADS))
     ap.add_argument("--scheme", type=int, default=0, help="0 auto, 1 atomic, 2 tiled")
     ap.add_argument("--tile", default=None, help="tiled scheme tile shape x,y,z (default: automatic)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the library time the atomic scheme and five tile shapes and keep the fastest")
     ap.add_argument("--batched", action="store_true",
                     help="measure 32 batched TOF-OSEM reconstructions (20 subsets) instead")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -169,7 +171,8 @@ def main():
 
     t0 = time.perf_counter()
     tile = tuple(int(v) for v in a.tile.split(",")) if a.tile else None
-    ctx = P.Mlem(fov, lors, tof=tof_obj, sensitivity=sens_img, n_subsets=subsets, scheme=a.scheme, tile=tile)
+    ctx = P.Mlem(fov, lors, tof=tof_obj, sensitivity=sens_img, n_subsets=subsets, scheme=a.scheme, tile=tile,
+                 autotune=a.autotune)
     log(f"[bench] context ready in {time.perf_counter() - t0:.2f} s")
     ws = ctx.walk_stats()
     log(f"[bench] walk stats {ws}")
@@ -178,7 +181,7 @@ def main():
         ctx.step(copy_image=False)
     P.lib().petalo_device_synchronize(0)
 
-    info = ctx.info()
+    info = dict(ctx.info(), tile=list(ctx.tile_shape()), autotune=bool(a.autotune))
     log(f"[bench] scheme {info}")
     ctx.set_timing(True)
     proj_ms = []
@@ -288,9 +291,29 @@ def main():
         O.mlem_update(fov.full_size, fov.n, sample, x, sens=s, tof=tof, threads=threads)
         t_cpu = time.perf_counter() - t0
         cpu_it_s = 1.0 / (t_cpu * n_lors / m)
+        # one thread, on a tenth of the sample
+        m1 = max(1, m // 10)
+        x1 = np.ones(fov.nvox, np.float32)
+        t0 = time.perf_counter()
+        O.mlem_update(fov.full_size, fov.n, lors[:m1], x1, sens=s, tof=tof, threads=1)
+        t_one = time.perf_counter() - t0
+        steps_per_lor = ws["n_steps"] / n_lors
+        model = None
+        try:
+            with open("/proc/cpuinfo") as fh:
+                model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+        except OSError:
+            pass
         cpu = {"value": cpu_it_s, "unit": "iter/s", "cores": threads, "kind": "port",
                "sample": f"one MLEM update over the first {m} of the {n_lors} LORs "
-                         f"({t_cpu:.2f} s), scaled by {n_lors / m:g} to one iteration"}
+                         f"({t_cpu:.2f} s), scaled by {n_lors / m:g} to one iteration",
+               "voxel_steps_per_s": steps_per_lor * m / t_cpu,
+               "one_thread": {"value": 1.0 / (t_one * n_lors / m1), "unit": "iter/s",
+                              "voxel_steps_per_s": steps_per_lor * m1 / t_one,
+                              "sample": f"the first {m1} LORs ({t_one:.2f} s)"},
+               "cpu_model": model}
+        log(f"[bench] CPU oracle: {cpu_it_s:.3f} iter/s on {threads} threads, "
+            f"{cpu['one_thread']['value']:.4f} iter/s on one ({model})")
         # Parity on the same sample: one update from ones on the GPU.
         g = P.Mlem(fov, sample, tof=tof_obj, sensitivity=sens_img, n_subsets=1)
         img, _ = g.step()

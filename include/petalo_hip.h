@@ -80,7 +80,7 @@ typedef struct {
 } petalo_stats_t;
 
 /* Backprojection schemes for petalo_mlem_options_t.scheme. */
-#define PETALO_SCHEME_AUTO 0     /* autotuned choice                        */
+#define PETALO_SCHEME_AUTO 0     /* tiled, or autotuned (options.autotune)  */
 #define PETALO_SCHEME_ATOMIC 1   /* plain: one lane per LOR, f32 global atomics */
 #define PETALO_SCHEME_TILE 2     /* image tiles in LDS, binary64 accumulation */
 
@@ -88,9 +88,18 @@ typedef struct {
   int32_t scheme;        /* PETALO_SCHEME_*                                   */
   int32_t sort_lors;     /* 1: reorder LORs inside each subset for locality   */
   int32_t device;        /* HIP device ordinal                                */
-  int32_t tile[3];       /* tiled scheme: tile shape in voxels (x, y, z); zeros
-                            choose the default 32 x 32 x 16.  At most 16384
-                            voxels per tile.                                  */
+  int32_t tile[3];       /* tiled scheme: tile shape in voxels (x, y, z); a
+                            zero takes the default (x, y: about 40, the axis
+                            split evenly; z: as many slices as fit 10240 LDS
+                            entries).  At most 20480 LDS entries per tile,
+                            rows padded to an odd pitch.                      */
+  int32_t autotune;      /* with scheme AUTO: 1 times the default tile, the
+                            atomic scheme and tiles of about 24, 32, 48 and
+                            64 voxels across on up to 2^21 LORs of the first
+                            subset, and keeps the default unless another is
+                            at least 3% faster (creation takes about a second
+                            longer); 0 takes the tiled scheme with the tile
+                            given or defaulted.                              */
 } petalo_mlem_options_t;
 
 /* ---- library ----------------------------------------------------------- */
@@ -161,6 +170,9 @@ enum {
 int petalo_mlem_last_kernel_timing(petalo_mlem_t* ctx, double* ms, size_t n);
 /* Scheme in use and, for the tiled scheme, the number of tile runs. */
 int petalo_mlem_info(petalo_mlem_t* ctx, int32_t* scheme, uint64_t* n_runs_total);
+/* The tile shape of a tiled context (zeros for the atomic scheme): after
+ * autotuning, the shape it chose. */
+int petalo_mlem_tile_shape(petalo_mlem_t* ctx, int32_t tile_out[3]);
 int petalo_mlem_destroy(petalo_mlem_t* ctx);
 
 /* ---- Batched reconstructions -------------------------------------------

@@ -67,7 +67,7 @@ class _Stats(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("sort_lors", C.c_int32), ("device", C.c_int32),
-                ("tile", C.c_int32 * 3)]
+                ("tile", C.c_int32 * 3), ("autotune", C.c_int32)]
 
 
 class _Ingest(C.Structure):
@@ -197,6 +197,7 @@ _SIGS = {
     "petalo_roi_stats": (C.c_int, [_P, _P, _P, C.c_size_t, _P]),
     "petalo_image_foms": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, C.c_float, _P, _P, _P]),
     "petalo_phantom_foms": (C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_size_t, _P]),
+    "petalo_mlem_tile_shape": (C.c_int, [_P, _P]),
     "petalo_config_ingest": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint64, _P]),
     "petalo_make_sensitivity_image_discrete": (C.c_int, [_P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float,
                                                          C.c_float, C.c_int32, C.c_uint64, _P, _P]),
@@ -347,7 +348,7 @@ class Mlem:
     subset update, like petalo::mlem::mlem (src/mlem.rs:8-31)."""
 
     def __init__(self, fov, lors, tof=None, sensitivity=None, n_subsets=1, scheme=SCHEME_AUTO,
-                 sort_lors=True, device=0, tile=None):
+                 sort_lors=True, device=0, tile=None, autotune=False):
         self.fov = fov
         lors = np.ascontiguousarray(lors, LOR_DTYPE)
         sens = None
@@ -357,7 +358,8 @@ class Mlem:
             if sens.size != fov.nvox:
                 raise PetaloError(6, "sensitivity image size does not match the FOV")
         self._ctx = C.c_void_p()
-        opt = _Options(int(scheme), 1 if sort_lors else 0, int(device), (C.c_int32 * 3)(*(tile or (0, 0, 0))))
+        opt = _Options(int(scheme), 1 if sort_lors else 0, int(device), (C.c_int32 * 3)(*(tile or (0, 0, 0))),
+                       1 if autotune else 0)
         _check(lib().petalo_mlem_create(C.byref(self._ctx), C.byref(fov._c()), _tof_ptr(tof),
                                         _ptr(lors), len(lors), _ptr(sens), int(n_subsets),
                                         C.byref(opt)))
@@ -386,6 +388,12 @@ class Mlem:
         if data.size != self.fov.nvox:
             raise PetaloError(6, "image size does not match the FOV")
         _check(lib().petalo_mlem_set_image(self._ctx, _ptr(data)))
+
+    def tile_shape(self):
+        """The tile shape in voxels, (0, 0, 0) for the atomic scheme."""
+        t = (C.c_int32 * 3)()
+        _check(lib().petalo_mlem_tile_shape(self._ctx, t))
+        return tuple(int(v) for v in t)
 
     def walk_stats(self, subset=0):
         s = _Stats()

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -633,18 +635,94 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+
+// The autotuner: times one subset update of each candidate on up to
+// kTuneLors LORs of the first subset (one warm-up update, then three) and
+// returns the options of the fastest, keeping the default tile shape unless
+// another candidate is at least kTuneMargin faster (the candidates often lie
+// within the timing noise of each other).  Candidates: the default tile, the
+// plain atomic scheme and tiles of about 24, 32, 48 and 64 voxels across,
+// each with its default depth; shapes the image does not admit are skipped.
+constexpr size_t kTuneLors = 1u << 21;
+constexpr double kTuneMargin = 0.03;
+
+petalo_mlem_options_t autotune(const petalo_fov_t* fov, const petalo_tof_t* tof, const petalo_lor_t* lors,
+                               size_t n_lors, const float* sensitivity, size_t n_subsets,
+                               petalo_mlem_options_t opt) {
+  opt.autotune = 0;
+  const size_t m = std::min(n_lors / n_subsets, kTuneLors);
+  petalo_mlem_options_t best = opt;
+  best.scheme = PETALO_SCHEME_TILE;
+  if (m == 0) return best;
+  std::vector<petalo_mlem_options_t> cands;
+  std::vector<std::array<int32_t, 3>> seen;
+  for (int32_t w : {0, 24, 32, 48, 64}) {
+    petalo_mlem_options_t t = opt;
+    t.scheme = PETALO_SCHEME_TILE;
+    for (int k = 0; k < 2; ++k) t.tile[k] = w ? even_split(fov->n[k], w) : 0;  // 0: the default
+    t.tile[2] = 0;  // default depth for this cross-section
+    const TileGeom g = tile_shape(*fov, t.tile);
+    const std::array<int32_t, 3> shape{g.tx, g.ty, g.tz};
+    if (std::find(seen.begin(), seen.end(), shape) != seen.end()) continue;
+    if (g.ntiles >= 65536 || g.tvox > kMaxTileVoxels || g.tx + g.ty + g.tz > (int32_t)kRunMaxSteps) continue;
+    seen.push_back(shape);
+    cands.push_back(t);
+    if (w == 0) {
+      petalo_mlem_options_t a = opt;
+      a.scheme = PETALO_SCHEME_ATOMIC;
+      cands.push_back(a);
+    }
+  }
+  double best_ms = 1e300;
+  for (const petalo_mlem_options_t& cand : cands) {
+    petalo_mlem* c = create_context(fov, tof, lors, m, sensitivity, 1, cand, nullptr);
+    double ms = 1e300;
+    try {
+      petalo_osem_t id;
+      auto step = [&] {
+        if (petalo_mlem_next(c, nullptr, &id) != PETALO_OK) throw std::runtime_error(petalo_last_error());
+      };
+      step();
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int k = 0; k < 3; ++k) step();
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / 3.0;
+    } catch (...) {
+      petalo_mlem_destroy(c);
+      throw;
+    }
+    petalo_mlem_destroy(c);
+    // the first candidate is the default; a later one must beat the best by the margin
+    if (&cand == &cands[0] || ms < best_ms * (1.0 - kTuneMargin)) {
+      best_ms = ms;
+      best = cand;
+    }
+  }
+  return best;
+}
+
+}  // namespace
+
+extern "C" {
+
 int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petalo_tof_t* tof,
                        const petalo_lor_t* lors, size_t n_lors, const float* sensitivity, size_t n_subsets,
                        const petalo_mlem_options_t* options) {
   return guarded([&] {
     if (!out) return fail(PETALO_ERR_ARG, "out is NULL");
     *out = nullptr;
-    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}};
+    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}, 0};
     if (options) opt = *options;
     const std::string why = check_mlem_args(fov, tof, lors, n_lors, n_subsets, opt);
     if (!why.empty()) return fail(PETALO_ERR_ARG, why);
     require_device();
     HIP_CHECK(hipSetDevice(opt.device));
+    if (opt.scheme == PETALO_SCHEME_AUTO && opt.autotune)
+      opt = autotune(fov, tof, lors, n_lors, sensitivity, n_subsets, opt);
     *out = create_context(fov, tof, lors, n_lors, sensitivity, n_subsets, opt, nullptr);
     return PETALO_OK;
   });
@@ -804,6 +882,15 @@ int petalo_mlem_info(petalo_mlem_t* c, int32_t* scheme, uint64_t* n_runs_total) 
     for (uint64_t v : c->n_runs) t += v;
     *n_runs_total = t;
   }
+  return PETALO_OK;
+}
+
+int petalo_mlem_tile_shape(petalo_mlem_t* c, int32_t tile[3]) {
+  if (!c || !tile) return fail(PETALO_ERR_ARG, "NULL argument");
+  const bool t = c->scheme == PETALO_SCHEME_TILE;
+  tile[0] = t ? c->tiles.tx : 0;
+  tile[1] = t ? c->tiles.ty : 0;
+  tile[2] = t ? c->tiles.tz : 0;
   return PETALO_OK;
 }
 

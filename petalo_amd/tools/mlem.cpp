@@ -2,6 +2,7 @@
 //
 //   mlem <config.toml> <output-directory>
 //        [--lors FILE | --records FILE --discretize "R_MIN DR DZ DA" | --synthetic N [--tof-fwhm PS]] [--seed S]
+//        [--autotune]
 //
 // Reads the config, copies it to <output-directory>/mlem-config.toml, loads
 // the sensitivity image named in [attenuation_correction] (sizes must match
@@ -74,7 +75,7 @@ int main(int argc, char** argv) {
   if (argc < 3 || help) {
     std::printf("Maximum Likelyhood Expectation Maximization\n\n"
                 "usage: mlem <config.toml> <output-directory> [--lors FILE | --records FILE --discretize "
-                "\"R_MIN DR DZ DA\" | --synthetic N [--tof-fwhm PS]] [--seed S]\n");
+                "\"R_MIN DR DZ DA\" | --synthetic N [--tof-fwhm PS]] [--seed S] [--autotune]\n");
     return help ? 0 : 1;
   }
   const std::string cfg_path = argv[1], outdir = argv[2];
@@ -82,6 +83,7 @@ int main(int argc, char** argv) {
   size_t synth_n = 0;
   uint64_t seed = 1;
   float fwhm = 0.0f;
+  bool tune = false;
   for (int i = 3; i < argc; ++i) {
     const std::string a = argv[i];
     auto val = [&]() -> std::string {
@@ -94,6 +96,7 @@ int main(int argc, char** argv) {
     else if (a == "--synthetic") synth_n = std::strtoull(val().c_str(), nullptr, 10);
     else if (a == "--seed") seed = std::strtoull(val().c_str(), nullptr, 10);
     else if (a == "--tof-fwhm") fwhm = std::strtof(val().c_str(), nullptr);
+    else if (a == "--autotune") tune = true;
     else if (a == "-j" || a == "--mlem-threads") (void)val();  // accepted for compatibility; the GPU ignores it
     else die("unknown argument " + a);
   }
@@ -172,8 +175,9 @@ int main(int argc, char** argv) {
   progress.done("Loaded " + std::to_string(lors.size()) + " LORs");
 
   petalo_mlem_t* ctx = nullptr;
+  petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}, tune ? 1 : 0};
   check(petalo_mlem_create(&ctx, &cfg.fov, cfg.has_tof ? &cfg.tof : nullptr, lors.data(), lors.size(),
-                           sens.empty() ? nullptr : sens.data(), cfg.iterations_subsets, nullptr),
+                           sens.empty() ? nullptr : sens.data(), cfg.iterations_subsets, &opt),
         "creating MLEM context");
   std::vector<float> image(nv);
   const uint64_t total = cfg.iterations_number * cfg.iterations_subsets;

@@ -378,6 +378,29 @@ def test_tile_shapes(gpu, tile):
         assert e <= 1e-5
 
 
+def test_autotuned_context_matches_oracle(gpu):
+    """scheme AUTO with autotune: the chosen configuration (atomic, or a tile
+    shape) reports its shape and reconstructs as the oracle does."""
+    full, n = [90.0, 90.0, 60.0], [45, 45, 30]
+    rng = np.random.default_rng(17)
+    lors = random_lors(rng, 60000, radius=150.0, zhalf=80.0, dt_sigma_ps=84.93)
+    fov = P.FOV(full, n)
+    tof = P.Tof(42.47, 3.0)
+    m = P.Mlem(fov, lors, tof=tof, n_subsets=2, autotune=True)
+    scheme = m.info()["scheme"]
+    shape = m.tile_shape()
+    _say(f"autotune chose scheme {scheme}, tile {shape}")
+    assert (scheme == P.SCHEME_ATOMIC and shape == (0, 0, 0)) or (scheme == P.SCHEME_TILE and min(shape) > 0)
+    o = _run_oracle(full, n, lors, 4, tof=(42.47, 3.0), subsets=2)
+    for k in range(4):
+        g, _ = m.step()
+        assert rel_l2_finite(g.data, o[k]) <= 1e-5
+    m.close()
+    d = P.Mlem(fov, lors, tof=tof, n_subsets=2, tile=(16, 16, 0))
+    assert d.tile_shape()[:2] == (16, 16) and d.tile_shape()[2] > 0
+    d.close()
+
+
 def test_graph_replays_match_direct_launches(gpu):
     """Subset updates replayed as captured graphs (the default) give the
     images of direct launches, replay after replay, with host reads between."""
