@@ -378,6 +378,58 @@ def test_tile_shapes(gpu, tile):
         assert e <= 1e-5
 
 
+def test_rows_match_golden_fixture(gpu):
+    """The committed 1000-LOR row fixture (tests/golden/oracle_rows.npz): voxel
+    sequences, step counts and ties identical, weights bit-identical without
+    TOF and within 8 ulp with TOF (200 LORs)."""
+    r = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_rows.npz"))
+    lors = np.ascontiguousarray(r["lors"], np.float32).view(P.LOR_DTYPE).reshape(-1)
+    off, toff = r["offsets"], r["tof_offsets"]
+    tof = P.Tof(*(float(v) for v in r["tof"]))
+    matched = ties = 0
+    for g in range(len(r["grids_n"])):
+        fov = P.FOV(list(r["grids_full"][g]), list(r["grids_n"][g]))
+        sel = np.flatnonzero(r["grid"] == g)
+        idx, w, cnt, steps = P.system_matrix_rows(fov, lors[sel])
+        for j, k in enumerate(sel):
+            n = int(cnt[j])
+            assert np.array_equal(idx[j, :n], r["idx"][off[k]:off[k + 1]]), k
+            assert np.array_equal(w[j, :n], r["w"][off[k]:off[k + 1]]), k
+            assert int(steps[j]) == r["steps"][k]
+            ties += int(steps[j]) - n if steps[j] > 0 else 0
+            matched += 1
+        tsel = np.array([k for k in r["tof_lors"] if r["grid"][k] == g], np.int64)
+        if len(tsel):
+            tidx, tw, tcnt, _ = P.system_matrix_rows(fov, lors[tsel], tof=tof)
+            pos = {int(k): j for j, k in enumerate(r["tof_lors"])}
+            for j, k in enumerate(tsel):
+                q = pos[int(k)]
+                n = int(tcnt[j])
+                assert np.array_equal(tidx[j, :n], r["tof_idx"][toff[q]:toff[q + 1]])
+                assert ulps(tw[j, :n], r["tof_w"][toff[q]:toff[q + 1]]).max(initial=0) <= 8
+    assert ties == int(r["ties"].sum())
+    _say(f"golden rows: {matched} of 1000 voxel sequences identical, {ties} ties")
+
+
+@pytest.mark.parametrize("scheme", [P.SCHEME_TILE, P.SCHEME_ATOMIC])
+def test_mlem_matches_golden_fixture(gpu, scheme):
+    """Three MLEM iterations on the committed 8^3 fixture, without and with
+    TOF: within 1e-5 of the binary32 oracle images, and reported against the
+    binary64 ones."""
+    m = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_mlem8.npz"))
+    lors = np.ascontiguousarray(m["lors"], np.float32).view(P.LOR_DTYPE).reshape(-1)
+    fov = P.FOV(list(m["full"]), list(m["n"]))
+    for name, tof in (("notof", None), ("tof", P.Tof(*(float(v) for v in m["tof"])))):
+        it = P.Mlem(fov, lors, tof=tof, scheme=scheme)
+        for k in range(3):
+            g, _ = it.step()
+            e32 = rel_l2(g.data, m[f"{name}_f32"][k])
+            e64 = rel_l2(g.data, m[f"{name}_f64"][k])
+            _say(f"golden MLEM {name} scheme {scheme} iteration {k + 1}: relL2 vs f32 {e32:.2e}, vs f64 {e64:.2e}")
+            assert e32 <= 1e-5
+        it.close()
+
+
 def test_autotuned_context_matches_oracle(gpu):
     """scheme AUTO with autotune: the chosen configuration (atomic, or a tile
     shape) reports its shape and reconstructs as the oracle does."""
