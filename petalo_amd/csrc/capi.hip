@@ -171,6 +171,7 @@ struct petalo_mlem {
   std::vector<uint64_t> n_runs;      // per subset
   std::vector<DevBuf<uint32_t>> run_off;    // per subset: LOR-major run offsets (n + 1)
   std::vector<DevBuf<uint32_t>> run_pos;    // per subset: sorted position of each LOR-major run
+  std::vector<DevBuf<uint4>> pos4;          // per subset: the first four of them per LOR (k_pack_pos4)
   std::vector<DevBuf<uint32_t>> irregular;  // per subset: LORs with unbounded walks
   DevBuf<float> partial;             // per-run partial projections (largest subset)
   DevBuf<float> rbuf;                // 1/(p*ac) per LOR of the current subset
@@ -327,6 +328,12 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   starts[g.ntiles] = total;
   for (int k = g.ntiles - 1; k >= 0; --k)
     if (starts[k] == UINT64_MAX) starts[k] = starts[k + 1];
+  c->pos4[sub].alloc(std::max<size_t>(count, 1));
+  if (count) {
+    k_pack_pos4<<<grid_for(count), kBlock, 0, st>>>(c->run_off[sub].p, c->run_pos[sub].p, (uint32_t)count,
+                                                    c->pos4[sub].p);
+    HIP_CHECK(hipGetLastError());
+  }
   // Work items: chunks of one tile's runs holding about equal numbers of
   // replayed steps (kWorkItems in total, at least kMinItemSteps each),
   // largest first so that the last workgroups to start are the short ones.
@@ -377,7 +384,7 @@ void launch_reduce(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   const Lor* L = c->lors.p + lo;
   hipStream_t st = c->stream;
   k_lor_reduce<false><<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
-                                                            c->run_pos[sub].p, c->partial.p, c->rbuf.p);
+                                                            c->run_pos[sub].p, c->pos4[sub].p, c->partial.p, c->rbuf.p);
   HIP_CHECK(hipGetLastError());
   const size_t n_irr = c->irregular[sub].n;
   if (n_irr) {
@@ -605,6 +612,7 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->work.resize(n_subsets);
         c->run_off.resize(n_subsets);
         c->run_pos.resize(n_subsets);
+        c->pos4.resize(n_subsets);
         c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
@@ -912,6 +920,7 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->work.clear();
   c->run_off.clear();
   c->run_pos.clear();
+  c->pos4.clear();
   c->irregular.clear();
   c->partial.release();
   c->ac.release();
@@ -1052,7 +1061,8 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
           d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->dfov, c->dtof,
                            c->tiles};
           const uint32_t m = (uint32_t)c->set_size;
-          rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->partial.p,
+          rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
+                             c->partial.p,
                              c->rbuf.p, m, rblocks};
           rblocks += (m + kBlock * kReduceLors - 1) / (kBlock * kReduceLors);
           std::vector<int4> h(c->work[sub].n);
@@ -1395,7 +1405,9 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
         DevBuf<float> partial(total);
         k_fwd_tile<false><<<nw, 1024, ((size_t)g.tvox + 1) * sizeof(float), s>>>(unsorted.p, work.p, dmu, f, notof, g,
                                                                            partial.p, perm2.p);
-        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, pos.p, partial.p, rbuf.p);
+        DevBuf<uint4> pos4(b);
+        k_pack_pos4<<<grid_for(b), kBlock, 0, s>>>(offs.p, pos.p, b, pos4.p);
+        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, pos.p, pos4.p, partial.p, rbuf.p);
         HIP_CHECK(hipGetLastError());
       }
       k_scatter<false><<<nw, 1024, (size_t)g.tvox * sizeof(double), s>>>(unsorted.p, work.p,

@@ -414,50 +414,72 @@ inline unsigned reduce_grid(size_t n) {
   const size_t per_group = 256 * kReduceLors;
   return (unsigned)(8 * ((share + per_group - 1) / per_group));
 }
+// Positions of a LOR's first four runs' partials (kNoRun where it has
+// fewer; bit 31 of the fourth set when it has more, which run_off/run_pos
+// then list), so that the reduction waits on one 16-byte load before its
+// gathers instead of two dependent ones.
+constexpr uint32_t kNoRun = 0xffffffffu;
+constexpr uint32_t kMoreRuns = 0x80000000u;
+
 // Reduces LORs base + first + 64*u (u < kReduceLors, those below base + end).
 template <bool SENS>
 __device__ __forceinline__ void reduce_lors(const float* __restrict__ ac, uint32_t base, uint32_t first,
                                             uint32_t end, const uint32_t* __restrict__ run_off,
-                                            const uint32_t* __restrict__ run_pos,
+                                            const uint32_t* __restrict__ run_pos, const uint4* __restrict__ pos4,
                                             const float* __restrict__ partial, float* __restrict__ r_out) {
-  uint32_t idx[kReduceLors], b[kReduceLors], e[kReduceLors];
+  uint32_t idx[kReduceLors];
   bool on[kReduceLors];
+  uint32_t q[kReduceLors][4];
 #pragma unroll
   for (uint32_t u = 0; u < kReduceLors; ++u) {
     const uint32_t local = first + 64u * u;
     idx[u] = base + local;
     on[u] = local < end;
-    b[u] = on[u] ? run_off[idx[u]] : 0u;
-    e[u] = on[u] ? run_off[idx[u] + 1] : 0u;
+    const uint4 p4 = on[u] ? pos4[idx[u]] : make_uint4(kNoRun, kNoRun, kNoRun, kNoRun);
+    q[u][0] = p4.x;
+    q[u][1] = p4.y;
+    q[u][2] = p4.z;
+    q[u][3] = p4.w;
   }
-  uint32_t q[kReduceLors][4];
-#pragma unroll
-  for (uint32_t u = 0; u < kReduceLors; ++u)
-#pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) q[u][k] = b[u] + k < e[u] ? run_pos[b[u] + k] : 0u;
   float v[kReduceLors][4];
 #pragma unroll
   for (uint32_t u = 0; u < kReduceLors; ++u)
 #pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) v[u][k] = b[u] + k < e[u] ? partial[q[u][k]] : 0.0f;
+    for (uint32_t k = 0; k < 4; ++k) v[u][k] = q[u][k] != kNoRun ? partial[q[u][k] & ~kMoreRuns] : 0.0f;
 #pragma unroll
   for (uint32_t u = 0; u < kReduceLors; ++u) {
     if (!on[u]) continue;
     float p = 0.0f;
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k)
-      if (b[u] + k < e[u]) p += v[u][k];
-    for (uint32_t k = b[u] + 4; k < e[u]; ++k) p += partial[run_pos[k]];
+      if (q[u][k] != kNoRun) p += v[u][k];
+    if (q[u][3] != kNoRun && (q[u][3] & kMoreRuns)) {
+      const uint32_t e = run_off[idx[u] + 1];
+      for (uint32_t k = run_off[idx[u]] + 4; k < e; ++k) p += partial[run_pos[k]];
+    }
     // MLEM: r = 1/(p*ac); sensitivity: p adapted to exp(-p), r = 1/exp(-p)
     // (projector.rs:67-74,144-149)
     r_out[idx[u]] = SENS ? 1.0f / expf(-p) : 1.0f / (p * ac[idx[u]]);
   }
 }
 
+// pos4 of LORs 0..n-1 from run_off and run_pos.
+__global__ void k_pack_pos4(const uint32_t* __restrict__ run_off, const uint32_t* __restrict__ run_pos, uint32_t n,
+                            uint4* __restrict__ pos4) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = run_off[i], e = run_off[i + 1];
+  uint32_t q[4];
+  for (uint32_t k = 0; k < 4; ++k) q[k] = b + k < e ? run_pos[b + k] : kNoRun;
+  if (e - b > 4) q[3] |= kMoreRuns;
+  pos4[i] = make_uint4(q[0], q[1], q[2], q[3]);
+}
+
 template <bool SENS>
 __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac, uint32_t n,
                                                     const uint32_t* __restrict__ run_off,
                                                     const uint32_t* __restrict__ run_pos,
+                                                    const uint4* __restrict__ pos4,
                                                     const float* __restrict__ partial, float* __restrict__ r_out) {
   const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
   const uint32_t share = (n + 7u) / 8u;
@@ -465,7 +487,7 @@ __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac
   const uint32_t first = slot * blockDim.x * kReduceLors + wave * 64u * kReduceLors + lane;
   const uint32_t base = xcd * share;
   const uint32_t end = base + share < n ? share : (base < n ? n - base : 0u);
-  reduce_lors<SENS>(ac, base, first, end, run_off, run_pos, partial, r_out);
+  reduce_lors<SENS>(ac, base, first, end, run_off, run_pos, pos4, partial, r_out);
 }
 
 // Per-reconstruction tables of the batched small kernels; blocks of one
@@ -474,6 +496,7 @@ struct ReduceDesc {
   const float* ac;
   const uint32_t* run_off;
   const uint32_t* run_pos;
+  const uint4* pos4;
   const float* partial;
   float* r;
   uint32_t n;
@@ -507,7 +530,7 @@ __global__ __launch_bounds__(256) void k_lor_reduce_batch(const ReduceDesc* __re
   const ReduceDesc q = d[r];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t first = (blockIdx.x - q.block0) * blockDim.x * kReduceLors + wave * 64u * kReduceLors + lane;
-  reduce_lors<false>(q.ac, 0u, first, q.n, q.run_off, q.run_pos, q.partial, q.r);
+  reduce_lors<false>(q.ac, 0u, first, q.n, q.run_off, q.run_pos, q.pos4, q.partial, q.r);
 }
 
 // LORs whose walk is not provably bounded: exact one-lane projection and
