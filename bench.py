@@ -36,6 +36,7 @@ WORKLOADS = {
     # name: (fov mm, voxels, n_lors, tof_fwhm_ps, tof (sigma_ps, cutoff) or None, sens_lors, subsets)
     "tof_mlem_120": (180.0, 120, 10_000_000, 200.0, (42.47, 3.0), 100_000_000, 1),
     "plain_mlem_60": (180.0, 60, 1_000_000, 0.0, None, 0, 1),
+    "precision_200": (200.0, 200, 50_000_000, 150.0, (31.85, 3.0), 0, 1),
 }
 
 
