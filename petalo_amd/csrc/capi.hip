@@ -104,7 +104,7 @@ template <class F> void parallel_for(size_t n, F&& f) {
 // Tiled scheme: number of work items per subset update (several rounds of
 // resident workgroups) and the least work per item, in replayed steps.
 constexpr uint64_t kWorkItems = 2048;
-constexpr uint64_t kScatterStepsPerVoxel = 16;
+constexpr uint64_t kItemStepsPerVoxel = 16;
 constexpr uint64_t kMinItemSteps = 8192;
 // Runs of a tile are sorted by blocks of 2^kLorBlockShift consecutive LORs
 // before length (see k_bin_runs).
@@ -169,7 +169,6 @@ struct petalo_mlem {
   TileGeom tiles{};
   std::vector<DevBuf<Run>> runs;     // per subset, sorted by tile
   std::vector<DevBuf<int4>> work;    // per subset: (tile, first run, end run, 0)
-  std::vector<DevBuf<int4>> scatter_work;  // per subset: coarser items for the scatter, or none
   std::vector<uint64_t> n_runs;      // per subset
   std::vector<DevBuf<uint32_t>> run_off;    // per subset: LOR-major run offsets (n + 1)
   std::vector<DevBuf<uint32_t>> run_pos;    // per subset: sorted position of each LOR-major run
@@ -366,15 +365,14 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     HIP_CHECK(hipGetLastError());
   }
   // Work items: chunks of one tile's runs holding about equal numbers of
-  // replayed steps (kWorkItems in total, at least kMinItemSteps each),
-  // largest first so that the last workgroups to start are the short ones.
-  // The scatter gets its own, coarser list where that many items would flush
-  // their tile often compared with the steps they replay: at least
-  // kScatterStepsPerVoxel replayed steps per LDS entry of the tile.
+  // replayed steps (kWorkItems in total), largest first so that the last
+  // workgroups to start are the short ones.  Each item loads (forward) and
+  // flushes (scatter) its whole tile, so it replays at least
+  // kItemStepsPerVoxel steps per LDS entry of the tile (and kMinItemSteps).
   uint64_t total_steps = 0;
   for (uint32_t v : summary) total_steps += v & kRunMaxSteps;
-  const uint64_t target = std::max<uint64_t>(kMinItemSteps, total_steps / kWorkItems + 1);
-  const uint64_t starget = std::max<uint64_t>(target, kScatterStepsPerVoxel * (uint64_t)g.tvox);
+  const uint64_t target = std::max<uint64_t>(std::max<uint64_t>(kMinItemSteps, total_steps / kWorkItems + 1),
+                                             kItemStepsPerVoxel * (uint64_t)g.tvox);
   const std::vector<std::pair<uint64_t, int4>> sized = tile_items(starts, summary, g.ntiles, target);
   std::vector<int4> items;
   items.reserve(sized.size());
@@ -387,14 +385,6 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   if (!items.empty())
     HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
   c->work[sub].n = items.size();
-  std::vector<int4> sitems;
-  if (starget > target)
-    for (auto& it : tile_items(starts, summary, g.ntiles, starget)) sitems.push_back(it.second);
-  c->scatter_work[sub].alloc(std::max<size_t>(sitems.size(), 1));
-  if (!sitems.empty())
-    HIP_CHECK(hipMemcpyAsync(c->scatter_work[sub].p, sitems.data(), sitems.size() * sizeof(int4),
-                             hipMemcpyHostToDevice, st));
-  c->scatter_work[sub].n = sitems.size();  // 0: the scatter takes the forward's items
   HIP_CHECK(hipStreamSynchronize(st));
   c->n_runs[sub] = total;
 }
@@ -439,9 +429,8 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count, bool timi
   if (timing) HIP_CHECK(hipEventRecord(c->ev[1], st));
   if (nw) {
     const size_t lds = (size_t)c->tiles.tvox * sizeof(double);
-    const bool coarse = c->scatter_work[sub].n > 0;
-    const int4* sw = coarse ? c->scatter_work[sub].p : c->work[sub].p;
-    const size_t nws = coarse ? c->scatter_work[sub].n : nw;
+    const int4* sw = c->work[sub].p;
+    const size_t nws = nw;
     if (c->tof_on)
       k_scatter<true><<<(unsigned)nws, 1024, lds, st>>>(c->runs[sub].p, sw, c->rbuf.p, c->dfov, c->dtof,
                                                        c->tiles, c->bp64.p);
@@ -635,7 +624,6 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->runs.resize(n_subsets);
         c->item_steps.resize(n_subsets);
         c->work.resize(n_subsets);
-        c->scatter_work.resize(n_subsets);
         c->run_off.resize(n_subsets);
         c->run_pos.resize(n_subsets);
         c->pos4.resize(n_subsets);
@@ -944,7 +932,6 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->counters.release();
   c->runs.clear();
   c->work.clear();
-  c->scatter_work.clear();
   c->run_off.clear();
   c->run_pos.clear();
   c->pos4.clear();
