@@ -561,9 +561,7 @@ def make_sensitivity_image_discrete(attenuation, detector_length, r_min, dr, dz,
         float(detector_length), float(r_min), float(dr), float(dz), float(da), int(adjust), int(seed),
         _ptr(out), C.byref(st)))
     img = Image(ofov, out)
-    if return_stats:
-        return img, Stats({f: int(getattr(st, f)) for f, _ in _Stats._fields_})
-    return img
+    return (img, _stats(st)) if return_stats else img
 
 
 # ---------------------------------------------------------------------------

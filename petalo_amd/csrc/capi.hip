@@ -700,7 +700,14 @@ petalo_mlem_options_t autotune(const petalo_fov_t* fov, const petalo_tof_t* tof,
   }
   double best_ms = 1e300;
   for (const petalo_mlem_options_t& cand : cands) {
-    petalo_mlem* c = create_context(fov, tof, lors, m, sensitivity, 1, cand, nullptr);
+    petalo_mlem* c = nullptr;
+    try {
+      c = create_context(fov, tof, lors, m, sensitivity, 1, cand, nullptr);
+    } catch (const HipError&) {
+      throw;  // a device failure is not a property of the candidate
+    } catch (const std::exception&) {
+      continue;  // e.g. out of host memory for this candidate's runs: skip it
+    }
     double ms = 1e300;
     try {
       petalo_osem_t id;
