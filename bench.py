@@ -315,6 +315,31 @@ def main():
                "cpu_model": model}
         log(f"[bench] CPU oracle: {cpu_it_s:.3f} iter/s on {threads} threads, "
             f"{cpu['one_thread']['value']:.4f} iter/s on one ({model})")
+        # the same oracle beside the paths either side of the reconstruction
+        if ingest is not None:
+            k = 1_000_000
+            t0 = time.perf_counter()
+            O.ingest(rec[:k], p, spec)
+            t_i = time.perf_counter() - t0
+            cpu["ingest"] = {"records_per_s": k / t_i, "threads": 1, "sample": f"the first {k} records ({t_i:.2f} s)"}
+            log(f"[bench] CPU oracle ingestion: {k / t_i:.3g} records/s")
+        if discrete is not None:
+            k = 1_000_000
+            pl, _ = O.discrete_lors(dgeom["r_min"], dgeom["dr"], dgeom["dz"], dgeom["da"], 600.0, 3, seed=0,
+                                    capacity=k)
+            t0 = time.perf_counter()
+            O.sensitivity(fov.full_size, fov.n, att.data, pl, threads=threads)
+            t_d = time.perf_counter() - t0
+            cpu["discrete_sensitivity"] = {"pairs_per_s": len(pl) / t_d, "threads": threads,
+                                           "sample": f"the first {len(pl)} element pairs ({t_d:.2f} s; "
+                                                     f"their generation not timed)"}
+            log(f"[bench] CPU oracle discrete sensitivity: {len(pl) / t_d:.3g} pairs/s")
+        if foms is not None:
+            t0 = time.perf_counter()
+            O.phantom_foms(img.fov.full_size, img.fov.n, img.data, 0)
+            t_f = time.perf_counter() - t0
+            cpu["foms"] = {"ms": t_f * 1e3, "threads": 1, "sample": "the same NEMA7 analysis, binary32 oracle"}
+            log(f"[bench] CPU oracle NEMA7 figures of merit: {t_f * 1e3:.0f} ms")
         # Parity on the same sample: one update from ones on the GPU.
         g = P.Mlem(fov, sample, tof=tof_obj, sensitivity=sens_img, n_subsets=1)
         img, _ = g.step()
