@@ -17,7 +17,6 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/petalo_hip.h"
@@ -55,52 +54,6 @@ DevTof to_dev(const petalo_tof_t* t) {
   return make_tof(true, t->sigma_ps, t->cutoff);
 }
 
-
-
-
-// Ordering key of a LOR inside its group: azimuth of its direction (folded
-// to [0, pi)), then signed transaxial distance from the axis, then axial
-// midpoint.  LORs close in this order are near-parallel neighbours: their
-// walks cross the same voxels, so a wavefront's gathers and LDS adds share
-// cache lines and its lanes take similar numbers of steps.
-uint64_t lor_order_key(const petalo_lor_t& l) {
-  const double dx = (double)l.p2[0] - l.p1[0], dy = (double)l.p2[1] - l.p1[1];
-  double phi = std::atan2(dy, dx);
-  double ux = dx, uy = dy;
-  if (phi < 0) { phi += M_PI; ux = -ux; uy = -uy; }
-  if (phi >= M_PI) phi -= M_PI;
-  const double ln = std::sqrt(ux * ux + uy * uy);
-  double s = 0.0;
-  if (ln > 0) s = ((double)l.p1[0] * uy - (double)l.p1[1] * ux) / ln;  // signed distance
-  const double zm = 0.5 * ((double)l.p1[2] + l.p2[2]);
-  auto q = [](double v, double lo, double hi, int bits) -> uint64_t {
-    double t = (v - lo) / (hi - lo);
-    t = std::min(std::max(t, 0.0), 1.0 - 1e-12);
-    return (uint64_t)(t * (double)(1ull << bits));
-  };
-  return (q(phi, 0.0, M_PI, 10) << 40) | (q(s, -400.0, 400.0, 20) << 20) | q(zm, -400.0, 400.0, 20);
-}
-
-// The LOR's TOF peak (its midpoint without TOF): where its weighted voxels
-// cluster.
-void lor_centre(const petalo_lor_t& l, bool tof, double out[3]) {
-  const double d[3] = {(double)l.p2[0] - l.p1[0], (double)l.p2[1] - l.p1[1], (double)l.p2[2] - l.p1[2]};
-  double t = 0.5;
-  const double L = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-  if (tof && L > 0) t = std::min(std::max(0.5 - 0.299792458 * (double)l.dt_ps / (2.0 * L), 0.0), 1.0);
-  for (int k = 0; k < 3; ++k) out[k] = (double)l.p1[k] + t * d[k];
-}
-
-template <class F> void parallel_for(size_t n, F&& f) {
-  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<std::thread> th;
-  for (unsigned t = 0; t < nt; ++t)
-    th.emplace_back([&, t] {
-      for (size_t k = t; k < n; k += nt) f(k);
-    });
-  for (auto& x : th) x.join();
-}
-
 // Tiled scheme: number of work items per subset update (several rounds of
 // resident workgroups) and the least work per item, in replayed steps.
 constexpr uint64_t kWorkItems = 2048;
@@ -115,32 +68,115 @@ constexpr int32_t kMaxTileVoxels = 20480;
 // Default tile budget: two workgroups' binary64 tiles per CU.
 constexpr int32_t kScatterEntries = 10240;
 
-// Reorders lors[lo, hi) (one subset) by the image tile that holds their TOF
-// peak, and within a tile by lor_order_key (direction, then offset, then
-// axial position).  LORs with nearby consecutive indices then cross the same
-// few tiles, so the per-LOR values that the tiled kernels read through the
-// runs of one tile come from a small, cache-resident index range.
-void order_lors(std::vector<petalo_lor_t>& lors, size_t lo, size_t hi, bool tof, const petalo_fov_t& fov,
-                const TileGeom& g) {
-  const size_t m = hi - lo;
-  if (m < 2) return;
-  std::vector<std::pair<std::pair<uint32_t, uint64_t>, uint32_t>> key(m);
-  parallel_for(m, [&](size_t k) {
-    const petalo_lor_t& l = lors[lo + k];
-    double c[3];
-    lor_centre(l, tof, c);
-    int32_t v[3];
-    for (int a = 0; a < 3; ++a) {
-      const double vs = (double)fov.full_size_mm[a] / fov.n[a];
-      const double x = std::floor((c[a] + 0.5 * fov.full_size_mm[a]) / vs);
-      v[a] = (int32_t)std::min(std::max(x, 0.0), (double)fov.n[a] - 1);
-    }
-    const uint32_t tile = (uint32_t)(v[0] / g.tx + g.ntx * (v[1] / g.ty + g.nty * (v[2] / g.tz)));
-    key[k] = {{tile, lor_order_key(l)}, (uint32_t)k};
-  });
-  std::sort(key.begin(), key.end());
-  std::vector<petalo_lor_t> tmp(lors.begin() + lo, lors.begin() + hi);
-  parallel_for(m, [&](size_t r) { lors[lo + r] = tmp[key[r].second]; });
+// LOR order inside a subset, computed on the device: by the image tile that
+// holds the LOR's TOF peak (its midpoint without TOF: where its weighted
+// voxels cluster), then by the azimuth of its direction (folded to [0, pi)),
+// its signed transaxial distance from the axis and its axial midpoint.  LORs
+// close in this order are near-parallel neighbours: their walks cross the
+// same voxels, so a wavefront's gathers and LDS adds share cache lines and its
+// lanes take similar numbers of steps.  Key: tile (bits 48-63, saturated),
+// azimuth (38-47), distance (20-37, +-400 mm), midpoint (0-19, +-400 mm).
+struct OrderParams {
+  double full[3];
+  uint32_t n[3];
+  int32_t tx, ty, tz, ntx, nty;
+  int32_t tof;
+};
+
+__device__ __forceinline__ uint64_t quantise(double v, double lo, double hi, int bits) {
+  double t = (v - lo) / (hi - lo);
+  t = fmin(fmax(t, 0.0), 1.0 - 1e-12);
+  return (uint64_t)(t * (double)(1ull << bits));
+}
+
+__global__ void k_order_keys(const Lor* __restrict__ lors, uint32_t n, OrderParams p, uint64_t* __restrict__ keys,
+                             uint32_t* __restrict__ idx) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const Lor l = lors[k];
+  const double d[3] = {(double)l.x2 - l.x1, (double)l.y2 - l.y1, (double)l.z2 - l.z1};
+  const double p1[3] = {l.x1, l.y1, l.z1};
+  double t = 0.5;
+  const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (p.tof && L > 0) t = fmin(fmax(0.5 - 0.299792458 * (double)l.dt / (2.0 * L), 0.0), 1.0);
+  int32_t v[3];
+  for (int a = 0; a < 3; ++a) {
+    const double c = p1[a] + t * d[a];
+    const double vs = p.full[a] / p.n[a];
+    const double x = floor((c + 0.5 * p.full[a]) / vs);
+    v[a] = (int32_t)fmin(fmax(x, 0.0), (double)p.n[a] - 1);
+  }
+  const uint64_t tile = (uint64_t)(v[0] / p.tx) + (uint64_t)p.ntx * ((uint64_t)(v[1] / p.ty) +
+                                                                     (uint64_t)p.nty * (uint64_t)(v[2] / p.tz));
+  double phi = atan2(d[1], d[0]);
+  double ux = d[0], uy = d[1];
+  if (phi < 0) {
+    phi += M_PI;
+    ux = -ux;
+    uy = -uy;
+  }
+  if (phi >= M_PI) phi -= M_PI;
+  const double ln = sqrt(ux * ux + uy * uy);
+  const double s = ln > 0 ? (p1[0] * uy - p1[1] * ux) / ln : 0.0;  // signed distance
+  const double zm = 0.5 * ((double)l.z1 + l.z2);
+  keys[k] = (min(tile, (uint64_t)0xffff) << 48) | (quantise(phi, 0.0, M_PI, 10) << 38) |
+            (quantise(s, -400.0, 400.0, 18) << 20) | quantise(zm, -400.0, 400.0, 20);
+  idx[k] = k;
+}
+
+__global__ void k_gather_lors(const Lor* __restrict__ in, const uint32_t* __restrict__ idx, uint32_t n,
+                              Lor* __restrict__ out) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) out[r] = in[idx[r]];
+}
+
+__global__ void k_lor_ac(const Lor* __restrict__ lors, size_t n, float* __restrict__ ac) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ac[i] = lors[i].ac;
+}
+
+// The context's LORs on the device: uploaded once, then (sort) each subset's
+// LORs reordered as above; the remainder past the last subset is kept as is.
+void place_lors(DevBuf<Lor>& out, hipStream_t st, const petalo_lor_t* lors, size_t n_lors, size_t n_subsets,
+                size_t m, bool tof, bool sort, const petalo_fov_t& fov, const TileGeom& g) {
+  out.alloc(std::max<size_t>(n_lors, 1));
+  if (!n_lors) return;
+  if (!sort || m < 2) {
+    HIP_CHECK(hipMemcpyAsync(out.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return;
+  }
+  DevBuf<Lor> raw(n_lors);
+  HIP_CHECK(hipMemcpyAsync(raw.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice, st));
+  OrderParams p;
+  for (int a = 0; a < 3; ++a) {
+    p.full[a] = fov.full_size_mm[a];
+    p.n[a] = fov.n[a];
+  }
+  p.tx = g.tx;
+  p.ty = g.ty;
+  p.tz = g.tz;
+  p.ntx = g.ntx;
+  p.nty = g.nty;
+  p.tof = tof ? 1 : 0;
+  DevBuf<uint64_t> keys(m), keys2(m);
+  DevBuf<uint32_t> idx(m), idx2(m);
+  size_t tmp_bytes = 0;
+  HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, idx.p, idx2.p, (int)m, 0, 64, st));
+  DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
+  for (size_t s = 0; s < n_subsets; ++s) {
+    const Lor* src = raw.p + s * m;
+    k_order_keys<<<grid_for(m), kBlock, 0, st>>>(src, (uint32_t)m, p, keys.p, idx.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, idx.p, idx2.p, (int)m, 0, 64, st));
+    k_gather_lors<<<grid_for(m), kBlock, 0, st>>>(src, idx2.p, (uint32_t)m, out.p + s * m);
+    HIP_CHECK(hipGetLastError());
+  }
+  const size_t done = n_subsets * m;
+  if (done < n_lors)
+    HIP_CHECK(hipMemcpyAsync(out.p + done, raw.p + done, (n_lors - done) * sizeof(Lor), hipMemcpyDeviceToDevice,
+                             st));
+  HIP_CHECK(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -600,12 +636,8 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
       const size_t nv = nvox(*fov);
 
       // LORs: reorder inside each subset (subset membership is unchanged).
-      std::vector<petalo_lor_t> h(lors, lors + n_lors);
-      if (opt.sort_lors)
-        for (size_t s = 0; s < n_subsets; ++s)
-          order_lors(h, s * c->set_size, (s + 1) * c->set_size, c->tof_on, *fov, tile_shape(*fov, opt.tile));
-      c->lors.alloc(std::max<size_t>(n_lors, 1));
-      if (n_lors) HIP_CHECK(hipMemcpy(c->lors.p, h.data(), n_lors * sizeof(Lor), hipMemcpyHostToDevice));
+      place_lors(c->lors, c->stream, lors, n_lors, n_subsets, c->set_size, c->tof_on, opt.sort_lors != 0, *fov,
+                 tile_shape(*fov, opt.tile));
 
       c->image.alloc(nv);
       c->bp.alloc(nv);
@@ -630,11 +662,10 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
-        {
-          std::vector<float> hac(n_lors);
-          for (size_t i = 0; i < n_lors; ++i) hac[i] = h[i].additive_correction;
-          c->ac.alloc(std::max<size_t>(n_lors, 1));
-          if (n_lors) HIP_CHECK(hipMemcpy(c->ac.p, hac.data(), n_lors * sizeof(float), hipMemcpyHostToDevice));
+        c->ac.alloc(std::max<size_t>(n_lors, 1));
+        if (n_lors) {
+          k_lor_ac<<<(unsigned)((n_lors + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(c->lors.p, n_lors, c->ac.p);
+          HIP_CHECK(hipGetLastError());
         }
         c->bp64.alloc(nv);
         fill_f64(c->bp64.p, 0.0, nv, c->stream);
