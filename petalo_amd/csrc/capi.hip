@@ -59,9 +59,9 @@ DevTof to_dev(const petalo_tof_t* t) {
 constexpr uint64_t kWorkItems = 2048;
 constexpr uint64_t kItemStepsPerVoxel = 16;
 constexpr uint64_t kMinItemSteps = 8192;
-// Runs of a tile are sorted by blocks of 2^kLorBlockShift consecutive LORs
-// before length (see k_bin_runs).
-constexpr uint32_t kLorBlockShift = 14;
+// Largest (tile, LOR block) table of the partial layout; past it the
+// partials stay in sorted order.
+constexpr uint64_t kMaxGroupTable = 1ull << 22;
 // Largest tile (LDS entries, row padding included): its binary64 copy fills
 // the 160 KiB of LDS of a CU.
 constexpr int32_t kMaxTileVoxels = 20480;
@@ -209,6 +209,8 @@ struct petalo_mlem {
   std::vector<DevBuf<uint32_t>> run_off;    // per subset: LOR-major run offsets (n + 1)
   std::vector<DevBuf<uint32_t>> run_pos;    // per subset: sorted position of each LOR-major run
   std::vector<DevBuf<uint4>> pos4;          // per subset: the first four of them per LOR (k_pack_pos4)
+  std::vector<DevBuf<int32_t>> group_delta; // per subset: partial layout (k_group_delta), empty for sorted order
+  std::vector<uint32_t> n_blocks;           // per subset: LOR blocks of the partial layout
   std::vector<DevBuf<uint32_t>> irregular;  // per subset: LORs with unbounded walks
   DevBuf<float> partial;             // per-run partial projections (largest subset)
   DevBuf<float> rbuf;                // 1/(p*ac) per LOR of the current subset
@@ -270,18 +272,59 @@ TileGeom make_tiles(const petalo_fov_t& fov, const int32_t shape[3]) {
   return g;
 }
 
-// Puts the runs in sorted order, records where each LOR-major run went and
+// Partial-projection layout.  The runs are replayed in (tile, LOR block)
+// order; their partials are stored in (LOR block, tile) order instead, each
+// (tile, block) group keeping its runs' replay order.  A group's runs are
+// consecutive in both orders, so the forward kernel's stores stay coalesced,
+// while the partials of one LOR block -- all that the reduction of its LORs
+// gathers -- lie together (about 4 runs x 2^14 LORs x 4 B) and stay
+// L2-resident while the block is reduced.  The position of sorted run j of
+// group (t, b) is j + delta[t * n_blocks + b].
+
+// Group sizes from the sorted keys (bits 32 and up tile, 12-31 LOR block):
+// count_bt[b * ntiles + t] = runs of group (t, b), start_tb[t * n_blocks + b]
+// = its first sorted index.  Both arrays start zeroed.
+__global__ void k_group_bounds(const uint64_t* __restrict__ sorted_keys, uint64_t n, uint32_t ntiles,
+                               uint32_t n_blocks, uint32_t* __restrict__ count_bt, uint32_t* __restrict__ start_tb) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint64_t gk = sorted_keys[j] >> 12;
+  const bool first = j == 0 || (sorted_keys[j - 1] >> 12) != gk;
+  const bool last = j + 1 == n || (sorted_keys[j + 1] >> 12) != gk;
+  if (!first && !last) return;
+  const uint32_t t = (uint32_t)(gk >> 20), b = (uint32_t)(gk & 0xfffffu);
+  // count = (last + 1) - first, added by the group's last and first runs (one
+  // lane for a one-run group); modulo 2^32 the order of the two adds is free
+  if (first) start_tb[(uint64_t)t * n_blocks + b] = (uint32_t)j;
+  if (last) atomicAdd(&count_bt[(uint64_t)b * ntiles + t], (uint32_t)(j + 1));
+  if (first) atomicSub(&count_bt[(uint64_t)b * ntiles + t], (uint32_t)j);
+}
+
+// delta[t * n_blocks + b] = stored start of group (t, b) - its sorted start.
+__global__ void k_group_delta(const uint32_t* __restrict__ stored_bt, const uint32_t* __restrict__ start_tb,
+                              uint32_t ntiles, uint32_t n_blocks, int32_t* __restrict__ delta) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (uint64_t)ntiles * n_blocks) return;
+  const uint32_t t = (uint32_t)(k / n_blocks), b = (uint32_t)(k % n_blocks);
+  delta[k] = (int32_t)(stored_bt[(uint64_t)b * ntiles + t] - start_tb[k]);
+}
+
+// Puts the runs in sorted order, records where each LOR-major run's partial
+// goes (its sorted index, moved by the group layout when delta is given) and
 // writes a (tile << 12) | length summary of the sorted runs for the host.
 __global__ void k_place_runs(const Run* __restrict__ in, const uint64_t* __restrict__ sorted_keys,
                              const uint32_t* __restrict__ perm, uint64_t n, Run* __restrict__ out,
-                             uint32_t* __restrict__ pos, uint32_t* __restrict__ summary) {
+                             uint32_t* __restrict__ pos, uint32_t* __restrict__ summary,
+                             const int32_t* __restrict__ delta, uint32_t n_blocks) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t src = perm[j];
   const Run ru = in[src];
   out[j] = ru;
-  pos[src] = (uint32_t)j;
-  summary[j] = ((uint32_t)(sorted_keys[j] >> 32) << 12) | run_len(ru.code);
+  const uint64_t key = sorted_keys[j];
+  const uint32_t t = (uint32_t)(key >> 32);
+  pos[src] = delta ? (uint32_t)((int64_t)j + delta[(uint64_t)t * n_blocks + ((key >> 12) & 0xfffffu)]) : (uint32_t)j;
+  summary[j] = (t << 12) | run_len(ru.code);
 }
 
 __global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
@@ -383,9 +426,35 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total, 0,
                                                  end_bit, st));
+    // partial layout by (LOR block, tile) where its table stays small
+    const uint32_t n_blocks = (uint32_t)((count + (1u << kLorBlockShift) - 1) >> kLorBlockShift);
+    const uint64_t n_groups = (uint64_t)g.ntiles * n_blocks;
+    c->group_delta[sub].alloc(0);
+    c->n_blocks[sub] = 0;
+    if (n_blocks > 1 && n_groups <= kMaxGroupTable) {
+      DevBuf<uint32_t> count_bt(n_groups), start_tb(n_groups), stored_bt(n_groups);
+      HIP_CHECK(hipMemsetAsync(count_bt.p, 0, n_groups * 4, st));
+      HIP_CHECK(hipMemsetAsync(start_tb.p, 0, n_groups * 4, st));
+      k_group_bounds<<<grid64(total), kBlock, 0, st>>>(keys2.p, total, (uint32_t)g.ntiles, n_blocks, count_bt.p,
+                                                       start_tb.p);
+      HIP_CHECK(hipGetLastError());
+      size_t scan_bytes = 0;
+      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count_bt.p, stored_bt.p, (int)n_groups, st));
+      DevBuf<unsigned char> scan_tmp(std::max<size_t>(scan_bytes, 1));
+      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, count_bt.p, stored_bt.p, (int)n_groups, st));
+      c->group_delta[sub].alloc(n_groups);
+      k_group_delta<<<grid64(n_groups), kBlock, 0, st>>>(stored_bt.p, start_tb.p, (uint32_t)g.ntiles, n_blocks,
+                                                        c->group_delta[sub].p);
+      HIP_CHECK(hipGetLastError());
+      c->n_blocks[sub] = n_blocks;
+      // the temporaries are freed at scope end: wait for their users first
+      HIP_CHECK(hipStreamSynchronize(st));
+    }
     DevBuf<uint32_t> dsum(total);
     k_place_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, keys2.p, perm2.p, total, c->runs[sub].p,
-                                                   c->run_pos[sub].p, dsum.p);
+                                                   c->run_pos[sub].p, dsum.p,
+                                                   c->n_blocks[sub] ? c->group_delta[sub].p : nullptr,
+                                                   c->n_blocks[sub]);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(summary.data(), dsum.p, total * 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
@@ -452,12 +521,15 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count, bool timi
   const size_t nw = c->work[sub].n;
   if (nw) {
     const size_t lds = ((size_t)c->tiles.tvox + 1) * sizeof(float);
+    const int32_t* delta = c->n_blocks[sub] ? c->group_delta[sub].p : nullptr;
     if (c->tof_on)
       k_fwd_tile<true><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->image.p, c->dfov,
-                                                        c->dtof, c->tiles, c->partial.p);
+                                                        c->dtof, c->tiles, c->partial.p, nullptr, delta,
+                                                        c->n_blocks[sub], kLorBlockShift);
     else
       k_fwd_tile<false><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->image.p, c->dfov,
-                                                         c->dtof, c->tiles, c->partial.p);
+                                                         c->dtof, c->tiles, c->partial.p, nullptr, delta,
+                                                         c->n_blocks[sub], kLorBlockShift);
     HIP_CHECK(hipGetLastError());
   }
   if (timing) HIP_CHECK(hipEventRecord(c->ev[4], st));
@@ -659,6 +731,8 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->run_off.resize(n_subsets);
         c->run_pos.resize(n_subsets);
         c->pos4.resize(n_subsets);
+        c->group_delta.resize(n_subsets);
+        c->n_blocks.resize(n_subsets, 0);
         c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
@@ -973,6 +1047,7 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->run_off.clear();
   c->run_pos.clear();
   c->pos4.clear();
+  c->group_delta.clear();
   c->irregular.clear();
   c->partial.release();
   c->ac.release();
@@ -1110,7 +1185,8 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
         uint64_t total_steps = 0;
         for (size_t r = 0; r < n; ++r) {
           petalo_mlem* c = b->ctx[r];
-          d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->dfov, c->dtof,
+          d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p,
+                           c->n_blocks[sub] ? c->group_delta[sub].p : nullptr, c->n_blocks[sub], c->dfov, c->dtof,
                            c->tiles};
           const uint32_t m = (uint32_t)c->set_size;
           rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,

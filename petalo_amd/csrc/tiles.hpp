@@ -176,17 +176,30 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
   return count;
 }
 
+// Length field of a run's sort key (bits 0-11): longer runs first, in
+// buckets of 4 steps.  Lanes of a wavefront get runs of about equal length;
+// inside a bucket the (stable) sort keeps LOR order, so neighbouring LORs'
+// partials land side by side and the reduction's gathers share cache lines
+// (exact lengths cost the reduction 15% more time; LOR order alone costs the
+// replays 25-60%).
+constexpr uint32_t kLenBucketShift = 2;
+__host__ __device__ __forceinline__ uint64_t len_key(uint32_t code) {
+  return (uint64_t)(((kRunMaxSteps - run_len(code)) >> kLenBucketShift) << kLenBucketShift);
+}
+
 // Binning walk: FILL = false counts runs per LOR into n_runs[i]; FILL = true
 // writes them at runs[offset[i] ...] (LOR-major) with their sort keys:
 // tile (bits 32 and up), then block of 2^lor_shift consecutive LORs (bits
 // 12-31; LORs are ordered spatially, so a block's 1/(p*ac) values stay
-// cache-resident while its runs in a tile are replayed), then longest run
-// first (bits 0-11; lanes of a wavefront get runs of similar length).  With
-// lor_shift >= 32 the key is (tile << 12) | length field only.
+// cache-resident while its runs in a tile are replayed), then len_key
+// (bits 0-11).  With lor_shift >= 32 the key is (tile << 12) | len_key only.
 // Each lane bins kBinLors LORs, 256 apart, one after the other: walks differ
 // in length, and a lane's total over several LORs varies less than one
 // walk's, so the lanes of a wavefront finish closer together.
 constexpr uint32_t kBinLors = 8;
+// Runs of a tile are sorted by blocks of 2^kLorBlockShift consecutive LORs
+// before length.
+constexpr uint32_t kLorBlockShift = 14;
 inline unsigned bin_grid(size_t n) { return (unsigned)((n + 256 * kBinLors - 1) / (256 * kBinLors)); }
 
 template <bool TOF, bool FILL>
@@ -211,7 +224,7 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
             // lor_shift >= 32: no LOR-block field, key = (tile << 12) | ...
             const uint64_t blk = lor_shift >= 32 ? 0u : (uint64_t)(i >> lor_shift);
             const uint32_t tshift = lor_shift >= 32 ? 12u : 32u;
-            keys[base + k] = ((uint64_t)tile << tshift) | (blk << 12) | (kRunMaxSteps - run_len(ru.code));
+            keys[base + k] = ((uint64_t)tile << tshift) | (blk << 12) | len_key(ru.code);
           }
           ++k;
         },
@@ -319,16 +332,19 @@ __device__ __forceinline__ float replay_project(const Run& ru, const DevTof& t, 
 // (dynamic LDS: tvox + 1 words)
 // into LDS and replays its runs, writing each run's partial projection
 // (sum of w*x over the run's pushed steps, in walk order) at the run's
-// sorted position (coalesced; writing them LOR-major instead scatters 4-byte
-// stores over the whole array and costs more than the gather in
-// k_lor_reduce).  Each lane loads its next run record before replaying the
-// current one, so the record's latency hides behind the replay (the replay
-// itself touches only LDS).
+// sorted position j, or at j + delta[tile * n_blocks + (lor >> lor_shift)]
+// when a group layout is given (both coalesced; writing them LOR-major
+// instead scatters 4-byte stores over the whole array and costs more than
+// the gather in k_lor_reduce).  Each lane loads its next run record before
+// replaying the current one, so the record's latency hides behind the
+// replay (the replay itself touches only LDS).
 template <bool TOF>
 __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, const int4 wi,
                                               const float* __restrict__ img, const DevFov& f, const DevTof& t,
                                               const TileGeom& g, float* __restrict__ partial,
-                                              const uint32_t* __restrict__ perm) {
+                                              const uint32_t* __restrict__ perm,
+                                              const int32_t* __restrict__ delta = nullptr, uint32_t n_blocks = 0,
+                                              uint32_t lor_shift = 0) {
   extern __shared__ float timg[];
   const int32_t tl = wi.x;
   const int32_t tzi = tl / (g.ntx * g.nty);
@@ -356,11 +372,14 @@ __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, cons
   const int32_t last = wi.z - 1;
   const int32_t stride = (int32_t)blockDim.x;
   Run cur = runs[perm ? perm[j] : j];
+  const int32_t* drow = delta ? delta + (size_t)tl * n_blocks : nullptr;
   for (;;) {
+    // the store position's table read is issued before the replay hides it
+    const int32_t at = drow ? j + drow[cur.lor >> lor_shift] : j;
     const int32_t jn = j + stride;
     const int32_t jc = jn < last ? jn : last;  // in bounds; unused past the end
     const Run nxt = runs[perm ? perm[jc] : jc];
-    partial[j] = guard ? replay_project<TOF, true>(cur, t, g, timg) : replay_project<TOF, false>(cur, t, g, timg);
+    partial[at] = guard ? replay_project<TOF, true>(cur, t, g, timg) : replay_project<TOF, false>(cur, t, g, timg);
     if (jn > last) break;
     cur = nxt;
     j = jn;
@@ -371,8 +390,10 @@ template <bool TOF>
 __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs, const int4* __restrict__ work,
                                                    const float* __restrict__ img, DevFov f, DevTof t, TileGeom g,
                                                    float* __restrict__ partial,
-                                                   const uint32_t* __restrict__ perm = nullptr) {
-  fwd_tile_body<TOF>(runs, work[blockIdx.x], img, f, t, g, partial, perm);
+                                                   const uint32_t* __restrict__ perm = nullptr,
+                                                   const int32_t* __restrict__ delta = nullptr,
+                                                   uint32_t n_blocks = 0, uint32_t lor_shift = 0) {
+  fwd_tile_body<TOF>(runs, work[blockIdx.x], img, f, t, g, partial, perm, delta, n_blocks, lor_shift);
 }
 
 // One reconstruction of a batch, as the batched tile kernels see it: the runs
@@ -383,6 +404,8 @@ struct ReconDesc {
   float* partial;
   const float* r;
   double* bp;
+  const int32_t* delta;  // partial layout (null: sorted order)
+  uint32_t n_blocks;
   DevFov f;
   DevTof t;
   TileGeom g;
@@ -394,7 +417,7 @@ __global__ __launch_bounds__(1024) void k_fwd_tile_batch(const ReconDesc* __rest
                                                          const int4* __restrict__ work) {
   const int4 wi = work[blockIdx.x];
   const ReconDesc& r = rd[wi.w];
-  fwd_tile_body<TOF>(r.runs, wi, r.image, r.f, r.t, r.g, r.partial, nullptr);
+  fwd_tile_body<TOF>(r.runs, wi, r.image, r.f, r.t, r.g, r.partial, nullptr, r.delta, r.n_blocks, kLorBlockShift);
 }
 
 // r = 1/(p*ac) per LOR (MLEM; 1/exp(-p) for the sensitivity image) with p the
