@@ -44,6 +44,33 @@ struct TileGeom {
   int32_t tvox;           // px*ty*tz: LDS entries per tile
 };
 
+// A lane's walk over the LDS entries k = first, first + stride, ... of a
+// tile with their local coordinates (k = lx + px * (ly + ty * lz)), advanced
+// by carries instead of a division per entry.
+struct TileCursor {
+  int32_t k, lx, ly, lz;
+  int32_t sx, sy, sz;  // stride = sx + px * (sy + ty * sz), sx < px, sy < ty
+  __device__ __forceinline__ TileCursor(const TileGeom& g, int32_t first, int32_t stride) {
+    k = first;
+    lx = first % g.px;
+    ly = (first / g.px) % g.ty;
+    lz = first / (g.px * g.ty);
+    sx = stride % g.px;
+    sy = (stride / g.px) % g.ty;
+    sz = stride / (g.px * g.ty);
+  }
+  __device__ __forceinline__ void next(const TileGeom& g, int32_t stride) {
+    k += stride;
+    lx += sx;
+    const int32_t cx = lx >= g.px ? 1 : 0;
+    lx -= cx ? g.px : 0;
+    ly += sy + cx;
+    const int32_t cy = ly >= g.ty ? 1 : 0;
+    ly -= cy ? g.ty : 0;
+    lz += sz + cy;
+  }
+};
+
 // One run: the walk state at the first pushed step of a LOR inside a tile
 // (40 bytes).
 struct __align__(8) Run {
@@ -328,6 +355,47 @@ __device__ __forceinline__ float replay_project(const Run& ru, const DevTof& t, 
   return TOF ? t.amp * p : p;
 }
 
+// Image voxels a lane of the forward kernel loads before writing them to LDS.
+constexpr int32_t kTileLoads = 8;
+
+// s_waitcnt vmcnt(0) (gfx9 encoding: vmcnt bits 0-3 and 14-15 zero, expcnt
+// and lgkmcnt at their maxima, so not waited for).
+constexpr int kWaitVmcnt0 = 0x0f70;
+
+// Store offset of the forward kernel without a partial layout (never
+// written; a global, not a constant, so that loads through either table
+// stay global loads).
+__device__ int32_t kZeroDelta[1] = {0};
+
+// A lane's runs j, j + stride, ... <= last of a forward work item: each
+// run's partial is stored at j + drow[lor >> dshift].  The run's table entry
+// and the next run's record are loaded before the replay, so neither load's
+// latency stalls it.  PERM: the runs are read through perm.
+template <bool TOF, bool PERM>
+__device__ __forceinline__ void fwd_runs(const Run* __restrict__ runs, const uint32_t* __restrict__ perm, int32_t j,
+                                         const int32_t last, const bool guard, const DevTof& t, const TileGeom& g,
+                                         const float* __restrict__ timg, float* __restrict__ partial,
+                                         const int32_t* __restrict__ drow, const uint32_t dshift) {
+  const int32_t stride = (int32_t)blockDim.x;
+  Run cur = runs[PERM ? perm[j] : j];
+  // Wait for the first record here: the loop's later records are register
+  // copies made after their loads completed, and without this wait the
+  // replay loop would wait on the vector-memory counter at every run (that
+  // counter also holds the previous run's partial store).
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  for (;;) {
+    const int32_t dcur = drow[cur.lor >> dshift];
+    const int32_t jn = j + stride;
+    const int32_t jc = jn < last ? jn : last;  // in bounds; unused past the end
+    const Run nxt = runs[PERM ? perm[jc] : jc];
+    const float p = guard ? replay_project<TOF, true>(cur, t, g, timg) : replay_project<TOF, false>(cur, t, g, timg);
+    partial[j + dcur] = p;
+    if (jn > last) break;
+    cur = nxt;
+    j = jn;
+  }
+}
+
 // Forward, tiled: one workgroup per work item copies its tile of the image
 // (dynamic LDS: tvox + 1 words)
 // into LDS and replays its runs, writing each run's partial projection
@@ -355,13 +423,27 @@ __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, cons
   volatile int32_t* odd_tile = reinterpret_cast<volatile int32_t*>(timg + g.tvox);
   if (threadIdx.x == 0) *odd_tile = 0;
   __syncthreads();
+  // the tile is read kTileLoads entries per lane at a time, all loads in
+  // flight before the first LDS write
   bool odd = false;
-  for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) {
-    const int32_t lx = k % g.px, ly = (k / g.px) % g.ty, lz = k / (g.px * g.ty);
-    const int32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
-    const float v = (lx < g.tx && x < f.nx && y < f.ny && z < f.nz) ? img[x + f.nx * (y + f.ny * z)] : 0.0f;
-    timg[k] = v;
-    odd |= !(fabsf(v) <= 3.402823466e38f);  // inf or NaN
+  const int32_t stride = (int32_t)blockDim.x;
+  TileCursor cu(g, (int32_t)threadIdx.x, stride);
+  while (cu.k < g.tvox) {
+    const int32_t k0 = cu.k;
+    float v[kTileLoads];
+#pragma unroll
+    for (int32_t u = 0; u < kTileLoads; ++u) {
+      const int32_t x = x0 + cu.lx, y = y0 + cu.ly, z = z0 + cu.lz;
+      const bool in = cu.k < g.tvox && cu.lx < g.tx && x < f.nx && y < f.ny && z < f.nz;
+      v[u] = in ? img[x + f.nx * (y + f.ny * z)] : 0.0f;
+      cu.next(g, stride);
+    }
+#pragma unroll
+    for (int32_t u = 0; u < kTileLoads; ++u) {
+      const int32_t k = k0 + u * stride;
+      if (k < g.tvox) timg[k] = v[u];
+      odd |= !(fabsf(v[u]) <= 3.402823466e38f);  // inf or NaN
+    }
   }
   if (odd) *odd_tile = 1;
   __syncthreads();
@@ -369,21 +451,13 @@ __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, cons
   const bool guard = *odd_tile != 0;
   int32_t j = wi.y + (int32_t)threadIdx.x;
   if (j >= wi.z) return;
-  const int32_t last = wi.z - 1;
-  const int32_t stride = (int32_t)blockDim.x;
-  Run cur = runs[perm ? perm[j] : j];
-  const int32_t* drow = delta ? delta + (size_t)tl * n_blocks : nullptr;
-  for (;;) {
-    // the store position's table read is issued before the replay hides it
-    const int32_t at = drow ? j + drow[cur.lor >> lor_shift] : j;
-    const int32_t jn = j + stride;
-    const int32_t jc = jn < last ? jn : last;  // in bounds; unused past the end
-    const Run nxt = runs[perm ? perm[jc] : jc];
-    partial[at] = guard ? replay_project<TOF, true>(cur, t, g, timg) : replay_project<TOF, false>(cur, t, g, timg);
-    if (jn > last) break;
-    cur = nxt;
-    j = jn;
-  }
+  // without a layout every run reads the zero entry
+  const int32_t* drow = delta ? delta + (size_t)tl * n_blocks : kZeroDelta;
+  const uint32_t dshift = delta ? lor_shift : 31u;
+  if (perm)
+    fwd_runs<TOF, true>(runs, perm, j, wi.z - 1, guard, t, g, timg, partial, drow, dshift);
+  else
+    fwd_runs<TOF, false>(runs, perm, j, wi.z - 1, guard, t, g, timg, partial, drow, dshift);
 }
 
 template <bool TOF>
@@ -607,6 +681,48 @@ __global__ __launch_bounds__(256) void k_fwd_irregular(const Lor* __restrict__ l
 // Backprojection, tiled: one workgroup per work item (tile, first run, end
 // run).  Lanes keep two runs in flight: the record of the run after next and
 // the 1/(p*ac) of the next run load while the current run is replayed.
+// r of every run when none is given (never written; a global, so that
+// loads through either table stay global loads).
+__device__ float kOneR[1] = {1.0f};
+
+// A lane's runs j, j + stride, ... <= last of a backprojection work item,
+// added to the LDS tile with r = r_src[lor & r_mask].  Records are loaded
+// two runs ahead and r one run ahead, so no load's latency stalls a replay;
+// the first records are waited for before the loop, as in fwd_runs.
+// PERM: the runs are read through perm.
+template <bool TOF, bool PERM>
+__device__ __forceinline__ void scatter_runs(const Run* __restrict__ runs, const uint32_t* __restrict__ perm,
+                                             int32_t j, const int32_t last, const float* __restrict__ r_src,
+                                             const uint32_t r_mask, const DevTof& t, const TileGeom& g,
+                                             double* tile) {
+  const int32_t stride = (int32_t)blockDim.x;
+  const int32_t j1 = j + stride;
+  const int32_t j1c = j1 < last ? j1 : last;
+  Run cur = runs[PERM ? perm[j] : j];
+  Run nxt = runs[PERM ? perm[j1c] : j1c];
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  float r = r_src[cur.lor & r_mask];
+  char* base = reinterpret_cast<char*>(tile);
+  auto add = [&](uint32_t off, float v) { atomicAdd(reinterpret_cast<double*>(base + off), (double)v); };
+  for (;;) {
+    const int32_t jn = j + stride;
+    const int32_t jnn = jn + stride;
+    const float rn = r_src[nxt.lor & r_mask];
+    const int32_t jnc = jnn < last ? jnn : last;
+    const Run nn = runs[PERM ? perm[jnc] : jnc];
+    // without a non-finite r in the wavefront, zero-length steps may add +0
+    if (__all(fabsf(r) <= 3.402823466e38f))
+      replay_run<TOF, false>(cur, r, t, g, add);
+    else
+      replay_run<TOF, true>(cur, r, t, g, add);
+    if (jn > last) break;
+    cur = nxt;
+    nxt = nn;
+    r = rn;
+    j = jn;
+  }
+}
+
 // r_in == nullptr backprojects the weights themselves (r = 1).  With perm,
 // sorted run j is runs[perm[j]] (runs left in LOR-major order).
 template <bool TOF>
@@ -618,34 +734,15 @@ __device__ __forceinline__ void scatter_body(const Run* __restrict__ runs, const
   const int32_t tl = wi.x;
   for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) tile[k] = 0.0;
   __syncthreads();
-  int32_t j = wi.y + (int32_t)threadIdx.x;
+  const int32_t j = wi.y + (int32_t)threadIdx.x;
   if (j < wi.z) {
-    const int32_t last = wi.z - 1;
-    const int32_t stride = (int32_t)blockDim.x;
-    const int32_t j1 = j + stride;
-    const int32_t j1c = j1 < last ? j1 : last;
-    Run cur = runs[perm ? perm[j] : j];
-    Run nxt = runs[perm ? perm[j1c] : j1c];
-    float r = r_in ? r_in[cur.lor] : 1.0f;
-    for (;;) {
-      const int32_t jn = j + stride;
-      const int32_t jnn = jn + stride;
-      const float rn = r_in ? r_in[nxt.lor] : 1.0f;
-      const int32_t jnc = jnn < last ? jnn : last;
-      const Run nn = runs[perm ? perm[jnc] : jnc];
-      char* base = reinterpret_cast<char*>(tile);
-      auto add = [&](uint32_t off, float v) { atomicAdd(reinterpret_cast<double*>(base + off), (double)v); };
-      // without a non-finite r in the wavefront, zero-length steps may add +0
-      if (__all(fabsf(r) <= 3.402823466e38f))
-        replay_run<TOF, false>(cur, r, t, g, add);
-      else
-        replay_run<TOF, true>(cur, r, t, g, add);
-      if (jn > last) break;
-      cur = nxt;
-      nxt = nn;
-      r = rn;
-      j = jn;
-    }
+    // without r every run reads the one entry of kOneR
+    const float* r_src = r_in ? r_in : kOneR;
+    const uint32_t r_mask = r_in ? 0xffffffffu : 0u;
+    if (perm)
+      scatter_runs<TOF, true>(runs, perm, j, wi.z - 1, r_src, r_mask, t, g, tile);
+    else
+      scatter_runs<TOF, false>(runs, perm, j, wi.z - 1, r_src, r_mask, t, g, tile);
   }
   __syncthreads();
   // flush the tile: consecutive lanes take consecutive x, so each wavefront
@@ -654,12 +751,10 @@ __device__ __forceinline__ void scatter_body(const Run* __restrict__ runs, const
   const int32_t tyi = (tl / g.ntx) % g.nty;
   const int32_t txi = tl % g.ntx;
   const int32_t x0 = txi * g.tx, y0 = tyi * g.ty, z0 = tzi * g.tz;
-  for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) {
-    const double v = tile[k];
-    if (v == 0.0) continue;
-    const int32_t lx = k % g.px, ly = (k / g.px) % g.ty, lz = k / (g.px * g.ty);
-    const int32_t x = x0 + lx, y = y0 + ly, z = z0 + lz;
-    if (lx < g.tx && x < f.nx && y < f.ny && z < f.nz) unsafeAtomicAdd(bp + x + f.nx * (y + f.ny * z), v);
+  for (TileCursor cu(g, (int32_t)threadIdx.x, (int32_t)blockDim.x); cu.k < g.tvox; cu.next(g, (int32_t)blockDim.x)) {
+    const double v = tile[cu.k];
+    const int32_t x = x0 + cu.lx, y = y0 + cu.ly, z = z0 + cu.lz;
+    if (v != 0.0 && cu.lx < g.tx && x < f.nx && y < f.ny && z < f.nz) unsafeAtomicAdd(bp + x + f.nx * (y + f.ny * z), v);
   }
 }
 
