@@ -61,7 +61,9 @@ constexpr uint64_t kItemStepsPerVoxel = 16;
 constexpr uint64_t kMinItemSteps = 8192;
 // Largest (tile, LOR block) table of the partial layout; past it the
 // partials stay in sorted order.
-constexpr uint64_t kMaxGroupTable = 1ull << 22;
+// LDS entries of k_lor_reduce_win (80 KiB: two workgroups per CU); larger
+// LOR-block windows are gathered from memory.
+constexpr uint32_t kReduceWindow = 20480;
 // Largest tile (LDS entries, row padding included): its binary64 copy fills
 // the 160 KiB of LDS of a CU.
 constexpr int32_t kMaxTileVoxels = 20480;
@@ -209,8 +211,10 @@ struct petalo_mlem {
   std::vector<DevBuf<uint32_t>> run_off;    // per subset: LOR-major run offsets (n + 1)
   std::vector<DevBuf<uint32_t>> run_pos;    // per subset: sorted position of each LOR-major run
   std::vector<DevBuf<uint4>> pos4;          // per subset: the first four of them per LOR (k_pack_pos4)
-  std::vector<DevBuf<int32_t>> group_delta; // per subset: partial layout (k_group_delta), empty for sorted order
-  std::vector<uint32_t> n_blocks;           // per subset: LOR blocks of the partial layout
+  std::vector<DevBuf<uint32_t>> lpos;       // per subset: partial position of each sorted run
+  std::vector<uint32_t> n_blocks;           // per subset: LOR blocks of 2^kWinShift of the partial layout
+  std::vector<DevBuf<uint32_t>> win;        // per subset: partial window of each LOR block (n_blocks + 1)
+  std::vector<uint32_t> max_win;            // per subset: largest window
   std::vector<DevBuf<uint32_t>> irregular;  // per subset: LORs with unbounded walks
   DevBuf<float> partial;             // per-run partial projections (largest subset)
   DevBuf<float> rbuf;                // 1/(p*ac) per LOR of the current subset
@@ -272,59 +276,54 @@ TileGeom make_tiles(const petalo_fov_t& fov, const int32_t shape[3]) {
   return g;
 }
 
-// Partial-projection layout.  The runs are replayed in (tile, LOR block)
-// order; their partials are stored in (LOR block, tile) order instead, each
-// (tile, block) group keeping its runs' replay order.  A group's runs are
-// consecutive in both orders, so the forward kernel's stores stay coalesced,
-// while the partials of one LOR block -- all that the reduction of its LORs
-// gathers -- lie together (about 4 runs x 2^14 LORs x 4 B) and stay
-// L2-resident while the block is reduced.  The position of sorted run j of
-// group (t, b) is j + delta[t * n_blocks + b].
+// Partial-projection layout.  The runs are replayed in (tile, block of
+// 2^kLorBlockShift LORs, length bucket) order; their partials are stored by
+// (block of 2^kWinShift LORs, tile) group instead, each group keeping its
+// runs' replay order.  A wavefront's runs share a tile and mostly a window
+// block, so the forward kernel's stores stay nearly contiguous, while the
+// partials of one window block -- all that the reduction of its LORs
+// gathers, about 4 runs x 4096 LORs x 4 B -- lie together and fit in LDS.
 
-// Group sizes from the sorted keys (bits 32 and up tile, 12-31 LOR block):
-// count_bt[b * ntiles + t] = runs of group (t, b), start_tb[t * n_blocks + b]
-// = its first sorted index.  Both arrays start zeroed.
-__global__ void k_group_bounds(const uint64_t* __restrict__ sorted_keys, uint64_t n, uint32_t ntiles,
-                               uint32_t n_blocks, uint32_t* __restrict__ count_bt, uint32_t* __restrict__ start_tb) {
+// wkey[j] = (window block << tile_bits) | tile of sorted run j.
+__global__ void k_window_keys(const Run* __restrict__ unsorted, const uint64_t* __restrict__ sorted_keys,
+                              const uint32_t* __restrict__ perm, uint64_t n, uint32_t tile_bits,
+                              uint64_t* __restrict__ wkey) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const uint64_t gk = sorted_keys[j] >> 12;
-  const bool first = j == 0 || (sorted_keys[j - 1] >> 12) != gk;
-  const bool last = j + 1 == n || (sorted_keys[j + 1] >> 12) != gk;
-  if (!first && !last) return;
-  const uint32_t t = (uint32_t)(gk >> 20), b = (uint32_t)(gk & 0xfffffu);
-  // count = (last + 1) - first, added by the group's last and first runs (one
-  // lane for a one-run group); modulo 2^32 the order of the two adds is free
-  if (first) start_tb[(uint64_t)t * n_blocks + b] = (uint32_t)j;
-  if (last) atomicAdd(&count_bt[(uint64_t)b * ntiles + t], (uint32_t)(j + 1));
-  if (first) atomicSub(&count_bt[(uint64_t)b * ntiles + t], (uint32_t)j);
+  const uint64_t blk = unsorted[perm[j]].lor >> kWinShift;
+  wkey[j] = (blk << tile_bits) | (sorted_keys[j] >> 32);
 }
 
-// delta[t * n_blocks + b] = stored start of group (t, b) - its sorted start.
-__global__ void k_group_delta(const uint32_t* __restrict__ stored_bt, const uint32_t* __restrict__ start_tb,
-                              uint32_t ntiles, uint32_t n_blocks, int32_t* __restrict__ delta) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (uint64_t)ntiles * n_blocks) return;
-  const uint32_t t = (uint32_t)(k / n_blocks), b = (uint32_t)(k % n_blocks);
-  delta[k] = (int32_t)(stored_bt[(uint64_t)b * ntiles + t] - start_tb[k]);
+// pos[perm[j]] = j.
+__global__ void k_invert_perm(const uint32_t* __restrict__ perm, uint64_t n, uint32_t* __restrict__ pos) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) pos[perm[j]] = (uint32_t)j;
+}
+
+// win[b] = first stored position of window block b (UINT32_MAX where the
+// block has no runs; the caller fills those in).
+__global__ void k_window_starts(const uint64_t* __restrict__ wkey, uint64_t n, uint32_t tile_bits,
+                                uint32_t* __restrict__ win) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint64_t b = wkey[j] >> tile_bits;
+  if (j == 0 || (wkey[j - 1] >> tile_bits) != b) win[b] = (uint32_t)j;
 }
 
 // Puts the runs in sorted order, records where each LOR-major run's partial
-// goes (its sorted index, moved by the group layout when delta is given) and
-// writes a (tile << 12) | length summary of the sorted runs for the host.
+// goes (lpos of its sorted index) and writes a (tile << 12) | length summary
+// of the sorted runs for the host.
 __global__ void k_place_runs(const Run* __restrict__ in, const uint64_t* __restrict__ sorted_keys,
                              const uint32_t* __restrict__ perm, uint64_t n, Run* __restrict__ out,
                              uint32_t* __restrict__ pos, uint32_t* __restrict__ summary,
-                             const int32_t* __restrict__ delta, uint32_t n_blocks) {
+                             const uint32_t* __restrict__ lpos) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t src = perm[j];
   const Run ru = in[src];
   out[j] = ru;
-  const uint64_t key = sorted_keys[j];
-  const uint32_t t = (uint32_t)(key >> 32);
-  pos[src] = delta ? (uint32_t)((int64_t)j + delta[(uint64_t)t * n_blocks + ((key >> 12) & 0xfffffu)]) : (uint32_t)j;
-  summary[j] = (t << 12) | run_len(ru.code);
+  pos[src] = lpos[j];
+  summary[j] = ((uint32_t)(sorted_keys[j] >> 32) << 12) | run_len(ru.code);
 }
 
 __global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
@@ -396,6 +395,10 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     off[count] = (uint32_t)total;
   }
   c->run_off[sub].alloc(count + 1);
+  c->n_blocks[sub] = (uint32_t)((count + (1u << kWinShift) - 1) >> kWinShift);
+  c->win[sub].alloc(c->n_blocks[sub] + 1);
+  c->max_win[sub] = 0;
+  HIP_CHECK(hipMemsetAsync(c->win[sub].p, 0, ((size_t)c->n_blocks[sub] + 1) * 4, st));  // empty windows
   HIP_CHECK(hipMemcpyAsync(c->run_off[sub].p, off.data(), (count + 1) * 4, hipMemcpyHostToDevice, st));
   c->irregular[sub].alloc(std::max<size_t>(irr.size(), 1));
   if (!irr.empty())
@@ -426,35 +429,46 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total, 0,
                                                  end_bit, st));
-    // partial layout by (LOR block, tile) where its table stays small
-    const uint32_t n_blocks = (uint32_t)((count + (1u << kLorBlockShift) - 1) >> kLorBlockShift);
-    const uint64_t n_groups = (uint64_t)g.ntiles * n_blocks;
-    c->group_delta[sub].alloc(0);
-    c->n_blocks[sub] = 0;
-    if (n_blocks > 1 && n_groups <= kMaxGroupTable) {
-      DevBuf<uint32_t> count_bt(n_groups), start_tb(n_groups), stored_bt(n_groups);
-      HIP_CHECK(hipMemsetAsync(count_bt.p, 0, n_groups * 4, st));
-      HIP_CHECK(hipMemsetAsync(start_tb.p, 0, n_groups * 4, st));
-      k_group_bounds<<<grid64(total), kBlock, 0, st>>>(keys2.p, total, (uint32_t)g.ntiles, n_blocks, count_bt.p,
-                                                       start_tb.p);
+    // partial layout by (window block, tile): a stable sort of the sorted
+    // runs by that key gives each run's stored position
+    int tile_bits = 1;
+    while ((1 << tile_bits) < g.ntiles) ++tile_bits;
+    int blk_bits = 1;
+    while ((1ull << blk_bits) < c->n_blocks[sub]) ++blk_bits;
+    c->lpos[sub].alloc(total);
+    std::vector<uint32_t> hw(c->n_blocks[sub] + 1, UINT32_MAX);
+    {
+      DevBuf<uint64_t> wkey(total), wkey2(total);
+      DevBuf<uint32_t> order(total);
+      k_window_keys<<<grid64(total), kBlock, 0, st>>>(unsorted.p, keys2.p, perm2.p, total, (uint32_t)tile_bits,
+                                                      wkey.p);
+      k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
       HIP_CHECK(hipGetLastError());
-      size_t scan_bytes = 0;
-      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count_bt.p, stored_bt.p, (int)n_groups, st));
-      DevBuf<unsigned char> scan_tmp(std::max<size_t>(scan_bytes, 1));
-      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, count_bt.p, stored_bt.p, (int)n_groups, st));
-      c->group_delta[sub].alloc(n_groups);
-      k_group_delta<<<grid64(n_groups), kBlock, 0, st>>>(stored_bt.p, start_tb.p, (uint32_t)g.ntiles, n_blocks,
-                                                        c->group_delta[sub].p);
+      size_t wbytes = 0;
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, wbytes, wkey.p, wkey2.p, perm.p, order.p, (int)total, 0,
+                                                   tile_bits + blk_bits, st));
+      DevBuf<unsigned char> wtmp(std::max<size_t>(wbytes, 1));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(wtmp.p, wbytes, wkey.p, wkey2.p, perm.p, order.p, (int)total, 0,
+                                                   tile_bits + blk_bits, st));
+      k_invert_perm<<<grid64(total), kBlock, 0, st>>>(order.p, total, c->lpos[sub].p);
+      HIP_CHECK(hipMemcpyAsync(c->win[sub].p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, st));
+      k_window_starts<<<grid64(total), kBlock, 0, st>>>(wkey2.p, total, (uint32_t)tile_bits, c->win[sub].p);
       HIP_CHECK(hipGetLastError());
-      c->n_blocks[sub] = n_blocks;
-      // the temporaries are freed at scope end: wait for their users first
+      HIP_CHECK(hipMemcpyAsync(hw.data(), c->win[sub].p, hw.size() * 4, hipMemcpyDeviceToHost, st));
+      // (also waits for the temporaries' users: they are freed at scope end)
       HIP_CHECK(hipStreamSynchronize(st));
     }
+    // blocks without runs start where the next one does
+    hw[c->n_blocks[sub]] = (uint32_t)total;
+    for (size_t b = c->n_blocks[sub]; b-- > 0;)
+      if (hw[b] == UINT32_MAX) hw[b] = hw[b + 1];
+    uint32_t mw = 0;
+    for (size_t b = 0; b < c->n_blocks[sub]; ++b) mw = std::max(mw, hw[b + 1] - hw[b]);
+    c->max_win[sub] = mw;
+    HIP_CHECK(hipMemcpyAsync(c->win[sub].p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, st));
     DevBuf<uint32_t> dsum(total);
     k_place_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, keys2.p, perm2.p, total, c->runs[sub].p,
-                                                   c->run_pos[sub].p, dsum.p,
-                                                   c->n_blocks[sub] ? c->group_delta[sub].p : nullptr,
-                                                   c->n_blocks[sub]);
+                                                   c->run_pos[sub].p, dsum.p, c->lpos[sub].p);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(summary.data(), dsum.p, total * 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
@@ -500,8 +514,16 @@ void launch_reduce(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   if (count == 0) return;
   const Lor* L = c->lors.p + lo;
   hipStream_t st = c->stream;
-  k_lor_reduce<false><<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
-                                                            c->run_pos[sub].p, c->pos4[sub].p, c->partial.p, c->rbuf.p);
+  if (c->n_blocks[sub]) {
+    const uint32_t lds = std::min(c->max_win[sub], kReduceWindow);
+    k_lor_reduce_win<false><<<c->n_blocks[sub], 1024, (size_t)lds * sizeof(float), st>>>(
+        c->ac.p + lo, (uint32_t)count, c->win[sub].p, lds, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
+        c->partial.p, c->rbuf.p);
+  } else {
+    k_lor_reduce<false><<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
+                                                              c->run_pos[sub].p, c->pos4[sub].p, c->partial.p,
+                                                              c->rbuf.p);
+  }
   HIP_CHECK(hipGetLastError());
   const size_t n_irr = c->irregular[sub].n;
   if (n_irr) {
@@ -521,15 +543,14 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count, bool timi
   const size_t nw = c->work[sub].n;
   if (nw) {
     const size_t lds = ((size_t)c->tiles.tvox + 1) * sizeof(float);
-    const int32_t* delta = c->n_blocks[sub] ? c->group_delta[sub].p : nullptr;
     if (c->tof_on)
       k_fwd_tile<true><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->image.p, c->dfov,
-                                                        c->dtof, c->tiles, c->partial.p, nullptr, delta,
-                                                        c->n_blocks[sub], kLorBlockShift);
+                                                        c->dtof, c->tiles, c->partial.p, nullptr,
+                                                        c->lpos[sub].p);
     else
       k_fwd_tile<false><<<(unsigned)nw, 1024, lds, st>>>(c->runs[sub].p, c->work[sub].p, c->image.p, c->dfov,
-                                                         c->dtof, c->tiles, c->partial.p, nullptr, delta,
-                                                         c->n_blocks[sub], kLorBlockShift);
+                                                         c->dtof, c->tiles, c->partial.p, nullptr,
+                                                         c->lpos[sub].p);
     HIP_CHECK(hipGetLastError());
   }
   if (timing) HIP_CHECK(hipEventRecord(c->ev[4], st));
@@ -580,7 +601,8 @@ void ensure_lds_attributes() {
   const void* fns[] = {(const void*)k_scatter<true>,        (const void*)k_scatter<false>,
                        (const void*)k_fwd_tile<true>,       (const void*)k_fwd_tile<false>,
                        (const void*)k_scatter_batch<true>,  (const void*)k_scatter_batch<false>,
-                       (const void*)k_fwd_tile_batch<true>, (const void*)k_fwd_tile_batch<false>};
+                       (const void*)k_fwd_tile_batch<true>, (const void*)k_fwd_tile_batch<false>,
+                       (const void*)k_lor_reduce_win<false>};
   for (const void* f : fns) HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   if (done.size() <= (size_t)dev) done.resize(dev + 1, false);
   done[dev] = true;
@@ -731,8 +753,10 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->run_off.resize(n_subsets);
         c->run_pos.resize(n_subsets);
         c->pos4.resize(n_subsets);
-        c->group_delta.resize(n_subsets);
+        c->lpos.resize(n_subsets);
         c->n_blocks.resize(n_subsets, 0);
+        c->win.resize(n_subsets);
+        c->max_win.resize(n_subsets, 0);
         c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
@@ -1047,7 +1071,8 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->run_off.clear();
   c->run_pos.clear();
   c->pos4.clear();
-  c->group_delta.clear();
+  c->lpos.clear();
+  c->win.clear();
   c->irregular.clear();
   c->partial.release();
   c->ac.release();
@@ -1185,9 +1210,8 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
         uint64_t total_steps = 0;
         for (size_t r = 0; r < n; ++r) {
           petalo_mlem* c = b->ctx[r];
-          d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p,
-                           c->n_blocks[sub] ? c->group_delta[sub].p : nullptr, c->n_blocks[sub], c->dfov, c->dtof,
-                           c->tiles};
+          d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->lpos[sub].p,
+                           c->dfov, c->dtof, c->tiles};
           const uint32_t m = (uint32_t)c->set_size;
           rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
                              c->partial.p,
@@ -1431,11 +1455,6 @@ __global__ void k_take_irregular(uint32_t* __restrict__ counts, uint32_t n, cons
   irr[atomicAdd(n_irr, 1u)] = lors[i];
 }
 
-// pos[perm[j]] = j.
-__global__ void k_invert_perm(const uint32_t* __restrict__ perm, uint64_t n, uint32_t* __restrict__ pos) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) pos[perm[j]] = (uint32_t)j;
-}
 
 // First sorted position of every tile; keys hold the tile from bit tshift.
 __global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uint32_t tshift,
@@ -1532,7 +1551,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       if (!mu_zero) {
         DevBuf<float> partial(total);
         k_fwd_tile<false><<<nw, 1024, ((size_t)g.tvox + 1) * sizeof(float), s>>>(unsorted.p, work.p, dmu, f, notof, g,
-                                                                           partial.p, perm2.p);
+                                                                           partial.p, perm2.p, nullptr);
         DevBuf<uint4> pos4(b);
         k_pack_pos4<<<grid_for(b), kBlock, 0, s>>>(offs.p, pos.p, b, pos4.p);
         k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, pos.p, pos4.p, partial.p, rbuf.p);

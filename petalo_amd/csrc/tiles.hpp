@@ -227,6 +227,9 @@ constexpr uint32_t kBinLors = 8;
 // Runs of a tile are sorted by blocks of 2^kLorBlockShift consecutive LORs
 // before length.
 constexpr uint32_t kLorBlockShift = 14;
+// The partial projections are stored by blocks of 2^kWinShift LORs (the
+// reduction's LDS windows).
+constexpr uint32_t kWinShift = 12;
 inline unsigned bin_grid(size_t n) { return (unsigned)((n + 256 * kBinLors - 1) / (256 * kBinLors)); }
 
 template <bool TOF, bool FILL>
@@ -362,20 +365,15 @@ constexpr int32_t kTileLoads = 8;
 // and lgkmcnt at their maxima, so not waited for).
 constexpr int kWaitVmcnt0 = 0x0f70;
 
-// Store offset of the forward kernel without a partial layout (never
-// written; a global, not a constant, so that loads through either table
-// stay global loads).
-__device__ int32_t kZeroDelta[1] = {0};
-
 // A lane's runs j, j + stride, ... <= last of a forward work item: each
-// run's partial is stored at j + drow[lor >> dshift].  The run's table entry
-// and the next run's record are loaded before the replay, so neither load's
-// latency stalls it.  PERM: the runs are read through perm.
+// run's partial is stored at lpos[j] (at j with PERM).  The position and the
+// next run's record are loaded before the replay, so neither load's latency
+// stalls it.  PERM: the runs are read through perm.
 template <bool TOF, bool PERM>
-__device__ __forceinline__ void fwd_runs(const Run* __restrict__ runs, const uint32_t* __restrict__ perm, int32_t j,
-                                         const int32_t last, const bool guard, const DevTof& t, const TileGeom& g,
-                                         const float* __restrict__ timg, float* __restrict__ partial,
-                                         const int32_t* __restrict__ drow, const uint32_t dshift) {
+__device__ __forceinline__ void fwd_runs(const Run* __restrict__ runs, const uint32_t* __restrict__ perm,
+                                         const uint32_t* __restrict__ lpos, int32_t j, const int32_t last,
+                                         const bool guard, const DevTof& t, const TileGeom& g,
+                                         const float* __restrict__ timg, float* __restrict__ partial) {
   const int32_t stride = (int32_t)blockDim.x;
   Run cur = runs[PERM ? perm[j] : j];
   // Wait for the first record here: the loop's later records are register
@@ -384,12 +382,12 @@ __device__ __forceinline__ void fwd_runs(const Run* __restrict__ runs, const uin
   // counter also holds the previous run's partial store).
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   for (;;) {
-    const int32_t dcur = drow[cur.lor >> dshift];
+    const uint32_t at = PERM ? (uint32_t)j : lpos[j];
     const int32_t jn = j + stride;
     const int32_t jc = jn < last ? jn : last;  // in bounds; unused past the end
     const Run nxt = runs[PERM ? perm[jc] : jc];
     const float p = guard ? replay_project<TOF, true>(cur, t, g, timg) : replay_project<TOF, false>(cur, t, g, timg);
-    partial[j + dcur] = p;
+    partial[at] = p;
     if (jn > last) break;
     cur = nxt;
     j = jn;
@@ -397,13 +395,12 @@ __device__ __forceinline__ void fwd_runs(const Run* __restrict__ runs, const uin
 }
 
 // Forward, tiled: one workgroup per work item copies its tile of the image
-// (dynamic LDS: tvox + 1 words)
-// into LDS and replays its runs, writing each run's partial projection
-// (sum of w*x over the run's pushed steps, in walk order) at the run's
-// sorted position j, or at j + delta[tile * n_blocks + (lor >> lor_shift)]
-// when a group layout is given (both coalesced; writing them LOR-major
-// instead scatters 4-byte stores over the whole array and costs more than
-// the gather in k_lor_reduce).  Each lane loads its next run record before
+// (dynamic LDS: tvox + 1 words) into LDS and replays its runs, writing each
+// run's partial projection (sum of w*x over the run's pushed steps, in walk
+// order) at lpos[j], its place in the (window block, tile) layout, or with
+// perm at its sorted position j (stores nearly contiguous either way;
+// writing them LOR-major instead scatters 4-byte stores over the whole
+// array and costs more than it saves in k_lor_reduce).  Each lane loads its next run record before
 // replaying the current one, so the record's latency hides behind the
 // replay (the replay itself touches only LDS).
 template <bool TOF>
@@ -411,8 +408,7 @@ __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, cons
                                               const float* __restrict__ img, const DevFov& f, const DevTof& t,
                                               const TileGeom& g, float* __restrict__ partial,
                                               const uint32_t* __restrict__ perm,
-                                              const int32_t* __restrict__ delta = nullptr, uint32_t n_blocks = 0,
-                                              uint32_t lor_shift = 0) {
+                                              const uint32_t* __restrict__ lpos) {
   extern __shared__ float timg[];
   const int32_t tl = wi.x;
   const int32_t tzi = tl / (g.ntx * g.nty);
@@ -451,23 +447,19 @@ __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, cons
   const bool guard = *odd_tile != 0;
   int32_t j = wi.y + (int32_t)threadIdx.x;
   if (j >= wi.z) return;
-  // without a layout every run reads the zero entry
-  const int32_t* drow = delta ? delta + (size_t)tl * n_blocks : kZeroDelta;
-  const uint32_t dshift = delta ? lor_shift : 31u;
   if (perm)
-    fwd_runs<TOF, true>(runs, perm, j, wi.z - 1, guard, t, g, timg, partial, drow, dshift);
+    fwd_runs<TOF, true>(runs, perm, lpos, j, wi.z - 1, guard, t, g, timg, partial);
   else
-    fwd_runs<TOF, false>(runs, perm, j, wi.z - 1, guard, t, g, timg, partial, drow, dshift);
+    fwd_runs<TOF, false>(runs, perm, lpos, j, wi.z - 1, guard, t, g, timg, partial);
 }
 
 template <bool TOF>
 __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs, const int4* __restrict__ work,
                                                    const float* __restrict__ img, DevFov f, DevTof t, TileGeom g,
                                                    float* __restrict__ partial,
-                                                   const uint32_t* __restrict__ perm = nullptr,
-                                                   const int32_t* __restrict__ delta = nullptr,
-                                                   uint32_t n_blocks = 0, uint32_t lor_shift = 0) {
-  fwd_tile_body<TOF>(runs, work[blockIdx.x], img, f, t, g, partial, perm, delta, n_blocks, lor_shift);
+                                                   const uint32_t* __restrict__ perm,
+                                                   const uint32_t* __restrict__ lpos) {
+  fwd_tile_body<TOF>(runs, work[blockIdx.x], img, f, t, g, partial, perm, lpos);
 }
 
 // One reconstruction of a batch, as the batched tile kernels see it: the runs
@@ -478,8 +470,7 @@ struct ReconDesc {
   float* partial;
   const float* r;
   double* bp;
-  const int32_t* delta;  // partial layout (null: sorted order)
-  uint32_t n_blocks;
+  const uint32_t* lpos;  // partial position of each sorted run
   DevFov f;
   DevTof t;
   TileGeom g;
@@ -491,7 +482,7 @@ __global__ __launch_bounds__(1024) void k_fwd_tile_batch(const ReconDesc* __rest
                                                          const int4* __restrict__ work) {
   const int4 wi = work[blockIdx.x];
   const ReconDesc& r = rd[wi.w];
-  fwd_tile_body<TOF>(r.runs, wi, r.image, r.f, r.t, r.g, r.partial, nullptr, r.delta, r.n_blocks, kLorBlockShift);
+  fwd_tile_body<TOF>(r.runs, wi, r.image, r.f, r.t, r.g, r.partial, nullptr, r.lpos);
 }
 
 // r = 1/(p*ac) per LOR (MLEM; 1/exp(-p) for the sensitivity image) with p the
@@ -585,6 +576,77 @@ __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac
   const uint32_t base = xcd * share;
   const uint32_t end = base + share < n ? share : (base < n ? n - base : 0u);
   reduce_lors<SENS>(ac, base, first, end, run_off, run_pos, pos4, partial, r_out);
+}
+
+// The same reduction for the (window block, tile) partial layout: all
+// partials of block b's LORs lie in [win[b], win[b + 1]).  One 1024-lane
+// workgroup per block of 2^kWinShift LORs (kWinLors per lane: b * block + lane +
+// 1024 * u) first issues its LORs' position and correction loads, then
+// copies the block's window into LDS with coalesced loads (kTileLoads in
+// flight per lane) and gathers from there: the partials cross the memory
+// system once, in whole cache lines, instead of as one L2 request per
+// partial.  A window over lds_entries entries is gathered from memory.
+constexpr uint32_t kWinLors = (1u << kWinShift) / 1024u;
+static_assert(kWinLors * 1024u == (1u << kWinShift), "LOR blocks of whole lanes");
+
+template <bool SENS>
+__global__ __launch_bounds__(1024) void k_lor_reduce_win(const float* __restrict__ ac, uint32_t n,
+                                                         const uint32_t* __restrict__ win, uint32_t lds_entries,
+                                                         const uint32_t* __restrict__ run_off,
+                                                         const uint32_t* __restrict__ run_pos,
+                                                         const uint4* __restrict__ pos4,
+                                                         const float* __restrict__ partial, float* __restrict__ r_out) {
+  extern __shared__ float wpart[];
+  const uint32_t b = blockIdx.x;
+  const uint32_t ws = win[b], size = win[b + 1] - ws;
+  const uint32_t stride = blockDim.x;
+  uint32_t q[kWinLors][4];
+  float a[kWinLors];
+#pragma unroll
+  for (uint32_t u = 0; u < kWinLors; ++u) {
+    const uint32_t i = (b << kWinShift) + threadIdx.x + u * stride;
+    const uint4 p4 = i < n ? pos4[i] : make_uint4(kNoRun, kNoRun, kNoRun, kNoRun);
+    q[u][0] = p4.x;
+    q[u][1] = p4.y;
+    q[u][2] = p4.z;
+    q[u][3] = p4.w;
+    a[u] = (!SENS && i < n) ? ac[i] : 1.0f;
+  }
+  const bool staged = size <= lds_entries;  // the same for the whole workgroup
+  if (staged) {
+    for (uint32_t k0 = threadIdx.x; k0 < size; k0 += kTileLoads * stride) {
+      float v[kTileLoads];
+#pragma unroll
+      for (uint32_t u = 0; u < (uint32_t)kTileLoads; ++u) {
+        const uint32_t k = k0 + u * stride;
+        v[u] = k < size ? partial[ws + k] : 0.0f;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < (uint32_t)kTileLoads; ++u) {
+        const uint32_t k = k0 + u * stride;
+        if (k < size) wpart[k] = v[u];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < kWinLors; ++u) {
+    const uint32_t i = (b << kWinShift) + threadIdx.x + u * stride;
+    if (i >= n) continue;
+    float p = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      if (q[u][k] != kNoRun) {
+        const uint32_t at = q[u][k] & ~kMoreRuns;
+        p += staged ? wpart[at - ws] : partial[at];
+      }
+    if (q[u][3] != kNoRun && (q[u][3] & kMoreRuns)) {
+      const uint32_t e = run_off[i + 1];
+      for (uint32_t k = run_off[i] + 4; k < e; ++k) p += staged ? wpart[run_pos[k] - ws] : partial[run_pos[k]];
+    }
+    // as reduce_lors (projector.rs:67-74,144-149)
+    r_out[i] = SENS ? 1.0f / expf(-p) : 1.0f / (p * a[u]);
+  }
 }
 
 // Per-reconstruction tables of the batched small kernels; blocks of one
