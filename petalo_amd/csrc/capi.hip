@@ -602,7 +602,7 @@ void ensure_lds_attributes() {
                        (const void*)k_fwd_tile<true>,       (const void*)k_fwd_tile<false>,
                        (const void*)k_scatter_batch<true>,  (const void*)k_scatter_batch<false>,
                        (const void*)k_fwd_tile_batch<true>, (const void*)k_fwd_tile_batch<false>,
-                       (const void*)k_lor_reduce_win<false>};
+                       (const void*)k_lor_reduce_win<false>, (const void*)k_lor_reduce_win_batch};
   for (const void* f : fns) HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   if (done.size() <= (size_t)dev) done.resize(dev + 1, false);
   done[dev] = true;
@@ -1096,7 +1096,8 @@ struct petalo_batch {
   std::vector<DevBuf<ReconDesc>> descs;   // per subset: one entry per reconstruction
   std::vector<DevBuf<int4>> items[2];     // [TOF][subset]: (tile, first, end, reconstruction)
   std::vector<DevBuf<ReduceDesc>> reduce; // per subset
-  std::vector<unsigned> reduce_blocks;    // per subset
+  std::vector<unsigned> reduce_blocks;    // per subset: window blocks of all reconstructions
+  std::vector<uint32_t> reduce_lds;       // per subset: LDS entries of k_lor_reduce_win_batch
   DevBuf<UpdateDesc> update;
   unsigned update_blocks = 0;
   size_t lds_f32 = 0, lds_f64 = 0;        // largest tile over the reconstructions
@@ -1121,7 +1122,9 @@ void batch_enqueue(petalo_batch* b, size_t sub) {
   if (b->items[0][sub].n)
     k_fwd_tile_batch<false><<<(unsigned)b->items[0][sub].n, 1024, b->lds_f32, st>>>(rd, b->items[0][sub].p);
   HIP_CHECK(hipGetLastError());
-  if (b->reduce_blocks[sub]) k_lor_reduce_batch<<<b->reduce_blocks[sub], kBlock, 0, st>>>(b->reduce[sub].p, n);
+  if (b->reduce_blocks[sub])
+    k_lor_reduce_win_batch<<<b->reduce_blocks[sub], 1024, (size_t)b->reduce_lds[sub] * sizeof(float), st>>>(
+        b->reduce[sub].p, n, b->reduce_lds[sub]);
   HIP_CHECK(hipGetLastError());
   for (petalo_mlem* c : b->ctx) {
     // the rare unbounded walks, one lane each
@@ -1186,6 +1189,7 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
       b->descs.resize(n_subsets);
       b->reduce.resize(n_subsets);
       b->reduce_blocks.assign(n_subsets, 0);
+      b->reduce_lds.assign(n_subsets, 0);
       for (int t = 0; t < 2; ++t) b->items[t].resize(n_subsets);
       {
         std::vector<UpdateDesc> u(n);
@@ -1214,9 +1218,9 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
                            c->dfov, c->dtof, c->tiles};
           const uint32_t m = (uint32_t)c->set_size;
           rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
-                             c->partial.p,
-                             c->rbuf.p, m, rblocks};
-          rblocks += (m + kBlock * kReduceLors - 1) / (kBlock * kReduceLors);
+                             c->partial.p, c->rbuf.p, c->win[sub].p, m, rblocks};
+          rblocks += c->n_blocks[sub];
+          b->reduce_lds[sub] = std::max(b->reduce_lds[sub], std::min(c->max_win[sub], kReduceWindow));
           std::vector<int4> h(c->work[sub].n);
           if (!h.empty())
             HIP_CHECK(hipMemcpy(h.data(), c->work[sub].p, h.size() * sizeof(int4), hipMemcpyDeviceToHost));

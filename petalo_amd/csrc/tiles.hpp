@@ -590,14 +590,12 @@ constexpr uint32_t kWinLors = (1u << kWinShift) / 1024u;
 static_assert(kWinLors * 1024u == (1u << kWinShift), "LOR blocks of whole lanes");
 
 template <bool SENS>
-__global__ __launch_bounds__(1024) void k_lor_reduce_win(const float* __restrict__ ac, uint32_t n,
-                                                         const uint32_t* __restrict__ win, uint32_t lds_entries,
-                                                         const uint32_t* __restrict__ run_off,
-                                                         const uint32_t* __restrict__ run_pos,
-                                                         const uint4* __restrict__ pos4,
-                                                         const float* __restrict__ partial, float* __restrict__ r_out) {
+__device__ __forceinline__ void reduce_window(const uint32_t b, const float* __restrict__ ac, uint32_t n,
+                                              const uint32_t* __restrict__ win, uint32_t lds_entries,
+                                              const uint32_t* __restrict__ run_off,
+                                              const uint32_t* __restrict__ run_pos, const uint4* __restrict__ pos4,
+                                              const float* __restrict__ partial, float* __restrict__ r_out) {
   extern __shared__ float wpart[];
-  const uint32_t b = blockIdx.x;
   const uint32_t ws = win[b], size = win[b + 1] - ws;
   const uint32_t stride = blockDim.x;
   uint32_t q[kWinLors][4];
@@ -649,6 +647,16 @@ __global__ __launch_bounds__(1024) void k_lor_reduce_win(const float* __restrict
   }
 }
 
+template <bool SENS>
+__global__ __launch_bounds__(1024) void k_lor_reduce_win(const float* __restrict__ ac, uint32_t n,
+                                                         const uint32_t* __restrict__ win, uint32_t lds_entries,
+                                                         const uint32_t* __restrict__ run_off,
+                                                         const uint32_t* __restrict__ run_pos,
+                                                         const uint4* __restrict__ pos4,
+                                                         const float* __restrict__ partial, float* __restrict__ r_out) {
+  reduce_window<SENS>(blockIdx.x, ac, n, win, lds_entries, run_off, run_pos, pos4, partial, r_out);
+}
+
 // Per-reconstruction tables of the batched small kernels; blocks of one
 // reconstruction are consecutive from block0.
 struct ReduceDesc {
@@ -658,6 +666,7 @@ struct ReduceDesc {
   const uint4* pos4;
   const float* partial;
   float* r;
+  const uint32_t* win;  // window blocks (k_lor_reduce_win_batch)
   uint32_t n;
   uint32_t block0;
 };
@@ -683,13 +692,14 @@ __device__ __forceinline__ uint32_t find_entry(const D* __restrict__ d, uint32_t
   return lo;
 }
 
-// Batched per-LOR reduction: 256 * kReduceLors LORs per block.
-__global__ __launch_bounds__(256) void k_lor_reduce_batch(const ReduceDesc* __restrict__ d, uint32_t n_rec) {
+// Batched window reduction: one workgroup per window block of every
+// reconstruction (blocks of one reconstruction consecutive from block0).
+__global__ __launch_bounds__(1024) void k_lor_reduce_win_batch(const ReduceDesc* __restrict__ d, uint32_t n_rec,
+                                                               uint32_t lds_entries) {
   const uint32_t r = find_entry(d, n_rec, blockIdx.x);
   const ReduceDesc q = d[r];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t first = (blockIdx.x - q.block0) * blockDim.x * kReduceLors + wave * 64u * kReduceLors + lane;
-  reduce_lors<false>(q.ac, 0u, first, q.n, q.run_off, q.run_pos, q.pos4, q.partial, q.r);
+  reduce_window<false>(blockIdx.x - q.block0, q.ac, q.n, q.win, lds_entries, q.run_off, q.run_pos, q.pos4, q.partial,
+                       q.r);
 }
 
 // LORs whose walk is not provably bounded: exact one-lane projection and
