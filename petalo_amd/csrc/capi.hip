@@ -210,7 +210,8 @@ struct petalo_mlem {
   std::vector<uint64_t> n_runs;      // per subset
   std::vector<DevBuf<uint32_t>> run_off;    // per subset: LOR-major run offsets (n + 1)
   std::vector<DevBuf<uint32_t>> run_pos;    // per subset: sorted position of each LOR-major run
-  std::vector<DevBuf<uint4>> pos4;          // per subset: the first four of them per LOR (k_pack_pos4)
+  std::vector<DevBuf<uint4>> pos4;          // per subset: the first four of them per LOR (k_pack_pos4) ...
+  std::vector<DevBuf<uint2>> pos2;          // ... or as 16-bit window offsets (k_pack_pos2)
   std::vector<DevBuf<uint32_t>> lpos;       // per subset: partial position of each sorted run
   std::vector<uint32_t> n_blocks;           // per subset: LOR blocks of 2^kWinShift of the partial layout
   std::vector<DevBuf<uint32_t>> win;        // per subset: partial window of each LOR block (n_blocks + 1)
@@ -477,12 +478,22 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   starts[g.ntiles] = total;
   for (int k = g.ntiles - 1; k >= 0; --k)
     if (starts[k] == UINT64_MAX) starts[k] = starts[k + 1];
-  c->pos4[sub].alloc(std::max<size_t>(count, 1));
-  if (count) {
-    k_pack_pos4<<<grid_for(count), kBlock, 0, st>>>(c->run_off[sub].p, c->run_pos[sub].p, (uint32_t)count,
-                                                    c->pos4[sub].p);
-    HIP_CHECK(hipGetLastError());
+  // the reduction's first four positions per LOR: 16-bit window offsets
+  // where every window allows them, else full positions
+  if (c->max_win[sub] <= kMaxCompactWindow) {
+    c->pos4[sub].alloc(0);
+    c->pos2[sub].alloc(std::max<size_t>(count, 1));
+    if (count)
+      k_pack_pos2<<<grid_for(count), kBlock, 0, st>>>(c->run_off[sub].p, c->run_pos[sub].p, (uint32_t)count,
+                                                      c->win[sub].p, c->pos2[sub].p);
+  } else {
+    c->pos2[sub].alloc(0);
+    c->pos4[sub].alloc(std::max<size_t>(count, 1));
+    if (count)
+      k_pack_pos4<<<grid_for(count), kBlock, 0, st>>>(c->run_off[sub].p, c->run_pos[sub].p, (uint32_t)count,
+                                                      c->pos4[sub].p);
   }
+  HIP_CHECK(hipGetLastError());
   // Work items: chunks of one tile's runs holding about equal numbers of
   // replayed steps (kWorkItems in total), largest first so that the last
   // workgroups to start are the short ones.  Each item loads (forward) and
@@ -514,16 +525,15 @@ void launch_reduce(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   if (count == 0) return;
   const Lor* L = c->lors.p + lo;
   hipStream_t st = c->stream;
-  if (c->n_blocks[sub]) {
-    const uint32_t lds = std::min(c->max_win[sub], kReduceWindow);
-    k_lor_reduce_win<false><<<c->n_blocks[sub], 1024, (size_t)lds * sizeof(float), st>>>(
+  const uint32_t lds = std::min(c->max_win[sub], kReduceWindow);
+  if (c->pos2[sub].p)
+    k_lor_reduce_win<false, true><<<c->n_blocks[sub], 1024, (size_t)lds * sizeof(float), st>>>(
         c->ac.p + lo, (uint32_t)count, c->win[sub].p, lds, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
-        c->partial.p, c->rbuf.p);
-  } else {
-    k_lor_reduce<false><<<reduce_grid(count), kBlock, 0, st>>>(c->ac.p + lo, (uint32_t)count, c->run_off[sub].p,
-                                                              c->run_pos[sub].p, c->pos4[sub].p, c->partial.p,
-                                                              c->rbuf.p);
-  }
+        c->pos2[sub].p, c->partial.p, c->rbuf.p);
+  else
+    k_lor_reduce_win<false, false><<<c->n_blocks[sub], 1024, (size_t)lds * sizeof(float), st>>>(
+        c->ac.p + lo, (uint32_t)count, c->win[sub].p, lds, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
+        c->pos2[sub].p, c->partial.p, c->rbuf.p);
   HIP_CHECK(hipGetLastError());
   const size_t n_irr = c->irregular[sub].n;
   if (n_irr) {
@@ -602,7 +612,8 @@ void ensure_lds_attributes() {
                        (const void*)k_fwd_tile<true>,       (const void*)k_fwd_tile<false>,
                        (const void*)k_scatter_batch<true>,  (const void*)k_scatter_batch<false>,
                        (const void*)k_fwd_tile_batch<true>, (const void*)k_fwd_tile_batch<false>,
-                       (const void*)k_lor_reduce_win<false>, (const void*)k_lor_reduce_win_batch};
+                       (const void*)k_lor_reduce_win<false, true>, (const void*)k_lor_reduce_win<false, false>,
+                       (const void*)k_lor_reduce_win_batch};
   for (const void* f : fns) HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   if (done.size() <= (size_t)dev) done.resize(dev + 1, false);
   done[dev] = true;
@@ -753,6 +764,7 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->run_off.resize(n_subsets);
         c->run_pos.resize(n_subsets);
         c->pos4.resize(n_subsets);
+        c->pos2.resize(n_subsets);
         c->lpos.resize(n_subsets);
         c->n_blocks.resize(n_subsets, 0);
         c->win.resize(n_subsets);
@@ -1071,6 +1083,7 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->run_off.clear();
   c->run_pos.clear();
   c->pos4.clear();
+  c->pos2.clear();
   c->lpos.clear();
   c->win.clear();
   c->irregular.clear();
@@ -1218,7 +1231,7 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
                            c->dfov, c->dtof, c->tiles};
           const uint32_t m = (uint32_t)c->set_size;
           rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
-                             c->partial.p, c->rbuf.p, c->win[sub].p, m, rblocks};
+                             c->partial.p, c->rbuf.p, c->win[sub].p, c->pos2[sub].p, m, rblocks};
           rblocks += c->n_blocks[sub];
           b->reduce_lds[sub] = std::max(b->reduce_lds[sub], std::min(c->max_win[sub], kReduceWindow));
           std::vector<int4> h(c->work[sub].n);

@@ -563,6 +563,24 @@ __global__ void k_pack_pos4(const uint32_t* __restrict__ run_off, const uint32_t
   pos4[i] = make_uint4(q[0], q[1], q[2], q[3]);
 }
 
+// pos2 of LORs 0..n-1: the first four positions as 16-bit offsets into the
+// LOR's window block (kNoRun16 where it has fewer; bit 15 of the fourth set
+// when it has more).  Valid while every window holds under 2^15 - 1 entries.
+constexpr uint32_t kNoRun16 = 0xffffu;
+constexpr uint32_t kMoreRuns16 = 0x8000u;
+constexpr uint32_t kMaxCompactWindow = 0x7fffu;
+__global__ void k_pack_pos2(const uint32_t* __restrict__ run_off, const uint32_t* __restrict__ run_pos, uint32_t n,
+                            const uint32_t* __restrict__ win, uint2* __restrict__ pos2) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t ws = win[i >> kWinShift];
+  const uint32_t b = run_off[i], e = run_off[i + 1];
+  uint32_t q[4];
+  for (uint32_t k = 0; k < 4; ++k) q[k] = b + k < e ? run_pos[b + k] - ws : kNoRun16;
+  if (e - b > 4) q[3] |= kMoreRuns16;
+  pos2[i] = make_uint2(q[0] | (q[1] << 16), q[2] | (q[3] << 16));
+}
+
 template <bool SENS>
 __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac, uint32_t n,
                                                     const uint32_t* __restrict__ run_off,
@@ -589,12 +607,15 @@ __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac
 constexpr uint32_t kWinLors = (1u << kWinShift) / 1024u;
 static_assert(kWinLors * 1024u == (1u << kWinShift), "LOR blocks of whole lanes");
 
-template <bool SENS>
+// COMPACT: the first four positions come from pos2 as 16-bit offsets into
+// the LOR's window (k_pack_pos2), else from pos4.
+template <bool SENS, bool COMPACT>
 __device__ __forceinline__ void reduce_window(const uint32_t b, const float* __restrict__ ac, uint32_t n,
                                               const uint32_t* __restrict__ win, uint32_t lds_entries,
                                               const uint32_t* __restrict__ run_off,
                                               const uint32_t* __restrict__ run_pos, const uint4* __restrict__ pos4,
-                                              const float* __restrict__ partial, float* __restrict__ r_out) {
+                                              const uint2* __restrict__ pos2, const float* __restrict__ partial,
+                                              float* __restrict__ r_out) {
   extern __shared__ float wpart[];
   const uint32_t ws = win[b], size = win[b + 1] - ws;
   const uint32_t stride = blockDim.x;
@@ -603,11 +624,19 @@ __device__ __forceinline__ void reduce_window(const uint32_t b, const float* __r
 #pragma unroll
   for (uint32_t u = 0; u < kWinLors; ++u) {
     const uint32_t i = (b << kWinShift) + threadIdx.x + u * stride;
-    const uint4 p4 = i < n ? pos4[i] : make_uint4(kNoRun, kNoRun, kNoRun, kNoRun);
-    q[u][0] = p4.x;
-    q[u][1] = p4.y;
-    q[u][2] = p4.z;
-    q[u][3] = p4.w;
+    if (COMPACT) {
+      const uint2 p2 = i < n ? pos2[i] : make_uint2(0xffffffffu, 0xffffffffu);
+      const uint32_t h[4] = {p2.x & 0xffffu, p2.x >> 16, p2.y & 0xffffu, p2.y >> 16};
+#pragma unroll
+      for (uint32_t k = 0; k < 3; ++k) q[u][k] = h[k] == kNoRun16 ? kNoRun : ws + h[k];
+      q[u][3] = h[3] == kNoRun16 ? kNoRun : ((ws + (h[3] & ~kMoreRuns16)) | ((h[3] & kMoreRuns16) ? kMoreRuns : 0u));
+    } else {
+      const uint4 p4 = i < n ? pos4[i] : make_uint4(kNoRun, kNoRun, kNoRun, kNoRun);
+      q[u][0] = p4.x;
+      q[u][1] = p4.y;
+      q[u][2] = p4.z;
+      q[u][3] = p4.w;
+    }
     a[u] = (!SENS && i < n) ? ac[i] : 1.0f;
   }
   const bool staged = size <= lds_entries;  // the same for the whole workgroup
@@ -647,14 +676,14 @@ __device__ __forceinline__ void reduce_window(const uint32_t b, const float* __r
   }
 }
 
-template <bool SENS>
+template <bool SENS, bool COMPACT>
 __global__ __launch_bounds__(1024) void k_lor_reduce_win(const float* __restrict__ ac, uint32_t n,
                                                          const uint32_t* __restrict__ win, uint32_t lds_entries,
                                                          const uint32_t* __restrict__ run_off,
                                                          const uint32_t* __restrict__ run_pos,
-                                                         const uint4* __restrict__ pos4,
+                                                         const uint4* __restrict__ pos4, const uint2* __restrict__ pos2,
                                                          const float* __restrict__ partial, float* __restrict__ r_out) {
-  reduce_window<SENS>(blockIdx.x, ac, n, win, lds_entries, run_off, run_pos, pos4, partial, r_out);
+  reduce_window<SENS, COMPACT>(blockIdx.x, ac, n, win, lds_entries, run_off, run_pos, pos4, pos2, partial, r_out);
 }
 
 // Per-reconstruction tables of the batched small kernels; blocks of one
@@ -667,6 +696,7 @@ struct ReduceDesc {
   const float* partial;
   float* r;
   const uint32_t* win;  // window blocks (k_lor_reduce_win_batch)
+  const uint2* pos2;    // compact positions, or null for pos4
   uint32_t n;
   uint32_t block0;
 };
@@ -698,8 +728,12 @@ __global__ __launch_bounds__(1024) void k_lor_reduce_win_batch(const ReduceDesc*
                                                                uint32_t lds_entries) {
   const uint32_t r = find_entry(d, n_rec, blockIdx.x);
   const ReduceDesc q = d[r];
-  reduce_window<false>(blockIdx.x - q.block0, q.ac, q.n, q.win, lds_entries, q.run_off, q.run_pos, q.pos4, q.partial,
-                       q.r);
+  if (q.pos2)
+    reduce_window<false, true>(blockIdx.x - q.block0, q.ac, q.n, q.win, lds_entries, q.run_off, q.run_pos, q.pos4,
+                               q.pos2, q.partial, q.r);
+  else
+    reduce_window<false, false>(blockIdx.x - q.block0, q.ac, q.n, q.win, lds_entries, q.run_off, q.run_pos, q.pos4,
+                                q.pos2, q.partial, q.r);
 }
 
 // LORs whose walk is not provably bounded: exact one-lane projection and

@@ -378,6 +378,26 @@ def test_tile_shapes(gpu, tile):
         assert e <= 1e-5
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [(4, 4, 4), (15, 15, 15), (30, 30, 20)])
+def test_reduction_window_forms(gpu, tile):
+    """The reduction's partial windows (4096 LORs each) at about 22, 6 and 3
+    runs per LOR (60^3 phantom LORs): over 2^15 partials per window (full
+    positions, gathered from memory), between 20480 and 2^15 (16-bit window
+    offsets, gathered from memory) and under 20480 (16-bit offsets, staged in
+    LDS) all give the oracle's images, with and without TOF."""
+    full, n = [180.0] * 3, [60] * 3
+    fov = P.FOV(full, n)
+    lors = P.synth_phantom_lors(5, 30000, 200.0)
+    for tof in (None, (42.47, 3.0)):
+        o = _run_oracle(full, n, lors, 2, tof=tof)
+        g = _run_gpu(fov, lors, 2, tof=P.Tof(*tof) if tof else None, scheme=P.SCHEME_TILE, tile=tile)
+        for k in range(2):
+            e = rel_l2(g[k], o[k])
+            _say(f"reduction windows tile {tile} tof={tof} iter {k + 1}: relL2 {e:.2e}")
+            assert e <= 1e-5
+
+
 def test_rows_match_golden_fixture(gpu):
     """The committed 1000-LOR row fixture (tests/golden/oracle_rows.npz): voxel
     sequences, step counts and ties identical, weights bit-identical without
