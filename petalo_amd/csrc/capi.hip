@@ -304,11 +304,11 @@ __global__ void k_invert_perm(const uint32_t* __restrict__ perm, uint64_t n, uin
 // win[b] = first stored position of window block b (UINT32_MAX where the
 // block has no runs; the caller fills those in).
 __global__ void k_window_starts(const uint64_t* __restrict__ wkey, uint64_t n, uint32_t tile_bits,
-                                uint32_t* __restrict__ win) {
+                                uint32_t n_blocks, uint32_t* __restrict__ win) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint64_t b = wkey[j] >> tile_bits;
-  if (j == 0 || (wkey[j - 1] >> tile_bits) != b) win[b] = (uint32_t)j;
+  if (b < n_blocks && (j == 0 || (wkey[j - 1] >> tile_bits) != b)) win[b] = (uint32_t)j;
 }
 
 // Puts the runs in sorted order, records where each LOR-major run's partial
@@ -453,7 +453,8 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
                                                    tile_bits + blk_bits, st));
       k_invert_perm<<<grid64(total), kBlock, 0, st>>>(order.p, total, c->lpos[sub].p);
       HIP_CHECK(hipMemcpyAsync(c->win[sub].p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, st));
-      k_window_starts<<<grid64(total), kBlock, 0, st>>>(wkey2.p, total, (uint32_t)tile_bits, c->win[sub].p);
+      k_window_starts<<<grid64(total), kBlock, 0, st>>>(wkey2.p, total, (uint32_t)tile_bits, c->n_blocks[sub],
+                                                        c->win[sub].p);
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipMemcpyAsync(hw.data(), c->win[sub].p, hw.size() * 4, hipMemcpyDeviceToHost, st));
       // (also waits for the temporaries' users: they are freed at scope end)
@@ -473,7 +474,10 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(summary.data(), dsum.p, total * 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    for (uint64_t j = total; j-- > 0;) starts[summary[j] >> 12] = j;
+    for (uint64_t j = total; j-- > 0;) {
+      if ((summary[j] >> 12) >= (uint32_t)g.ntiles) throw std::runtime_error("run sort key without a valid tile");
+      starts[summary[j] >> 12] = j;
+    }
   }
   starts[g.ntiles] = total;
   for (int k = g.ntiles - 1; k >= 0; --k)
@@ -1473,13 +1477,14 @@ __global__ void k_take_irregular(uint32_t* __restrict__ counts, uint32_t n, cons
 }
 
 
-// First sorted position of every tile; keys hold the tile from bit tshift.
-__global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uint32_t tshift,
+// First sorted position of every tile; keys hold the tile from bit tshift
+// (values past ntiles, which a well-formed key never holds, are ignored).
+__global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uint32_t tshift, uint32_t ntiles,
                               uint64_t* __restrict__ start) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const uint32_t t = (uint32_t)(keys[j] >> tshift);
-  if (j == 0 || (uint32_t)(keys[j - 1] >> tshift) != t) start[t] = j;
+  const uint64_t t = keys[j] >> tshift;
+  if (t < ntiles && (j == 0 || (keys[j - 1] >> tshift) != t)) start[t] = j;
 }
 
 // Sensitivity image (one FOV) with the tiled kernels: per batch, bin the
@@ -1548,7 +1553,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       }
       DevBuf<uint64_t> dstart(g.ntiles);
       HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, (size_t)g.ntiles * 8, s));
-      k_tile_starts<<<grid64(total), kBlock, 0, s>>>(keys2.p, total, tshift, dstart.p);
+      k_tile_starts<<<grid64(total), kBlock, 0, s>>>(keys2.p, total, tshift, (uint32_t)g.ntiles, dstart.p);
       HIP_CHECK(hipGetLastError());
       std::vector<uint64_t> starts(g.ntiles + 1);
       HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, (size_t)g.ntiles * 8, hipMemcpyDeviceToHost, s));
