@@ -184,27 +184,26 @@ def main():
 
     info = dict(ctx.info(), tile=list(ctx.tile_shape()), autotune=bool(a.autotune))
     log(f"[bench] scheme {info}")
+    # HIP events bracket every kernel of every timed update; the updates run
+    # back to back (their timings are read once, after the timed region,
+    # or every 256 updates, the context's ring of event sets)
     ctx.set_timing(True)
-    proj_ms = []
-    upd_ms = []
-    fwd_ms = []
-    sct_ms = []
     kern = []
     P.lib().petalo_device_synchronize(0)
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        for _ in range(subsets):
-            ctx.step(copy_image=False)
-            pm, um = ctx.last_timing()
-            fm, sm, _ = ctx.last_phase_timing()
-            kern.append(ctx.last_kernel_timing())
-            proj_ms.append(pm)
-            upd_ms.append(um)
-            fwd_ms.append(fm)
-            sct_ms.append(sm)
+    for k in range(a.steps * subsets):
+        ctx.step(copy_image=False)
+        if (k + 1) % 256 == 0:
+            kern += ctx.drain_kernel_timings(256)
     P.lib().petalo_device_synchronize(0)
     elapsed = time.perf_counter() - t0
+    kern += ctx.drain_kernel_timings(256)
     ctx.set_timing(False)
+    assert len(kern) == a.steps * subsets
+    fwd_ms = [d["forward_tiles"] + d["forward_reduce"] for d in kern]
+    sct_ms = [d["scatter"] for d in kern]
+    proj_ms = [f + s_ for f, s_ in zip(fwd_ms, sct_ms)]
+    upd_ms = [d["update"] for d in kern]
 
     it_per_s = a.steps / elapsed
     steps_per_s = ws["n_steps"] * a.steps / elapsed

@@ -168,6 +168,13 @@ enum {
   PETALO_N_TIMES = 4
 };
 int petalo_mlem_last_kernel_timing(petalo_mlem_t* ctx, double* ms, size_t n);
+/* The same PETALO_N_TIMES timings for each of the most recent timed subset
+ * updates since timing was switched on or last drained (at most
+ * max_updates, and at most the 256 the context keeps), oldest first, into
+ * ms[u * PETALO_N_TIMES + k]; *n_updates is how many.  Waits for the
+ * context's stream.  Timed updates do not wait for each other, so a loop of
+ * them runs back to back on the device.  No counterpart in the reference. */
+int petalo_mlem_drain_kernel_timings(petalo_mlem_t* ctx, double* ms, size_t max_updates, size_t* n_updates);
 /* Scheme in use and, for the tiled scheme, the number of tile runs. */
 int petalo_mlem_info(petalo_mlem_t* ctx, int32_t* scheme, uint64_t* n_runs_total);
 /* The tile shape of a tiled context (zeros for the atomic scheme): after

@@ -170,6 +170,7 @@ _SIGS = {
     "petalo_mlem_last_timing": (C.c_int, [_P, _P, _P]),
     "petalo_mlem_last_phase_timing": (C.c_int, [_P, _P, _P, _P]),
     "petalo_mlem_last_kernel_timing": (C.c_int, [_P, _P, C.c_size_t]),
+    "petalo_mlem_drain_kernel_timings": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "petalo_mlem_info": (C.c_int, [_P, _P, _P]),
     "petalo_mlem_destroy": (C.c_int, [_P]),
     "petalo_mlem_set_graphs": (C.c_int, [_P, C.c_int]),
@@ -427,6 +428,15 @@ class Mlem:
         buf = (C.c_double * 4)()
         _check(lib().petalo_mlem_last_kernel_timing(self._ctx, buf, 4))
         return dict(zip(("forward_tiles", "forward_reduce", "scatter", "update"), list(buf)))
+
+    def drain_kernel_timings(self, max_updates=256):
+        """Kernel timings of the timed updates since timing was switched on or
+        last drained, oldest first: a list of dicts as last_kernel_timing."""
+        buf = (C.c_double * (4 * max_updates))()
+        n = C.c_size_t(0)
+        _check(lib().petalo_mlem_drain_kernel_timings(self._ctx, buf, max_updates, C.byref(n)))
+        names = ("forward_tiles", "forward_reduce", "scatter", "update")
+        return [dict(zip(names, list(buf[4 * u:4 * u + 4]))) for u in range(n.value)]
 
     def info(self):
         sch, nr = C.c_int32(0), C.c_uint64(0)

@@ -64,6 +64,8 @@ constexpr uint64_t kMinItemSteps = 8192;
 // LDS entries of k_lor_reduce_win (80 KiB: two workgroups per CU); larger
 // LOR-block windows are gathered from memory.
 constexpr uint32_t kReduceWindow = 20480;
+// Timed subset updates whose kernel timings a context keeps.
+constexpr size_t kTimingRing = 256;
 // Largest tile (LDS entries, row padding included): its binary64 copy fills
 // the 160 KiB of LDS of a CU.
 constexpr int32_t kMaxTileVoxels = 20480;
@@ -200,8 +202,14 @@ struct petalo_mlem {
   DevBuf<unsigned long long> counters;
   petalo_osem_t osem{};
   bool timing = false;
-  // timing: start, forward end, scatter end, update end, forward tile kernel end
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // timing: per timed update, events at its start, forward end, scatter end,
+  // update end and forward tile kernel end, in a ring of kTimingRing sets
+  // (created when timing is first switched on) so that updates need not wait
+  // for each other's timings; ev is the set of the last timed update
+  std::vector<std::array<hipEvent_t, 5>> evr;
+  hipEvent_t* ev = nullptr;
+  size_t ev_next = 0;   // set of the next timed update
+  size_t ev_count = 0;  // timed updates since the last drain (at most the ring)
   bool have_timing = false;
   // tiled scheme
   TileGeom tiles{};
@@ -630,7 +638,12 @@ void enqueue_update(petalo_mlem* c, bool timing) {
   const size_t sub = c->osem.subset - 1;
   const size_t lo = sub * c->set_size;  // Osem::subset (mlem.rs:90-95)
   zero_counters(c->counters.p, c->stream);
-  if (timing) HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
+  if (timing) {
+    c->ev = c->evr[c->ev_next].data();
+    c->ev_next = (c->ev_next + 1) % c->evr.size();
+    c->ev_count = std::min(c->ev_count + 1, c->evr.size());
+    HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
+  }
   if (c->scheme == PETALO_SCHEME_TILE) {
     launch_tiled(c, sub, lo, c->set_size, timing);
     if (timing) HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
@@ -758,7 +771,6 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->sens.alloc(nv);
         HIP_CHECK(hipMemcpy(c->sens.p, sensitivity, nv * sizeof(float), hipMemcpyHostToDevice));
       }
-      for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
       if (c->scheme == PETALO_SCHEME_TILE) {
         c->tiles = make_tiles(*fov, opt.tile);
         ensure_lds_attributes();
@@ -988,10 +1000,20 @@ int petalo_mlem_last_skipped(petalo_mlem_t* c, uint64_t* skipped) {
 }
 
 int petalo_mlem_set_timing(petalo_mlem_t* c, int on) {
-  if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
-  c->timing = on != 0;
-  c->have_timing = false;
-  return PETALO_OK;
+  return guarded([&] {
+    if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
+    if (on && c->evr.empty()) {
+      HIP_CHECK(hipSetDevice(c->device));
+      c->evr.resize(kTimingRing);
+      for (auto& set : c->evr) set.fill(nullptr);
+      for (auto& set : c->evr)
+        for (auto& e : set) HIP_CHECK(hipEventCreate(&e));
+    }
+    c->timing = on != 0;
+    c->have_timing = false;
+    c->ev_count = 0;
+    return PETALO_OK;
+  });
 }
 
 int petalo_mlem_set_graphs(petalo_mlem_t* c, int on) {
@@ -1048,6 +1070,29 @@ int petalo_mlem_last_kernel_timing(petalo_mlem_t* c, double* ms, size_t n) {
   });
 }
 
+int petalo_mlem_drain_kernel_timings(petalo_mlem_t* c, double* ms, size_t max_updates, size_t* n_updates) {
+  return guarded([&] {
+    if (!c || !n_updates || (!ms && max_updates)) return fail(PETALO_ERR_ARG, "NULL argument");
+    HIP_CHECK(hipSetDevice(c->device));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const size_t n = std::min(c->ev_count, max_updates);
+    const size_t ring = c->evr.size();
+    for (size_t u = 0; u < n; ++u) {
+      // the n most recent timed updates, oldest first
+      const hipEvent_t* e = c->evr[(c->ev_next + ring - n + u) % ring].data();
+      float t[PETALO_N_TIMES] = {0, 0, 0, 0};
+      HIP_CHECK(hipEventElapsedTime(&t[PETALO_TIME_FORWARD_TILES], e[0], e[4]));
+      HIP_CHECK(hipEventElapsedTime(&t[PETALO_TIME_FORWARD_REDUCE], e[4], e[1]));
+      HIP_CHECK(hipEventElapsedTime(&t[PETALO_TIME_SCATTER], e[1], e[2]));
+      HIP_CHECK(hipEventElapsedTime(&t[PETALO_TIME_UPDATE], e[2], e[3]));
+      for (size_t k = 0; k < PETALO_N_TIMES; ++k) ms[u * PETALO_N_TIMES + k] = t[k];
+    }
+    *n_updates = n;
+    c->ev_count = 0;
+    return PETALO_OK;
+  });
+}
+
 int petalo_mlem_info(petalo_mlem_t* c, int32_t* scheme, uint64_t* n_runs_total) {
   if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
   if (scheme) *scheme = c->scheme;
@@ -1072,8 +1117,9 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   if (!c) return PETALO_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto& e : c->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto& set : c->evr)
+    for (auto& e : set)
+      if (e) (void)hipEventDestroy(e);
   for (auto& g : c->graphs)
     if (g) (void)hipGraphExecDestroy(g);
   c->graphs.clear();

@@ -497,6 +497,33 @@ def test_graph_replays_match_direct_launches(gpu):
     b.close()
 
 
+def test_timed_updates_back_to_back(gpu):
+    """Timed updates drained after a loop: one timing per update, oldest
+    first, the last equal to last_kernel_timing; images unchanged by timing;
+    more updates than the ring keeps yields the most recent 256."""
+    full, n = [60.0, 60.0, 48.0], [20, 20, 16]
+    rng = np.random.default_rng(33)
+    lors = random_lors(rng, 20000, radius=120.0, zhalf=60.0, dt_sigma_ps=84.93)
+    fov = P.FOV(full, n)
+    a = P.Mlem(fov, lors, tof=P.Tof(42.47, 3.0), n_subsets=2)
+    b = P.Mlem(fov, lors, tof=P.Tof(42.47, 3.0), n_subsets=2)
+    a.set_timing(True)
+    for _ in range(5):
+        a.step(copy_image=False)
+        b.step(copy_image=False)
+    last = a.last_kernel_timing()
+    t = a.drain_kernel_timings(16)
+    assert len(t) == 5 and t[-1] == last
+    assert all(v >= 0 for d in t for v in d.values()) and all(d["forward_tiles"] > 0 for d in t)
+    assert a.drain_kernel_timings(16) == []
+    assert np.array_equal(a.image().data, b.image().data, equal_nan=True)
+    for _ in range(300):
+        a.step(copy_image=False)
+    assert len(a.drain_kernel_timings(512)) == 256
+    a.close()
+    b.close()
+
+
 def test_batch_matches_single_reconstructions(gpu):
     """A batch of reconstructions of different shapes, TOF settings, LOR
     counts (one empty) and sensitivity images gives each one's own images."""
