@@ -1513,6 +1513,39 @@ constexpr uint32_t kSensItemRuns = 8192;
 
 // Unbounded walks (count 0xffffffff): copied out for the one-lane kernel,
 // their count set to 0.  counts[n] is set to 0 for the exclusive scan.
+// Sort key of a sensitivity LOR: about the number of steps of its walk (the
+// crossings of each axis before the first axis runs out, from the boundary
+// arithmetic; an estimate, for ordering only), so that the binning kernel's
+// lanes, which take several LORs each, get walks of similar lengths.
+__global__ void k_walk_len_keys(const Lor* __restrict__ lors, uint32_t n, DevFov f, uint32_t* __restrict__ keys,
+                                uint32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Lor l = load_lor(lors, i);
+  WalkC c;
+  WalkS w;
+  float est = 0.0f;
+  if (walk_setup(l, f, c, w) && bounded(w)) {
+    const float nb[3] = {w.nb_x, w.nb_y, w.nb_z}, vl[3] = {c.vl_x, c.vl_y, c.vl_z};
+    const int32_t r[3] = {w.r_x, w.r_y, w.r_z};
+    float d = 3.0e38f;
+    for (int a = 0; a < 3; ++a) d = fminf(d, nb[a] + (float)(r[a] - 1) * vl[a]);
+    for (int a = 0; a < 3; ++a) {
+      float q = (d - nb[a]) / vl[a];     // NaN or negative for an axis that is never crossed
+      q = q >= 0.0f ? fminf(q + 1.0f, (float)r[a]) : 0.0f;
+      est += q;
+    }
+  }
+  keys[i] = (uint32_t)fminf(est, 65535.0f);
+  idx[i] = i;
+}
+
+__global__ void k_gather_lors_by(const Lor* __restrict__ in, const uint32_t* __restrict__ idx, uint32_t n,
+                                 Lor* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = in[idx[j]];
+}
+
 __global__ void k_take_irregular(uint32_t* __restrict__ counts, uint32_t n, const Lor* __restrict__ lors,
                                  Lor* __restrict__ irr, uint32_t* __restrict__ n_irr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1540,10 +1573,13 @@ __global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uin
 // with mu == 0 everywhere every LOR has p = 0 exactly and r = 1, so the
 // forward half is skipped.  Unbounded walks go to the one-lane kernel.
 // produce(dst, first, count, stream) writes LORs first .. first + count - 1.
+// by_length: bin each batch in order of walk length (LORs in random order:
+// the binning lanes then walk similar lengths); false keeps the source's
+// order (neighbouring element pairs share cache lines and LDS banks).
 using LorSource = std::function<void(Lor*, uint64_t, uint32_t, hipStream_t)>;
 
 void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const float* mu_host, const float* dmu,
-                uint64_t n_lors, const LorSource& produce, double* acc, unsigned long long* cnt) {
+                uint64_t n_lors, const LorSource& produce, double* acc, unsigned long long* cnt, bool by_length) {
   const int32_t no_shape[3] = {0, 0, 0};
   const TileGeom g = make_tiles(fov, no_shape);
   const size_t na = nvox(fov);
@@ -1557,14 +1593,27 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   const DevTof notof = no_tof();
   ensure_lds_attributes();
   const size_t bmax = (size_t)std::min<uint64_t>(n_lors, kSensBatch);
-  DevBuf<Lor> lb(bmax), irr(bmax);
+  DevBuf<Lor> lb(bmax), lraw(by_length ? bmax : 0), irr(bmax);
   DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1);
   DevBuf<float> rbuf;
   if (!mu_zero) rbuf.alloc(bmax);
   for (uint64_t first = 0; first < n_lors; first += kSensBatch) {
     const uint32_t b = (uint32_t)std::min<uint64_t>(kSensBatch, n_lors - first);
-    produce(lb.p, first, b, s);
+    produce(by_length ? lraw.p : lb.p, first, b, s);
     HIP_CHECK(hipGetLastError());
+    if (by_length) {
+      // the batch in order of walk length (the image is a sum over LORs)
+      DevBuf<uint32_t> lk(b), lk2(b), li(b), li2(b);
+      k_walk_len_keys<<<grid_for(b), kBlock, 0, s>>>(lraw.p, b, f, lk.p, li.p);
+      HIP_CHECK(hipGetLastError());
+      size_t sb = 0;
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, lk.p, lk2.p, li.p, li2.p, (int)b, 0, 16, s));
+      DevBuf<unsigned char> st(std::max<size_t>(sb, 1));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(st.p, sb, lk.p, lk2.p, li.p, li2.p, (int)b, 0, 16, s));
+      k_gather_lors_by<<<grid_for(b), kBlock, 0, s>>>(lraw.p, li2.p, b, lb.p);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipStreamSynchronize(s));  // the temporaries are freed at scope end
+    }
     k_bin_runs<false, false><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr,
                                                              nullptr, shift, cnt);
     HIP_CHECK(hipMemsetAsync(n_irr.p, 0, 4, s));
@@ -1697,7 +1746,7 @@ int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const fl
       k_pair_lors<<<grid_for(n), kBlock, 0, st>>>(first, n, el, seed, dst);
     };
     if (!out_fov) {
-      sens_tiled(s, *att_fov, fa, mu, dmu.p, n_pairs, src, acc.p, cnt.p);
+      sens_tiled(s, *att_fov, fa, mu, dmu.p, n_pairs, src, acc.p, cnt.p, false);
     } else {
       // two grids: one lane per LOR, binary64 global atomics
       DevBuf<Lor> lb((size_t)std::min<uint64_t>(n_pairs, kSensBatch));
@@ -1767,7 +1816,7 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
                              : LorSource([&](Lor* dst, uint64_t first, uint32_t cnt_, hipStream_t st) {
           k_sens_lors<<<grid_for(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst);
         });
-        sens_tiled(s, *att_fov, fa, mu, dmu.p, n_lors, src, acc.p, cnt.p);
+        sens_tiled(s, *att_fov, fa, mu, dmu.p, n_lors, src, acc.p, cnt.p, true);
       } else {
         // two FOVs: one lane per LOR, binary64 global atomics
         if (lors) k_sens<false, true><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
