@@ -203,6 +203,192 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
   return count;
 }
 
+// bin_lor without TOF, without stepping through the walk: each axis's
+// boundary sequence nb, nb + vl, nb + vl + vl, ... (the walk's values, bit for
+// bit: the walk adds vl to an axis's boundary each time it crosses it) is
+// generated on its own, and the axes are merged only where the walk leaves a
+// tile or the FOV.  The walk crosses boundaries in order of (value, axis x <
+// y < z), so a tile segment ends at the smallest (value, axis) among the
+// axes' segment-ending crossings (the crossing that leaves the tile along
+// that axis, or the axis's last one); the other axes cross the boundaries
+// that precede it in that order.  A segment's run is all of its steps when
+// its first and last steps have positive length (no TOF: those are the
+// pushed steps); otherwise the segment is stepped through as by bin_lor.
+// Adds the walk's step count to *n_steps.  Emits exactly bin_lor<false>'s
+// runs.
+template <class Emit>
+__host__ __device__ __forceinline__ uint32_t bin_lor_axes(const Lor& l, uint32_t i, const DevFov& f,
+                                                          const TileGeom& g, Emit&& emit,
+                                                          uint32_t* n_steps = nullptr) {
+  WalkC c;
+  WalkS w;
+  if (!walk_setup(l, f, c, w) || !bounded(w)) return 0;
+  // a degenerate LOR (p1 == p2) has NaN boundaries, which order no crossings
+  if (!(w.nb_x == w.nb_x && w.nb_y == w.nb_y && w.nb_z == w.nb_z && c.vl_x == c.vl_x && c.vl_y == c.vl_y &&
+        c.vl_z == c.vl_z))
+    return bin_lor<false>(l, i, f, no_tof(), g, emit, n_steps, nullptr);
+  int32_t cc[3], sgn[3], tile;
+  {
+    const int32_t nxy = f.nx * f.ny;
+    const int32_t z = w.index / nxy;
+    const int32_t rem = w.index - z * nxy;
+    const int32_t y = rem / f.nx;
+    const int32_t x = rem - y * f.nx;
+    sgn[0] = c.d_x < 0 ? -1 : 1;
+    sgn[1] = c.d_y < 0 ? -1 : 1;
+    sgn[2] = c.d_z < 0 ? -1 : 1;
+    cc[0] = sgn[0] > 0 ? g.tx - x % g.tx : x % g.tx + 1;
+    cc[1] = sgn[1] > 0 ? g.ty - y % g.ty : y % g.ty + 1;
+    cc[2] = sgn[2] > 0 ? g.tz - z % g.tz : z % g.tz + 1;
+    tile = x / g.tx + g.ntx * (y / g.ty + g.nty * (z / g.tz));
+  }
+  const float vl[3] = {c.vl_x, c.vl_y, c.vl_z};
+  const int32_t tt[3] = {g.tx, g.ty, g.tz};
+  const int32_t dtile[3] = {sgn[0], sgn[1] * g.ntx, sgn[2] * g.ntx * g.nty};
+  const uint32_t flips = (sgn[0] < 0 ? 1u : 0u) | (sgn[1] < 0 ? 2u : 0u) | (sgn[2] < 0 ? 4u : 0u);
+  float V[3] = {w.nb_x, w.nb_y, w.nb_z};  // next boundary per axis
+  int32_t r[3] = {w.r_x, w.r_y, w.r_z};   // crossings left per axis (the last one leaves the FOV)
+  float E[3] = {0.0f, 0.0f, 0.0f};        // value of the axis's segment-ending crossing
+  float Ep[3] = {0.0f, 0.0f, 0.0f};       // value of the crossing before it, -inf if none in the lookahead
+  bool have[3] = {false, false, false};
+  float here = w.here;
+  uint32_t count = 0;
+  uint32_t total = 0;
+  auto make_run = [&](const float nb[3], float h, const int32_t cq[3], uint32_t len) {
+    Coord q;
+    q.cx = cq[0];
+    q.cy = cq[1];
+    q.cz = cq[2];
+    q.sx = sgn[0];
+    q.sy = sgn[1];
+    q.sz = sgn[2];
+    Run ru;
+    ru.nb_x = nb[0];
+    ru.nb_y = nb[1];
+    ru.nb_z = nb[2];
+    ru.here = h;
+    ru.vl_x = c.vl_x;
+    ru.vl_y = c.vl_y;
+    ru.vl_z = c.vl_z;
+    ru.tof_peak = c.tof_peak;
+    ru.code = (local_of(g, q) << 15) | (flips << 12) | len;
+    ru.lor = i;
+    emit(ru, (uint32_t)tile);
+    ++count;
+  };
+  for (;;) {
+    int32_t m[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      m[a] = cc[a] < r[a] ? cc[a] : r[a];
+      if (!have[a]) {
+        float v = V[a], prev = -INFINITY;
+        for (int32_t j = 1; j < m[a]; ++j) {
+          prev = v;
+          v = v + vl[a];
+        }
+        E[a] = v;
+        Ep[a] = prev;
+        have[a] = true;
+      }
+    }
+    int e = 0;
+    if (E[1] < E[e]) e = 1;
+    if (E[2] < E[e]) e = 2;
+    float Ee = E[0], Epe = Ep[0];
+    int32_t me = m[0];
+#pragma unroll
+    for (int a = 1; a < 3; ++a)
+      if (a == e) {
+        Ee = E[a];
+        Epe = Ep[a];
+        me = m[a];
+      }
+    // crossings of the other axes before the end crossing
+    int32_t n[3];
+    float Vn[3];
+    float P = here > Epe ? here : Epe;  // start of the segment's last step
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a == e) {
+        n[a] = me;
+        Vn[a] = V[a];
+        continue;
+      }
+      float v = V[a];
+      int32_t k = 0;
+      const int32_t lim = m[a] - 1;
+      while (k < lim && (v < Ee || (v == Ee && a < e))) {
+        P = P > v ? P : v;
+        v = v + vl[a];
+        ++k;
+      }
+      n[a] = k;
+      Vn[a] = v;
+    }
+    const uint32_t steps = (uint32_t)(n[0] + n[1] + n[2]);
+    total += steps;
+    const float F = fminf(fminf(V[0], V[1]), V[2]);  // end of the segment's first step
+    if (F > here && Ee > P) {
+      make_run(V, here, cc, steps);
+    } else {
+      // step through the segment (ties at its ends): first and last pushed step
+      WalkS s = w;
+      s.nb_x = V[0];
+      s.nb_y = V[1];
+      s.nb_z = V[2];
+      s.here = here;
+      WalkC cs = c;
+      int32_t q3[3] = {cc[0], cc[1], cc[2]};
+      float nb0[3] = {0.0f, 0.0f, 0.0f};
+      float h0 = 0.0f;
+      int32_t c0[3] = {0, 0, 0};
+      int64_t first = -1, last = -1;
+      for (uint32_t k = 0; k < steps; ++k) {
+        const Step st = next_boundary(s);
+        if (st.bnd - s.here > 0.0f) {
+          if (first < 0) {
+            first = k;
+            nb0[0] = s.nb_x;
+            nb0[1] = s.nb_y;
+            nb0[2] = s.nb_z;
+            h0 = s.here;
+            c0[0] = q3[0];
+            c0[1] = q3[1];
+            c0[2] = q3[2];
+          }
+          last = k;
+        }
+        q3[0] -= st.ax ? 1 : 0;
+        q3[1] -= st.ay ? 1 : 0;
+        q3[2] -= st.az ? 1 : 0;
+        move(s, cs, st);
+      }
+      if (first >= 0) make_run(nb0, h0, c0, (uint32_t)(last - first + 1));
+    }
+    // the segment's crossings
+    bool out = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      r[a] -= n[a];
+      if (a == e) {
+        V[a] = Ee + vl[a];
+        out = r[a] == 0;
+        cc[a] = tt[a];
+        tile += dtile[a];
+        have[a] = false;
+      } else {
+        V[a] = Vn[a];
+        cc[a] -= n[a];
+      }
+    }
+    here = Ee;
+    if (out) break;
+  }
+  if (n_steps) *n_steps += total;
+  return count;
+}
+
 // Length field of a run's sort key (bits 0-11): longer runs first, in
 // buckets of 4 steps.  Lanes of a wavefront get runs of about equal length;
 // inside a bucket the (stable) sort keeps LOR order, so neighbouring LORs'
@@ -246,19 +432,22 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
     const Lor l = load_lor(lors, i);
     const uint32_t base = FILL ? offset[i] : 0;
     uint32_t k = 0;
-    const uint32_t count = bin_lor<TOF>(
-        l, i, f, t, g,
-        [&](const Run& ru, uint32_t tile) {
-          if (FILL) {
-            runs[base + k] = ru;
-            // lor_shift >= 32: no LOR-block field, key = (tile << 12) | ...
-            const uint64_t blk = lor_shift >= 32 ? 0u : (uint64_t)(i >> lor_shift);
-            const uint32_t tshift = lor_shift >= 32 ? 12u : 32u;
-            keys[base + k] = ((uint64_t)tile << tshift) | (blk << 12) | len_key(ru.code);
-          }
-          ++k;
-        },
-        FILL ? nullptr : &steps, FILL ? nullptr : &nz);
+    auto emit = [&](const Run& ru, uint32_t tile) {
+      if (FILL) {
+        runs[base + k] = ru;
+        // lor_shift >= 32: no LOR-block field, key = (tile << 12) | ...
+        const uint64_t blk = lor_shift >= 32 ? 0u : (uint64_t)(i >> lor_shift);
+        const uint32_t tshift = lor_shift >= 32 ? 12u : 32u;
+        keys[base + k] = ((uint64_t)tile << tshift) | (blk << 12) | len_key(ru.code);
+      }
+      ++k;
+    };
+    // filling without TOF: the per-axis form (the same runs, 25% faster on
+    // the GPU; counting, where the step walk's single loop keeps the lanes
+    // together and also gives the pushed-step count, it is 7% slower)
+    const uint32_t count = (!TOF && FILL) ? bin_lor_axes(l, i, f, g, emit)
+                                          : bin_lor<TOF>(l, i, f, t, g, emit, FILL ? nullptr : &steps,
+                                                         FILL ? nullptr : &nz);
     if (!FILL) {
       // unbounded walks are projected by the one-lane kernels instead
       WalkC c;

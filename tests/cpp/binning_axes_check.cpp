@@ -1,0 +1,92 @@
+// Host-side check that the per-axis binning (bin_lor_axes) emits exactly the
+// runs and step counts of the step-by-step binning walk (bin_lor<false>),
+// bit for bit, over random LORs and LORs built to tie: axis-parallel, on
+// voxel planes, through voxel edges and corners, diagonal, degenerate
+// (p1 == p2), grazing a face, and the irregular LORs of the sensitivity
+// tests.  Usage: binning_axes_check [lors per case]; exit status 1 on any
+// mismatch.
+#include "tiles.hpp"
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+using namespace petalo;
+struct R { Run ru; uint32_t tile; };
+static bool same(const R& a, const R& b) { return std::memcmp(&a.ru, &b.ru, sizeof(Run)) == 0 && a.tile == b.tile; }
+int main(int argc, char** argv) {
+  const int n_lors = argc > 1 ? atoi(argv[1]) : 200000;
+  std::mt19937_64 rng(12345);
+  std::uniform_real_distribution<float> U(-1.0f, 1.0f);
+  long bad = 0, runs = 0, lors_hit = 0, fallback_like = 0;
+  uint64_t steps_a = 0, steps_b = 0;
+  struct Case { float full[3]; uint32_t n[3]; int32_t tile[3]; };
+  std::vector<Case> cases = {
+    {{30, 30, 30}, {6, 6, 6}, {6, 6, 6}}, {{30, 30, 30}, {6, 6, 6}, {2, 3, 4}},
+    {{180, 180, 180}, {120, 120, 120}, {40, 40, 6}}, {{180, 180, 180}, {60, 60, 60}, {4, 4, 4}},
+    {{100, 130, 170}, {37, 51, 29}, {5, 7, 3}},       {{30, 30, 30}, {3, 3, 3}, {2, 2, 2}},
+    {{307.2f, 307.2f, 153.6f}, {256, 256, 128}, {40, 40, 6}}, {{60, 60, 48}, {20, 20, 16}, {8, 16, 4}}};
+  for (const Case& cs : cases) {
+    const DevFov f = make_fov(cs.full, cs.n);
+    TileGeom g{};
+    g.tx = cs.tile[0]; g.ty = cs.tile[1]; g.tz = cs.tile[2]; g.px = g.tx | 1;
+    g.ntx = (int32_t)((cs.n[0] + g.tx - 1) / g.tx); g.nty = (int32_t)((cs.n[1] + g.ty - 1) / g.ty);
+    g.ntz = (int32_t)((cs.n[2] + g.tz - 1) / g.tz); g.ntiles = g.ntx * g.nty * g.ntz; g.tvox = g.px * g.ty * g.tz;
+    const DevTof t = no_tof();
+    const float vsx = cs.full[0] / cs.n[0], vsy = cs.full[1] / cs.n[1], vsz = cs.full[2] / cs.n[2];
+    for (int i = 0; i < n_lors; ++i) {
+      Lor l;
+      const int kind = i % 8;
+      float R0 = 1.2f * cs.full[0];
+      l.x1 = U(rng) * R0; l.y1 = U(rng) * R0; l.z1 = U(rng) * cs.full[2];
+      l.x2 = U(rng) * R0; l.y2 = U(rng) * R0; l.z2 = U(rng) * cs.full[2];
+      if (kind == 1) l.y2 = l.y1;                      // parallel to a plane
+      if (kind == 2) { l.z2 = l.z1; l.y2 = l.y1; }     // parallel to x
+      if (kind == 3) {                                  // through voxel corners (ties)
+        const int a = (int)((U(rng) + 1) * 5), b = (int)((U(rng) + 1) * 5), h = (int)((U(rng) + 1) * 3);
+        l.x1 = -cs.full[0]; l.y1 = -cs.full[1] / 2 + a * vsy; l.z1 = -cs.full[2] / 2 + h * vsz;
+        l.x2 = cs.full[0]; l.y2 = l.y1 + 2 * cs.full[0] / vsx * vsy * (b % 3 == 0 ? 1.0f : 0.5f); l.z2 = l.z1 + (b % 2) * 2 * cs.full[0];
+      }
+      if (kind == 4) {                                  // exact diagonals
+        const float d = (U(rng) + 1) * 4;
+        l.x1 = -cs.full[0]; l.y1 = -cs.full[0] + d * vsy; l.z1 = -cs.full[0] * vsz / vsx;
+        l.x2 = cs.full[0]; l.y2 = cs.full[0] + d * vsy; l.z2 = cs.full[0] * vsz / vsx;
+      }
+      if (kind == 5) { l.x1 = (int)(U(rng) * 10) * vsx; l.x2 = l.x1; }  // on a voxel plane
+      if (kind == 6 && (i % 64) == 6) { l.x2 = l.x1; l.y2 = l.y1; l.z2 = l.z1; }  // degenerate
+      if (kind == 7 && (i % 64) == 7) { l.x1 = -cs.full[0]; l.x2 = cs.full[0]; l.y1 = l.y2 = cs.full[1] / 2; l.z1 = l.z2 = 0; }  // grazing a face
+      if (i < 10) {  // the irregular LORs of the sensitivity test
+        const float P1[10][3] = {{-100, 5, 0}, {5, 5, 5}, {-100, 0, 0}, {15.0f, -100, 1.0f}, {-100, -100, -100}, {-100, 2, 3},
+                                 {15.0f, 0, 0}, {-15.0f, -100.0f, -15.0f}, {0, 0, -100.0f}, {-100, 50, 0}};
+        const float P2[10][3] = {{100, 5, 0}, {5, 5, 5}, {100, 0, 0}, {15.0f, 100, 1.0f}, {100, 100, 100}, {100, -2, -3},
+                                 {100.0f, 0, 0}, {-15.0f, 100.0f, -15.0f}, {0, 0, 100.0f}, {100, 50, 0}};
+        l.x1 = P1[i][0]; l.y1 = P1[i][1]; l.z1 = P1[i][2]; l.x2 = P2[i][0]; l.y2 = P2[i][1]; l.z2 = P2[i][2];
+      }
+      l.dt = 0.0f; l.ac = 1.0f;
+      std::vector<R> a, b;
+      uint32_t sa = 0, sb = 0;
+      bin_lor<false>(l, (uint32_t)i, f, t, g, [&](const Run& ru, uint32_t tl) { a.push_back({ru, tl}); }, &sa, nullptr);
+      bin_lor_axes(l, (uint32_t)i, f, g, [&](const Run& ru, uint32_t tl) { b.push_back({ru, tl}); }, &sb);
+      WalkC c; WalkS w;
+      const bool bnd = walk_setup(l, f, c, w) && bounded(w);
+      if (!bnd) { sa = 0; }  // bin_lor counts nothing for unbounded walks either
+      lors_hit += a.empty() ? 0 : 1;
+      runs += (long)a.size();
+      steps_a += sa; steps_b += sb;
+      bool ok = a.size() == b.size() && sa == sb;
+      for (size_t k = 0; ok && k < a.size(); ++k) ok = same(a[k], b[k]);
+      if (!ok) {
+        if (bad < 5) {
+          printf("MISMATCH case %ux%ux%u tile %dx%dx%d lor %d kind %d: runs %zu vs %zu steps %u vs %u\n", cs.n[0], cs.n[1], cs.n[2],
+                 g.tx, g.ty, g.tz, i, kind, a.size(), b.size(), sa, sb);
+          for (size_t k = 0; k < a.size() || k < b.size(); ++k) {
+            if (k < a.size()) printf("  A %zu: tile %u len %u local %u nb %.9g %.9g %.9g here %.9g\n", k, a[k].tile, run_len(a[k].ru.code), run_local(a[k].ru.code), a[k].ru.nb_x, a[k].ru.nb_y, a[k].ru.nb_z, a[k].ru.here);
+            if (k < b.size()) printf("  B %zu: tile %u len %u local %u nb %.9g %.9g %.9g here %.9g\n", k, b[k].tile, run_len(b[k].ru.code), run_local(b[k].ru.code), b[k].ru.nb_x, b[k].ru.nb_y, b[k].ru.nb_z, b[k].ru.here);
+          }
+        }
+        ++bad;
+      }
+    }
+  }
+  printf("lors with runs %ld, runs %ld, steps %llu vs %llu, mismatches %ld\n", lors_hit, runs, (unsigned long long)steps_a, (unsigned long long)steps_b, bad);
+  return bad ? 1 : 0;
+}
