@@ -539,11 +539,11 @@ void launch_reduce(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   hipStream_t st = c->stream;
   const uint32_t lds = std::min(c->max_win[sub], kReduceWindow);
   if (c->pos2[sub].p)
-    k_lor_reduce_win<false, true><<<c->n_blocks[sub], 1024, (size_t)lds * sizeof(float), st>>>(
+    k_lor_reduce_win<false, true><<<c->n_blocks[sub], kWinThreads, (size_t)lds * sizeof(float), st>>>(
         c->ac.p + lo, (uint32_t)count, c->win[sub].p, lds, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
         c->pos2[sub].p, c->partial.p, c->rbuf.p);
   else
-    k_lor_reduce_win<false, false><<<c->n_blocks[sub], 1024, (size_t)lds * sizeof(float), st>>>(
+    k_lor_reduce_win<false, false><<<c->n_blocks[sub], kWinThreads, (size_t)lds * sizeof(float), st>>>(
         c->ac.p + lo, (uint32_t)count, c->win[sub].p, lds, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
         c->pos2[sub].p, c->partial.p, c->rbuf.p);
   HIP_CHECK(hipGetLastError());
@@ -1186,7 +1186,7 @@ void batch_enqueue(petalo_batch* b, size_t sub) {
     k_fwd_tile_batch<false><<<(unsigned)b->items[0][sub].n, 1024, b->lds_f32, st>>>(rd, b->items[0][sub].p);
   HIP_CHECK(hipGetLastError());
   if (b->reduce_blocks[sub])
-    k_lor_reduce_win_batch<<<b->reduce_blocks[sub], 1024, (size_t)b->reduce_lds[sub] * sizeof(float), st>>>(
+    k_lor_reduce_win_batch<<<b->reduce_blocks[sub], kWinThreads, (size_t)b->reduce_lds[sub] * sizeof(float), st>>>(
         b->reduce[sub].p, n, b->reduce_lds[sub]);
   HIP_CHECK(hipGetLastError());
   for (petalo_mlem* c : b->ctx) {

@@ -793,8 +793,9 @@ __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac
 // flight per lane) and gathers from there: the partials cross the memory
 // system once, in whole cache lines, instead of as one L2 request per
 // partial.  A window over lds_entries entries is gathered from memory.
-constexpr uint32_t kWinLors = (1u << kWinShift) / 1024u;
-static_assert(kWinLors * 1024u == (1u << kWinShift), "LOR blocks of whole lanes");
+constexpr uint32_t kWinThreads = 1024;  // lanes of a k_lor_reduce_win workgroup
+constexpr uint32_t kWinLors = (1u << kWinShift) / kWinThreads;
+static_assert(kWinLors * kWinThreads == (1u << kWinShift), "LOR blocks of whole lanes");
 
 // COMPACT: the first four positions come from pos2 as 16-bit offsets into
 // the LOR's window (k_pack_pos2), else from pos4.
