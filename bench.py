@@ -378,7 +378,8 @@ def main():
         "sensitivity_ms": sens_ms,
         "sensitivity": ({"lors": n_sens, "ms": sens_ms, "n_steps": sens_stats["n_steps"], "n_nz": sens_stats["n_nz"],
                          "algorithmic_bytes": 8 * sens_stats["n_nz"] + 4 * fov.nvox,
-                         "achieved_gbs": (8 * sens_stats["n_nz"] + 4 * fov.nvox) / (sens_ms * 1e-3) / 1e9}
+                         "achieved_gbs": (8 * sens_stats["n_nz"] + 4 * fov.nvox) / (sens_ms * 1e-3) / 1e9,
+                         "frac": (8 * sens_stats["n_nz"] + 4 * fov.nvox) / (sens_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
                         if sens_stats else None),
         "discrete_sensitivity": discrete,
         "parity_rel_l2_sample": parity,
