@@ -77,6 +77,11 @@ typedef struct {
   uint64_t n_skipped;  /* LORs dropped by the index check (projector.rs:103-118) */
   uint64_t n_missed;   /* LORs missing the FOV                               */
   uint64_t n_runaway;  /* walks stopped at the step cap                      */
+  uint64_t n_shifted;  /* walks that start one voxel short of their FOV entry:
+                          an entry coordinate in voxel units still negative
+                          after the reference's 1e-5 snap (siddon.rs:134-153),
+                          which floor + the saturating cast (:312-313) put in
+                          voxel 0; the reference's own behaviour, counted     */
 } petalo_stats_t;
 
 /* Backprojection schemes for petalo_mlem_options_t.scheme. */
@@ -266,8 +271,11 @@ int petalo_ingest_lors(const float* records, size_t n_records, const petalo_inge
  * out_fov (NULL = att_fov), normalised by n_lors.  With lors == NULL the
  * LORs are generated on the device: both ends uniform on the detector
  * cylinder (length, diameter), rejected until they enter att_fov, from a
- * Philox4x32-10 stream keyed by seed and counted by LOR index.
- * Accumulation is in FP64. */
+ * Philox4x32-10 stream keyed by seed and counted by LOR index (attempt k of
+ * LOR i: block (i, k, 0, 0x5e45); z = L (u0 - 1/2), theta = TAU u1 for p1,
+ * u2, u3 for p2; u = top 24 bits / 2^24).  A LOR that finds no FOV-crossing
+ * pair within 4096 attempts fails the call with PETALO_ERR_ARG (the reference
+ * draws forever).  Accumulation is in FP64. */
 int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu,
                                   const petalo_fov_t* out_fov_or_null, const petalo_lor_t* lors_or_null,
                                   size_t n_lors, uint64_t seed, float detector_length_mm,

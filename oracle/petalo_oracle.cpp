@@ -31,6 +31,7 @@
 #include <cstdint>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -77,6 +78,18 @@ template <class T> Fov<T> fov_new(const float full[3], const uint32_t n[3]) {
   return f;
 }
 
+// Entry-point arithmetic used by fov_entry (orc_set_entry_variant):
+//   0  the restated ncollide3d slab cast: entry = p1 + dir * toi per component
+//      (the reference's FOV::entry as far as its third-party arithmetic is
+//      known; the default);
+//   1  the same, then the coordinate of the face that set toi replaced by that
+//      face's exact position (-half_width: after flip_axes every direction
+//      component is >= 0, so a LOR enters through a lower face).  This is the
+//      entry an exact cast gives; it never lands outside the box, so no walk
+//      starts shifted (see Hit::shifted).  It bounds how much the unpinned
+//      binary32 order of the cast can move an image.
+int g_entry_variant = 0;
+
 // src/fov.rs:55-71 -> ncollide3d 0.33.0 Cuboid::toi_with_ray (solid = true),
 // i.e. the AABB slab cast with tmin = 0, tmax = max_toi; identity isometry.
 template <class T> bool fov_entry(const Fov<T>& f, const T p1[3], const T p2[3], T out[3]) {
@@ -85,6 +98,7 @@ template <class T> bool fov_entry(const Fov<T>& f, const T p1[3], const T p2[3],
   const T dir[3] = {d[0] / len, d[1] / len, d[2] / len};  // normalize: v / norm
   T tmin = T(0);
   T tmax = len;  // max_toi = (p2 - p1).norm()
+  int face = -1;  // axis whose slab set tmin (-1: p1 inside the box)
   for (int i = 0; i < 3; ++i) {
     if (dir[i] == T(0)) {
       if (p1[i] < -f.hw[i] || p1[i] > f.hw[i]) return false;
@@ -93,12 +107,14 @@ template <class T> bool fov_entry(const Fov<T>& f, const T p1[3], const T p2[3],
       T a = (-f.hw[i] - p1[i]) * denom;
       T b = (f.hw[i] - p1[i]) * denom;
       if (a > b) std::swap(a, b);
+      if (a > tmin) face = i;
       tmin = fmax_(tmin, a);
       tmax = fmin_(tmax, b);
       if (tmin > tmax) return false;
     }
   }
   for (int i = 0; i < 3; ++i) out[i] = p1[i] + dir[i] * tmin;
+  if (g_entry_variant == 1 && face >= 0) out[face] = -f.hw[face];
   return true;
 }
 
@@ -143,6 +159,12 @@ template <class T> struct Hit {
   int32_t delta[3];
   int32_t remaining[3];
   T tof_peak;
+  T frac[3];     // e - floor(e): how far into its first voxel the walk starts, per axis
+  T entry[3];    // entry point (flipped coordinates)
+  bool shifted;  // an entry coordinate e is negative after the EPS snap: floor(e) = -1
+                 // saturates to voxel 0 (siddon.rs:312-313), so the walk starts
+                 // one voxel short of the entry along that axis and every index
+                 // after its first (sliver) step is one voxel further along
 };
 
 // siddon.rs:190-229 lor_fov_hit and helpers :138-185, :233-273, :322-340.
@@ -174,9 +196,12 @@ template <class T> bool lor_fov_hit(const orc_lor_t& lor, const Fov<T>& f, Hit<T
 
   // find_entry_point :138-153
   T e[3];
+  h.shifted = false;
   for (int i = 0; i < 3; ++i) {
+    h.entry[i] = entry[i];
     e[i] = (entry[i] + f.hw[i]) / f.vs[i];
     if (std::fabs(e[i]) < K<T>::EPS) e[i] = T(0);
+    if (e[i] < T(0)) h.shifted = true;
   }
 
   // index_trackers :233-273
@@ -202,6 +227,7 @@ template <class T> bool lor_fov_hit(const orc_lor_t& lor, const Fov<T>& f, Hit<T
   // first_boundaries :166-174
   for (int i = 0; i < 3; ++i) {
     const T frac = e[i] - std::floor(e[i]);
+    h.frac[i] = frac;
     h.next[i] = (T(1) - frac) * h.vs[i];
   }
   return true;
@@ -210,13 +236,20 @@ template <class T> bool lor_fov_hit(const orc_lor_t& lor, const Fov<T>& f, Hit<T
 template <class T> struct Row {
   std::vector<int64_t> idx;
   std::vector<T> w;
+  std::vector<uint8_t> axis;  // crossed axis of every walk step (filled on request)
+  std::vector<uint8_t> push;  // whether the step pushed an element (filled on request)
   uint64_t steps = 0;
   bool runaway = false;
+  bool shifted = false;
+  bool record = false;        // fill axis/push
   void clear() {
     idx.clear();
     w.clear();
+    axis.clear();
+    push.clear();
     steps = 0;
     runaway = false;
+    shifted = false;
   }
 };
 
@@ -256,6 +289,10 @@ template <class T> void walk(const Hit<T>& hit, const Gauss<T>* g, const int64_t
       row.idx.push_back((int64_t)index);
       row.w.push_back(weight);
     }
+    if (row.record) {
+      row.axis.push_back((uint8_t)d);
+      row.push.push_back(weight > T(0) ? 1 : 0);
+    }
     here = bnd;
     next[d] += hit.vs[d];
     index = (int32_t)((uint32_t)index + (uint32_t)hit.delta[d]);
@@ -268,9 +305,78 @@ template <class T> bool make_row(const orc_lor_t& lor, const Fov<T>& f, const Ga
   row.clear();
   Hit<T> h;
   if (!lor_fov_hit(lor, f, h)) return false;
+  row.shifted = h.shifted;
   walk(h, g, f.n, row);
   return true;
 }
+
+// "binary32 walk decisions, binary64 arithmetic": the binary32 walk (the
+// reference's) decides everything discrete -- whether the LOR hits, the first
+// voxel, how far into it the walk starts (the binary32 fraction frac, which
+// carries the entry quirk of Hit::shifted), the crossed axis of every step,
+// which steps push an element and when the walk ends -- and the weights are
+// then evaluated in binary64 along that same step sequence: binary64 voxel
+// lengths along the LOR, boundaries accumulated in binary64 from
+// (1 - frac) * length, the binary64 TOF peak and Gaussian.  Images from these
+// rows differ from the binary32 oracle's by rounding alone and from the
+// binary64 oracle's by the walk's discrete choices alone.
+bool make_row_mixed(const orc_lor_t& lor, const Fov<float>& f32, const Fov<double>& f64, const Gauss<double>* g,
+                    const Gauss<float>* g32, Row<double>& row, Row<float>& scratch) {
+  row.clear();
+  Hit<float> h;
+  if (!lor_fov_hit(lor, f32, h)) return false;
+  scratch.clear();
+  scratch.record = true;
+  walk(h, g32, f32.n, scratch);
+  row.steps = scratch.steps;
+  row.runaway = scratch.runaway;
+  row.shifted = h.shifted;
+  // binary64 geometry of the flipped LOR
+  double p1[3], p2[3];
+  for (int i = 0; i < 3; ++i) {
+    p1[i] = lor.p1[i];
+    p2[i] = lor.p2[i];
+    if (h.delta[i] < 0) {  // flipped in binary32 (flip_axes); the same here
+      p1[i] = -p1[i];
+      p2[i] = -p2[i];
+    }
+  }
+  const double d[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+  const double len = norm3(d[0], d[1], d[2]);
+  double vl[3], next[3];
+  for (int i = 0; i < 3; ++i) {
+    vl[i] = f64.vs[i] / (d[i] / len);
+    next[i] = (1.0 - (double)h.frac[i]) * vl[i];
+  }
+  double entry[3];
+  if (!fov_entry(f64, p1, p2, entry))
+    for (int i = 0; i < 3; ++i) entry[i] = h.entry[i];
+  const double half = norm3(p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]) / 2.0;
+  const double tof_peak = (half - K<double>::C * (double)lor.dt_ps / 2.0) -
+                          norm3(entry[0] - p1[0], entry[1] - p1[1], entry[2] - p1[2]);
+  double here = 0.0;
+  int32_t index = h.index;
+  for (size_t k = 0; k < scratch.axis.size(); ++k) {
+    const int a = scratch.axis[k];
+    const double bnd = next[a];
+    double w = bnd - here;
+    if (g) w *= K<double>::TOF_FACTOR * gauss_call(*g, here - tof_peak);
+    if (scratch.push[k]) {
+      row.idx.push_back((int64_t)index);
+      row.w.push_back(w);
+    }
+    here = bnd;
+    next[a] += vl[a];
+    index = (int32_t)((uint32_t)index + (uint32_t)h.delta[a]);
+  }
+  return true;
+}
+
+// The binary32 FOV and Gaussian behind a mixed-precision projection.
+struct Mixed {
+  Fov<float> f;
+  const Gauss<float>* g;
+};
 
 // projector.rs:104-116: `index as usize >= len` (negative i32 -> huge usize).
 template <class T> bool row_in_range(const Row<T>& r, int64_t len) {
@@ -285,6 +391,7 @@ template <class T> void count(orc_stats_t& s, const Row<T>& r, bool hit) {
   s.n_nz += r.idx.size();
   if (!hit) s.n_missed += 1;
   if (r.runaway) s.n_runaway += 1;
+  if (hit && r.shifted) s.n_shifted += 1;
 }
 
 template <class T> T mlem_adapt(T p, const orc_lor_t& lor) { return p * T(lor.additive_correction); }
@@ -293,10 +400,11 @@ template <class T> T sens_adapt(T p, const orc_lor_t&) { return std::exp(-p); }
 // project_lors (projector.rs:33-63) with 10 000-LOR jobs (mlem.rs:39,57-66):
 // job c goes to thread c % n_threads; each thread folds into a private
 // backprojection; the private buffers are summed in thread order.
+// With mx (T = double only), rows come from make_row_mixed.
 template <class T, class Adapt>
 void project_lors(const Fov<T>& ffwd, const Fov<T>* fbck_opt, const Gauss<T>* g, const orc_lor_t* lors,
                   size_t n_lors, const T* image, int n_threads, Adapt adapt, std::vector<T>& bp_out,
-                  orc_stats_t& stats) {
+                  orc_stats_t& stats, const Mixed* mx = nullptr) {
   const Fov<T>& fbck = fbck_opt ? *fbck_opt : ffwd;
   const size_t chunk = 10000;
   const size_t n_chunks = (n_lors + chunk - 1) / chunk;
@@ -308,12 +416,18 @@ void project_lors(const Fov<T>& ffwd, const Fov<T>* fbck_opt, const Gauss<T>* g,
   for (auto& s : st) std::memset(&s, 0, sizeof s);
   auto work = [&](int t) {
     Row<T> rf, rb;
+    Row<float> scratch;
     std::vector<T>& bp = bps[t];
     for (size_t c = (size_t)t; c < n_chunks; c += (size_t)n_threads) {
       const size_t lo = c * chunk, hi = std::min(n_lors, lo + chunk);
       for (size_t i = lo; i < hi; ++i) {
         const orc_lor_t& lor = lors[i];
-        bool hit = make_row(lor, ffwd, g, rf);
+        bool hit;
+        if constexpr (std::is_same<T, double>::value) {
+          hit = mx ? make_row_mixed(lor, mx->f, ffwd, g, mx->g, rf, scratch) : make_row(lor, ffwd, g, rf);
+        } else {
+          hit = make_row(lor, ffwd, g, rf);
+        }
         count(st[t], rf, hit);
         const Row<T>* rbp = &rf;
         if (fbck_opt) {
@@ -353,19 +467,24 @@ void project_lors(const Fov<T>& ffwd, const Fov<T>* fbck_opt, const Gauss<T>* g,
     stats.n_skipped += s.n_skipped;
     stats.n_missed += s.n_missed;
     stats.n_runaway += s.n_runaway;
+    stats.n_shifted += s.n_shifted;
   }
 }
 
 template <class T>
 int mlem_update(const float full[3], const uint32_t n[3], const orc_tof_t* tof, const orc_lor_t* lors,
-                size_t n_lors, const T* sens, T* image, int n_threads, orc_stats_t* stats) {
+                size_t n_lors, const T* sens, T* image, int n_threads, orc_stats_t* stats, bool mixed = false) {
   const Fov<T> f = fov_new<T>(full, n);
   Gauss<T> g;
   if (tof) g = gauss_new<T>(T(tof->sigma_ps), T(tof->cutoff));
+  Gauss<float> g32;
+  if (tof) g32 = gauss_new<float>(tof->sigma_ps, tof->cutoff);
+  const Mixed mx{fov_new<float>(full, n), tof ? &g32 : nullptr};
   orc_stats_t st;
   std::memset(&st, 0, sizeof st);
   std::vector<T> bp;
-  project_lors<T>(f, nullptr, tof ? &g : nullptr, lors, n_lors, image, n_threads, mlem_adapt<T>, bp, st);
+  project_lors<T>(f, nullptr, tof ? &g : nullptr, lors, n_lors, image, n_threads, mlem_adapt<T>, bp, st,
+                  mixed ? &mx : nullptr);
   // apply_sensitivity_image mlem.rs:45-52
   const int64_t nv = f.nvox();
   for (int64_t j = 0; j < nv; ++j) {
@@ -567,6 +686,87 @@ int orc_mlem_update_f64(const float full_mm[3], const uint32_t n[3], const orc_t
   return mlem_update<double>(full_mm, n, tof, lors, n_lors, sensitivity, image, n_threads, stats);
 }
 
+int orc_mlem_update_mixed(const float full_mm[3], const uint32_t n[3], const orc_tof_t* tof, const orc_lor_t* lors,
+                          size_t n_lors, const double* sensitivity, double* image, int n_threads,
+                          orc_stats_t* stats) {
+  return mlem_update<double>(full_mm, n, tof, lors, n_lors, sensitivity, image, n_threads, stats, true);
+}
+
+int64_t orc_row_mixed(const float full_mm[3], const uint32_t n[3], const orc_tof_t* tof, const orc_lor_t* lor,
+                      int64_t* idx, double* w, int64_t cap, uint64_t* steps) {
+  const Fov<float> f32 = fov_new<float>(full_mm, n);
+  const Fov<double> f64 = fov_new<double>(full_mm, n);
+  Gauss<double> g;
+  Gauss<float> g32;
+  if (tof) {
+    g = gauss_new<double>(tof->sigma_ps, tof->cutoff);
+    g32 = gauss_new<float>(tof->sigma_ps, tof->cutoff);
+  }
+  Row<double> r;
+  Row<float> scratch;
+  make_row_mixed(*lor, f32, f64, tof ? &g : nullptr, tof ? &g32 : nullptr, r, scratch);
+  const int64_t m = (int64_t)r.idx.size();
+  for (int64_t k = 0; k < m && k < cap; ++k) {
+    idx[k] = r.idx[k];
+    w[k] = r.w[k];
+  }
+  if (steps) *steps = r.steps;
+  return r.runaway ? -1 - m : m;
+}
+
+// find_lors / random_point_on_cylinder (continuous.rs:20-47) as the library
+// draws them: attempt k of LOR i takes Philox4x32-10 block (i, k, 0, 0x5e45)
+// under key (seed lo, seed hi); uniforms are the top 24 bits / 2^24;
+// p1 = (R cos t1, R sin t1, z1), p2 likewise, z = L (u - 1/2), t = TAU u,
+// with (z1, t1, z2, t2) from the block's four words; a pair is kept when
+// FOV::entry finds the box (unflipped endpoints, as find_lors calls it).
+int64_t orc_sensitivity_lors(const float full_mm[3], const uint32_t n[3], size_t n_lors, uint64_t seed, float L,
+                             float D, orc_lor_t* out, uint32_t* attempts) {
+  const Fov<float> f = fov_new<float>(full_mm, n);
+  const float R = D / 2.0f;
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  int64_t found_all = 0;
+  for (size_t i = 0; i < n_lors; ++i) {
+    bool found = false;
+    for (uint32_t k = 0; k < 4096 && !found; ++k) {
+      const uint32_t ctr[4] = {(uint32_t)i, k, 0u, 0x5e45u};
+      uint32_t r[4];
+      orc_philox4x32_10(ctr, key, r);
+      auto u = [](uint32_t v) { return (float)(v >> 8) * (1.0f / 16777216.0f); };
+      const float z1 = L * (u(r[0]) - 0.5f), t1 = K<float>::TAU * u(r[1]);
+      const float z2 = L * (u(r[2]) - 0.5f), t2 = K<float>::TAU * u(r[3]);
+      const float p1[3] = {R * std::cos(t1), R * std::sin(t1), z1};
+      const float p2[3] = {R * std::cos(t2), R * std::sin(t2), z2};
+      float e[3];
+      if (fov_entry(f, p1, p2, e)) {
+        out[i] = orc_lor_t{{p1[0], p1[1], p1[2]}, {p2[0], p2[1], p2[2]}, 0.0f, 1.0f};
+        if (attempts) attempts[i] = k + 1;
+        found = true;
+      }
+    }
+    if (!found) {
+      out[i] = orc_lor_t{{0, 0, 0}, {0, 0, 0}, 0.0f, 1.0f};
+      if (attempts) attempts[i] = 0;
+    } else {
+      ++found_all;
+    }
+  }
+  return found_all;
+}
+
+int orc_set_entry_variant(int variant) {
+  if (variant < 0 || variant > 1) return -1;
+  g_entry_variant = variant;
+  return 0;
+}
+
+int orc_entry_shifted_f32(const float full_mm[3], const uint32_t n[3], const orc_lor_t* lor) {
+  const Fov<float> f = fov_new<float>(full_mm, n);
+  Hit<float> h;
+  if (!lor_fov_hit(*lor, f, h)) return -1;
+  return h.shifted ? 1 : 0;
+}
+
 int orc_backproject_mlem_f32(const float full_mm[3], const uint32_t n[3], const orc_tof_t* tof,
                              const orc_lor_t* lors, size_t n_lors, const float* image, float* bp, int n_threads,
                              orc_stats_t* stats) {
@@ -636,6 +836,7 @@ int orc_walk_stats_f32(const float full_mm[3], const uint32_t n[3], const orc_to
     stats->n_skipped += s.n_skipped;
     stats->n_missed += s.n_missed;
     stats->n_runaway += s.n_runaway;
+    stats->n_shifted += s.n_shifted;
   }
   return 0;
 }

@@ -53,6 +53,8 @@ typedef struct {
   uint64_t n_skipped;    /* LORs dropped by the index check (projector.rs:103-118) */
   uint64_t n_missed;     /* LORs that miss the FOV (empty row)            */
   uint64_t n_runaway;    /* walks stopped at the step cap (reference would not terminate) */
+  uint64_t n_shifted;    /* LORs whose walk starts one voxel short of its entry (an entry
+                            coordinate e in (-1, -1e-5]: floor saturates to voxel 0) */
 } orc_stats_t;
 
 /* FOV derivation, binary32 (src/fov.rs:22-39). */
@@ -76,6 +78,39 @@ int64_t orc_row_f32(const float full_mm[3], const uint32_t n[3],
 int64_t orc_row_f64(const float full_mm[3], const uint32_t n[3],
                     const orc_tof_t* tof, const orc_lor_t* lor,
                     int64_t* idx, double* w, int64_t cap, uint64_t* steps);
+
+/* Rows and MLEM updates with binary32 walk decisions and binary64 arithmetic:
+ * the binary32 walk fixes hit, first voxel, start fraction (with its entry
+ * quirk), the crossed axis of every step and which steps push; weights,
+ * projections and the image are then binary64 along that step sequence.
+ * Against the binary32 oracle this differs by rounding only, against the
+ * binary64 oracle by the walk's discrete choices only. */
+int64_t orc_row_mixed(const float full_mm[3], const uint32_t n[3],
+                      const orc_tof_t* tof, const orc_lor_t* lor,
+                      int64_t* idx, double* w, int64_t cap, uint64_t* steps);
+int orc_mlem_update_mixed(const float full_mm[3], const uint32_t n[3],
+                          const orc_tof_t* tof, const orc_lor_t* lors, size_t n_lors,
+                          const double* sensitivity, double* image, int n_threads,
+                          orc_stats_t* stats);
+
+/* The library's sensitivity-LOR stream restated (continuous.rs:20-47 with
+ * the library's Philox draws): attempt k of LOR i takes block (i, k, 0,
+ * 0x5e45); z = L (u - 1/2), theta = TAU u, r = D/2 for both ends; kept when
+ * FOV::entry hits.  attempts[i] = draws used (0: none found in 4096).
+ * Returns the number of LORs found. */
+int64_t orc_sensitivity_lors(const float full_mm[3], const uint32_t n[3], size_t n_lors, uint64_t seed,
+                             float detector_length_mm, float detector_diameter_mm, orc_lor_t* out,
+                             uint32_t* attempts);
+
+/* FOV entry arithmetic of every later call (process-wide; test use only):
+ * 0 (default) the restated ncollide3d 0.33 slab cast, entry = p1 + dir*toi;
+ * 1 the same with the hit face's coordinate set to exactly -half_width (an
+ * exact cast: no walk starts shifted).  Returns -1 for an unknown variant. */
+int orc_set_entry_variant(int variant);
+/* 1 when the binary32 walk of this LOR starts shifted (an entry coordinate e
+ * in (-1, -1e-5] after the EPS snap of siddon.rs:134-153, floor saturating to
+ * 0 at siddon.rs:312-313), 0 when not, -1 on a miss. */
+int orc_entry_shifted_f32(const float full_mm[3], const uint32_t n[3], const orc_lor_t* lor);
 
 /* Independent classic Siddon in binary64: returns the number of segments
  * (voxel index in the reference's 1-D layout, and length in mm), in

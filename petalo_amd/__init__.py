@@ -62,7 +62,7 @@ class _Osem(C.Structure):
 
 class _Stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in
-                ("n_lors", "n_steps", "n_nz", "n_skipped", "n_missed", "n_runaway")]
+                ("n_lors", "n_steps", "n_nz", "n_skipped", "n_missed", "n_runaway", "n_shifted")]
 
 
 class _Options(C.Structure):

@@ -371,6 +371,8 @@ std::vector<std::pair<uint64_t, int4>> tile_items(const std::vector<uint64_t>& s
   return sized;
 }
 
+void zero_counters(unsigned long long* p, hipStream_t s);
+
 // Cuts the subset's LOR walks into per-tile runs, sorted by tile, and the
 // per-workgroup work items over them; lists the LORs with unbounded walks.
 void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
@@ -422,13 +424,25 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     DevBuf<Run> unsorted(total);
     DevBuf<uint64_t> keys(total), keys2(total);
     DevBuf<uint32_t> perm(total), perm2(total);
+    DevBuf<unsigned long long> chk(CNT_N);
+    zero_counters(chk.p, st);
     if (c->tof_on)
       k_bin_runs<true, true><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
-                                                                 c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift);
+                                                                 c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift,
+                                                                 chk.p);
     else
       k_bin_runs<false, true><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
-                                                                  c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift);
+                                                                  c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift,
+                                                                  chk.p);
     HIP_CHECK(hipGetLastError());
+    {
+      unsigned long long h[CNT_N];
+      HIP_CHECK(hipMemcpyAsync(h, chk.p, sizeof h, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      if (h[CNT_RUN_MISMATCH])
+        throw std::runtime_error("tile binning: fill and count passes disagree on " + std::to_string(h[CNT_RUN_MISMATCH]) +
+                                 " LORs");
+    }
     k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
     int end_bit = 33;
     while ((1 << (end_bit - 32)) < g.ntiles) ++end_bit;
@@ -982,7 +996,8 @@ int petalo_mlem_walk_stats(petalo_mlem_t* c, size_t subset, petalo_stats_t* out)
     unsigned long long h[CNT_N];
     HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
-    *out = petalo_stats_t{h[CNT_LORS], h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED], h[CNT_RUNAWAY]};
+    *out = petalo_stats_t{h[CNT_LORS], h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED], h[CNT_RUNAWAY],
+                          h[CNT_SHIFTED]};
     return PETALO_OK;
   });
 }
@@ -1224,9 +1239,10 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
     *out = nullptr;
     if (n == 0) return fail(PETALO_ERR_ARG, "a batch needs at least one reconstruction");
     if (!fovs || !lors || !n_lors) return fail(PETALO_ERR_ARG, "NULL argument");
-    petalo_mlem_options_t opt{PETALO_SCHEME_TILE, 1, 0, {0, 0, 0}};
+    petalo_mlem_options_t opt{PETALO_SCHEME_TILE, 1, 0, {0, 0, 0}, 0};
     if (options) opt = *options;
     if (opt.scheme == PETALO_SCHEME_ATOMIC) return fail(PETALO_ERR_ARG, "batches run the tiled scheme");
+    if (opt.autotune) return fail(PETALO_ERR_ARG, "options.autotune is not supported by petalo_batch_create");
     opt.scheme = PETALO_SCHEME_TILE;
     for (size_t r = 0; r < n; ++r) {
       const std::string why = check_mlem_args(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],
@@ -1495,9 +1511,16 @@ int petalo_sensitivity_lors(const petalo_fov_t* fov, size_t n_lors, uint64_t see
     if (n_lors == 0) return PETALO_OK;
     require_device();
     DevBuf<Lor> d(n_lors);
-    k_sens_lors<<<grid_for(n_lors), kBlock>>>((uint32_t)n_lors, 0u, seed, to_dev(*fov), L, D / 2.0f, d.p);
+    DevBuf<unsigned long long> cnt(CNT_N);
+    zero_counters(cnt.p, nullptr);
+    k_sens_lors<<<grid_for(n_lors), kBlock>>>((uint32_t)n_lors, 0u, seed, to_dev(*fov), L, D / 2.0f, d.p, cnt.p);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(out, d.p, n_lors * sizeof(Lor), hipMemcpyDeviceToHost));
+    unsigned long long h[CNT_N];
+    HIP_CHECK(hipMemcpy(h, cnt.p, sizeof h, hipMemcpyDeviceToHost));
+    if (h[CNT_UNFOUND])
+      return fail(PETALO_ERR_ARG, std::to_string(h[CNT_UNFOUND]) + " sensitivity LORs found no line through the FOV in " +
+                                      std::to_string(kSensMaxAttempts) + " draws (FOV outside the detector's reach)");
     return PETALO_OK;
   });
 }
@@ -1633,7 +1656,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       DevBuf<uint64_t> keys(total), keys2(total);
       DevBuf<uint32_t> perm(total), perm2(total), pos;
       k_bin_runs<false, true><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, unsorted.p,
-                                                              keys.p, shift);
+                                                              keys.p, shift, cnt);
       k_iota<<<grid64(total), kBlock, 0, s>>>(perm.p, total);
       HIP_CHECK(hipGetLastError());
       size_t sort_bytes = 0;
@@ -1765,8 +1788,12 @@ int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const fl
     unsigned long long h[CNT_N];
     HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    if (h[CNT_RUN_MISMATCH])
+      throw std::runtime_error("tile binning: fill and count passes disagree on " + std::to_string(h[CNT_RUN_MISMATCH]) +
+                               " LORs");
     if (stats)
-      *stats = petalo_stats_t{n_pairs, h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED], h[CNT_RUNAWAY]};
+      *stats = petalo_stats_t{n_pairs, h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED], h[CNT_RUNAWAY],
+                              h[CNT_SHIFTED]};
     return PETALO_OK;
   });
 }
@@ -1814,7 +1841,7 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
           HIP_CHECK(hipMemcpyAsync(dst, dl.p + first, (size_t)cnt_ * sizeof(Lor), hipMemcpyDeviceToDevice, st));
         })
                              : LorSource([&](Lor* dst, uint64_t first, uint32_t cnt_, hipStream_t st) {
-          k_sens_lors<<<grid_for(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst);
+          k_sens_lors<<<grid_for(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst, cnt.p);
         });
         sens_tiled(s, *att_fov, fa, mu, dmu.p, n_lors, src, acc.p, cnt.p, true);
       } else {
@@ -1830,9 +1857,15 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
     unsigned long long h[CNT_N];
     HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    if (h[CNT_RUN_MISMATCH])
+      throw std::runtime_error("tile binning: fill and count passes disagree on " + std::to_string(h[CNT_RUN_MISMATCH]) +
+                               " LORs");
+    if (h[CNT_UNFOUND])
+      return fail(PETALO_ERR_ARG, std::to_string(h[CNT_UNFOUND]) + " sensitivity LORs found no line through the FOV in " +
+                                      std::to_string(kSensMaxAttempts) + " draws (FOV outside the detector's reach)");
     if (stats)
       *stats = petalo_stats_t{(uint64_t)n_lors, h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED],
-                              h[CNT_RUNAWAY]};
+                              h[CNT_RUNAWAY], h[CNT_SHIFTED]};
     return PETALO_OK;
   });
 }

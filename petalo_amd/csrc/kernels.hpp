@@ -31,7 +31,16 @@
 namespace petalo {
 
 // Counter slots.
-enum : int { CNT_LORS = 0, CNT_STEPS, CNT_NZ, CNT_SKIPPED, CNT_MISSED, CNT_RUNAWAY, CNT_N };
+// CNT_SHIFTED: walks that start one voxel short of their entry (walk_setup);
+// CNT_UNFOUND: generated sensitivity LORs that found no FOV-crossing pair of
+// endpoints within kSensMaxAttempts draws.
+// CNT_RUN_MISMATCH: LORs whose fill pass of k_bin_runs emitted a different
+// number of runs than its count pass counted (never expected; the context
+// build fails if it happens).
+enum : int {
+  CNT_LORS = 0, CNT_STEPS, CNT_NZ, CNT_SKIPPED, CNT_MISSED, CNT_RUNAWAY, CNT_SHIFTED, CNT_UNFOUND, CNT_RUN_MISMATCH,
+  CNT_N
+};
 
 // Sum a per-lane value over the wavefront; lane 0 adds it to *dst.  Every
 // lane of the wavefront must call it.
@@ -146,15 +155,17 @@ template <bool TOF>
 __global__ __launch_bounds__(256) void k_walk_stats(const Lor* __restrict__ lors, uint32_t n, DevFov f, DevTof t,
                                                     unsigned long long* __restrict__ counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t c_lor = 0, c_steps = 0, c_nz = 0, c_skip = 0, c_miss = 0, c_run = 0;
+  uint32_t c_lor = 0, c_steps = 0, c_nz = 0, c_skip = 0, c_miss = 0, c_run = 0, c_shift = 0;
   if (i < n) {
     c_lor = 1;
     const Lor l = load_lor(lors, i);
     WalkC c;
     WalkS w;
-    if (!walk_setup(l, f, c, w)) {
+    bool sh = false;
+    if (!walk_setup(l, f, c, w, &sh)) {
       c_miss = 1;
     } else {
+      c_shift = sh ? 1 : 0;
       const uint32_t cap = step_cap(f);
       bool bad = false;
       for (;;) {
@@ -177,6 +188,7 @@ __global__ __launch_bounds__(256) void k_walk_stats(const Lor* __restrict__ lors
   wave_add_u64(counters + CNT_SKIPPED, c_skip);
   wave_add_u64(counters + CNT_MISSED, c_miss);
   wave_add_u64(counters + CNT_RUNAWAY, c_run);
+  wave_add_u64(counters + CNT_SHIFTED, c_shift);
 }
 
 // ---------------------------------------------------------------------------
@@ -216,7 +228,10 @@ __global__ __launch_bounds__(256) void k_rows(const Lor* __restrict__ lors, uint
 // ---------------------------------------------------------------------------
 // Sensitivity-image LORs (continuous.rs:20-47): both ends uniform on the
 // detector cylinder, z = L*(u - 1/2), theta = TAU*u; rejected until the LOR
-// enters the FOV.  Attempt k of LOR i draws Philox block (i, k, 0, tag).
+// enters the FOV.  Attempt k of LOR i draws Philox block (i, k, 0, tag).  The
+// reference draws until it finds one; here a LOR that finds none within
+// kSensMaxAttempts draws (a FOV that the detector's LORs almost never cross)
+// is counted in CNT_UNFOUND and the call fails.
 constexpr uint32_t kSensTag = 0x5e45u;
 constexpr uint32_t kSensMaxAttempts = 4096;
 
@@ -242,16 +257,22 @@ __device__ __forceinline__ Lor sens_lor(uint64_t seed, uint32_t i, const DevFov&
   return Lor{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
 }
 
-// LORs first .. first + n - 1 of the stream.
+// LORs first .. first + n - 1 of the stream; LORs without a hit counted in
+// counters[CNT_UNFOUND].
 __global__ __launch_bounds__(256) void k_sens_lors(uint32_t n, uint32_t first, uint64_t seed, DevFov f, float L,
-                                                   float R, Lor* __restrict__ out) {
+                                                   float R, Lor* __restrict__ out,
+                                                   unsigned long long* __restrict__ counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  bool found;
-  const Lor l = sens_lor(seed, first + i, f, L, R, found);
-  float4* p = reinterpret_cast<float4*>(out + i);
-  p[0] = make_float4(l.x1, l.y1, l.z1, l.x2);
-  p[1] = make_float4(l.y2, l.z2, l.dt, l.ac);
+  uint32_t unfound = 0;
+  if (i < n) {
+    bool found;
+    const Lor l = sens_lor(seed, first + i, f, L, R, found);
+    unfound = found ? 0u : 1u;
+    float4* p = reinterpret_cast<float4*>(out + i);
+    p[0] = make_float4(l.x1, l.y1, l.z1, l.x2);
+    p[1] = make_float4(l.y2, l.z2, l.dt, l.ac);
+  }
+  wave_add_u64(counters + CNT_UNFOUND, unfound);
 }
 
 // Sensitivity backprojection (project_one_lor_sens, projector.rs:72-74):
@@ -263,7 +284,7 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
                                               float R, DevFov fa, const float* __restrict__ mu, DevFov fo,
                                               double* __restrict__ acc, unsigned long long* __restrict__ counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t c_lor = 0, c_steps = 0, c_nz = 0, c_skip = 0, c_miss = 0, c_run = 0;
+  uint32_t c_lor = 0, c_steps = 0, c_nz = 0, c_skip = 0, c_miss = 0, c_run = 0, c_shift = 0, c_unfound = 0;
   const DevTof notof = no_tof();
   if (i < n) {
     c_lor = 1;
@@ -271,12 +292,15 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
     if (GEN) {
       bool found;
       l = sens_lor(seed, i, fa, L, R, found);
+      c_unfound = found ? 0u : 1u;
     } else {
       l = load_lor(lors, i);
     }
     WalkC c;
     WalkS w;
-    const bool hit = walk_setup(l, fa, c, w);
+    bool sh = false;
+    const bool hit = walk_setup(l, fa, c, w, &sh);
+    c_shift = (hit && sh) ? 1u : 0u;
     bool bad = false;
     float p = 0.0f;
     WalkS start = w;
@@ -348,6 +372,8 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
   wave_add_u64(counters + CNT_SKIPPED, c_skip);
   wave_add_u64(counters + CNT_MISSED, c_miss);
   wave_add_u64(counters + CNT_RUNAWAY, c_run);
+  wave_add_u64(counters + CNT_SHIFTED, c_shift);
+  wave_add_u64(counters + CNT_UNFOUND, c_unfound);
 }
 
 // normalize (main.rs:77): out = (float)acc / (float)n.

@@ -166,8 +166,15 @@ __host__ __device__ __forceinline__ int32_t floor_index(float e) {
 // lor_fov_hit (siddon.rs:190-229) with flip_axes (:322-340),
 // find_tof_peak (:156-163), find_entry_point (:138-153),
 // index_trackers (:233-273), voxel_size (:179-185), first_boundaries
-// (:166-174).  Returns false when the LOR misses the FOV.
-__host__ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, WalkC& c, WalkS& w) {
+// (:166-174).  Returns false when the LOR misses the FOV.  *shifted (when
+// given) is set when an entry coordinate in voxel units is still negative
+// after the EPS snap: floor gives -1, which the saturating cast turns into
+// voxel 0, so the walk starts one voxel short of its entry along that axis
+// (its first step is a sliver) and every later index is one voxel further
+// on -- the reference's own behaviour, kept; counted so that it can be
+// reported.
+__host__ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, WalkC& c, WalkS& w,
+                                                    bool* shifted = nullptr) {
   float x1 = l.x1, y1 = l.y1, z1 = l.z1, x2 = l.x2, y2 = l.y2, z2 = l.z2;
   const bool fx = (x2 - x1) < 0.0f, fy = (y2 - y1) < 0.0f, fz = (z2 - z1) < 0.0f;
   if (fx) { x1 = -x1; x2 = -x2; }
@@ -189,6 +196,7 @@ __host__ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& 
   if (fabsf(vx) < kEntryEps) vx = 0.0f;
   if (fabsf(vy) < kEntryEps) vy = 0.0f;
   if (fabsf(vz) < kEntryEps) vz = 0.0f;
+  if (shifted) *shifted = vx < 0.0f || vy < 0.0f || vz < 0.0f;
 
   const int32_t ix = floor_index(vx), iy = floor_index(vy), iz = floor_index(vz);
   c.d_x = fx ? -1 : 1;

@@ -401,7 +401,9 @@ __host__ __device__ __forceinline__ uint64_t len_key(uint32_t code) {
 }
 
 // Binning walk: FILL = false counts runs per LOR into n_runs[i]; FILL = true
-// writes them at runs[offset[i] ...] (LOR-major) with their sort keys:
+// writes them at runs[offset[i] .. offset[i + 1]) (LOR-major; a run past the
+// end of that range is not written, and a LOR whose run count differs from
+// the range is counted in counters[CNT_RUN_MISMATCH]) with their sort keys:
 // tile (bits 32 and up), then block of 2^lor_shift consecutive LORs (bits
 // 12-31; LORs are ordered spatially, so a block's 1/(p*ac) values stay
 // cache-resident while its runs in a tile are replayed), then len_key
@@ -424,16 +426,17 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
                                                   const uint32_t* __restrict__ offset, Run* __restrict__ runs,
                                                   uint64_t* __restrict__ keys, uint32_t lor_shift,
                                                   unsigned long long* __restrict__ counters = nullptr) {
-  uint32_t steps = 0, nz = 0, missed = 0;
+  uint32_t steps = 0, nz = 0, missed = 0, shifted = 0, mismatch = 0;
   const uint32_t first = blockIdx.x * blockDim.x * kBinLors + threadIdx.x;
   for (uint32_t u = 0; u < kBinLors; ++u) {
     const uint32_t i = first + u * blockDim.x;
     if (i >= n) break;
     const Lor l = load_lor(lors, i);
     const uint32_t base = FILL ? offset[i] : 0;
+    const uint32_t room = FILL ? offset[i + 1] - base : 0;
     uint32_t k = 0;
     auto emit = [&](const Run& ru, uint32_t tile) {
-      if (FILL) {
+      if (FILL && k < room) {
         runs[base + k] = ru;
         // lor_shift >= 32: no LOR-block field, key = (tile << 12) | ...
         const uint64_t blk = lor_shift >= 32 ? 0u : (uint64_t)(i >> lor_shift);
@@ -448,12 +451,15 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
     const uint32_t count = (!TOF && FILL) ? bin_lor_axes(l, i, f, g, emit)
                                           : bin_lor<TOF>(l, i, f, t, g, emit, FILL ? nullptr : &steps,
                                                          FILL ? nullptr : &nz);
+    if (FILL) mismatch += count != room ? 1u : 0u;
     if (!FILL) {
       // unbounded walks are projected by the one-lane kernels instead
       WalkC c;
       WalkS w;
-      const bool hit = walk_setup(l, f, c, w);
+      bool sh = false;
+      const bool hit = walk_setup(l, f, c, w, &sh);
       missed += hit ? 0u : 1u;
+      shifted += (hit && sh) ? 1u : 0u;
       n_runs[i] = (hit && !bounded(w)) ? 0xffffffffu : count;
     }
   }
@@ -463,7 +469,9 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
     wave_add_u64(counters + CNT_STEPS, steps);
     wave_add_u64(counters + CNT_NZ, nz);
     wave_add_u64(counters + CNT_MISSED, missed);
+    wave_add_u64(counters + CNT_SHIFTED, shifted);
   }
+  if (FILL && counters) wave_add_u64(counters + CNT_RUN_MISMATCH, mismatch);
 }
 
 // Walk state of a run for a replay against an LDS tile of elements of

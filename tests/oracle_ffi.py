@@ -19,7 +19,7 @@ LOR_DTYPE = np.dtype([("p1", "<f4", 3), ("p2", "<f4", 3), ("dt", "<f4"), ("ac", 
 
 class Stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in
-                ("n_lors", "n_steps", "n_nz", "n_skipped", "n_missed", "n_runaway")]
+                ("n_lors", "n_steps", "n_nz", "n_skipped", "n_missed", "n_runaway", "n_shifted")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -69,6 +69,14 @@ def lib():
         L.orc_rho_to_mu_factor_f32.argtypes = [C.c_float]
         L.orc_walk_stats_f32.restype = C.c_int
         L.orc_walk_stats_f32.argtypes = [P, P, P, P, C.c_size_t, C.c_int, P]
+        L.orc_row_mixed.restype = C.c_int64
+        L.orc_row_mixed.argtypes = [P, P, P, P, P, P, C.c_int64, P]
+        L.orc_mlem_update_mixed.restype = C.c_int
+        L.orc_mlem_update_mixed.argtypes = [P, P, P, P, C.c_size_t, P, P, C.c_int, P]
+        L.orc_set_entry_variant.restype = C.c_int
+        L.orc_set_entry_variant.argtypes = [C.c_int]
+        L.orc_entry_shifted_f32.restype = C.c_int
+        L.orc_entry_shifted_f32.argtypes = [P, P, P]
         _lib = L
     return _lib
 
@@ -98,15 +106,16 @@ def make_lors(p1, p2, dt=0.0, ac=1.0):
     return out
 
 
-def row(full, n, lor, tof=None, f64=False):
-    """One system-matrix row: (indices, weights, steps); runaway -> steps < 0."""
+def row(full, n, lor, tof=None, f64=False, mixed=False):
+    """One system-matrix row: (indices, weights, steps); runaway -> steps < 0.
+    mixed: binary32 walk decisions with binary64 weights."""
     full, n = _fov(full, n)
     cap = int(2 * n.sum() + 64)
     idx = np.zeros(cap, np.int64)
-    w = np.zeros(cap, np.float64 if f64 else np.float32)
+    w = np.zeros(cap, np.float64 if (f64 or mixed) else np.float32)
     steps = C.c_uint64(0)
     lor = np.ascontiguousarray(np.asarray(lor, LOR_DTYPE).reshape(1))
-    fn = lib().orc_row_f64 if f64 else lib().orc_row_f32
+    fn = lib().orc_row_mixed if mixed else (lib().orc_row_f64 if f64 else lib().orc_row_f32)
     m = fn(_p(full), _p(n), _tof(tof), _p(lor), _p(idx), _p(w), cap, C.byref(steps))
     runaway = m < 0
     if runaway:
@@ -160,15 +169,16 @@ def gauss(sigma_ps, cutoff, dx, f64=False):
     return (lib().orc_gauss_f64 if f64 else lib().orc_gauss_f32)(sigma_ps, cutoff, dx)
 
 
-def mlem_update(full, n, lors, image, sens=None, tof=None, threads=1, f64=False):
-    """One subset update in place; returns the work counters."""
+def mlem_update(full, n, lors, image, sens=None, tof=None, threads=1, f64=False, mixed=False):
+    """One subset update in place; returns the work counters.  mixed: binary32
+    walk decisions with binary64 arithmetic (binary64 image)."""
     full, n = _fov(full, n)
-    dt = np.float64 if f64 else np.float32
+    dt = np.float64 if (f64 or mixed) else np.float32
     assert image.dtype == dt and image.flags.c_contiguous
     lors = np.ascontiguousarray(lors, LOR_DTYPE)
     st = Stats()
     s = None if sens is None else _p(np.ascontiguousarray(sens, dt))
-    fn = lib().orc_mlem_update_f64 if f64 else lib().orc_mlem_update_f32
+    fn = lib().orc_mlem_update_mixed if mixed else (lib().orc_mlem_update_f64 if f64 else lib().orc_mlem_update_f32)
     fn(_p(full), _p(n), _tof(tof), _p(lors), len(lors), s, _p(image), threads, C.byref(st))
     return st.as_dict()
 
@@ -299,3 +309,42 @@ def walk_stats(full, n, lors, tof=None, threads=8):
     st = Stats()
     lib().orc_walk_stats_f32(_p(full), _p(n), _tof(tof), _p(lors), len(lors), threads, C.byref(st))
     return st.as_dict()
+
+
+def set_entry_variant(v):
+    """0: the restated ncollide3d cast (default); 1: the hit face's coordinate exact."""
+    assert lib().orc_set_entry_variant(int(v)) == 0
+
+
+class entry_variant:
+    """Context manager selecting an entry variant for the calls inside it."""
+
+    def __init__(self, v):
+        self.v = v
+
+    def __enter__(self):
+        set_entry_variant(self.v)
+
+    def __exit__(self, *a):
+        set_entry_variant(0)
+
+
+def entry_shifted(full, n, lors):
+    """Per LOR: 1 if the binary32 walk starts shifted, 0 if not, -1 on a miss."""
+    full, n = _fov(full, n)
+    lors = np.ascontiguousarray(lors, LOR_DTYPE)
+    return np.array([lib().orc_entry_shifted_f32(_p(full), _p(n), _p(lors[i:i + 1])) for i in range(len(lors))],
+                    np.int32)
+
+
+def sensitivity_lors(full, n, n_lors, seed, length, diameter):
+    """(LORs, attempts per LOR) of the library's sensitivity stream, restated."""
+    f, nn = _fov(full, n)
+    L = lib()
+    L.orc_sensitivity_lors.restype = C.c_int64
+    L.orc_sensitivity_lors.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint64, C.c_float, C.c_float,
+                                       C.c_void_p, C.c_void_p]
+    out = np.zeros(int(n_lors), LOR_DTYPE)
+    att = np.zeros(int(n_lors), np.uint32)
+    L.orc_sensitivity_lors(_p(f), _p(nn), int(n_lors), int(seed), float(length), float(diameter), _p(out), _p(att))
+    return out, att
