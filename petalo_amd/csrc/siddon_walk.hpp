@@ -173,8 +173,16 @@ __host__ __device__ __forceinline__ int32_t floor_index(float e) {
 // (its first step is a sliver) and every later index is one voxel further
 // on -- the reference's own behaviour, kept; counted so that it can be
 // reported.
+// *extra (when given) receives the per-axis start fractions e - floor(e) and
+// the entry point in flipped coordinates (for walks that replay these
+// decisions in binary64).
+struct SetupExtra {
+  float frac_x, frac_y, frac_z;
+  float ex, ey, ez;
+};
+
 __host__ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& f, WalkC& c, WalkS& w,
-                                                    bool* shifted = nullptr) {
+                                                    bool* shifted = nullptr, SetupExtra* extra = nullptr) {
   float x1 = l.x1, y1 = l.y1, z1 = l.z1, x2 = l.x2, y2 = l.y2, z2 = l.z2;
   const bool fx = (x2 - x1) < 0.0f, fy = (y2 - y1) < 0.0f, fz = (z2 - z1) < 0.0f;
   if (fx) { x1 = -x1; x2 = -x2; }
@@ -216,10 +224,12 @@ __host__ __device__ __forceinline__ bool walk_setup(const Lor& l, const DevFov& 
   c.vl_y = f.vs_y / (dy / len);
   c.vl_z = f.vs_z / (dz / len);
 
-  w.nb_x = (1.0f - (vx - floorf(vx))) * c.vl_x;
-  w.nb_y = (1.0f - (vy - floorf(vy))) * c.vl_y;
-  w.nb_z = (1.0f - (vz - floorf(vz))) * c.vl_z;
+  const float frx = vx - floorf(vx), fry = vy - floorf(vy), frz = vz - floorf(vz);
+  w.nb_x = (1.0f - frx) * c.vl_x;
+  w.nb_y = (1.0f - fry) * c.vl_y;
+  w.nb_z = (1.0f - frz) * c.vl_z;
   w.here = 0.0f;
+  if (extra) *extra = SetupExtra{frx, fry, frz, ex, ey, ez};
   return true;
 }
 
