@@ -89,6 +89,15 @@ typedef struct {
 #define PETALO_SCHEME_ATOMIC 1   /* plain: one lane per LOR, f32 global atomics */
 #define PETALO_SCHEME_TILE 2     /* image tiles in LDS, binary64 accumulation */
 
+/* Arithmetic of a context (petalo_mlem_options_t.precision). */
+#define PETALO_PRECISION_F32 0       /* the reference's binary32 walk and arithmetic */
+#define PETALO_PRECISION_F64 1       /* everything in binary64: walk setup, walk,
+                                        TOF weights, projections, image, update   */
+#define PETALO_PRECISION_F32_WALK 2  /* the binary32 walk's discrete decisions
+                                        (first voxel, start fraction, crossed axis
+                                        of every step, pushes, end) with binary64
+                                        weights, projections, image and update    */
+
 typedef struct {
   int32_t scheme;        /* PETALO_SCHEME_*                                   */
   int32_t sort_lors;     /* 1: reorder LORs inside each subset for locality   */
@@ -105,6 +114,10 @@ typedef struct {
                             at least 3% faster (creation takes about a second
                             longer); 0 takes the tiled scheme with the tile
                             given or defaulted.                              */
+  int32_t precision;     /* PETALO_PRECISION_*; the binary64 modes run one lane
+                            per LOR with binary64 global atomics whatever the
+                            scheme (reference modes for precision studies;
+                            not with autotune, not in batches)               */
 } petalo_mlem_options_t;
 
 /* ---- library ----------------------------------------------------------- */
@@ -138,8 +151,10 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
  * the Osem counter *before* it advances; image_out (N_vox floats, may be
  * NULL) receives the updated image. */
 int petalo_mlem_next(petalo_mlem_t* ctx, float* image_out_or_null, petalo_osem_t* id_out);
-/* Copies the current image to the host. */
+/* Copies the current image to the host (a binary64 image rounded). */
 int petalo_mlem_get_image(petalo_mlem_t* ctx, float* image_out);
+/* The current image in binary64 (binary64 precision modes only). */
+int petalo_mlem_get_image_f64(petalo_mlem_t* ctx, double* image_out);
 /* Replaces the current image (for resuming from a written subset image). */
 int petalo_mlem_set_image(petalo_mlem_t* ctx, const float* image);
 /* Exact walk statistics of the LOR set (full reference walk per LOR), for

@@ -26,7 +26,7 @@ __all__ = [
     "make_sensitivity_image", "sensitivity_lors", "system_matrix_rows", "density_to_attenuation",
     "synth_phantom_lors", "synth_density", "philox4x32_10", "read_config", "parse_config", "parse_quantity",
     "write_image3d", "read_image3d", "device_count", "make_lors",
-    "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE", "Batch", "ingest_lors", "ingest_params",
+    "SCHEME_AUTO", "SCHEME_ATOMIC", "SCHEME_TILE", "PRECISION_F32", "PRECISION_F64", "PRECISION_F32_WALK", "Batch", "ingest_lors", "ingest_params",
     "scattergram_spec", "H5LOR_DTYPE", "make_sensitivity_image_discrete",
     "roi", "roi_stats", "image_foms", "phantom_foms", "ROI_DTYPE", "ROI_STATS_DTYPE",
 ]
@@ -67,7 +67,10 @@ class _Stats(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("sort_lors", C.c_int32), ("device", C.c_int32),
-                ("tile", C.c_int32 * 3), ("autotune", C.c_int32)]
+                ("tile", C.c_int32 * 3), ("autotune", C.c_int32), ("precision", C.c_int32)]
+
+
+PRECISION_F32, PRECISION_F64, PRECISION_F32_WALK = 0, 1, 2
 
 
 class _Ingest(C.Structure):
@@ -163,6 +166,7 @@ _SIGS = {
     "petalo_mlem_create": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, _P]),
     "petalo_mlem_next": (C.c_int, [_P, _P, _P]),
     "petalo_mlem_get_image": (C.c_int, [_P, _P]),
+    "petalo_mlem_get_image_f64": (C.c_int, [_P, _P]),
     "petalo_mlem_set_image": (C.c_int, [_P, _P]),
     "petalo_mlem_walk_stats": (C.c_int, [_P, C.c_size_t, _P]),
     "petalo_mlem_last_skipped": (C.c_int, [_P, _P]),
@@ -349,7 +353,7 @@ class Mlem:
     subset update, like petalo::mlem::mlem (src/mlem.rs:8-31)."""
 
     def __init__(self, fov, lors, tof=None, sensitivity=None, n_subsets=1, scheme=SCHEME_AUTO,
-                 sort_lors=True, device=0, tile=None, autotune=False):
+                 sort_lors=True, device=0, tile=None, autotune=False, precision=PRECISION_F32):
         self.fov = fov
         lors = np.ascontiguousarray(lors, LOR_DTYPE)
         sens = None
@@ -360,7 +364,7 @@ class Mlem:
                 raise PetaloError(6, "sensitivity image size does not match the FOV")
         self._ctx = C.c_void_p()
         opt = _Options(int(scheme), 1 if sort_lors else 0, int(device), (C.c_int32 * 3)(*(tile or (0, 0, 0))),
-                       1 if autotune else 0)
+                       1 if autotune else 0, int(precision))
         _check(lib().petalo_mlem_create(C.byref(self._ctx), C.byref(fov._c()), _tof_ptr(tof),
                                         _ptr(lors), len(lors), _ptr(sens), int(n_subsets),
                                         C.byref(opt)))
@@ -383,6 +387,12 @@ class Mlem:
         out = np.empty(self.fov.nvox, np.float32)
         _check(lib().petalo_mlem_get_image(self._ctx, _ptr(out)))
         return Image(self.fov, out)
+
+    def image_f64(self):
+        """The binary64 image of a binary64-precision context."""
+        out = np.empty(self.fov.nvox, np.float64)
+        _check(lib().petalo_mlem_get_image_f64(self._ctx, _ptr(out)))
+        return out
 
     def set_image(self, data):
         data = np.ascontiguousarray(data, np.float32).reshape(-1)

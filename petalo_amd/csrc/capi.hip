@@ -24,6 +24,7 @@
 #include "hip_util.hpp"
 #include "kernels.hpp"
 #include "tiles.hpp"
+#include "walk64.hpp"
 #include "ingest.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -230,6 +231,11 @@ struct petalo_mlem {
   DevBuf<float> ac;                  // additive corrections, in context LOR order
   DevBuf<double> bp64;               // binary64 backprojection
   std::vector<std::vector<uint64_t>> item_steps;  // per subset: replayed steps of each work item
+  // binary64 precision modes (PETALO_PRECISION_F64 / _F32_WALK)
+  int precision = PETALO_PRECISION_F32;
+  DevFov64 dfov64{};
+  DevTof64 dtof64{};
+  DevBuf<double> image64, sens64;
   // execution
   bool own_stream = true;            // false inside a batch (the batch's stream)
   bool use_graphs = true;            // replay each subset update as a captured HIP graph
@@ -237,6 +243,32 @@ struct petalo_mlem {
 };
 
 namespace {
+
+// The binary64 modes: one lane per LOR, binary64 walk/weights/image.
+void launch_project64(petalo_mlem* c, size_t lo, size_t count) {
+  if (count == 0) return;
+  const Lor* L = c->lors.p + lo;
+  const unsigned g = grid_for(count);
+  const bool mixed = c->precision == PETALO_PRECISION_F32_WALK;
+  hipStream_t st = c->stream;
+  const uint32_t n = (uint32_t)count;
+  if (c->tof_on) {
+    if (mixed)
+      k_mlem_project64<true, true><<<g, kBlock, 0, st>>>(L, n, c->dfov, c->dtof, c->dfov64, c->dtof64,
+                                                         c->image64.p, c->bp64.p, c->counters.p);
+    else
+      k_mlem_project64<true, false><<<g, kBlock, 0, st>>>(L, n, c->dfov, c->dtof, c->dfov64, c->dtof64,
+                                                          c->image64.p, c->bp64.p, c->counters.p);
+  } else {
+    if (mixed)
+      k_mlem_project64<false, true><<<g, kBlock, 0, st>>>(L, n, c->dfov, c->dtof, c->dfov64, c->dtof64,
+                                                          c->image64.p, c->bp64.p, c->counters.p);
+    else
+      k_mlem_project64<false, false><<<g, kBlock, 0, st>>>(L, n, c->dfov, c->dtof, c->dfov64, c->dtof64,
+                                                           c->image64.p, c->bp64.p, c->counters.p);
+  }
+  HIP_CHECK(hipGetLastError());
+}
 
 void launch_project(petalo_mlem* c, size_t lo, size_t count) {
   if (count == 0) return;
@@ -658,7 +690,15 @@ void enqueue_update(petalo_mlem* c, bool timing) {
     c->ev_count = std::min(c->ev_count + 1, c->evr.size());
     HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
   }
-  if (c->scheme == PETALO_SCHEME_TILE) {
+  if (c->precision != PETALO_PRECISION_F32) {
+    launch_project64(c, lo, c->set_size);
+    if (timing) {
+      HIP_CHECK(hipEventRecord(c->ev[4], c->stream));
+      HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
+      HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
+    }
+    k_update64<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image64.p, c->bp64.p, c->sens64.p, (uint32_t)nv);
+  } else if (c->scheme == PETALO_SCHEME_TILE) {
     launch_tiled(c, sub, lo, c->set_size, timing);
     if (timing) HIP_CHECK(hipEventRecord(c->ev[2], c->stream));
     k_update_f64<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->bp64.p, c->sens.p, (uint32_t)nv);
@@ -743,6 +783,8 @@ std::string check_mlem_args(const petalo_fov_t* fov, const petalo_tof_t* tof, co
     return "unsupported backprojection scheme";
   for (int a = 0; a < 3; ++a)
     if (opt.tile[a] < 0) return "negative tile dimension";
+  if (opt.precision < PETALO_PRECISION_F32 || opt.precision > PETALO_PRECISION_F32_WALK) return "unknown precision";
+  if (opt.precision != PETALO_PRECISION_F32 && opt.autotune) return "autotune applies to the binary32 path only";
   return "";
 }
 
@@ -761,7 +803,10 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
       c->n_lors = n_lors;
       c->n_subsets = n_subsets;
       c->set_size = n_lors / n_subsets;  // remainder LORs ignored (mlem.rs:91)
-      c->scheme = opt.scheme == PETALO_SCHEME_ATOMIC ? PETALO_SCHEME_ATOMIC : PETALO_SCHEME_TILE;
+      c->precision = opt.precision;
+      // the binary64 modes are one lane per LOR with global atomics
+      c->scheme = (opt.scheme == PETALO_SCHEME_ATOMIC || opt.precision != PETALO_PRECISION_F32) ? PETALO_SCHEME_ATOMIC
+                                                                                                  : PETALO_SCHEME_TILE;
       c->osem = petalo_osem_t{n_subsets, 1, 1};
       if (shared) {
         c->stream = shared;
@@ -785,7 +830,19 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->sens.alloc(nv);
         HIP_CHECK(hipMemcpy(c->sens.p, sensitivity, nv * sizeof(float), hipMemcpyHostToDevice));
       }
-      if (c->scheme == PETALO_SCHEME_TILE) {
+      if (c->precision != PETALO_PRECISION_F32) {
+        c->dfov64 = make_fov64(fov->full_size_mm, fov->n);
+        c->dtof64 = c->tof_on ? make_tof64(tof->sigma_ps, tof->cutoff) : DevTof64{1.0, 0.0, 0.0};
+        c->image64.alloc(nv);
+        c->bp64.alloc(nv);
+        fill_f64(c->image64.p, 1.0, nv, c->stream);  // Image::ones (mlem.rs:17)
+        fill_f64(c->bp64.p, 0.0, nv, c->stream);
+        if (sensitivity) {
+          c->sens64.alloc(nv);
+          k_widen<<<grid_for(nv), kBlock, 0, c->stream>>>(c->sens.p, c->sens64.p, nv);
+          HIP_CHECK(hipGetLastError());
+        }
+      } else if (c->scheme == PETALO_SCHEME_TILE) {
         c->tiles = make_tiles(*fov, opt.tile);
         ensure_lds_attributes();
         c->runs.resize(n_subsets);
@@ -915,7 +972,7 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
   return guarded([&] {
     if (!out) return fail(PETALO_ERR_ARG, "out is NULL");
     *out = nullptr;
-    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}, 0};
+    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}, 0, PETALO_PRECISION_F32};
     if (options) opt = *options;
     const std::string why = check_mlem_args(fov, tof, lors, n_lors, n_subsets, opt);
     if (!why.empty()) return fail(PETALO_ERR_ARG, why);
@@ -948,6 +1005,10 @@ int petalo_mlem_next(petalo_mlem_t* c, float* image_out, petalo_osem_t* id_out) 
       c->osem.iteration += 1;
     }
     if (image_out) {
+      if (c->precision != PETALO_PRECISION_F32) {
+        k_narrow<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image64.p, c->image.p, nv);
+        HIP_CHECK(hipGetLastError());
+      }
       HIP_CHECK(hipMemcpyAsync(image_out, c->image.p, nv * sizeof(float), hipMemcpyDeviceToHost, c->stream));
       HIP_CHECK(hipStreamSynchronize(c->stream));
     }
@@ -959,7 +1020,23 @@ int petalo_mlem_get_image(petalo_mlem_t* c, float* out) {
   return guarded([&] {
     if (!c || !out) return fail(PETALO_ERR_ARG, "NULL argument");
     HIP_CHECK(hipSetDevice(c->device));
-    HIP_CHECK(hipMemcpyAsync(out, c->image.p, nvox(c->fov) * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    const size_t nv = nvox(c->fov);
+    if (c->precision != PETALO_PRECISION_F32) {
+      k_narrow<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image64.p, c->image.p, nv);
+      HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipMemcpyAsync(out, c->image.p, nv * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PETALO_OK;
+  });
+}
+
+int petalo_mlem_get_image_f64(petalo_mlem_t* c, double* out) {
+  return guarded([&] {
+    if (!c || !out) return fail(PETALO_ERR_ARG, "NULL argument");
+    if (c->precision == PETALO_PRECISION_F32) return fail(PETALO_ERR_ARG, "not a binary64 context");
+    HIP_CHECK(hipSetDevice(c->device));
+    HIP_CHECK(hipMemcpyAsync(out, c->image64.p, nvox(c->fov) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     return PETALO_OK;
   });
@@ -969,7 +1046,12 @@ int petalo_mlem_set_image(petalo_mlem_t* c, const float* img) {
   return guarded([&] {
     if (!c || !img) return fail(PETALO_ERR_ARG, "NULL argument");
     HIP_CHECK(hipSetDevice(c->device));
-    HIP_CHECK(hipMemcpyAsync(c->image.p, img, nvox(c->fov) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const size_t nv = nvox(c->fov);
+    HIP_CHECK(hipMemcpyAsync(c->image.p, img, nv * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (c->precision != PETALO_PRECISION_F32) {
+      k_widen<<<grid_for(nv), kBlock, 0, c->stream>>>(c->image.p, c->image64.p, nv);
+      HIP_CHECK(hipGetLastError());
+    }
     HIP_CHECK(hipStreamSynchronize(c->stream));
     return PETALO_OK;
   });
@@ -1156,6 +1238,8 @@ int petalo_mlem_destroy(petalo_mlem_t* c) {
   c->ac.release();
   c->rbuf.release();
   c->bp64.release();
+  c->image64.release();
+  c->sens64.release();
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PETALO_OK;
@@ -1239,10 +1323,11 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
     *out = nullptr;
     if (n == 0) return fail(PETALO_ERR_ARG, "a batch needs at least one reconstruction");
     if (!fovs || !lors || !n_lors) return fail(PETALO_ERR_ARG, "NULL argument");
-    petalo_mlem_options_t opt{PETALO_SCHEME_TILE, 1, 0, {0, 0, 0}, 0};
+    petalo_mlem_options_t opt{PETALO_SCHEME_TILE, 1, 0, {0, 0, 0}, 0, PETALO_PRECISION_F32};
     if (options) opt = *options;
     if (opt.scheme == PETALO_SCHEME_ATOMIC) return fail(PETALO_ERR_ARG, "batches run the tiled scheme");
     if (opt.autotune) return fail(PETALO_ERR_ARG, "options.autotune is not supported by petalo_batch_create");
+    if (opt.precision != PETALO_PRECISION_F32) return fail(PETALO_ERR_ARG, "batches run the binary32 path");
     opt.scheme = PETALO_SCHEME_TILE;
     for (size_t r = 0; r < n; ++r) {
       const std::string why = check_mlem_args(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],

@@ -30,6 +30,10 @@ pub const PETALO_SCHEME_AUTO: i32 = 0;
 pub const PETALO_SCHEME_ATOMIC: i32 = 1;
 pub const PETALO_SCHEME_TILE: i32 = 2;
 
+pub const PETALO_PRECISION_F32: i32 = 0;
+pub const PETALO_PRECISION_F64: i32 = 1;
+pub const PETALO_PRECISION_F32_WALK: i32 = 2;
+
 pub const PETALO_TIME_FORWARD_TILES: usize = 0;
 pub const PETALO_TIME_FORWARD_REDUCE: usize = 1;
 pub const PETALO_TIME_SCATTER: usize = 2;
@@ -107,6 +111,7 @@ pub struct petalo_mlem_options_t {
     pub device: i32,
     pub tile: [i32; 3],
     pub autotune: i32,
+    pub precision: i32,
 }
 
 /// Opaque MLEM/OSEM context.
@@ -247,6 +252,7 @@ extern "C" {
     ) -> c_int;
     pub fn petalo_mlem_next(ctx: *mut petalo_mlem_t, image_out_or_null: *mut f32, id_out: *mut petalo_osem_t) -> c_int;
     pub fn petalo_mlem_get_image(ctx: *mut petalo_mlem_t, image_out: *mut f32) -> c_int;
+    pub fn petalo_mlem_get_image_f64(ctx: *mut petalo_mlem_t, image_out: *mut f64) -> c_int;
     pub fn petalo_mlem_set_image(ctx: *mut petalo_mlem_t, image: *const f32) -> c_int;
     pub fn petalo_mlem_walk_stats(ctx: *mut petalo_mlem_t, subset: usize, out: *mut petalo_stats_t) -> c_int;
     pub fn petalo_mlem_last_skipped(ctx: *mut petalo_mlem_t, skipped: *mut u64) -> c_int;

@@ -175,7 +175,7 @@ int main(int argc, char** argv) {
   progress.done("Loaded " + std::to_string(lors.size()) + " LORs");
 
   petalo_mlem_t* ctx = nullptr;
-  petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}, tune ? 1 : 0};
+  petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}, tune ? 1 : 0, PETALO_PRECISION_F32};
   check(petalo_mlem_create(&ctx, &cfg.fov, cfg.has_tof ? &cfg.tof : nullptr, lors.data(), lors.size(),
                            sens.empty() ? nullptr : sens.data(), cfg.iterations_subsets, &opt),
         "creating MLEM context");
