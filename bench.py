@@ -38,34 +38,70 @@ WORKLOADS = {
     "plain_mlem_60": (180.0, 60, 1_000_000, 0.0, None, 0, 1),
     "precision_200": (200.0, 200, 50_000_000, 150.0, (31.85, 3.0), 0, 1),
 }
+METRICS = {
+    "tof_mlem_120": "TOF-MLEM iter/s + Siddon voxel-steps/s (1e7 LORs, 120^3) at <=1e-4 rel L2",
+    "plain_mlem_60": "MLEM iter/s + Siddon voxel-steps/s (1e6 LORs, 60^3, no TOF)",
+    "precision_200": "TOF-MLEM iter/s + Siddon voxel-steps/s (5e7 LORs, 200^3, 150 ps)",
+}
+
+
+def host_cpus():
+    """(logical CPUs of the host, physical cores, CPUs this process may use)."""
+    phys = set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ":" in ln:
+                    k, v = (t.strip() for t in ln.split(":", 1))
+                    if k in ("physical id", "core id"):
+                        cur[k] = v
+                elif cur:
+                    phys.add((cur.get("physical id"), cur.get("core id")))
+                    cur = {}
+        if cur:
+            phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    return os.cpu_count(), (len(phys) or None), len(os.sched_getaffinity(0))
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_batched(P, a):
-    """32 independent TOF-OSEM reconstructions (LOR counts log-uniform in
-    [1e5, 1e7], images 60^3 .. 120^3 over a 180 mm FOV, 20 subsets) advanced
-    together; a step is one subset update of all 32, the metric full
-    iterations (20 steps) per second."""
-    rng = np.random.default_rng(4)
-    n_rec, subsets = 32, 20
+def batch_workload(P, n_rec=32, seed=4):
+    """The batched workload: 32 reconstructions, LOR counts log-uniform in
+    [1e5, 1e7], images 60^3 .. 120^3 over a 180 mm FOV, TOF 200 ps."""
+    rng = np.random.default_rng(seed)
     sizes = rng.choice([60, 72, 84, 96, 108, 120], n_rec)
     counts = np.round(10 ** (5 + 2 * rng.uniform(size=n_rec))).astype(np.int64)
     fovs = [P.FOV([180.0] * 3, [int(v)] * 3) for v in sizes]
-    t0 = time.time()
     lors = [P.synth_phantom_lors(100 + k, int(counts[k]), 200.0) for k in range(n_rec)]
+    return fovs, lors, sizes, counts
+
+
+def measure_batched(P, steps, warmup, singles=True):
+    """32 independent TOF-OSEM reconstructions (20 subsets) advanced together;
+    a step is one full iteration (20 subset updates of all 32).  With singles,
+    the same reconstructions run one at a time as single contexts, on the
+    default (tiled) path and on the plain one-lane atomic path: one warm-up
+    full iteration each (captures its graphs), then one timed full iteration
+    each; their sums are the per-iteration times the batch is compared with."""
+    n_rec, subsets = 32, 20
+    t0 = time.time()
+    fovs, lors, sizes, counts = batch_workload(P, n_rec)
     log(f"[bench] batched: {counts.sum()} LORs over {n_rec} reconstructions in {time.time() - t0:.1f} s")
     tof = P.Tof(42.47, 3.0)
-    t0 = time.time()
+    t0 = time.perf_counter()
     b = P.Batch(fovs, lors, tofs=[tof] * n_rec, n_subsets=subsets)
-    log(f"[bench] batch ready in {time.time() - t0:.1f} s")
-    for _ in range(a.warmup * subsets):
+    create_s = time.perf_counter() - t0
+    log(f"[bench] batch ready in {create_s:.1f} s")
+    for _ in range(warmup * subsets):
         b.step()
     P.lib().petalo_device_synchronize(0)
     t0 = time.perf_counter()
-    for _ in range(a.steps * subsets):
+    for _ in range(steps * subsets):
         b.step()
     P.lib().petalo_device_synchronize(0)
     el = time.perf_counter() - t0
@@ -75,36 +111,99 @@ def run_batched(P, a):
         b.step()
         ms_direct.append(b.last_ms())
     b.close()
-    # The plain drop-in for comparison: one reconstruction at a time, one
-    # lane per LOR with global atomics (PETALO_SCHEME_ATOMIC), one full
-    # iteration each after one warm-up subset update.
-    plain_s = 0.0
-    if not a.no_cpu_baseline:
-        for k in range(n_rec):
-            c = P.Mlem(fovs[k], lors[k], tof=tof, n_subsets=subsets, scheme=P.SCHEME_ATOMIC)
-            c.step(copy_image=False)
-            P.lib().petalo_device_synchronize(0)
-            t1 = time.perf_counter()
-            for _ in range(subsets - 1):
-                c.step(copy_image=False)
-            c.step(copy_image=False)
-            P.lib().petalo_device_synchronize(0)
-            plain_s += time.perf_counter() - t1
-            c.close()
-        log(f"[bench] plain one-at-a-time atomic path: {plain_s * 1e3:.1f} ms per full iteration of all 32")
+    full_ms = el / steps * 1e3
+    log(f"[bench] batched: {full_ms:.2f} ms per full iteration of all {n_rec}")
+    out = {"value": steps / el, "unit": "iter/s", "steps": steps, "warmup": warmup,
+           "ms_per_full_iteration": full_ms, "subset_update_ms_graphs": full_ms / subsets,
+           "subset_update_ms_direct_launches": float(np.mean(ms_direct)), "create_s": create_s,
+           "config": {"workload": "batched_tof_osem_32", "reconstructions": n_rec, "subsets": subsets,
+                      "voxels": [int(v) for v in sizes], "n_lors": [int(c) for c in counts],
+                      "total_lors": int(counts.sum()), "tof_sigma_ps": 42.47, "tof_cutoff": 3.0}}
+    if singles:
+        for name, scheme in (("tiled", P.SCHEME_TILE), ("atomic", P.SCHEME_ATOMIC)):
+            tot = 0.0
+            for k in range(n_rec):
+                c = P.Mlem(fovs[k], lors[k], tof=tof, n_subsets=subsets, scheme=scheme)
+                for _ in range(subsets):
+                    c.step(copy_image=False)
+                P.lib().petalo_device_synchronize(0)
+                t1 = time.perf_counter()
+                for _ in range(subsets):
+                    c.step(copy_image=False)
+                P.lib().petalo_device_synchronize(0)
+                tot += time.perf_counter() - t1
+                c.close()
+            out[f"single_{name}_full_iteration_ms"] = tot * 1e3
+            out[f"speedup_vs_single_{name}"] = tot * 1e3 / full_ms
+            log(f"[bench] batched: one at a time on the {name} path {tot * 1e3:.1f} ms per full iteration "
+                f"({tot * 1e3 / full_ms:.1f}x the batch)")
+    return out
+
+
+def run_batched(P, a):
+    """The batched workload on its own line."""
+    r = measure_batched(P, a.steps, a.warmup, singles=not a.no_cpu_baseline)
     out = {
         "metric": "batched TOF-OSEM full iterations/s (32 reconstructions, 20 subsets)",
-        "value": a.steps / el, "unit": "iter/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded phantom LORs)",
-        "config": {"workload": "batched_tof_osem_32", "reconstructions": n_rec, "subsets": subsets,
-                   "voxels": [int(v) for v in sizes], "n_lors": [int(c) for c in counts],
-                   "total_lors": int(counts.sum())},
-        "subset_update_ms_graphs": el / (a.steps * subsets) * 1e3,
-        "subset_update_ms_direct_launches": float(np.mean(ms_direct)),
-        "plain_atomic_full_iteration_ms": plain_s * 1e3 if plain_s else None,
-        "speedup_vs_plain_atomic": (plain_s / (el / a.steps)) if plain_s else None,
+        "value": r["value"], "unit": "iter/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": r["ms_per_full_iteration"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded phantom LORs)", "config": r["config"],
+        "batched": r,
     }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_precision_study(P, a):
+    """The precision workload (200^3 x 1 mm, 5e7 TOF LORs, 150 ps FWHM data,
+    reconstruction sigma 31.85 ps) for `iters` iterations on three contexts:
+    the binary32 path (tiled, the product), the full binary64 GPU path and the
+    binary32-walk/binary64-arithmetic path; per iteration the relative L2 and
+    normalised max difference of the binary32 image against each binary64
+    image, and between the two binary64 images (walk geometry alone)."""
+    full, nvx, n_lors, fwhm, tof, _, _ = WORKLOADS["precision_200"]
+    fov = P.FOV([full] * 3, [nvx] * 3)
+    t0 = time.time()
+    lors = P.synth_phantom_lors(1, n_lors, fwhm)
+    log(f"[precision] {n_lors} LORs in {time.time() - t0:.1f} s")
+    tof_obj = P.Tof(*tof)
+    ctx = {name: P.Mlem(fov, lors, tof=tof_obj, precision=prec) for name, prec in
+           (("f32", P.PRECISION_F32), ("f64", P.PRECISION_F64), ("f32_walk_f64", P.PRECISION_F32_WALK))}
+    ws = ctx["f32"].walk_stats()
+
+    def rel(u, v):
+        d = u - v
+        return float(np.linalg.norm(d) / np.linalg.norm(v)), float(np.max(np.abs(d)) / np.max(np.abs(v)))
+
+    rows = []
+    times = {k: 0.0 for k in ctx}
+    for it in range(1, a.precision_iters + 1):
+        for k, c in ctx.items():
+            P.lib().petalo_device_synchronize(0)
+            t1 = time.perf_counter()
+            c.step(copy_image=False)
+            P.lib().petalo_device_synchronize(0)
+            times[k] += time.perf_counter() - t1
+        x32 = ctx["f32"].image().data.astype(np.float64)
+        x64 = ctx["f64"].image_f64()
+        xm = ctx["f32_walk_f64"].image_f64()
+        r = {"iteration": it}
+        for name, (u, v) in {"f32_vs_f64": (x32, x64), "f32_vs_f32_walk_f64": (x32, xm),
+                             "f32_walk_f64_vs_f64": (xm, x64)}.items():
+            l2, mx = rel(u, v)
+            r[name] = {"rel_l2": l2, "max": mx}
+        rows.append(r)
+        log(f"[precision] iteration {it}: " + ", ".join(f"{k} {r[k]['rel_l2']:.2e} (max {r[k]['max']:.2e})"
+                                                         for k in r if k != "iteration"))
+    for c in ctx.values():
+        c.close()
+    out = {"metric": "precision study: relative L2 / max error per iteration of the binary32 path",
+           "config": {"workload": "precision_200", "fov_mm": full, "voxels": [nvx] * 3, "n_lors": n_lors,
+                      "tof_fwhm_ps": fwhm, "tof_sigma_ps": tof[0], "tof_cutoff": tof[1],
+                      "iterations": a.precision_iters},
+           "walk": ws, "shifted_fraction": ws["n_shifted"] / ws["n_lors"],
+           "ms_per_iteration": {k: v / a.precision_iters * 1e3 for k, v in times.items()},
+           "iterations": rows}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -121,6 +220,11 @@ def main():
                     help="let the library time the atomic scheme and five tile shapes and keep the fastest")
     ap.add_argument("--batched", action="store_true",
                     help="measure 32 batched TOF-OSEM reconstructions (20 subsets) instead")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="leave the batched workload out of the headline line")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="binary32 vs the binary64 GPU paths on the 200^3 precision workload instead")
+    ap.add_argument("--precision-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-lors", type=int, default=1_000_000)
     ap.add_argument("--no-sensitivity", action="store_true")
@@ -137,6 +241,8 @@ def main():
 
     if a.batched:
         return run_batched(P, a)
+    if a.precision_study:
+        return run_precision_study(P, a)
 
     if a.copy_probe_mb:
         ms = P.copy_probe(a.copy_probe_mb << 20, 3)
@@ -207,6 +313,21 @@ def main():
 
     it_per_s = a.steps / elapsed
     steps_per_s = ws["n_steps"] * a.steps / elapsed
+
+    # End to end, as the reference's caller runs it: create the context from
+    # host LORs (upload, LOR ordering, tile binning), 10 iterations, read the
+    # image back.
+    ctx.close()
+    P.lib().petalo_device_synchronize(0)
+    t0 = time.perf_counter()
+    e2e = P.Mlem(fov, lors, tof=tof_obj, sensitivity=sens_img, n_subsets=subsets, scheme=a.scheme, tile=tile)
+    t_create = time.perf_counter() - t0
+    for _ in range(10 * subsets - 1):
+        e2e.step(copy_image=False)
+    e2e.step(copy_image=True)
+    end_to_end_ms = (time.perf_counter() - t0) * 1e3
+    e2e.close()
+    log(f"[bench] end to end (create {t_create * 1e3:.0f} ms, 10 iterations, image read back): {end_to_end_ms:.1f} ms")
     proj_avg = float(np.mean(proj_ms))
     # Algorithmic bytes: per LOR its 32-byte record, per pushed element one
     # 4-byte image gather (forward) and one 4-byte backprojection add (scatter).
@@ -276,14 +397,19 @@ def main():
                 "crc": [round(float(c), 2) for c in fv[:, 1]]}
         log(f"[bench] NEMA7 figures of merit ({foms['regions']} regions, {nema.nvox} voxels): {fms:.2f} ms")
 
+    batched = None
+    if not a.no_batched and a.workload == "tof_mlem_120":
+        batched = measure_batched(P, 2, 1)
+
     cpu = None
     parity = None
+    rows = None
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_ffi as O
         m = min(a.cpu_sample_lors, n_lors)
         sample = lors[:m]
-        threads = len(os.sched_getaffinity(0))
+        n_cpus, n_phys, threads = host_cpus()
         x = np.ones(fov.nvox, np.float32)
         s = sens_img.data if sens_img is not None else None
         log(f"[bench] CPU oracle: one update over {m} LORs on {threads} threads")
@@ -304,7 +430,8 @@ def main():
                 model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
         except OSError:
             pass
-        cpu = {"value": cpu_it_s, "unit": "iter/s", "cores": threads, "kind": "port",
+        cpu = {"value": cpu_it_s, "unit": "iter/s", "cores": threads, "host_logical_cpus": n_cpus,
+               "host_physical_cores": n_phys, "kind": "port",
                "sample": f"one MLEM update over the first {m} of the {n_lors} LORs "
                          f"({t_cpu:.2f} s), scaled by {n_lors / m:g} to one iteration",
                "voxel_steps_per_s": steps_per_lor * m / t_cpu,
@@ -339,6 +466,21 @@ def main():
             t_f = time.perf_counter() - t0
             cpu["foms"] = {"ms": t_f * 1e3, "threads": 1, "sample": "the same NEMA7 analysis, binary32 oracle"}
             log(f"[bench] CPU oracle NEMA7 figures of merit: {t_f * 1e3:.0f} ms")
+        # Row dump on a sample: the GPU's voxel sequences against the oracle's
+        # walk (identical sequences and step counts, non-TOF weights bit for
+        # bit), with the boundary ties of the sample (steps of zero length).
+        k = 2000
+        idx, w, cnt, stp = P.system_matrix_rows(fov, lors[:k])
+        same = ties = 0
+        for i in range(k):
+            oi, ow, os_ = O.row(fov.full_size, fov.n, lors[i])
+            mm = int(cnt[i])
+            ok = mm == len(oi) and np.array_equal(idx[i, :mm], oi) and np.array_equal(w[i, :mm], ow) and stp[i] == os_
+            same += int(ok)
+            ties += int(stp[i]) - mm
+        rows = {"sample": k, "tof": False, "sequence_match_rate": same / k, "ties": ties,
+                "shifted": int(ws["n_shifted"])}
+        log(f"[bench] row dump: {same} of {k} rows identical to the oracle's, {ties} boundary ties")
         # Parity on the same sample: one update from ones on the GPU.
         g = P.Mlem(fov, sample, tof=tof_obj, sensitivity=sens_img, n_subsets=1)
         img, _ = g.step()
@@ -348,7 +490,7 @@ def main():
         log(f"[bench] parity on sample: relL2 {parity:.2e}")
 
     out = {
-        "metric": "TOF-MLEM iter/s + Siddon voxel-steps/s (1e7 LORs, 120^3) at <=1e-4 rel L2",
+        "metric": METRICS[a.workload],
         "value": it_per_s,
         "unit": "iter/s",
         "n_gpus": 1,
@@ -368,7 +510,13 @@ def main():
             "scheme": a.scheme, "tile": list(tile) if tile else None,
         },
         "voxel_steps_per_s": steps_per_s,
-        "walk": {"n_steps": ws["n_steps"], "n_nz": ws["n_nz"], "n_skipped": ws["n_skipped"]},
+        "walk": {"n_steps": ws["n_steps"], "n_nz": ws["n_nz"], "n_skipped": ws["n_skipped"],
+                 "n_shifted": ws["n_shifted"], "shifted_fraction": ws["n_shifted"] / max(ws["n_lors"], 1)},
+        "end_to_end_ms": end_to_end_ms,
+        "end_to_end": {"what": "context creation from host LORs + 10 iterations + image read back",
+                       "create_ms": t_create * 1e3, "iter_per_s": 10.0 / (end_to_end_ms * 1e-3)},
+        "rows": rows,
+        "batched": batched,
         "projection_ms": proj_avg,
         "kernel_ms": kt,
         "forward_ms": float(np.mean(fwd_ms)),
