@@ -308,6 +308,19 @@ int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const fl
                                            const petalo_fov_t* out_fov_or_null, float detector_length_mm,
                                            float r_min_mm, float dr_mm, float dz_mm, float da_mm, int32_t adjust,
                                            uint64_t seed, float* out, petalo_stats_t* stats_or_null);
+/* petalo_make_sensitivity_image for caller-given LORs on one FOV, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * mu (N_vox floats), lors (n_lors records) and out (N_vox floats) are DEVICE
+ * pointers on the current device; stream is a hipStream_t (NULL: the default
+ * stream) on which the work is enqueued and not waited for.  The binary64
+ * accumulator and the counters live in the library's static device scratch
+ * (one per device, kept until the process ends): images of at most 2^21
+ * voxels (PETALO_ERR_SIZE beyond).  Calls that overlap on one device race on
+ * that scratch.  Each LOR is projected by the one-lane kernel (skip and miss
+ * rules as petalo_make_sensitivity_image); no statistics are returned. */
+int petalo_sensitivity_image_noalloc(const petalo_fov_t* fov, const float* mu, const petalo_lor_t* lors,
+                                     size_t n_lors, float* out, void* stream);
 /* The LORs petalo_make_sensitivity_image generates for (seed, fov, cylinder). */
 int petalo_sensitivity_lors(const petalo_fov_t* att_fov, size_t n_lors, uint64_t seed,
                             float detector_length_mm, float detector_diameter_mm, petalo_lor_t* out);

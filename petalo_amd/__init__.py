@@ -187,6 +187,7 @@ _SIGS = {
     "petalo_make_sensitivity_image": (C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_uint64, C.c_float,
                                                 C.c_float, _P, _P]),
     "petalo_sensitivity_lors": (C.c_int, [_P, C.c_size_t, C.c_uint64, C.c_float, C.c_float, _P]),
+    "petalo_sensitivity_image_noalloc": (C.c_int, [_P, _P, _P, C.c_size_t, _P, _P]),
     "petalo_density_to_attenuation": (C.c_int, [_P, C.c_size_t, C.c_float]),
     "petalo_system_matrix_rows": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, _P, _P, _P, _P]),
     "petalo_image3d_write": (C.c_int, [C.c_char_p, _P, _P]),

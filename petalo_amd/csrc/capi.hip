@@ -1956,3 +1956,77 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The sensitivity image without any allocation, for callers that cannot
+// allocate device memory (code under HIP graph capture, serving loops):
+// petalo_sensitivity_image_noalloc.
+//
+// It is petalo_make_sensitivity_image for caller-given LORs on one FOV, with
+// device pointers in and out and the caller's stream: the one-lane kernel
+// k_sens (forward walk over mu, p -> exp(-p), backprojection of w / exp(-p)
+// with binary64 atomics) and k_sens_finalize (division by n).  Its scratch --
+// the binary64 accumulator and the work counters -- is the file-scope array
+// `scratch` below, at most kNoAllocMaxVoxels voxels; every device that runs
+// it holds its own copy until the process ends.  Host code never uses the
+// array's name as an address: on a device's first call the address is looked
+// up with hipGetSymbolAddress and kept per device.  Calls on one device share
+// the array and clear it themselves (fill kernels, no memset nodes), so calls
+// that overlap on one device race.
+namespace {
+
+// 2^21 voxels (a 128^3 image); the counters follow the accumulator.
+constexpr size_t kNoAllocMaxVoxels = size_t(1) << 21;
+constexpr size_t kCounterOffset = kNoAllocMaxVoxels * sizeof(double);
+constexpr size_t kScratchBytes = kCounterOffset + 256;
+static_assert(CNT_N * sizeof(unsigned long long) <= 256, "counters fit the scratch tail");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_mu;
+void* g_scratch[kMaxDevices] = {};  // device address of `scratch` per device, looked up once
+
+// The scratch array's device address on the current device.
+void* scratch_address(int dev) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = p;
+  }
+  return g_scratch[dev];
+}
+
+}  // namespace
+
+extern "C" int petalo_sensitivity_image_noalloc(const petalo_fov_t* fov, const float* mu, const petalo_lor_t* lors,
+                                                size_t n_lors, float* out, void* stream) {
+  return guarded([&] {
+    std::string why;
+    if (!fov_ok(fov, why)) return fail(PETALO_ERR_ARG, why);
+    if (!mu || !out || (n_lors && !lors)) return fail(PETALO_ERR_ARG, "NULL argument");
+    const size_t nv = (size_t)fov->n[0] * fov->n[1] * fov->n[2];
+    if (nv > kNoAllocMaxVoxels)
+      return fail(PETALO_ERR_SIZE, "image of " + std::to_string(nv) + " voxels exceeds the " +
+                                       std::to_string(kNoAllocMaxVoxels) +
+                                       " voxels of petalo_sensitivity_image_noalloc's scratch");
+    if (n_lors >= (1ull << 32)) return fail(PETALO_ERR_ARG, "more than 2^32-1 LORs");
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(PETALO_ERR_ARG, "device ordinal beyond the supported 64");
+    unsigned char* base = static_cast<unsigned char*>(scratch_address(dev));
+    double* acc = reinterpret_cast<double*>(base);
+    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + kCounterOffset);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const DevFov f = make_fov(fov->full_size_mm, fov->n);
+    k_fill_f64<<<grid_for(nv), kBlock, 0, s>>>(acc, 0.0, nv);
+    k_zero_u64<<<1, 64, 0, s>>>(cnt, CNT_N);
+    if (n_lors)
+      k_sens<false, false><<<grid_for(n_lors), kBlock, 0, s>>>(reinterpret_cast<const Lor*>(lors), (uint32_t)n_lors, 0,
+                                                               0.0f, 0.0f, f, mu, f, acc, cnt);
+    k_sens_finalize<<<grid_for(nv), kBlock, 0, s>>>(acc, out, (float)n_lors, nv);
+    HIP_CHECK(hipGetLastError());
+    return PETALO_OK;
+  });
+}

@@ -14,7 +14,7 @@
 //! calls these functions behind it.
 #![allow(non_camel_case_types)]
 
-use std::os::raw::{c_char, c_int};
+use std::os::raw::{c_char, c_int, c_void};
 
 // ---- status codes ---------------------------------------------------------
 pub const PETALO_OK: c_int = 0;
@@ -329,6 +329,14 @@ extern "C" {
         seed: u64,
         out: *mut f32,
         stats_or_null: *mut petalo_stats_t,
+    ) -> c_int;
+    pub fn petalo_sensitivity_image_noalloc(
+        fov: *const petalo_fov_t,
+        mu: *const f32,
+        lors: *const petalo_lor_t,
+        n_lors: usize,
+        out: *mut f32,
+        stream: *mut c_void,
     ) -> c_int;
     pub fn petalo_sensitivity_lors(
         att_fov: *const petalo_fov_t,
