@@ -1598,7 +1598,8 @@ int petalo_sensitivity_lors(const petalo_fov_t* fov, size_t n_lors, uint64_t see
     DevBuf<Lor> d(n_lors);
     DevBuf<unsigned long long> cnt(CNT_N);
     zero_counters(cnt.p, nullptr);
-    k_sens_lors<<<grid_for(n_lors), kBlock>>>((uint32_t)n_lors, 0u, seed, to_dev(*fov), L, D / 2.0f, d.p, cnt.p);
+    k_sens_lors<<<sens_lors_grid(n_lors), kBlock>>>((uint32_t)n_lors, 0u, seed, to_dev(*fov), L, D / 2.0f, d.p,
+                                                    cnt.p);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(out, d.p, n_lors * sizeof(Lor), hipMemcpyDeviceToHost));
     unsigned long long h[CNT_N];
@@ -1686,6 +1687,50 @@ __global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uin
 // order (neighbouring element pairs share cache lines and LDS banks).
 using LorSource = std::function<void(Lor*, uint64_t, uint32_t, hipStream_t)>;
 
+// A device buffer that only grows (kept across the batches of one call).
+template <class T> struct Grow {
+  DevBuf<T> b;
+  T* ensure(size_t n) {
+    if (n > b.n) b.alloc(n + n / 4);
+    return b.p;
+  }
+};
+
+// Work items of a sensitivity batch, built on the device from the tile
+// starts of the sorted runs (UINT64_MAX where a tile has none): every tile's
+// runs in chunks of `per` -- the full chunks of all tiles first, then each
+// tile's remainder, so that the largest items start first -- and empty items
+// (first == end) up to cap, which their workgroups skip.  One workgroup;
+// thread 0 walks the tiles (a few hundred to a few thousand).
+__global__ void k_sens_items(const uint64_t* __restrict__ start, uint32_t ntiles, uint64_t total, uint32_t per,
+                             uint32_t cap, int4* __restrict__ items) {
+  __shared__ uint32_t n_items;
+  if (threadIdx.x == 0) {
+    uint32_t n = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      int32_t pend = -1;  // the last non-empty tile seen, whose end is the next one's start
+      uint64_t pend_start = 0;
+      for (uint32_t t = 0; t <= ntiles; ++t) {
+        const uint64_t st = t < ntiles ? start[t] : total;
+        if (t < ntiles && st == UINT64_MAX) continue;
+        if (pend >= 0) {
+          const uint64_t a = pend_start, e = st, full = a + (e - a) / per * per;
+          if (pass == 0) {
+            for (uint64_t x = a; x < full && n < cap; x += per) items[n++] = make_int4(pend, (int)x, (int)(x + per), 0);
+          } else if (full < e && n < cap) {
+            items[n++] = make_int4(pend, (int)full, (int)e, 0);
+          }
+        }
+        pend = (int32_t)t;
+        pend_start = st;
+      }
+    }
+    n_items = n;
+  }
+  __syncthreads();
+  for (uint32_t k = n_items + threadIdx.x; k < cap; k += blockDim.x) items[k] = make_int4(0, 0, 0, 0);
+}
+
 void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const float* mu_host, const float* dmu,
                 uint64_t n_lors, const LorSource& produce, double* acc, unsigned long long* cnt, bool by_length) {
   const int32_t no_shape[3] = {0, 0, 0};
@@ -1701,98 +1746,98 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   const DevTof notof = no_tof();
   ensure_lds_attributes();
   const size_t bmax = (size_t)std::min<uint64_t>(n_lors, kSensBatch);
+  // buffers of the whole call: batch-sized ones now, run-sized ones grown
+  // to the largest batch (no allocation, and one wait, per batch)
   DevBuf<Lor> lb(bmax), lraw(by_length ? bmax : 0), irr(bmax);
   DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1);
+  DevBuf<uint32_t> lk(by_length ? bmax : 0), lk2(by_length ? bmax : 0), li(by_length ? bmax : 0),
+      li2(by_length ? bmax : 0);
   DevBuf<float> rbuf;
-  if (!mu_zero) rbuf.alloc(bmax);
+  DevBuf<uint4> pos4;
+  if (!mu_zero) {
+    rbuf.alloc(bmax);
+    pos4.alloc(bmax);
+  }
+  size_t lsort_bytes = 0, scan_bytes = 0;
+  if (by_length)
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, lsort_bytes, lk.p, lk2.p, li.p, li2.p, (int)bmax, 0, 16, s));
+  HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts.p, offs.p, (int)bmax + 1, s));
+  DevBuf<unsigned char> lsort_tmp(std::max<size_t>(lsort_bytes, 1)), scan_tmp(std::max<size_t>(scan_bytes, 1));
+  DevBuf<uint64_t> dstart(g.ntiles);
+  Grow<Run> unsorted;
+  Grow<uint64_t> keys, keys2;
+  Grow<uint32_t> perm, perm2, pos;
+  Grow<float> partial;
+  Grow<unsigned char> sort_tmp;
+  Grow<int4> work;
   for (uint64_t first = 0; first < n_lors; first += kSensBatch) {
     const uint32_t b = (uint32_t)std::min<uint64_t>(kSensBatch, n_lors - first);
     produce(by_length ? lraw.p : lb.p, first, b, s);
     HIP_CHECK(hipGetLastError());
     if (by_length) {
       // the batch in order of walk length (the image is a sum over LORs)
-      DevBuf<uint32_t> lk(b), lk2(b), li(b), li2(b);
       k_walk_len_keys<<<grid_for(b), kBlock, 0, s>>>(lraw.p, b, f, lk.p, li.p);
       HIP_CHECK(hipGetLastError());
-      size_t sb = 0;
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, lk.p, lk2.p, li.p, li2.p, (int)b, 0, 16, s));
-      DevBuf<unsigned char> st(std::max<size_t>(sb, 1));
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(st.p, sb, lk.p, lk2.p, li.p, li2.p, (int)b, 0, 16, s));
+      size_t sb = lsort_bytes;
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(lsort_tmp.p, sb, lk.p, lk2.p, li.p, li2.p, (int)b, 0, 16, s));
       k_gather_lors_by<<<grid_for(b), kBlock, 0, s>>>(lraw.p, li2.p, b, lb.p);
       HIP_CHECK(hipGetLastError());
-      HIP_CHECK(hipStreamSynchronize(s));  // the temporaries are freed at scope end
     }
     k_bin_runs<false, false><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr,
                                                              nullptr, shift, cnt);
     HIP_CHECK(hipMemsetAsync(n_irr.p, 0, 4, s));
     k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts.p, b, lb.p, irr.p, n_irr.p);
     HIP_CHECK(hipGetLastError());
-    size_t tmp_bytes = 0;
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts.p, offs.p, (int)b + 1, s));
-    DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, counts.p, offs.p, (int)b + 1, s));
+    size_t tb = scan_bytes;
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tb, counts.p, offs.p, (int)b + 1, s));
     uint32_t total = 0, h_irr = 0;
     HIP_CHECK(hipMemcpyAsync(&total, offs.p + b, 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&h_irr, n_irr.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipStreamSynchronize(s));  // the batch's run count sizes the sort
     if (total >= (1u << 31)) throw std::runtime_error("2^31 or more tile runs in one sensitivity batch");
     if (total) {
-      DevBuf<Run> unsorted(total);
-      DevBuf<uint64_t> keys(total), keys2(total);
-      DevBuf<uint32_t> perm(total), perm2(total), pos;
-      k_bin_runs<false, true><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, unsorted.p,
-                                                              keys.p, shift, cnt);
-      k_iota<<<grid64(total), kBlock, 0, s>>>(perm.p, total);
+      Run* un = unsorted.ensure(total);
+      uint64_t* k1 = keys.ensure(total);
+      uint64_t* k2 = keys2.ensure(total);
+      uint32_t* p1 = perm.ensure(total);
+      uint32_t* p2 = perm2.ensure(total);
+      k_bin_runs<false, true><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, un, k1, shift,
+                                                              cnt);
+      k_iota<<<grid64(total), kBlock, 0, s>>>(p1, total);
       HIP_CHECK(hipGetLastError());
       size_t sort_bytes = 0;
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys.p, keys2.p, perm.p, perm2.p,
-                                                   (int)total, 0, end_bit, s));
-      DevBuf<unsigned char> stmp(std::max<size_t>(sort_bytes, 1));
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp.p, sort_bytes, keys.p, keys2.p, perm.p, perm2.p,
-                                                   (int)total, 0, end_bit, s));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k1, k2, p1, p2, (int)total, 0, end_bit, s));
+      unsigned char* stmp = sort_tmp.ensure(std::max<size_t>(sort_bytes, 1));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp, sort_bytes, k1, k2, p1, p2, (int)total, 0, end_bit, s));
+      uint32_t* ps = nullptr;
       if (!mu_zero) {
-        pos.alloc(total);
-        k_invert_perm<<<grid64(total), kBlock, 0, s>>>(perm2.p, total, pos.p);
+        ps = pos.ensure(total);
+        k_invert_perm<<<grid64(total), kBlock, 0, s>>>(p2, total, ps);
       }
-      DevBuf<uint64_t> dstart(g.ntiles);
       HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, (size_t)g.ntiles * 8, s));
-      k_tile_starts<<<grid64(total), kBlock, 0, s>>>(keys2.p, total, tshift, (uint32_t)g.ntiles, dstart.p);
+      k_tile_starts<<<grid64(total), kBlock, 0, s>>>(k2, total, tshift, (uint32_t)g.ntiles, dstart.p);
+      const uint32_t cap = total / kSensItemRuns + (uint32_t)g.ntiles + 1;
+      int4* items = work.ensure(cap);
+      k_sens_items<<<1, 256, 0, s>>>(dstart.p, (uint32_t)g.ntiles, total, kSensItemRuns, cap, items);
       HIP_CHECK(hipGetLastError());
-      std::vector<uint64_t> starts(g.ntiles + 1);
-      HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, (size_t)g.ntiles * 8, hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      starts[g.ntiles] = total;
-      for (int k = g.ntiles - 1; k >= 0; --k)
-        if (starts[k] == UINT64_MAX) starts[k] = starts[k + 1];
-      std::vector<int4> items;
-      for (int k = 0; k < g.ntiles; ++k)
-        for (uint64_t a = starts[k]; a < starts[k + 1]; a += kSensItemRuns)
-          items.push_back(make_int4(k, (int)a, (int)std::min<uint64_t>(starts[k + 1], a + kSensItemRuns), 0));
-      std::stable_sort(items.begin(), items.end(),
-                       [](const int4& u, const int4& v) { return u.z - u.y > v.z - v.y; });
-      DevBuf<int4> work(items.size());
-      HIP_CHECK(hipMemcpyAsync(work.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, s));
-      const unsigned nw = (unsigned)items.size();
       if (!mu_zero) {
-        DevBuf<float> partial(total);
-        k_fwd_tile<false><<<nw, 1024, ((size_t)g.tvox + 1) * sizeof(float), s>>>(unsorted.p, work.p, dmu, f, notof, g,
-                                                                           partial.p, perm2.p, nullptr);
-        DevBuf<uint4> pos4(b);
-        k_pack_pos4<<<grid_for(b), kBlock, 0, s>>>(offs.p, pos.p, b, pos4.p);
-        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, pos.p, pos4.p, partial.p, rbuf.p);
+        float* part = partial.ensure(total);
+        k_fwd_tile<false><<<cap, 1024, ((size_t)g.tvox + 1) * sizeof(float), s>>>(un, items, dmu, f, notof, g, part,
+                                                                                   p2, nullptr);
+        k_pack_pos4<<<grid_for(b), kBlock, 0, s>>>(offs.p, ps, b, pos4.p);
+        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, ps, pos4.p, part, rbuf.p);
         HIP_CHECK(hipGetLastError());
       }
-      k_scatter<false><<<nw, 1024, (size_t)g.tvox * sizeof(double), s>>>(unsorted.p, work.p,
-                                                                         mu_zero ? nullptr : rbuf.p, f, notof,
-                                                                         g, acc, perm2.p);
+      k_scatter<false><<<cap, 1024, (size_t)g.tvox * sizeof(double), s>>>(un, items, mu_zero ? nullptr : rbuf.p, f,
+                                                                          notof, g, acc, p2);
       HIP_CHECK(hipGetLastError());
-      HIP_CHECK(hipStreamSynchronize(s));  // batch buffers are released at scope end
     }
     if (h_irr) {
       k_sens<false, false><<<grid_for(h_irr), kBlock, 0, s>>>(irr.p, h_irr, 0, 0.0f, 0.0f, f, dmu, f, acc, cnt);
       HIP_CHECK(hipGetLastError());
     }
   }
+  HIP_CHECK(hipStreamSynchronize(s));  // the buffers above are released at scope end
 }
 
 }  // namespace
@@ -1926,7 +1971,7 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
           HIP_CHECK(hipMemcpyAsync(dst, dl.p + first, (size_t)cnt_ * sizeof(Lor), hipMemcpyDeviceToDevice, st));
         })
                              : LorSource([&](Lor* dst, uint64_t first, uint32_t cnt_, hipStream_t st) {
-          k_sens_lors<<<grid_for(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst, cnt.p);
+          k_sens_lors<<<sens_lors_grid(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst, cnt.p);
         });
         sens_tiled(s, *att_fov, fa, mu, dmu.p, n_lors, src, acc.p, cnt.p, true);
       } else {

@@ -235,42 +235,86 @@ __global__ __launch_bounds__(256) void k_rows(const Lor* __restrict__ lors, uint
 constexpr uint32_t kSensTag = 0x5e45u;
 constexpr uint32_t kSensMaxAttempts = 4096;
 
+__device__ __forceinline__ bool sens_attempt(uint64_t seed, uint32_t i, uint32_t k, const DevFov& f, float L, float R,
+                                             Lor& l);
+
 __device__ __forceinline__ Lor sens_lor(uint64_t seed, uint32_t i, const DevFov& f, float L, float R,
                                         bool& found) {
-  for (uint32_t k = 0; k < kSensMaxAttempts; ++k) {
-    const U32x4 r = philox(seed, i, k, 0, kSensTag);
-    const float z1 = L * (unit_f32(r.x) - 0.5f);
-    const float t1 = kTau * unit_f32(r.y);
-    const float z2 = L * (unit_f32(r.z) - 0.5f);
-    const float t2 = kTau * unit_f32(r.w);
-    float s1, c1, s2, c2;
-    sincosf(t1, &s1, &c1);
-    sincosf(t2, &s2, &c2);
-    const float x1 = R * c1, y1 = R * s1, x2 = R * c2, y2 = R * s2;
-    float ex, ey, ez;
-    if (fov_entry(f, x1, y1, z1, x2, y2, z2, ex, ey, ez)) {
+  Lor l;
+  for (uint32_t k = 0; k < kSensMaxAttempts; ++k)
+    if (sens_attempt(seed, i, k, f, L, R, l)) {
       found = true;
-      return Lor{x1, y1, z1, x2, y2, z2, 0.0f, 1.0f};
+      return l;
     }
-  }
   found = false;
   return Lor{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
 }
 
-// LORs first .. first + n - 1 of the stream; LORs without a hit counted in
-// counters[CNT_UNFOUND].
+// One draw of sens_lor: attempt k of LOR i.  Returns true when it enters the FOV.
+__device__ __forceinline__ bool sens_attempt(uint64_t seed, uint32_t i, uint32_t k, const DevFov& f, float L, float R,
+                                             Lor& l) {
+  const U32x4 r = philox(seed, i, k, 0, kSensTag);
+  const float z1 = L * (unit_f32(r.x) - 0.5f);
+  const float t1 = kTau * unit_f32(r.y);
+  const float z2 = L * (unit_f32(r.z) - 0.5f);
+  const float t2 = kTau * unit_f32(r.w);
+  float s1, c1, s2, c2;
+  sincosf(t1, &s1, &c1);
+  sincosf(t2, &s2, &c2);
+  l = Lor{R * c1, R * s1, z1, R * c2, R * s2, z2, 0.0f, 1.0f};
+  float ex, ey, ez;
+  return fov_entry(f, l.x1, l.y1, l.z1, l.x2, l.y2, l.z2, ex, ey, ez);
+}
+
+// LORs first .. first + n - 1 of the stream (each exactly sens_lor's: attempt
+// k of LOR i draws block (i, k, 0, tag), the first hit is kept), written to
+// out[0 .. n); LORs without a hit counted in counters[CNT_UNFOUND].  The
+// rejection loop takes a geometric number of draws (5.4 on average for a
+// 180 mm FOV in a 700 mm x 300 mm detector), so one LOR per lane would keep a
+// wavefront for its slowest lane's draws; instead each wavefront takes
+// kSensWaveLors consecutive LORs and a lane that has found its LOR takes the
+// wavefront's next one, in lane order.  Launch with sens_lors_grid(n) blocks
+// of 256.
+constexpr uint32_t kSensWaveLors = 1024;
+inline unsigned sens_lors_grid(size_t n) {
+  const size_t waves = (n + kSensWaveLors - 1) / kSensWaveLors;
+  return (unsigned)((waves + 3) / 4);
+}
+
 __global__ __launch_bounds__(256) void k_sens_lors(uint32_t n, uint32_t first, uint64_t seed, DevFov f, float L,
                                                    float R, Lor* __restrict__ out,
                                                    unsigned long long* __restrict__ counters) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t unfound = 0;
-  if (i < n) {
-    bool found;
-    const Lor l = sens_lor(seed, first + i, f, L, R, found);
-    unfound = found ? 0u : 1u;
-    float4* p = reinterpret_cast<float4*>(out + i);
-    p[0] = make_float4(l.x1, l.y1, l.z1, l.x2);
-    p[1] = make_float4(l.y2, l.z2, l.dt, l.ac);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t c0 = (uint64_t)wave * kSensWaveLors;
+  if (c0 >= n) return;  // the whole wavefront
+  const uint32_t c1 = (uint32_t)(c0 + kSensWaveLors < n ? c0 + kSensWaveLors : n);
+  uint32_t i = (uint32_t)c0 + lane, k = 0, next = (uint32_t)c0 + 64u, unfound = 0;
+  bool active = i < c1;
+  while (__any(active)) {
+    bool done = false;
+    if (active) {
+      Lor l;
+      const bool hit = sens_attempt(seed, first + i, k, f, L, R, l);
+      ++k;
+      if (hit || k == kSensMaxAttempts) {
+        if (!hit) {
+          l = Lor{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+          ++unfound;
+        }
+        float4* p = reinterpret_cast<float4*>(out + i);
+        p[0] = make_float4(l.x1, l.y1, l.z1, l.x2);
+        p[1] = make_float4(l.y2, l.z2, l.dt, l.ac);
+        done = true;
+      }
+    }
+    const unsigned long long fin = __ballot(done);
+    if (done) {
+      i = next + (uint32_t)__popcll(fin & ((1ull << lane) - 1ull));
+      k = 0;
+      active = i < c1;
+    }
+    next += (uint32_t)__popcll(fin);
   }
   wave_add_u64(counters + CNT_UNFOUND, unfound);
 }

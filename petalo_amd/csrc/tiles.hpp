@@ -656,7 +656,9 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(const Run* __restrict__ runs,
                                                    float* __restrict__ partial,
                                                    const uint32_t* __restrict__ perm,
                                                    const uint32_t* __restrict__ lpos) {
-  fwd_tile_body<TOF>(runs, work[blockIdx.x], img, f, t, g, partial, perm, lpos);
+  const int4 wi = work[blockIdx.x];
+  if (wi.z <= wi.y) return;  // an empty item (the grid may be an upper bound)
+  fwd_tile_body<TOF>(runs, wi, img, f, t, g, partial, perm, lpos);
 }
 
 // One reconstruction of a batch, as the batched tile kernels see it: the runs
@@ -1067,7 +1069,9 @@ __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, 
                                                   const float* __restrict__ r_in, DevFov f, DevTof t, TileGeom g,
                                                   double* __restrict__ bp,
                                                   const uint32_t* __restrict__ perm = nullptr) {
-  scatter_body<TOF>(runs, work[blockIdx.x], r_in, f, t, g, bp, perm);
+  const int4 wi = work[blockIdx.x];
+  if (wi.z <= wi.y) return;  // an empty item (the grid may be an upper bound)
+  scatter_body<TOF>(runs, wi, r_in, f, t, g, bp, perm);
 }
 
 // Batched backprojection: work item w = (tile, first run, end run, reconstruction).
