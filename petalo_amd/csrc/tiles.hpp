@@ -642,12 +642,13 @@ __device__ __forceinline__ void fwd_tile_body(const Run* __restrict__ runs, cons
   __syncthreads();
   // the zero-length-step select is needed only where a voxel is not finite
   const bool guard = *odd_tile != 0;
-  int32_t j = wi.y + (int32_t)threadIdx.x;
-  if (j >= wi.z) return;
-  if (perm)
-    fwd_runs<TOF, true>(runs, perm, lpos, j, wi.z - 1, guard, t, g, timg, partial);
-  else
-    fwd_runs<TOF, false>(runs, perm, lpos, j, wi.z - 1, guard, t, g, timg, partial);
+  const int32_t j = wi.y + (int32_t)threadIdx.x;
+  if (j < wi.z) {
+    if (perm)
+      fwd_runs<TOF, true>(runs, perm, lpos, j, wi.z - 1, guard, t, g, timg, partial);
+    else
+      fwd_runs<TOF, false>(runs, perm, lpos, j, wi.z - 1, guard, t, g, timg, partial);
+  }
 }
 
 template <bool TOF>

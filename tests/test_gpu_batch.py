@@ -79,3 +79,32 @@ def test_batched_workload_matches_oracle(gpu):
             if it == 1 and sub < 3:
                 assert live >= 1_000_000
     b.close()
+
+
+@pytest.mark.skip(reason="the persistent batch launch (one task queue per subset update, tasks waiting on "
+                         "per-reconstruction completion counters) stalled on its first launch on the GPU (no output "
+                         "for 180 s); no cause was shown by reading the code, so the schedule was taken out and "
+                         "batches run one launch per phase")
+def test_persistent_and_phase_schedules_agree(gpu):
+    """The persistent launch (one task queue per subset update) and the
+    launch-per-phase schedule give the same images (binary64 sums in another
+    order, rounded to binary32), for a batch mixing TOF and non-TOF
+    reconstructions, graphs on and off."""
+    fovs, lors, sizes, counts = batch_workload(scale=0.02, n_rec=12, seed=9)
+    tofs = [P.Tof(42.47, 3.0) if k % 3 else None for k in range(len(fovs))]
+    imgs = {}
+    for persistent in (True, False):
+        for graphs in (True, False):
+            b = P.Batch(fovs, lors, tofs=tofs, n_subsets=4)
+            b.set_schedule(persistent)
+            b.set_options(timing=False, graphs=graphs)
+            for _ in range(6):
+                b.step()
+            imgs[(persistent, graphs)] = [b.image(k).data.copy() for k in range(len(fovs))]
+            b.close()
+    ref = imgs[(False, False)]
+    for key, got in imgs.items():
+        for k in range(len(fovs)):
+            e = rel_l2_finite(got[k], ref[k])
+            assert e <= 1e-6, (key, k, e)
+    _say(f"schedules agree over {len(fovs)} reconstructions ({counts.sum()} LORs), 6 subset updates")
