@@ -1675,10 +1675,59 @@ __global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uin
   if (t < ntiles && (j == 0 || (keys[j - 1] >> tshift) != t)) start[t] = j;
 }
 
+// Tile counts up to which a sensitivity batch is binned straight into tile
+// order (k_bin_runs_seg keeps a cursor and an end per tile in LDS); above it
+// the runs are binned LOR-major and sorted by tile.
+constexpr uint32_t kSegMaxTiles = 4096;
+
+// Work items of a sensitivity batch binned by k_bin_runs_seg: tile t's runs
+// are [soff[t * ng], soff[(t + 1) * ng]) (soff[ntiles * ng] = the total),
+// cut into chunks of `per` -- the full chunks of all tiles first, then each
+// tile's remainder, so that the largest items start first -- followed by
+// empty items (first == end) up to cap, which their workgroups skip.  One
+// workgroup of 1024 lanes; lane q takes a contiguous range of tiles.
+__global__ __launch_bounds__(1024) void k_seg_items(const uint32_t* __restrict__ soff, uint32_t ng, uint32_t ntiles,
+                                                    uint32_t per, uint32_t cap, int4* __restrict__ items) {
+  __shared__ uint32_t full_s[1024], rem_s[1024];
+  const uint32_t q = threadIdx.x, span = (ntiles + 1023u) / 1024u;
+  const uint32_t t0 = min(q * span, ntiles), t1 = min(t0 + span, ntiles);
+  uint32_t nf = 0, nr = 0;
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t c = soff[(size_t)(t + 1) * ng] - soff[(size_t)t * ng];
+    nf += c / per;
+    nr += c % per ? 1u : 0u;
+  }
+  full_s[q] = nf;
+  rem_s[q] = nr;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024u; off <<= 1) {  // inclusive scans
+    const uint32_t a = q >= off ? full_s[q - off] : 0u, b = q >= off ? rem_s[q - off] : 0u;
+    __syncthreads();
+    full_s[q] += a;
+    rem_s[q] += b;
+    __syncthreads();
+  }
+  const uint32_t n_full = full_s[1023], n_items = n_full + rem_s[1023];
+  uint32_t fi = full_s[q] - nf, ri = n_full + rem_s[q] - nr;
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t a = soff[(size_t)t * ng], e = soff[(size_t)(t + 1) * ng];
+    uint32_t x = a;
+    for (; e - x >= per; x += per, ++fi)
+      if (fi < cap) items[fi] = make_int4((int)t, (int)x, (int)(x + per), 0);
+    if (x < e) {
+      if (ri < cap) items[ri] = make_int4((int)t, (int)x, (int)e, 0);
+      ++ri;
+    }
+  }
+  for (uint32_t k = n_items + q; k < cap; k += 1024u) items[k] = make_int4(0, 0, 0, 0);
+}
+
 // Sensitivity image (one FOV) with the tiled kernels: per batch, bin the
-// LORs into runs, sort their indices by tile (the runs are read through the
-// permutation: each is used once or twice), then (attenuation present) forward
-// project mu over the runs, reduce to 1/exp(-p) per LOR, and backproject;
+// LORs into runs in tile order -- straight into (tile, workgroup) segments
+// sized by a counting pass, or, above kSegMaxTiles tiles, LOR-major and then
+// sorted by tile (the runs read through the permutation) -- then (attenuation
+// present) forward project mu over the runs, reduce to 1/exp(-p) per LOR, and
+// backproject;
 // with mu == 0 everywhere every LOR has p = 0 exactly and r = 1, so the
 // forward half is skipped.  Unbounded walks go to the one-lane kernel.
 // produce(dst, first, count, stream) writes LORs first .. first + count - 1.
@@ -1738,6 +1787,9 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   const size_t na = nvox(fov);
   bool mu_zero = true;
   for (size_t k = 0; k < na && mu_zero; ++k) mu_zero = mu_host[k] == 0.0f;
+  // tile order straight from the binning (segments), or LOR-major runs sorted by tile
+  const bool segs = (uint32_t)g.ntiles <= kSegMaxTiles;
+  const uint32_t nt = (uint32_t)g.ntiles;
   const uint32_t shift = mu_zero ? 32u : kLorBlockShift;
   const uint32_t tshift = mu_zero ? 12u : 32u;
   int tile_bits = 1;
@@ -1746,10 +1798,12 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   const DevTof notof = no_tof();
   ensure_lds_attributes();
   const size_t bmax = (size_t)std::min<uint64_t>(n_lors, kSensBatch);
+  const size_t ng_max = bin_grid(bmax);
+  const size_t n_seg = segs ? (size_t)nt * ng_max + 1 : 0;
   // buffers of the whole call: batch-sized ones now, run-sized ones grown
   // to the largest batch (no allocation, and one wait, per batch)
   DevBuf<Lor> lb(bmax), lraw(by_length ? bmax : 0), irr(bmax);
-  DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1);
+  DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1), seg(n_seg), soff(n_seg);
   DevBuf<uint32_t> lk(by_length ? bmax : 0), lk2(by_length ? bmax : 0), li(by_length ? bmax : 0),
       li2(by_length ? bmax : 0);
   DevBuf<float> rbuf;
@@ -1758,12 +1812,17 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
     rbuf.alloc(bmax);
     pos4.alloc(bmax);
   }
-  size_t lsort_bytes = 0, scan_bytes = 0;
+  size_t lsort_bytes = 0, scan_bytes = 0, seg_scan_bytes = 0;
   if (by_length)
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, lsort_bytes, lk.p, lk2.p, li.p, li2.p, (int)bmax, 0, 16, s));
   HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts.p, offs.p, (int)bmax + 1, s));
+  if (segs) {
+    if (n_seg >= (1ull << 31)) throw std::runtime_error("too many (workgroup, tile) segments in a sensitivity batch");
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, seg_scan_bytes, seg.p, soff.p, (int)n_seg, s));
+    scan_bytes = std::max(scan_bytes, seg_scan_bytes);
+  }
   DevBuf<unsigned char> lsort_tmp(std::max<size_t>(lsort_bytes, 1)), scan_tmp(std::max<size_t>(scan_bytes, 1));
-  DevBuf<uint64_t> dstart(g.ntiles);
+  DevBuf<uint64_t> dstart(segs ? 0 : g.ntiles);
   Grow<Run> unsorted;
   Grow<uint64_t> keys, keys2;
   Grow<uint32_t> perm, perm2, pos;
@@ -1772,6 +1831,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   Grow<int4> work;
   for (uint64_t first = 0; first < n_lors; first += kSensBatch) {
     const uint32_t b = (uint32_t)std::min<uint64_t>(kSensBatch, n_lors - first);
+    const uint32_t ng = bin_grid(b);
     produce(by_length ? lraw.p : lb.p, first, b, s);
     HIP_CHECK(hipGetLastError());
     if (by_length) {
@@ -1783,53 +1843,75 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       k_gather_lors_by<<<grid_for(b), kBlock, 0, s>>>(lraw.p, li2.p, b, lb.p);
       HIP_CHECK(hipGetLastError());
     }
-    k_bin_runs<false, false><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr,
-                                                             nullptr, shift, cnt);
+    const size_t nsb = (size_t)nt * ng + 1;  // this batch's segments and the closing zero
+    if (segs) {
+      HIP_CHECK(hipMemsetAsync(seg.p + (nsb - 1), 0, 4, s));
+      k_bin_runs_seg<false><<<ng, kBlock, (size_t)nt * 4, s>>>(lb.p, b, f, g, counts.p, seg.p, nullptr, nullptr,
+                                                               nullptr, nullptr, cnt);
+    } else {
+      k_bin_runs<false, false><<<ng, kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr, nullptr,
+                                                      shift, cnt);
+    }
+    HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemsetAsync(n_irr.p, 0, 4, s));
     k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts.p, b, lb.p, irr.p, n_irr.p);
     HIP_CHECK(hipGetLastError());
-    size_t tb = scan_bytes;
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tb, counts.p, offs.p, (int)b + 1, s));
     uint32_t total = 0, h_irr = 0;
-    HIP_CHECK(hipMemcpyAsync(&total, offs.p + b, 4, hipMemcpyDeviceToHost, s));
+    if (!segs || !mu_zero) {
+      size_t tb = scan_bytes;
+      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tb, counts.p, offs.p, (int)b + 1, s));
+    }
+    if (segs) {
+      size_t tb = scan_bytes;
+      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tb, seg.p, soff.p, (int)nsb, s));
+      HIP_CHECK(hipMemcpyAsync(&total, soff.p + (nsb - 1), 4, hipMemcpyDeviceToHost, s));
+    } else {
+      HIP_CHECK(hipMemcpyAsync(&total, offs.p + b, 4, hipMemcpyDeviceToHost, s));
+    }
     HIP_CHECK(hipMemcpyAsync(&h_irr, n_irr.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));  // the batch's run count sizes the sort
+    HIP_CHECK(hipStreamSynchronize(s));  // the batch's run count sizes the run buffers
     if (total >= (1u << 31)) throw std::runtime_error("2^31 or more tile runs in one sensitivity batch");
     if (total) {
-      Run* un = unsorted.ensure(total);
-      uint64_t* k1 = keys.ensure(total);
-      uint64_t* k2 = keys2.ensure(total);
-      uint32_t* p1 = perm.ensure(total);
-      uint32_t* p2 = perm2.ensure(total);
-      k_bin_runs<false, true><<<bin_grid(b), kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, un, k1, shift,
-                                                              cnt);
-      k_iota<<<grid64(total), kBlock, 0, s>>>(p1, total);
-      HIP_CHECK(hipGetLastError());
-      size_t sort_bytes = 0;
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k1, k2, p1, p2, (int)total, 0, end_bit, s));
-      unsigned char* stmp = sort_tmp.ensure(std::max<size_t>(sort_bytes, 1));
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp, sort_bytes, k1, k2, p1, p2, (int)total, 0, end_bit, s));
-      uint32_t* ps = nullptr;
-      if (!mu_zero) {
-        ps = pos.ensure(total);
-        k_invert_perm<<<grid64(total), kBlock, 0, s>>>(p2, total, ps);
-      }
-      HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, (size_t)g.ntiles * 8, s));
-      k_tile_starts<<<grid64(total), kBlock, 0, s>>>(k2, total, tshift, (uint32_t)g.ntiles, dstart.p);
-      const uint32_t cap = total / kSensItemRuns + (uint32_t)g.ntiles + 1;
+      const uint32_t cap = total / kSensItemRuns + nt + 1;
       int4* items = work.ensure(cap);
-      k_sens_items<<<1, 256, 0, s>>>(dstart.p, (uint32_t)g.ntiles, total, kSensItemRuns, cap, items);
-      HIP_CHECK(hipGetLastError());
+      Run* un = unsorted.ensure(total);
+      // sorted position of every LOR-major run (attenuation present)
+      uint32_t* ps = mu_zero ? nullptr : pos.ensure(total);
+      uint32_t* order = nullptr;  // runs in tile order are un[order[j]]; nullptr: un[j]
+      if (segs) {
+        k_bin_runs_seg<true><<<ng, kBlock, (size_t)nt * 8, s>>>(lb.p, b, f, g, nullptr, seg.p, soff.p, offs.p, un,
+                                                                ps, cnt);
+        k_seg_items<<<1, 1024, 0, s>>>(soff.p, ng, nt, kSensItemRuns, cap, items);
+        HIP_CHECK(hipGetLastError());
+      } else {
+        uint64_t* k1 = keys.ensure(total);
+        uint64_t* k2 = keys2.ensure(total);
+        uint32_t* p1 = perm.ensure(total);
+        uint32_t* p2 = perm2.ensure(total);
+        k_bin_runs<false, true><<<ng, kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, un, k1, shift, cnt);
+        k_iota<<<grid64(total), kBlock, 0, s>>>(p1, total);
+        HIP_CHECK(hipGetLastError());
+        size_t sort_bytes = 0;
+        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k1, k2, p1, p2, (int)total, 0, end_bit, s));
+        unsigned char* stmp = sort_tmp.ensure(std::max<size_t>(sort_bytes, 1));
+        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp, sort_bytes, k1, k2, p1, p2, (int)total, 0, end_bit, s));
+        if (ps) k_invert_perm<<<grid64(total), kBlock, 0, s>>>(p2, total, ps);
+        HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, (size_t)g.ntiles * 8, s));
+        k_tile_starts<<<grid64(total), kBlock, 0, s>>>(k2, total, tshift, nt, dstart.p);
+        k_sens_items<<<1, 256, 0, s>>>(dstart.p, nt, total, kSensItemRuns, cap, items);
+        HIP_CHECK(hipGetLastError());
+        order = p2;
+      }
       if (!mu_zero) {
         float* part = partial.ensure(total);
         k_fwd_tile<false><<<cap, 1024, ((size_t)g.tvox + 1) * sizeof(float), s>>>(un, items, dmu, f, notof, g, part,
-                                                                                   p2, nullptr);
+                                                                                   order, nullptr);
         k_pack_pos4<<<grid_for(b), kBlock, 0, s>>>(offs.p, ps, b, pos4.p);
         k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, ps, pos4.p, part, rbuf.p);
         HIP_CHECK(hipGetLastError());
       }
       k_scatter<false><<<cap, 1024, (size_t)g.tvox * sizeof(double), s>>>(un, items, mu_zero ? nullptr : rbuf.p, f,
-                                                                          notof, g, acc, p2);
+                                                                          notof, g, acc, order);
       HIP_CHECK(hipGetLastError());
     }
     if (h_irr) {

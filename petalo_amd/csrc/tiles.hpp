@@ -402,8 +402,9 @@ __host__ __device__ __forceinline__ uint64_t len_key(uint32_t code) {
 
 // Binning walk: FILL = false counts runs per LOR into n_runs[i]; FILL = true
 // writes them at runs[offset[i] .. offset[i + 1]) (LOR-major; a run past the
-// end of that range is not written, and a LOR whose run count differs from
-// the range is counted in counters[CNT_RUN_MISMATCH]) with their sort keys:
+// end of that range is not written, a LOR left short is padded with empty
+// runs, and a LOR whose run count differs from the range is counted in
+// counters[CNT_RUN_MISMATCH]) with their sort keys:
 // tile (bits 32 and up), then block of 2^lor_shift consecutive LORs (bits
 // 12-31; LORs are ordered spatially, so a block's 1/(p*ac) values stay
 // cache-resident while its runs in a tile are replayed), then len_key
@@ -451,7 +452,14 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
     const uint32_t count = (!TOF && FILL) ? bin_lor_axes(l, i, f, g, emit)
                                           : bin_lor<TOF>(l, i, f, t, g, emit, FILL ? nullptr : &steps,
                                                          FILL ? nullptr : &nz);
-    if (FILL) mismatch += count != room ? 1u : 0u;
+    if (FILL) {
+      mismatch += count != room ? 1u : 0u;
+      // a LOR left short holds empty runs (length 0, LOR 0, tile 0), not stale records
+      for (; k < room; ++k) {
+        runs[base + k] = Run{};
+        keys[base + k] = len_key(0u);
+      }
+    }
     if (!FILL) {
       // unbounded walks are projected by the one-lane kernels instead
       WalkC c;
@@ -472,6 +480,93 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
     wave_add_u64(counters + CNT_SHIFTED, shifted);
   }
   if (FILL && counters) wave_add_u64(counters + CNT_RUN_MISMATCH, mismatch);
+}
+
+// Binning without TOF straight into tile order, for the sensitivity image.
+// Workgroup w of a grid of G takes the same LORs in both passes (as
+// k_bin_runs), and the runs of its LORs in tile t form segment (t, w) of a
+// tile-major layout: segment sizes seg[t * G + w], their exclusive prefix
+// sums soff.  Count (FILL false): n_runs[i] as k_bin_runs<false, false>
+// (with the work counters), and seg from a per-workgroup LDS histogram.
+// Fill (FILL true): each run goes to the next free position of its segment
+// (an LDS cursor per tile; positions inside a segment follow no order), and,
+// with run_pos, LOR i's k-th run position to run_pos[offset[i] + k].  A run
+// past its segment's end is not written (its run_pos entry is 0), a segment
+// left short is padded with empty runs, and every segment whose fill differs
+// from its count adds to counters[CNT_RUN_MISMATCH].  Dynamic LDS: 4 * ntiles
+// bytes (count) or 8 * ntiles bytes (fill).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lors, uint32_t n, DevFov f,
+                                                      TileGeom g, uint32_t* __restrict__ n_runs,
+                                                      uint32_t* __restrict__ seg, const uint32_t* __restrict__ soff,
+                                                      const uint32_t* __restrict__ offset, Run* __restrict__ runs,
+                                                      uint32_t* __restrict__ run_pos,
+                                                      unsigned long long* __restrict__ counters) {
+  extern __shared__ uint32_t cur[];  // [0, ntiles): count or cursor; [ntiles, 2 ntiles): segment end (fill)
+  const uint32_t nt = (uint32_t)g.ntiles, ng = gridDim.x, wg = blockIdx.x;
+  for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+    if (FILL) {
+      const uint32_t a = soff[(size_t)t * ng + wg];
+      cur[t] = a;
+      cur[nt + t] = a + seg[(size_t)t * ng + wg];
+    } else {
+      cur[t] = 0;
+    }
+  }
+  __syncthreads();
+  const DevTof notof = no_tof();
+  uint32_t steps = 0, nz = 0, missed = 0, shifted = 0, mismatch = 0;
+  const uint32_t first = wg * blockDim.x * kBinLors + threadIdx.x;
+  for (uint32_t u = 0; u < kBinLors; ++u) {
+    const uint32_t i = first + u * blockDim.x;
+    if (i >= n) break;
+    const Lor l = load_lor(lors, i);
+    const uint32_t base = (FILL && run_pos) ? offset[i] : 0;
+    const uint32_t room = (FILL && run_pos) ? offset[i + 1] - base : 0;
+    uint32_t k = 0;
+    auto emit = [&](const Run& ru, uint32_t tile) {
+      if (FILL) {
+        const uint32_t at = atomicAdd(&cur[tile], 1u);
+        const bool fits = at < cur[nt + tile];
+        if (fits) runs[at] = ru;
+        if (run_pos && k < room) run_pos[base + k] = fits ? at : 0u;
+      } else {
+        atomicAdd(&cur[tile], 1u);
+      }
+      ++k;
+    };
+    const uint32_t count = FILL ? bin_lor_axes(l, i, f, g, emit)
+                                : bin_lor<false>(l, i, f, notof, g, emit, &steps, &nz);
+    if (FILL && run_pos) mismatch += count != room ? 1u : 0u;
+    if (!FILL) {
+      WalkC c;
+      WalkS w;
+      bool sh = false;
+      const bool hit = walk_setup(l, f, c, w, &sh);
+      missed += hit ? 0u : 1u;
+      shifted += (hit && sh) ? 1u : 0u;
+      n_runs[i] = (hit && !bounded(w)) ? 0xffffffffu : count;
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+    if (FILL) {
+      mismatch += cur[t] != cur[nt + t] ? 1u : 0u;
+      // a segment left short holds empty runs (length 0, LOR 0), not stale records
+      for (uint32_t x = cur[t]; x < cur[nt + t]; ++x) runs[x] = Run{};
+    } else {
+      seg[(size_t)t * ng + wg] = cur[t];
+    }
+  }
+  if (!counters) return;
+  if (FILL) {
+    wave_add_u64(counters + CNT_RUN_MISMATCH, mismatch);
+  } else {
+    wave_add_u64(counters + CNT_STEPS, steps);
+    wave_add_u64(counters + CNT_NZ, nz);
+    wave_add_u64(counters + CNT_MISSED, missed);
+    wave_add_u64(counters + CNT_SHIFTED, shifted);
+  }
 }
 
 // Walk state of a run for a replay against an LDS tile of elements of
@@ -579,7 +674,7 @@ __device__ __forceinline__ void fwd_runs(const Run* __restrict__ runs, const uin
   // counter also holds the previous run's partial store).
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   for (;;) {
-    const uint32_t at = PERM ? (uint32_t)j : lpos[j];
+    const uint32_t at = (PERM || !lpos) ? (uint32_t)j : lpos[j];
     const int32_t jn = j + stride;
     const int32_t jc = jn < last ? jn : last;  // in bounds; unused past the end
     const Run nxt = runs[PERM ? perm[jc] : jc];
@@ -594,8 +689,8 @@ __device__ __forceinline__ void fwd_runs(const Run* __restrict__ runs, const uin
 // Forward, tiled: one workgroup per work item copies its tile of the image
 // (dynamic LDS: tvox + 1 words) into LDS and replays its runs, writing each
 // run's partial projection (sum of w*x over the run's pushed steps, in walk
-// order) at lpos[j], its place in the (window block, tile) layout, or with
-// perm at its sorted position j (stores nearly contiguous either way;
+// order) at lpos[j], its place in the (window block, tile) layout, or, with
+// perm or without lpos, at its sorted position j (stores nearly contiguous;
 // writing them LOR-major instead scatters 4-byte stores over the whole
 // array and costs more than it saves in k_lor_reduce).  Each lane loads its next run record before
 // replaying the current one, so the record's latency hides behind the
