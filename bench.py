@@ -258,21 +258,39 @@ def main():
     lors = P.synth_phantom_lors(1, n_lors, fwhm)
     log(f"[bench] LORs ready in {time.time() - t0:.1f} s")
 
-    sens_img, sens_ms, sens_stats, discrete = None, None, None, None
+    sens_img, sens_ms, sens_stats, discrete, sens_att = None, None, None, None, None
     if n_sens and not a.no_sensitivity:
+        # each image is timed as a caller makes it (no work counters: those
+        # cost a walk over every step), median of three calls; the counters
+        # behind the algorithmic bytes come from one more call with them
+        def timed(fn, reps=3):
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                out = fn()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            return out, float(np.median(ts)), ts
         mu = P.Image(fov, np.zeros(fov.nvox, np.float32))
         P.make_sensitivity_image(mu, 1_000_000, 300.0, 700.0, seed=7)  # warm-up
+        sens_img, sens_ms, ts = timed(lambda: P.make_sensitivity_image(mu, n_sens, 300.0, 700.0, seed=7))
         t0 = time.perf_counter()
-        sens_img, sens_stats = P.make_sensitivity_image(mu, n_sens, 300.0, 700.0, seed=7, return_stats=True)
-        sens_ms = (time.perf_counter() - t0) * 1e3
-        log(f"[bench] sensitivity image from {n_sens} LORs: {sens_ms:.1f} ms")
-        # discrete detector: all pairs of 10 mm elements of a 700 mm ring, 600 mm long
+        img_st, sens_stats = P.make_sensitivity_image(mu, n_sens, 300.0, 700.0, seed=7, return_stats=True)
+        stats_ms = (time.perf_counter() - t0) * 1e3
+        ref = sens_img.data.astype(np.float64)
+        same = float(np.linalg.norm(img_st.data - ref) / max(np.linalg.norm(ref), 1e-300))
+        assert same < 1e-6, f"sensitivity images with and without work counters differ: relL2 {same:.2e}"
+        log(f"[bench] sensitivity image from {n_sens} LORs: {sens_ms:.1f} ms "
+            f"({', '.join(f'{t:.1f}' for t in ts)}; {stats_ms:.1f} ms with work counters, relL2 {same:.1e})")
+        # the same LORs through the synthetic body's attenuation (forward half included)
         att = P.density_to_attenuation(P.synth_density(fov), 0.096)
+        _, att_ms, ts = timed(lambda: P.make_sensitivity_image(att, n_sens, 300.0, 700.0, seed=7))
+        sens_att = {"lors": n_sens, "ms": att_ms, "ms_calls": ts}
+        log(f"[bench] attenuated sensitivity image from {n_sens} LORs: {att_ms:.1f} ms")
+        # discrete detector: all pairs of 10 mm elements of a 700 mm ring, 600 mm long
         dgeom = dict(r_min=350.0, dr=20.0, dz=10.0, da=10.0)
-        t0 = time.perf_counter()
+        _, dms, ts = timed(lambda: P.make_sensitivity_image_discrete(att, 600.0, **dgeom))
         _, dst = P.make_sensitivity_image_discrete(att, 600.0, return_stats=True, **dgeom)
-        dms = (time.perf_counter() - t0) * 1e3
-        discrete = {"detector": dict(dgeom, length=600.0), "pairs": dst["n_lors"], "ms": dms,
+        discrete = {"detector": dict(dgeom, length=600.0), "pairs": dst["n_lors"], "ms": dms, "ms_calls": ts,
                     "n_steps": dst["n_steps"], "n_nz": dst["n_nz"], "n_missed": dst["n_missed"]}
         log(f"[bench] discrete sensitivity image, {dst['n_lors']} element pairs: {dms:.1f} ms")
 
@@ -529,6 +547,7 @@ def main():
                          "achieved_gbs": (8 * sens_stats["n_nz"] + 4 * fov.nvox) / (sens_ms * 1e-3) / 1e9,
                          "frac": (8 * sens_stats["n_nz"] + 4 * fov.nvox) / (sens_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
                         if sens_stats else None),
+        "sensitivity_attenuated": sens_att,
         "discrete_sensitivity": discrete,
         "parity_rel_l2_sample": parity,
         "ingest": ingest,

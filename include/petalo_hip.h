@@ -290,7 +290,9 @@ int petalo_ingest_lors(const float* records, size_t n_records, const petalo_inge
  * LOR i: block (i, k, 0, 0x5e45); z = L (u0 - 1/2), theta = TAU u1 for p1,
  * u2, u3 for p2; u = top 24 bits / 2^24).  A LOR that finds no FOV-crossing
  * pair within 4096 attempts fails the call with PETALO_ERR_ARG (the reference
- * draws forever).  Accumulation is in FP64. */
+ * draws forever).  Accumulation is in FP64.  A non-NULL stats also counts
+ * every step and pushed step of every LOR, which costs one more walk over
+ * all of them; with NULL the binning jumps from tile to tile. */
 int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu,
                                   const petalo_fov_t* out_fov_or_null, const petalo_lor_t* lors_or_null,
                                   size_t n_lors, uint64_t seed, float detector_length_mm,
@@ -303,7 +305,8 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu,
  * element (PETALO_ADJUST_*, draws from Philox keyed by seed and counted by
  * pair index); projected as the continuous image; normalised by n (n - 1) / 2
  * over the n elements (discrete.rs:16-20, which counts neither self pairs nor
- * the FOV filter).  stats->n_lors = pairs. */
+ * the FOV filter).  stats->n_lors = pairs; a non-NULL stats costs a walk over
+ * every step, as above. */
 int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const float* mu,
                                            const petalo_fov_t* out_fov_or_null, float detector_length_mm,
                                            float r_min_mm, float dr_mm, float dz_mm, float da_mm, int32_t adjust,

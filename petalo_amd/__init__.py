@@ -548,6 +548,8 @@ def make_sensitivity_image(attenuation, n_lors, detector_length=None, detector_d
 
     ``attenuation`` is an Image of mu [1/mm].  With ``lors`` None the LORs are
     generated on the device from (seed, detector_length, detector_diameter).
+    ``return_stats`` also counts every step and pushed step, which costs a
+    walk over every step of every LOR.
     """
     mu = np.ascontiguousarray(attenuation.data, np.float32)
     out_fov = output_fov or attenuation.fov
@@ -563,7 +565,7 @@ def make_sensitivity_image(attenuation, n_lors, detector_length=None, detector_d
     _check(lib().petalo_make_sensitivity_image(
         C.byref(attenuation.fov._c()), _ptr(mu),
         C.byref(output_fov._c()) if output_fov is not None else None,
-        lp, int(n_lors), int(seed), L, D, _ptr(out), C.byref(st)))
+        lp, int(n_lors), int(seed), L, D, _ptr(out), C.byref(st) if return_stats else None))
     img = Image(out_fov, out)
     return (img, _stats(st)) if return_stats else img
 
@@ -580,7 +582,7 @@ def make_sensitivity_image_discrete(attenuation, detector_length, r_min, dr, dz,
     _check(lib().petalo_make_sensitivity_image_discrete(
         C.byref(att_fov._c()), _ptr(mu), C.byref(output_fov._c()) if output_fov is not None else None,
         float(detector_length), float(r_min), float(dr), float(dz), float(da), int(adjust), int(seed),
-        _ptr(out), C.byref(st)))
+        _ptr(out), C.byref(st) if return_stats else None))
     img = Image(ofov, out)
     return (img, _stats(st)) if return_stats else img
 

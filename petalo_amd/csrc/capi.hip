@@ -1734,6 +1734,8 @@ __global__ __launch_bounds__(1024) void k_seg_items(const uint32_t* __restrict__
 // by_length: bin each batch in order of walk length (LORs in random order:
 // the binning lanes then walk similar lengths); false keeps the source's
 // order (neighbouring element pairs share cache lines and LDS banks).
+// want_stats: count every step and pushed step (the counting pass then walks
+// every step instead of jumping from tile to tile).
 using LorSource = std::function<void(Lor*, uint64_t, uint32_t, hipStream_t)>;
 
 // A device buffer that only grows (kept across the batches of one call).
@@ -1781,7 +1783,8 @@ __global__ void k_sens_items(const uint64_t* __restrict__ start, uint32_t ntiles
 }
 
 void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const float* mu_host, const float* dmu,
-                uint64_t n_lors, const LorSource& produce, double* acc, unsigned long long* cnt, bool by_length) {
+                uint64_t n_lors, const LorSource& produce, double* acc, unsigned long long* cnt, bool by_length,
+                bool want_stats) {
   const int32_t no_shape[3] = {0, 0, 0};
   const TileGeom g = make_tiles(fov, no_shape);
   const size_t na = nvox(fov);
@@ -1846,8 +1849,12 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
     const size_t nsb = (size_t)nt * ng + 1;  // this batch's segments and the closing zero
     if (segs) {
       HIP_CHECK(hipMemsetAsync(seg.p + (nsb - 1), 0, 4, s));
-      k_bin_runs_seg<false><<<ng, kBlock, (size_t)nt * 4, s>>>(lb.p, b, f, g, counts.p, seg.p, nullptr, nullptr,
-                                                               nullptr, nullptr, cnt);
+      if (want_stats)
+        k_bin_runs_seg<false, true><<<ng, kBlock, (size_t)nt * 4, s>>>(lb.p, b, f, g, counts.p, seg.p, nullptr,
+                                                                       nullptr, nullptr, nullptr, cnt);
+      else
+        k_bin_runs_seg<false, false><<<ng, kBlock, (size_t)nt * 4, s>>>(lb.p, b, f, g, counts.p, seg.p, nullptr,
+                                                                        nullptr, nullptr, nullptr, cnt);
     } else {
       k_bin_runs<false, false><<<ng, kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr, nullptr,
                                                       shift, cnt);
@@ -1879,7 +1886,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       uint32_t* ps = mu_zero ? nullptr : pos.ensure(total);
       uint32_t* order = nullptr;  // runs in tile order are un[order[j]]; nullptr: un[j]
       if (segs) {
-        k_bin_runs_seg<true><<<ng, kBlock, (size_t)nt * 8, s>>>(lb.p, b, f, g, nullptr, seg.p, soff.p, offs.p, un,
+        k_bin_runs_seg<true, false><<<ng, kBlock, (size_t)nt * 8, s>>>(lb.p, b, f, g, nullptr, seg.p, soff.p, offs.p, un,
                                                                 ps, cnt);
         k_seg_items<<<1, 1024, 0, s>>>(soff.p, ng, nt, kSensItemRuns, cap, items);
         HIP_CHECK(hipGetLastError());
@@ -1981,7 +1988,7 @@ int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const fl
       k_pair_lors<<<grid_for(n), kBlock, 0, st>>>(first, n, el, seed, dst);
     };
     if (!out_fov) {
-      sens_tiled(s, *att_fov, fa, mu, dmu.p, n_pairs, src, acc.p, cnt.p, false);
+      sens_tiled(s, *att_fov, fa, mu, dmu.p, n_pairs, src, acc.p, cnt.p, false, stats != nullptr);
     } else {
       // two grids: one lane per LOR, binary64 global atomics
       DevBuf<Lor> lb((size_t)std::min<uint64_t>(n_pairs, kSensBatch));
@@ -2055,7 +2062,7 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
                              : LorSource([&](Lor* dst, uint64_t first, uint32_t cnt_, hipStream_t st) {
           k_sens_lors<<<sens_lors_grid(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst, cnt.p);
         });
-        sens_tiled(s, *att_fov, fa, mu, dmu.p, n_lors, src, acc.p, cnt.p, true);
+        sens_tiled(s, *att_fov, fa, mu, dmu.p, n_lors, src, acc.p, cnt.p, true, stats != nullptr);
       } else {
         // two FOVs: one lane per LOR, binary64 global atomics
         if (lors) k_sens<false, true><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);

@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <cstring>
 
 namespace petalo {
 
@@ -354,6 +355,70 @@ __host__ __device__ __forceinline__ bool advance_packed(WalkS& w, const WalkC c,
   move(w, c, s);
   rem -= s.ax ? 1u : (s.ay ? (1u << 10) : (1u << 20));
   return (rem & kPackGuards) != kPackGuards;
+}
+
+// v after m additions of d, each rounded to binary32 (round to nearest,
+// ties to even): the value the walk's boundary of one axis reaches after m
+// crossings of that axis, nb, nb + vl, ..., computed in O(binades crossed)
+// instead of O(m), bit for bit.  For v, d > 0 (+inf allowed).  While v lies
+// in the binade [2^e, 2^(e+1)) with ulp u = 2^(e-23), write v = V u (V an
+// integer, 2^23 <= V < 2^24) and d / u = q + r (q an integer, 0 <= r < 1,
+// exact: a power-of-two scaling).  If V + q <= 2^24 - 1, the exact sum
+// (V + q + r) u lies below 2^(e+1), where binary32 values are the multiples
+// of u, so it rounds to (V + Q) u with Q = q + (r > 1/2) -- for r != 1/2;
+// r == 1/2 rounds to the even neighbour, which depends on V, and is done by
+// one hardware addition.  So k additions jump V to V + kQ while the start of
+// the last one, V + (k-1)Q, still satisfies the bound; the addition that
+// leaves the binade is done by the hardware, and the next binade repeats
+// the argument.  v < d (only at the start of a walk) takes hardware
+// additions until v >= d, which keeps d / u below 2^24.
+// floor(num / den) for num < 2^24, 0 < den < 2^24: a binary32 estimate
+// (within a few units: both operands are exact, the reciprocal and the
+// product are each within an ulp) corrected in integers.
+__host__ __device__ __forceinline__ uint32_t div_small(uint32_t num, uint32_t den) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float rcp = __builtin_amdgcn_rcpf((float)den);
+#else
+  const float rcp = 1.0f / (float)den;
+#endif
+  uint32_t k = (uint32_t)((float)num * rcp);
+  while (k * den > num) --k;
+  while ((k + 1u) * den <= num) ++k;
+  return k;
+}
+
+__host__ __device__ __forceinline__ float repeat_add(float v, float d, uint32_t m) {
+  if (m <= 4u) {  // a few additions cost less than the binade arithmetic
+    for (uint32_t j = 0; j < m; ++j) v = v + d;
+    return v;
+  }
+  if (!(v < INFINITY && d < INFINITY)) return v + d;
+  while (m > 0) {
+    if (!(v >= d)) {
+      v = v + d;
+      --m;
+      continue;
+    }
+    uint32_t bits;
+    memcpy(&bits, &v, 4);
+    const int32_t e = (int32_t)(bits >> 23) - 127;  // v >= d > 0 is normal here
+    const uint32_t V = (bits & 0x7fffffu) | 0x800000u;
+    const float t = ldexpf(d, 23 - e);  // d / u, exact (or below 1/2 when it underflows)
+    const float qf = floorf(t);
+    const float r = t - qf;
+    const uint32_t q = (uint32_t)qf;
+    if (r == 0.5f || V + q > 0xffffffu) {  // a tie, or an addition that leaves the binade
+      v = v + d;
+      --m;
+      continue;
+    }
+    const uint32_t Q = q + (r > 0.5f ? 1u : 0u);
+    uint32_t k = m;
+    if (Q != 0u && (uint64_t)V + (uint64_t)(m - 1u) * Q + q > 0xffffffu) k = div_small(0xffffffu - q - V, Q) + 1u;
+    v = ldexpf((float)(V + k * Q), e - 23);  // V + kQ <= 2^24: exact
+    m -= k;
+  }
+  return v;
 }
 
 // True when every remaining count is positive: then each step decrements

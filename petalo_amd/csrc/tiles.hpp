@@ -389,6 +389,142 @@ __host__ __device__ __forceinline__ uint32_t bin_lor_axes(const Lor& l, uint32_t
   return count;
 }
 
+// The tiles in which bin_lor<false> emits a run for LOR i, in walk order,
+// without walking its steps: calls on_run(tile) for each and returns their
+// number.  The walk leaves a tile at the first, in (position, axis x < y <
+// z) order, of the three axes' tile-leaving crossings, and the FOV at the
+// first of their FOV-leaving ones; each of these is one axis's boundary
+// after a known number m of crossings of that axis.  Its binary32 value
+// (m additions of vl to nb, each rounded) lies within m 2^-23 b of the
+// binary64 estimate b = nb + m vl: every addition rounds by at most half an
+// ulp, 2^-24 of a value at most the result.  The order of the three is
+// taken from the estimates where their error intervals are disjoint, and
+// from the exact values (repeat_add) where they are not.  The walk's
+// positions never decrease and its step lengths are the differences of
+// consecutive positions, so a tile has a pushed step (weight > 0, no TOF)
+// exactly when the crossing that leaves it lies beyond the one that
+// entered it; that test too is decided on the estimates where it can be.
+// Walks whose boundaries are not positive (degenerate LORs) are stepped
+// through by bin_lor.
+__host__ __device__ __forceinline__ double crossing_estimate(float nb, float vl, uint32_t m, double& err) {
+  const double b = (double)nb + (double)m * (double)vl;
+  err = b < (double)INFINITY ? (double)m * 0x1p-23 * b : 0.0;
+  return b;
+}
+
+template <class OnRun>
+__host__ __device__ __forceinline__ uint32_t tile_runs(const Lor& l, uint32_t i, const DevFov& f, const TileGeom& g,
+                                                       OnRun&& on_run) {
+  WalkC c;
+  WalkS w;
+  if (!walk_setup(l, f, c, w) || !bounded(w)) return 0;
+  if (!(w.nb_x > 0.0f && w.nb_y > 0.0f && w.nb_z > 0.0f && c.vl_x > 0.0f && c.vl_y > 0.0f && c.vl_z > 0.0f))
+    return bin_lor<false>(l, i, f, no_tof(), g, [&](const Run&, uint32_t tl) { on_run(tl); });
+  const float nb[3] = {w.nb_x, w.nb_y, w.nb_z}, vl[3] = {c.vl_x, c.vl_y, c.vl_z};
+  int32_t tile;
+  uint32_t idx[3], last[3];
+  bool out[3];
+  double b[3], err[3];
+  {
+    const int32_t nxy = f.nx * f.ny;
+    const int32_t z = w.index / nxy;
+    const int32_t rem = w.index - z * nxy;
+    const int32_t y = rem / f.nx;
+    const int32_t x = rem - y * f.nx;
+    const bool px = c.d_x > 0, py = c.d_y > 0, pz = c.d_z > 0;
+    // crossings of each axis until the walk leaves the tile along it
+    const uint32_t cc[3] = {(uint32_t)(px ? g.tx - x % g.tx : x % g.tx + 1),
+                            (uint32_t)(py ? g.ty - y % g.ty : y % g.ty + 1),
+                            (uint32_t)(pz ? g.tz - z % g.tz : z % g.tz + 1)};
+    tile = x / g.tx + g.ntx * (y / g.ty + g.nty * (z / g.tz));
+    const int32_t rr[3] = {w.r_x, w.r_y, w.r_z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      last[a] = (uint32_t)rr[a] - 1u;  // index of the crossing that leaves the FOV
+      idx[a] = cc[a] - 1u < last[a] ? cc[a] - 1u : last[a];
+      out[a] = cc[a] - 1u >= last[a];
+      b[a] = crossing_estimate(nb[a], vl[a], idx[a], err[a]);
+    }
+  }
+  const uint32_t step[3] = {(uint32_t)g.tx, (uint32_t)g.ty, (uint32_t)g.tz};
+  const int32_t dtile[3] = {c.d_x > 0 ? 1 : -1, (c.d_y > 0 ? 1 : -1) * g.ntx, (c.d_z > 0 ? 1 : -1) * g.ntx * g.nty};
+  double h = 0.0, herr = 0.0;  // the crossing that entered the tile (the walk starts at here = 0)
+  bool hstart = true;          // no crossing yet; else its axis's nb, vl and index give its exact value
+  float hnb = 0.0f, hvl = 0.0f;
+  uint32_t hidx = 0;
+  uint32_t count = 0;
+  for (;;) {
+    int e = 0;
+    if (b[1] < b[e]) e = 1;
+    if (b[2] < b[e]) e = 2;
+    bool sure = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a != e) sure = sure && b[e] + err[e] < b[a] - err[a];
+    float Be = 0.0f;
+    bool exact = false;
+    if (!sure) {
+      float B[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) B[a] = repeat_add(nb[a], vl[a], idx[a]);
+      e = 0;
+      if (B[1] < B[e]) e = 1;
+      if (B[2] < B[e]) e = 2;
+      Be = e == 0 ? B[0] : (e == 1 ? B[1] : B[2]);
+      exact = true;
+    }
+    uint32_t ie = idx[0], le = last[0], se = step[0];
+    bool oe = out[0];
+    float nbe = nb[0], vle = vl[0];
+    int32_t de = dtile[0];
+    double bee = b[0], eee = err[0];
+#pragma unroll
+    for (int a = 1; a < 3; ++a)
+      if (a == e) {
+        ie = idx[a];
+        le = last[a];
+        se = step[a];
+        oe = out[a];
+        nbe = nb[a];
+        vle = vl[a];
+        de = dtile[a];
+        bee = b[a];
+        eee = err[a];
+      }
+    bool push = bee - eee > h + herr;
+    if (!push) {
+      if (!exact) Be = repeat_add(nbe, vle, ie);
+      const float hx = hstart ? w.here : repeat_add(hnb, hvl, hidx);
+      push = Be > hx;
+    }
+    if (push) {
+      on_run((uint32_t)tile);
+      ++count;
+    }
+    if (oe) break;
+    h = bee;
+    herr = eee;
+    hstart = false;
+    hnb = nbe;
+    hvl = vle;
+    hidx = ie;
+    tile += de;
+    const uint32_t nx = ie + se < le ? ie + se : le;
+    double en;
+    const double bn = crossing_estimate(nbe, vle, nx, en);
+    const bool on = ie + se >= le;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a == e) {
+        idx[a] = nx;
+        out[a] = on;
+        b[a] = bn;
+        err[a] = en;
+      }
+  }
+  return count;
+}
+
 // Length field of a run's sort key (bits 0-11): longer runs first, in
 // buckets of 4 steps.  Lanes of a wavefront get runs of about equal length;
 // inside a bucket the (stable) sort keeps LOR order, so neighbouring LORs'
@@ -486,16 +622,18 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
 // Workgroup w of a grid of G takes the same LORs in both passes (as
 // k_bin_runs), and the runs of its LORs in tile t form segment (t, w) of a
 // tile-major layout: segment sizes seg[t * G + w], their exclusive prefix
-// sums soff.  Count (FILL false): n_runs[i] as k_bin_runs<false, false>
-// (with the work counters), and seg from a per-workgroup LDS histogram.
-// Fill (FILL true): each run goes to the next free position of its segment
+// sums soff.  Count (FILL false): n_runs[i] as k_bin_runs<false, false>,
+// and seg from a per-workgroup LDS histogram; STATS walks every step to
+// count the steps and pushed steps too (bin_lor), otherwise the tiles come
+// from the tile jump (tile_runs) and only the miss and shifted-entry
+// counters are kept.  Fill (FILL true): each run goes to the next free position of its segment
 // (an LDS cursor per tile; positions inside a segment follow no order), and,
 // with run_pos, LOR i's k-th run position to run_pos[offset[i] + k].  A run
 // past its segment's end is not written (its run_pos entry is 0), a segment
 // left short is padded with empty runs, and every segment whose fill differs
 // from its count adds to counters[CNT_RUN_MISMATCH].  Dynamic LDS: 4 * ntiles
 // bytes (count) or 8 * ntiles bytes (fill).
-template <bool FILL>
+template <bool FILL, bool STATS>
 __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lors, uint32_t n, DevFov f,
                                                       TileGeom g, uint32_t* __restrict__ n_runs,
                                                       uint32_t* __restrict__ seg, const uint32_t* __restrict__ soff,
@@ -535,8 +673,13 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
       }
       ++k;
     };
-    const uint32_t count = FILL ? bin_lor_axes(l, i, f, g, emit)
-                                : bin_lor<false>(l, i, f, notof, g, emit, &steps, &nz);
+    uint32_t count;
+    if (FILL)
+      count = bin_lor_axes(l, i, f, g, emit);
+    else if (STATS)
+      count = bin_lor<false>(l, i, f, notof, g, emit, &steps, &nz);
+    else
+      count = tile_runs(l, i, f, g, [&](uint32_t tile) { atomicAdd(&cur[tile], 1u); });
     if (FILL && run_pos) mismatch += count != room ? 1u : 0u;
     if (!FILL) {
       WalkC c;
@@ -562,8 +705,10 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
   if (FILL) {
     wave_add_u64(counters + CNT_RUN_MISMATCH, mismatch);
   } else {
-    wave_add_u64(counters + CNT_STEPS, steps);
-    wave_add_u64(counters + CNT_NZ, nz);
+    if (STATS) {
+      wave_add_u64(counters + CNT_STEPS, steps);
+      wave_add_u64(counters + CNT_NZ, nz);
+    }
     wave_add_u64(counters + CNT_MISSED, missed);
     wave_add_u64(counters + CNT_SHIFTED, shifted);
   }

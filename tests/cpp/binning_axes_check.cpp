@@ -1,12 +1,13 @@
 // Host-side check that the per-axis binning (bin_lor_axes) emits exactly the
 // runs and step counts of the step-by-step binning walk (bin_lor<false>),
-// bit for bit, over random LORs and LORs built to tie: axis-parallel, on
+// bit for bit, and that the tile jump (tile_runs) finds the same tiles, over random LORs and LORs built to tie: axis-parallel, on
 // voxel planes, through voxel edges and corners, diagonal, degenerate
 // (p1 == p2), grazing a face, and the irregular LORs of the sensitivity
 // tests.  Usage: binning_axes_check [lors per case]; exit status 1 on any
 // mismatch.
 #include "tiles.hpp"
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -17,7 +18,40 @@ int main(int argc, char** argv) {
   const int n_lors = argc > 1 ? atoi(argv[1]) : 200000;
   std::mt19937_64 rng(12345);
   std::uniform_real_distribution<float> U(-1.0f, 1.0f);
-  long bad = 0, runs = 0, lors_hit = 0, fallback_like = 0;
+  long bad = 0, bad_tiles = 0, runs = 0, lors_hit = 0, fallback_like = 0;
+  // repeat_add against m binary32 additions one by one: values and
+  // increments across binades, increments with a half-ulp remainder (ties),
+  // increments above the value, and long runs that cross several binades
+  long bad_add = 0, adds = 0;
+  {
+    std::uniform_real_distribution<float> L(-30.0f, 12.0f), M(0.0f, 1.0f);
+    for (int k = 0; k < 400000; ++k) {
+      float v = std::ldexp(1.0f + M(rng), (int)L(rng));
+      float d = std::ldexp(1.0f + M(rng), (int)L(rng));
+      const int kind = k % 5;
+      if (kind == 1) d = v * (0.001f + M(rng));                 // a fraction of v
+      if (kind == 2) {                                           // d / ulp(v) ends in exactly 1/2
+        int e;
+        std::frexp(v, &e);
+        const float u = std::ldexp(1.0f, e - 24);
+        d = ((float)(1 + (int)(M(rng) * 5000)) + 0.5f) * u;
+      }
+      if (kind == 3) v = std::nextafter(std::ldexp(1.0f, (int)L(rng)), 0.0f);  // just below a power of two
+      if (kind == 4) { v = 1e-7f * (1.0f + M(rng)); d = 0.5f + M(rng); }        // a walk's first boundary
+      const uint32_t m = (uint32_t)(M(rng) * (k % 7 == 0 ? 5000.0f : 60.0f));
+      float ref = v;
+      for (uint32_t j = 0; j < m; ++j) ref = ref + d;
+      const float got = repeat_add(v, d, m);
+      ++adds;
+      if (std::memcmp(&ref, &got, 4) != 0) {
+        if (bad_add < 5) printf("REPEAT_ADD MISMATCH v %.9g d %.9g m %u: %.9g vs %.9g\n", v, d, m, ref, got);
+        ++bad_add;
+      }
+    }
+    const float inf = INFINITY;
+    if (!(repeat_add(inf, 1.0f, 3) == inf && repeat_add(1.0f, inf, 2) == inf && repeat_add(2.5f, inf, 0) == 2.5f)) ++bad_add;
+  }
+  printf("repeat_add: %ld cases, repeat_add mismatches %ld\n", adds, bad_add);
   uint64_t steps_a = 0, steps_b = 0;
   struct Case { float full[3]; uint32_t n[3]; int32_t tile[3]; };
   std::vector<Case> cases = {
@@ -74,6 +108,20 @@ int main(int argc, char** argv) {
       steps_a += sa; steps_b += sb;
       bool ok = a.size() == b.size() && sa == sb;
       for (size_t k = 0; ok && k < a.size(); ++k) ok = same(a[k], b[k]);
+      // the tile jump: the same tiles, in walk order
+      std::vector<uint32_t> tj;
+      const uint32_t nj = tile_runs(l, (uint32_t)i, f, g, [&](uint32_t tl) { tj.push_back(tl); });
+      bool tok = nj == tj.size() && tj.size() == a.size();
+      for (size_t k = 0; tok && k < a.size(); ++k) tok = tj[k] == a[k].tile;
+      if (!tok) {
+        if (bad_tiles < 5) {
+          printf("TILE MISMATCH case %ux%ux%u tile %dx%dx%d lor %d kind %d: runs %zu vs %zu\n", cs.n[0], cs.n[1], cs.n[2],
+                 g.tx, g.ty, g.tz, i, kind, a.size(), tj.size());
+          for (size_t k = 0; k < a.size() || k < tj.size(); ++k)
+            printf("  step walk %d  jump %d\n", k < a.size() ? (int)a[k].tile : -1, k < tj.size() ? (int)tj[k] : -1);
+        }
+        ++bad_tiles;
+      }
       if (!ok) {
         if (bad < 5) {
           printf("MISMATCH case %ux%ux%u tile %dx%dx%d lor %d kind %d: runs %zu vs %zu steps %u vs %u\n", cs.n[0], cs.n[1], cs.n[2],
@@ -87,6 +135,7 @@ int main(int argc, char** argv) {
       }
     }
   }
-  printf("lors with runs %ld, runs %ld, steps %llu vs %llu, mismatches %ld\n", lors_hit, runs, (unsigned long long)steps_a, (unsigned long long)steps_b, bad);
-  return bad ? 1 : 0;
+  printf("lors with runs %ld, runs %ld, steps %llu vs %llu, mismatches %ld, tile mismatches %ld\n", lors_hit, runs,
+         (unsigned long long)steps_a, (unsigned long long)steps_b, bad, bad_tiles);
+  return (bad || bad_tiles || bad_add) ? 1 : 0;
 }

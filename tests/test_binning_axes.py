@@ -1,7 +1,10 @@
 """The per-axis binning of the tiled scheme (bin_lor_axes, used to fill the
-runs of walks without TOF) against the step-by-step binning walk (bin_lor),
-compiled for the host from the library's own headers: identical runs (every
-field, bit for bit) and step counts on 8 FOV/tile cases x 20 000 LORs."""
+runs of walks without TOF) and the tile jump (tile_runs, which counts them)
+against the step-by-step binning walk (bin_lor), compiled for the host from
+the library's own headers: identical runs (every field, bit for bit), step
+counts and tile sequences on 8 FOV/tile cases x 20 000 LORs; and the jump's
+repeated binary32 addition (repeat_add) against one addition at a time on
+400 000 cases (ties, binade crossings, long runs)."""
 import os
 import shutil
 import subprocess
@@ -23,4 +26,5 @@ def test_axes_binning_matches_step_walk():
         out = subprocess.run([exe, "20000"], capture_output=True, text=True, timeout=600)
         print(out.stdout)
         assert out.returncode == 0, out.stdout
-        assert "mismatches 0" in out.stdout
+        assert "repeat_add mismatches 0" in out.stdout
+        assert ", mismatches 0, tile mismatches 0" in out.stdout

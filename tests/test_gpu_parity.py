@@ -188,12 +188,13 @@ def test_sensitivity_matches_oracle(gpu):
     gen = P.sensitivity_lors(att, n, seed, L, D)
     img, st = P.make_sensitivity_image(mu, n, L, D, seed=seed, return_stats=True)
     img2 = P.make_sensitivity_image(mu, n, lors=gen)
+    img3 = P.make_sensitivity_image(mu, n, L, D, seed=seed)  # no work counters: the tile jump bins
     ref, ost = O.sensitivity(att.full_size, att.n, mu.data, gen, threads=16)
     ref64, _ = O.sensitivity(att.full_size, att.n, mu.data, gen, threads=16, f64=True)
     assert st["n_steps"] == ost["n_steps"] and st["n_nz"] == ost["n_nz"]
-    e1, e2, e64 = rel_l2(img.data, ref), rel_l2(img2.data, ref), rel_l2(img.data, ref64)
-    _say(f"sensitivity: relL2 gen {e1:.2e} list {e2:.2e} vs f64 {e64:.2e}")
-    assert e1 <= 1e-5 and e2 <= 1e-5 and e64 <= 1e-5
+    e1, e2, e3, e64 = rel_l2(img.data, ref), rel_l2(img2.data, ref), rel_l2(img3.data, ref), rel_l2(img.data, ref64)
+    _say(f"sensitivity: relL2 gen {e1:.2e} list {e2:.2e} gen without counters {e3:.2e} vs f64 {e64:.2e}")
+    assert e1 <= 1e-5 and e2 <= 1e-5 and e3 <= 1e-5 and e64 <= 1e-5
 
 
 def test_sensitivity_two_fovs(gpu):
@@ -214,11 +215,12 @@ def test_sensitivity_no_attenuation(gpu):
     n, seed, L, D = 300_000, 3, 300.0, 700.0
     gen = P.sensitivity_lors(att, n, seed, L, D)
     img, st = P.make_sensitivity_image(mu, n, L, D, seed=seed, return_stats=True)
+    img2 = P.make_sensitivity_image(mu, n, L, D, seed=seed)
     ref, ost = O.sensitivity(att.full_size, att.n, mu.data, gen, threads=16)
     assert st["n_steps"] == ost["n_steps"] and st["n_nz"] == ost["n_nz"] and st["n_lors"] == n
-    e = rel_l2(img.data, ref)
-    _say(f"sensitivity mu=0: relL2 {e:.2e}")
-    assert e <= 1e-5
+    e, e2 = rel_l2(img.data, ref), rel_l2(img2.data, ref)
+    _say(f"sensitivity mu=0: relL2 {e:.2e}, without work counters {e2:.2e}")
+    assert e <= 1e-5 and e2 <= 1e-5
 
 
 def test_sensitivity_irregular_lors(gpu):
@@ -286,10 +288,12 @@ def test_discrete_sensitivity_matches_oracle(gpu, adjust):
     geom = dict(r_min=100.0, dr=10.0, dz=10.0, da=10.0)
     L, seed = 100.0, 5
     img, st = P.make_sensitivity_image_discrete(mu, L, adjust=adjust, seed=seed, return_stats=True, **geom)
+    img2 = P.make_sensitivity_image_discrete(mu, L, adjust=adjust, seed=seed, **geom)
     lors, n = O.discrete_lors(geom["r_min"], geom["dr"], geom["dz"], geom["da"], L, adjust, seed=seed)
     ref, ost = O.sensitivity(att.full_size, att.n, mu.data, lors, threads=16)
     ref = ref.astype(np.float64) * len(lors) / (n * (n - 1) / 2)
     assert st["n_lors"] == len(lors) == n * (n + 1) // 2
+    assert rel_l2(img2.data, img.data) <= 1e-6  # without work counters: the same runs
     e = rel_l2(img.data, ref)
     ds = abs(st["n_steps"] - ost["n_steps"]) / ost["n_steps"]
     _say(f"discrete sensitivity adjust {adjust}: {n} elements, {len(lors)} pairs, relL2 {e:.2e}, steps {ds:.1e}")
