@@ -1656,12 +1656,13 @@ __global__ void k_gather_lors_by(const Lor* __restrict__ in, const uint32_t* __r
 }
 
 __global__ void k_take_irregular(uint32_t* __restrict__ counts, uint32_t n, const Lor* __restrict__ lors,
-                                 Lor* __restrict__ irr, uint32_t* __restrict__ n_irr) {
+                                 const uint32_t* __restrict__ order, Lor* __restrict__ irr,
+                                 uint32_t* __restrict__ n_irr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == n) counts[n] = 0;
   if (i >= n || counts[i] != 0xffffffffu) return;
   counts[i] = 0;
-  irr[atomicAdd(n_irr, 1u)] = lors[i];
+  irr[atomicAdd(n_irr, 1u)] = lors[order ? order[i] : i];
 }
 
 
@@ -1805,7 +1806,8 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   const size_t n_seg = segs ? (size_t)nt * ng_max + 1 : 0;
   // buffers of the whole call: batch-sized ones now, run-sized ones grown
   // to the largest batch (no allocation, and one wait, per batch)
-  DevBuf<Lor> lb(bmax), lraw(by_length ? bmax : 0), irr(bmax);
+  // by length, the segment binning reads lraw through the sort's order and needs no copy in lb
+  DevBuf<Lor> lb(by_length && segs ? 0 : bmax), lraw(by_length ? bmax : 0), irr(bmax);
   DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1), seg(n_seg), soff(n_seg);
   DevBuf<uint32_t> lk(by_length ? bmax : 0), lk2(by_length ? bmax : 0), li(by_length ? bmax : 0),
       li2(by_length ? bmax : 0);
@@ -1843,25 +1845,29 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       HIP_CHECK(hipGetLastError());
       size_t sb = lsort_bytes;
       HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(lsort_tmp.p, sb, lk.p, lk2.p, li.p, li2.p, (int)b, 0, 16, s));
-      k_gather_lors_by<<<grid_for(b), kBlock, 0, s>>>(lraw.p, li2.p, b, lb.p);
+      // the segment binning reads the batch through the order; the sorted
+      // binning walks a reordered copy
+      if (!segs) k_gather_lors_by<<<grid_for(b), kBlock, 0, s>>>(lraw.p, li2.p, b, lb.p);
       HIP_CHECK(hipGetLastError());
     }
+    const Lor* src_lors = (by_length && segs) ? lraw.p : lb.p;
+    const uint32_t* lor_order = (by_length && segs) ? li2.p : nullptr;
     const size_t nsb = (size_t)nt * ng + 1;  // this batch's segments and the closing zero
     if (segs) {
       HIP_CHECK(hipMemsetAsync(seg.p + (nsb - 1), 0, 4, s));
       if (want_stats)
-        k_bin_runs_seg<false, true><<<ng, kBlock, (size_t)nt * 4, s>>>(lb.p, b, f, g, counts.p, seg.p, nullptr,
-                                                                       nullptr, nullptr, nullptr, cnt);
+        k_bin_runs_seg<false, true><<<ng, kBlock, (size_t)nt * 4, s>>>(src_lors, lor_order, b, f, g, counts.p, seg.p,
+                                                                       nullptr, nullptr, nullptr, nullptr, cnt);
       else
-        k_bin_runs_seg<false, false><<<ng, kBlock, (size_t)nt * 4, s>>>(lb.p, b, f, g, counts.p, seg.p, nullptr,
-                                                                        nullptr, nullptr, nullptr, cnt);
+        k_bin_runs_seg<false, false><<<ng, kBlock, (size_t)nt * 4, s>>>(src_lors, lor_order, b, f, g, counts.p, seg.p,
+                                                                        nullptr, nullptr, nullptr, nullptr, cnt);
     } else {
       k_bin_runs<false, false><<<ng, kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr, nullptr,
                                                       shift, cnt);
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemsetAsync(n_irr.p, 0, 4, s));
-    k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts.p, b, lb.p, irr.p, n_irr.p);
+    k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts.p, b, src_lors, lor_order, irr.p, n_irr.p);
     HIP_CHECK(hipGetLastError());
     uint32_t total = 0, h_irr = 0;
     if (!segs || !mu_zero) {
@@ -1886,8 +1892,8 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       uint32_t* ps = mu_zero ? nullptr : pos.ensure(total);
       uint32_t* order = nullptr;  // runs in tile order are un[order[j]]; nullptr: un[j]
       if (segs) {
-        k_bin_runs_seg<true, false><<<ng, kBlock, (size_t)nt * 8, s>>>(lb.p, b, f, g, nullptr, seg.p, soff.p, offs.p, un,
-                                                                ps, cnt);
+        k_bin_runs_seg<true, false><<<ng, kBlock, (size_t)nt * 8, s>>>(src_lors, lor_order, b, f, g, nullptr, seg.p,
+                                                                       soff.p, offs.p, un, ps, cnt);
         k_seg_items<<<1, 1024, 0, s>>>(soff.p, ng, nt, kSensItemRuns, cap, items);
         HIP_CHECK(hipGetLastError());
       } else {

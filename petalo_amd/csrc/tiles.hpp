@@ -619,7 +619,8 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
 }
 
 // Binning without TOF straight into tile order, for the sensitivity image.
-// Workgroup w of a grid of G takes the same LORs in both passes (as
+// LOR i is lors[order[i]] (lors[i] without order).  Workgroup w of a grid of
+// G takes the same LORs in both passes (as
 // k_bin_runs), and the runs of its LORs in tile t form segment (t, w) of a
 // tile-major layout: segment sizes seg[t * G + w], their exclusive prefix
 // sums soff.  Count (FILL false): n_runs[i] as k_bin_runs<false, false>,
@@ -634,8 +635,8 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
 // from its count adds to counters[CNT_RUN_MISMATCH].  Dynamic LDS: 4 * ntiles
 // bytes (count) or 8 * ntiles bytes (fill).
 template <bool FILL, bool STATS>
-__global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lors, uint32_t n, DevFov f,
-                                                      TileGeom g, uint32_t* __restrict__ n_runs,
+__global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lors, const uint32_t* __restrict__ order,
+                                                      uint32_t n, DevFov f, TileGeom g, uint32_t* __restrict__ n_runs,
                                                       uint32_t* __restrict__ seg, const uint32_t* __restrict__ soff,
                                                       const uint32_t* __restrict__ offset, Run* __restrict__ runs,
                                                       uint32_t* __restrict__ run_pos,
@@ -658,7 +659,7 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
   for (uint32_t u = 0; u < kBinLors; ++u) {
     const uint32_t i = first + u * blockDim.x;
     if (i >= n) break;
-    const Lor l = load_lor(lors, i);
+    const Lor l = load_lor(lors, order ? order[i] : i);
     const uint32_t base = (FILL && run_pos) ? offset[i] : 0;
     const uint32_t room = (FILL && run_pos) ? offset[i + 1] - base : 0;
     uint32_t k = 0;
