@@ -1677,9 +1677,9 @@ __global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uin
 }
 
 // Tile counts up to which a sensitivity batch is binned straight into tile
-// order (k_bin_runs_seg keeps a cursor and an end per tile in LDS); above it
-// the runs are binned LOR-major and sorted by tile.
-constexpr uint32_t kSegMaxTiles = 4096;
+// order (k_bin_runs_seg keeps two cursors and the bounds of each tile's
+// segment in LDS); above it the runs are binned LOR-major and sorted by tile.
+constexpr uint32_t kSegMaxTiles = 2048;
 
 // Work items of a sensitivity batch binned by k_bin_runs_seg: tile t's runs
 // are [soff[t * ng], soff[(t + 1) * ng]) (soff[ntiles * ng] = the total),
@@ -1794,6 +1794,8 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   // tile order straight from the binning (segments), or LOR-major runs sorted by tile
   const bool segs = (uint32_t)g.ntiles <= kSegMaxTiles;
   const uint32_t nt = (uint32_t)g.ntiles;
+  // runs this long or longer fill their segment from the start, shorter ones from the end
+  const uint32_t long_run = (uint32_t)(g.tx + g.ty + g.tz) / 4u;
   const uint32_t shift = mu_zero ? 32u : kLorBlockShift;
   const uint32_t tshift = mu_zero ? 12u : 32u;
   int tile_bits = 1;
@@ -1857,10 +1859,10 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       HIP_CHECK(hipMemsetAsync(seg.p + (nsb - 1), 0, 4, s));
       if (want_stats)
         k_bin_runs_seg<false, true><<<ng, kBlock, (size_t)nt * 4, s>>>(src_lors, lor_order, b, f, g, counts.p, seg.p,
-                                                                       nullptr, nullptr, nullptr, nullptr, cnt);
+                                                                       nullptr, nullptr, nullptr, nullptr, 0u, cnt);
       else
         k_bin_runs_seg<false, false><<<ng, kBlock, (size_t)nt * 4, s>>>(src_lors, lor_order, b, f, g, counts.p, seg.p,
-                                                                        nullptr, nullptr, nullptr, nullptr, cnt);
+                                                                        nullptr, nullptr, nullptr, nullptr, 0u, cnt);
     } else {
       k_bin_runs<false, false><<<ng, kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr, nullptr,
                                                       shift, cnt);
@@ -1892,8 +1894,8 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       uint32_t* ps = mu_zero ? nullptr : pos.ensure(total);
       uint32_t* order = nullptr;  // runs in tile order are un[order[j]]; nullptr: un[j]
       if (segs) {
-        k_bin_runs_seg<true, false><<<ng, kBlock, (size_t)nt * 8, s>>>(src_lors, lor_order, b, f, g, nullptr, seg.p,
-                                                                       soff.p, offs.p, un, ps, cnt);
+        k_bin_runs_seg<true, false><<<ng, kBlock, (size_t)nt * 16, s>>>(src_lors, lor_order, b, f, g, nullptr, seg.p,
+                                                                        soff.p, offs.p, un, ps, long_run, cnt);
         k_seg_items<<<1, 1024, 0, s>>>(soff.p, ng, nt, kSensItemRuns, cap, items);
         HIP_CHECK(hipGetLastError());
       } else {

@@ -627,27 +627,34 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
 // and seg from a per-workgroup LDS histogram; STATS walks every step to
 // count the steps and pushed steps too (bin_lor), otherwise the tiles come
 // from the tile jump (tile_runs) and only the miss and shifted-entry
-// counters are kept.  Fill (FILL true): each run goes to the next free position of its segment
-// (an LDS cursor per tile; positions inside a segment follow no order), and,
+// counters are kept.  Fill (FILL true): each run goes to the next free
+// position of its segment, runs of long_run steps or more from its start and
+// shorter ones from its end (two LDS cursors per tile), so that the lanes of
+// a replay mostly get runs of one class (inside a class positions follow no
+// order), and,
 // with run_pos, LOR i's k-th run position to run_pos[offset[i] + k].  A run
 // past its segment's end is not written (its run_pos entry is 0), a segment
 // left short is padded with empty runs, and every segment whose fill differs
 // from its count adds to counters[CNT_RUN_MISMATCH].  Dynamic LDS: 4 * ntiles
-// bytes (count) or 8 * ntiles bytes (fill).
+// bytes (count) or 16 * ntiles bytes (fill).
 template <bool FILL, bool STATS>
 __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lors, const uint32_t* __restrict__ order,
                                                       uint32_t n, DevFov f, TileGeom g, uint32_t* __restrict__ n_runs,
                                                       uint32_t* __restrict__ seg, const uint32_t* __restrict__ soff,
                                                       const uint32_t* __restrict__ offset, Run* __restrict__ runs,
-                                                      uint32_t* __restrict__ run_pos,
+                                                      uint32_t* __restrict__ run_pos, uint32_t long_run,
                                                       unsigned long long* __restrict__ counters) {
-  extern __shared__ uint32_t cur[];  // [0, ntiles): count or cursor; [ntiles, 2 ntiles): segment end (fill)
+  // count: [0, nt) runs per tile; fill: [0, nt) front cursor, [nt, 2 nt)
+  // back cursor, [2 nt, 3 nt) segment start, [3 nt, 4 nt) segment end
+  extern __shared__ uint32_t cur[];
   const uint32_t nt = (uint32_t)g.ntiles, ng = gridDim.x, wg = blockIdx.x;
   for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
     if (FILL) {
-      const uint32_t a = soff[(size_t)t * ng + wg];
+      const uint32_t a = soff[(size_t)t * ng + wg], e = a + seg[(size_t)t * ng + wg];
       cur[t] = a;
-      cur[nt + t] = a + seg[(size_t)t * ng + wg];
+      cur[nt + t] = e;
+      cur[2 * nt + t] = a;
+      cur[3 * nt + t] = e;
     } else {
       cur[t] = 0;
     }
@@ -665,8 +672,10 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
     uint32_t k = 0;
     auto emit = [&](const Run& ru, uint32_t tile) {
       if (FILL) {
-        const uint32_t at = atomicAdd(&cur[tile], 1u);
-        const bool fits = at < cur[nt + tile];
+        // long runs fill the segment from its start, short ones from its end
+        const bool lng = run_len(ru.code) >= long_run;
+        const uint32_t at = lng ? atomicAdd(&cur[tile], 1u) : atomicSub(&cur[nt + tile], 1u) - 1u;
+        const bool fits = at >= cur[2 * nt + tile] && at < cur[3 * nt + tile];
         if (fits) runs[at] = ru;
         if (run_pos && k < room) run_pos[base + k] = fits ? at : 0u;
       } else {
@@ -695,6 +704,7 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
     if (FILL) {
+      // the two cursors meet exactly when the segment holds its count
       mismatch += cur[t] != cur[nt + t] ? 1u : 0u;
       // a segment left short holds empty runs (length 0, LOR 0), not stale records
       for (uint32_t x = cur[t]; x < cur[nt + t]; ++x) runs[x] = Run{};
