@@ -309,7 +309,9 @@ def main():
         ctx.step(copy_image=False)
     P.lib().petalo_device_synchronize(0)
 
-    info = dict(ctx.info(), tile=list(ctx.tile_shape()), autotune=bool(a.autotune))
+    steps_per_item, n_items = ctx.work_items()
+    info = dict(ctx.info(), tile=list(ctx.tile_shape()), autotune=bool(a.autotune),
+                item_steps=steps_per_item, work_items=n_items)
     log(f"[bench] scheme {info}")
     # HIP events bracket every kernel of every timed update; the updates run
     # back to back (their timings are read once, after the timed region,

@@ -112,12 +112,21 @@ typedef struct {
                             64 voxels across on up to 2^21 LORs of the first
                             subset, and keeps the default unless another is
                             at least 3% faster (creation takes about a second
-                            longer); 0 takes the tiled scheme with the tile
-                            given or defaulted.                              */
+                            longer), then, with item_steps 0, the work
+                            granularity (below); 0 takes the tiled scheme
+                            with the tile given or defaulted.                */
   int32_t precision;     /* PETALO_PRECISION_*; the binary64 modes run one lane
                             per LOR with binary64 global atomics whatever the
                             scheme (reference modes for precision studies;
                             not with autotune, not in batches)               */
+  int32_t item_steps;    /* tiled scheme: replayed steps per work item (one
+                            workgroup each; at least 8192); 0 takes the
+                            automatic rule (about 2048 items per subset
+                            update, at least 16 steps per LDS entry of the
+                            tile), and with autotune the tuner then also
+                            times 2^14 .. 2^20 on the context's first
+                            subsets and keeps the rule unless one is 3%
+                            faster                                           */
 } petalo_mlem_options_t;
 
 /* ---- library ----------------------------------------------------------- */
@@ -200,6 +209,10 @@ int petalo_mlem_info(petalo_mlem_t* ctx, int32_t* scheme, uint64_t* n_runs_total
 /* The tile shape of a tiled context (zeros for the atomic scheme): after
  * autotuning, the shape it chose. */
 int petalo_mlem_tile_shape(petalo_mlem_t* ctx, int32_t tile_out[3]);
+/* The work granularity of a tiled context: its replayed steps per work item
+ * (0: the automatic rule; after autotuning, the choice) and its work items
+ * over all subsets. */
+int petalo_mlem_work_items(petalo_mlem_t* ctx, uint64_t* item_steps_out, uint64_t* n_items_total_out);
 int petalo_mlem_destroy(petalo_mlem_t* ctx);
 
 /* ---- Batched reconstructions -------------------------------------------
@@ -210,8 +223,11 @@ int petalo_mlem_destroy(petalo_mlem_t* ctx);
  * them (work items of all their image tiles, largest first), replayed as a
  * HIP graph per subset.  tofs_or_null: n entries or NULL (no TOF);
  * sensitivities_or_null: n pointers (entries may be NULL) or NULL.  Tiled
- * scheme only.  No counterpart in the reference, whose mlem runs one
- * reconstruction per call (src/mlem.rs:8-31). */
+ * scheme only; options.tile applies to every reconstruction, options.item_steps
+ * to every reconstruction's work items and to their merging, and
+ * options.autotune (with item_steps 0) times the work granularity as for one
+ * context, on the whole batch.  No counterpart in the reference, whose mlem
+ * runs one reconstruction per call (src/mlem.rs:8-31). */
 typedef struct petalo_batch petalo_batch_t;
 int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs, const petalo_tof_t* tofs_or_null,
                         const petalo_lor_t* const* lors, const size_t* n_lors,
@@ -219,6 +235,9 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
                         const petalo_mlem_options_t* options);
 /* One subset update of every reconstruction; id_out: the Osem before advance. */
 int petalo_batch_next(petalo_batch_t* batch, petalo_osem_t* id_out_or_null);
+/* The batch's replayed steps per merged work item (0: the automatic rule;
+ * after autotuning, the choice) and its work items over all subsets. */
+int petalo_batch_work_items(petalo_batch_t* batch, uint64_t* item_steps_out, uint64_t* n_items_total_out);
 int petalo_batch_get_image(petalo_batch_t* batch, size_t r, float* image_out);
 /* timing: HIP events around each petalo_batch_next (petalo_batch_last_ms);
  * use_graphs: replay captured graphs (default 1) or launch directly. */

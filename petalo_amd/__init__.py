@@ -67,7 +67,8 @@ class _Stats(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("sort_lors", C.c_int32), ("device", C.c_int32),
-                ("tile", C.c_int32 * 3), ("autotune", C.c_int32), ("precision", C.c_int32)]
+                ("tile", C.c_int32 * 3), ("autotune", C.c_int32), ("precision", C.c_int32),
+                ("item_steps", C.c_int32)]
 
 
 PRECISION_F32, PRECISION_F64, PRECISION_F32_WALK = 0, 1, 2
@@ -180,6 +181,8 @@ _SIGS = {
     "petalo_mlem_set_graphs": (C.c_int, [_P, C.c_int]),
     "petalo_batch_create": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P, _P, C.c_size_t, _P]),
     "petalo_batch_next": (C.c_int, [_P, _P]),
+    "petalo_batch_work_items": (C.c_int, [_P, _P, _P]),
+    "petalo_mlem_work_items": (C.c_int, [_P, _P, _P]),
     "petalo_batch_get_image": (C.c_int, [_P, C.c_size_t, _P]),
     "petalo_batch_set_options": (C.c_int, [_P, C.c_int, C.c_int]),
     "petalo_batch_last_ms": (C.c_int, [_P, _P]),
@@ -354,7 +357,7 @@ class Mlem:
     subset update, like petalo::mlem::mlem (src/mlem.rs:8-31)."""
 
     def __init__(self, fov, lors, tof=None, sensitivity=None, n_subsets=1, scheme=SCHEME_AUTO,
-                 sort_lors=True, device=0, tile=None, autotune=False, precision=PRECISION_F32):
+                 sort_lors=True, device=0, tile=None, autotune=False, precision=PRECISION_F32, item_steps=0):
         self.fov = fov
         lors = np.ascontiguousarray(lors, LOR_DTYPE)
         sens = None
@@ -365,7 +368,7 @@ class Mlem:
                 raise PetaloError(6, "sensitivity image size does not match the FOV")
         self._ctx = C.c_void_p()
         opt = _Options(int(scheme), 1 if sort_lors else 0, int(device), (C.c_int32 * 3)(*(tile or (0, 0, 0))),
-                       1 if autotune else 0, int(precision))
+                       1 if autotune else 0, int(precision), int(item_steps))
         _check(lib().petalo_mlem_create(C.byref(self._ctx), C.byref(fov._c()), _tof_ptr(tof),
                                         _ptr(lors), len(lors), _ptr(sens), int(n_subsets),
                                         C.byref(opt)))
@@ -406,6 +409,12 @@ class Mlem:
         t = (C.c_int32 * 3)()
         _check(lib().petalo_mlem_tile_shape(self._ctx, t))
         return tuple(int(v) for v in t)
+
+    def work_items(self):
+        """(replayed steps per work item, 0 for the automatic rule; work items over all subsets)."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().petalo_mlem_work_items(self._ctx, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
 
     def walk_stats(self, subset=0):
         s = _Stats()
@@ -471,7 +480,7 @@ class Batch:
     each per step(), with one forward and one backprojection launch for all."""
 
     def __init__(self, fovs, lors_list, tofs=None, sensitivities=None, n_subsets=1, sort_lors=True, device=0,
-                 tile=None):
+                 tile=None, autotune=False, item_steps=0):
         n = len(fovs)
         if len(lors_list) != n:
             raise PetaloError(1, "one LOR array per FOV")
@@ -498,7 +507,8 @@ class Batch:
                     raise PetaloError(6, "sensitivity image size does not match the FOV")
                 self._sens.append(a)
                 csens[k] = a.ctypes.data
-        opt = _Options(SCHEME_TILE, 1 if sort_lors else 0, int(device), (C.c_int32 * 3)(*(tile or (0, 0, 0))))
+        opt = _Options(SCHEME_TILE, 1 if sort_lors else 0, int(device), (C.c_int32 * 3)(*(tile or (0, 0, 0))),
+                       1 if autotune else 0, PRECISION_F32, int(item_steps))
         self._b = C.c_void_p()
         _check(lib().petalo_batch_create(C.byref(self._b), n, cfov, ctof, ptrs, counts, csens, int(n_subsets),
                                          C.byref(opt)))
@@ -510,6 +520,12 @@ class Batch:
         osem = _Osem()
         _check(lib().petalo_batch_next(self._b, C.byref(osem)))
         return Osem(osem.n_subsets, osem.iteration, osem.subset)
+
+    def work_items(self):
+        """(replayed steps per merged work item, 0 for the automatic rule; work items over all subsets)."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        _check(lib().petalo_batch_work_items(self._b, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
 
     def image(self, r):
         out = np.empty(self.fovs[r].nvox, np.float32)

@@ -231,6 +231,12 @@ struct petalo_mlem {
   DevBuf<float> ac;                  // additive corrections, in context LOR order
   DevBuf<double> bp64;               // binary64 backprojection
   std::vector<std::vector<uint64_t>> item_steps;  // per subset: replayed steps of each work item
+  uint64_t item_target = 0;          // replayed steps per work item; 0: the automatic rule (cut_items)
+  // kept while the autotuner re-cuts the work items: per subset, the first
+  // sorted run of every tile and (tile << 12) | length of every sorted run
+  bool keep_cuts = false;
+  std::vector<std::vector<uint64_t>> cut_starts;
+  std::vector<std::vector<uint32_t>> cut_summary;
   // binary64 precision modes (PETALO_PRECISION_F64 / _F32_WALK)
   int precision = PETALO_PRECISION_F32;
   DevFov64 dfov64{};
@@ -405,6 +411,36 @@ std::vector<std::pair<uint64_t, int4>> tile_items(const std::vector<uint64_t>& s
 
 void zero_counters(unsigned long long* p, hipStream_t s);
 
+// Work items of one subset: chunks of one tile's runs holding about equal
+// numbers of replayed steps, largest first so that the last workgroups to
+// start are the short ones.  With c->item_target the chunks hold about that
+// many steps (and at least kMinItemSteps); automatically, kWorkItems in
+// total, and, since each item loads (forward) and flushes (scatter) its
+// whole tile, at least kItemStepsPerVoxel steps per LDS entry of the tile.
+void cut_items(petalo_mlem* c, size_t sub, const std::vector<uint64_t>& starts, const std::vector<uint32_t>& summary) {
+  const TileGeom& g = c->tiles;
+  uint64_t total_steps = 0;
+  for (uint32_t v : summary) total_steps += v & kRunMaxSteps;
+  const uint64_t target =
+      c->item_target ? std::max<uint64_t>(kMinItemSteps, c->item_target)
+                     : std::max<uint64_t>(std::max<uint64_t>(kMinItemSteps, total_steps / kWorkItems + 1),
+                                          kItemStepsPerVoxel * (uint64_t)g.tvox);
+  const std::vector<std::pair<uint64_t, int4>> sized = tile_items(starts, summary, g.ntiles, target);
+  std::vector<int4> items;
+  items.reserve(sized.size());
+  c->item_steps[sub].clear();
+  for (auto& it : sized) {
+    items.push_back(it.second);
+    c->item_steps[sub].push_back(it.first);
+  }
+  c->work[sub].alloc(std::max<size_t>(items.size(), 1));
+  if (!items.empty())
+    HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
+                             c->stream));
+  c->work[sub].n = items.size();
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
 // Cuts the subset's LOR walks into per-tile runs, sorted by tile, and the
 // per-workgroup work items over them; lists the LORs with unbounded walks.
 void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
@@ -552,29 +588,12 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
                                                       c->pos4[sub].p);
   }
   HIP_CHECK(hipGetLastError());
-  // Work items: chunks of one tile's runs holding about equal numbers of
-  // replayed steps (kWorkItems in total), largest first so that the last
-  // workgroups to start are the short ones.  Each item loads (forward) and
-  // flushes (scatter) its whole tile, so it replays at least
-  // kItemStepsPerVoxel steps per LDS entry of the tile (and kMinItemSteps).
-  uint64_t total_steps = 0;
-  for (uint32_t v : summary) total_steps += v & kRunMaxSteps;
-  const uint64_t target = std::max<uint64_t>(std::max<uint64_t>(kMinItemSteps, total_steps / kWorkItems + 1),
-                                             kItemStepsPerVoxel * (uint64_t)g.tvox);
-  const std::vector<std::pair<uint64_t, int4>> sized = tile_items(starts, summary, g.ntiles, target);
-  std::vector<int4> items;
-  items.reserve(sized.size());
-  c->item_steps[sub].clear();
-  for (auto& it : sized) {
-    items.push_back(it.second);
-    c->item_steps[sub].push_back(it.first);
-  }
-  c->work[sub].alloc(std::max<size_t>(items.size(), 1));
-  if (!items.empty())
-    HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
-  c->work[sub].n = items.size();
-  HIP_CHECK(hipStreamSynchronize(st));
+  cut_items(c, sub, starts, summary);
   c->n_runs[sub] = total;
+  if (c->keep_cuts) {
+    c->cut_starts[sub] = std::move(starts);
+    c->cut_summary[sub] = std::move(summary);
+  }
 }
 
 // 1/(p*ac) per LOR from the partials of the forward kernel, then the
@@ -785,6 +804,7 @@ std::string check_mlem_args(const petalo_fov_t* fov, const petalo_tof_t* tof, co
     if (opt.tile[a] < 0) return "negative tile dimension";
   if (opt.precision < PETALO_PRECISION_F32 || opt.precision > PETALO_PRECISION_F32_WALK) return "unknown precision";
   if (opt.precision != PETALO_PRECISION_F32 && opt.autotune) return "autotune applies to the binary32 path only";
+  if (opt.item_steps < 0) return "negative item_steps";
   return "";
 }
 
@@ -792,10 +812,12 @@ std::string check_mlem_args(const petalo_fov_t* fov, const petalo_tof_t* tof, co
 // (a batch's), else its own.  Throws on failure.
 petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, const petalo_lor_t* lors,
                             size_t n_lors, const float* sensitivity, size_t n_subsets,
-                            const petalo_mlem_options_t& opt, hipStream_t shared) {
+                            const petalo_mlem_options_t& opt, hipStream_t shared, bool keep_cuts = false) {
     auto* c = new petalo_mlem();
     try {
       c->device = opt.device;
+      c->item_target = (uint64_t)opt.item_steps;
+      c->keep_cuts = keep_cuts;
       c->fov = *fov;
       c->dfov = to_dev(*fov);
       c->dtof = to_dev(tof);
@@ -858,6 +880,10 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->max_win.resize(n_subsets, 0);
         c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
+        if (keep_cuts) {
+          c->cut_starts.resize(n_subsets);
+          c->cut_summary.resize(n_subsets);
+        }
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
         c->ac.alloc(std::max<size_t>(n_lors, 1));
         if (n_lors) {
@@ -962,6 +988,66 @@ petalo_mlem_options_t autotune(const petalo_fov_t* fov, const petalo_tof_t* tof,
   return best;
 }
 
+// The autotuner's second dimension: the work granularity of a tiled context,
+// timed on the context itself (created with keep_cuts).  Candidates: the
+// automatic rule of cut_items and targets of 2^14 .. 2^20 replayed steps per
+// work item, those that cut the first subset into a number of items no
+// earlier candidate gave; each re-cuts the items of the timed subsets (the
+// first min(n_subsets, 4)), runs them once untimed and three times timed
+// (the best counts), and the automatic rule stays unless another is at least
+// kTuneMargin faster.  The
+// items of every subset are then cut with the choice, the image, counters
+// and subset position reset, and the run summaries released.
+void tune_items(petalo_mlem* c) {
+  if (c->scheme != PETALO_SCHEME_TILE || c->cut_summary.empty()) return;
+  const size_t nt = std::min<size_t>(c->n_subsets, 4);
+  const bool graphs = c->use_graphs;
+  c->use_graphs = false;
+  auto recut = [&](uint64_t target, size_t upto) {
+    c->item_target = target;
+    for (size_t sub = 0; sub < upto; ++sub) cut_items(c, sub, c->cut_starts[sub], c->cut_summary[sub]);
+  };
+  auto updates = [&]() {
+    c->osem = petalo_osem_t{c->n_subsets, 1, 1};
+    petalo_osem_t id;
+    for (size_t k = 0; k < nt; ++k)
+      if (petalo_mlem_next(c, nullptr, &id) != PETALO_OK) throw std::runtime_error(petalo_last_error());
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  };
+  std::vector<size_t> seen;
+  uint64_t best = 0;
+  double best_ms = 1e300;
+  for (uint64_t target : {(uint64_t)0, (uint64_t)1 << 14, (uint64_t)1 << 15, (uint64_t)1 << 16, (uint64_t)1 << 17,
+                          (uint64_t)1 << 18, (uint64_t)1 << 19, (uint64_t)1 << 20}) {
+    recut(target, nt);
+    const size_t n_items = c->work[0].n;
+    if (std::find(seen.begin(), seen.end(), n_items) != seen.end()) continue;
+    seen.push_back(n_items);
+    updates();  // warm-up
+    double ms = 1e300;  // the best of three: a few updates time noisily
+    for (int rep = 0; rep < 3; ++rep) {
+      const auto t0 = std::chrono::steady_clock::now();
+      updates();
+      ms = std::min(ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    if (target == 0 || ms < best_ms * (1.0 - kTuneMargin)) {
+      best_ms = ms;
+      best = target;
+    }
+  }
+  recut(best, c->n_subsets);
+  const size_t nv = nvox(c->fov);
+  fill_f32(c->image.p, 1.0f, nv, c->stream);  // Image::ones (mlem.rs:17)
+  fill_f64(c->bp64.p, 0.0, nv, c->stream);
+  zero_counters(c->counters.p, c->stream);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->osem = petalo_osem_t{c->n_subsets, 1, 1};
+  c->use_graphs = graphs;
+  c->keep_cuts = false;
+  std::vector<std::vector<uint64_t>>().swap(c->cut_starts);
+  std::vector<std::vector<uint32_t>>().swap(c->cut_summary);
+}
+
 }  // namespace
 
 extern "C" {
@@ -972,15 +1058,23 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
   return guarded([&] {
     if (!out) return fail(PETALO_ERR_ARG, "out is NULL");
     *out = nullptr;
-    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}, 0, PETALO_PRECISION_F32};
+    petalo_mlem_options_t opt{PETALO_SCHEME_AUTO, 1, 0, {0, 0, 0}, 0, PETALO_PRECISION_F32, 0};
     if (options) opt = *options;
     const std::string why = check_mlem_args(fov, tof, lors, n_lors, n_subsets, opt);
     if (!why.empty()) return fail(PETALO_ERR_ARG, why);
     require_device();
     HIP_CHECK(hipSetDevice(opt.device));
-    if (opt.scheme == PETALO_SCHEME_AUTO && opt.autotune)
-      opt = autotune(fov, tof, lors, n_lors, sensitivity, n_subsets, opt);
-    *out = create_context(fov, tof, lors, n_lors, sensitivity, n_subsets, opt, nullptr);
+    const bool tune = opt.scheme == PETALO_SCHEME_AUTO && opt.autotune;
+    if (tune) opt = autotune(fov, tof, lors, n_lors, sensitivity, n_subsets, opt);
+    petalo_mlem* c = create_context(fov, tof, lors, n_lors, sensitivity, n_subsets, opt, nullptr,
+                                    tune && opt.item_steps == 0);
+    try {
+      if (tune && opt.item_steps == 0) tune_items(c);
+    } catch (...) {
+      petalo_mlem_destroy(c);
+      throw;
+    }
+    *out = c;
     return PETALO_OK;
   });
 }
@@ -1210,6 +1304,17 @@ int petalo_mlem_tile_shape(petalo_mlem_t* c, int32_t tile[3]) {
   return PETALO_OK;
 }
 
+int petalo_mlem_work_items(petalo_mlem_t* c, uint64_t* item_steps, uint64_t* n_items_total) {
+  if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
+  if (item_steps) *item_steps = c->item_target;
+  if (n_items_total) {
+    uint64_t t = 0;
+    for (const auto& w : c->work) t += w.n;
+    *n_items_total = t;
+  }
+  return PETALO_OK;
+}
+
 int petalo_mlem_destroy(petalo_mlem_t* c) {
   if (!c) return PETALO_OK;
   (void)hipSetDevice(c->device);
@@ -1267,6 +1372,7 @@ struct petalo_batch {
   std::vector<hipGraphExec_t> graphs;     // per subset
   bool timing = false, have_timing = false;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  uint64_t item_target = 0;               // replayed steps per merged item; 0: the automatic rule
 };
 
 namespace {
@@ -1311,6 +1417,142 @@ void batch_enqueue(petalo_batch* b, size_t sub) {
 
 }  // namespace
 
+namespace {
+
+// Descriptors and work items of one subset update of a batch: every
+// reconstruction's items of that subset (as its context cut them), adjacent
+// items of one tile merged up to item_target replayed steps (0: the batch's
+// share, total / kBatchItems), so that small reconstructions do not pay a
+// tile load and flush per few runs; then largest first over the whole batch.
+void batch_build_subset(petalo_batch* b, size_t sub, uint64_t item_target) {
+  const size_t n = b->ctx.size();
+  std::vector<ReconDesc> d(n);
+  std::vector<ReduceDesc> rd(n);
+  uint32_t rblocks = 0;
+  // (reconstruction, tile, first, end) with steps, from each context's items
+  struct Item { int r, tile, a, e; uint64_t steps; bool tof; };
+  std::vector<Item> all;
+  uint64_t total_steps = 0;
+  for (size_t r = 0; r < n; ++r) {
+    petalo_mlem* c = b->ctx[r];
+    d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->lpos[sub].p,
+                     c->dfov, c->dtof, c->tiles};
+    const uint32_t m = (uint32_t)c->set_size;
+    rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
+                       c->partial.p, c->rbuf.p, c->win[sub].p, c->pos2[sub].p, m, rblocks};
+    rblocks += c->n_blocks[sub];
+    b->reduce_lds[sub] = std::max(b->reduce_lds[sub], std::min(c->max_win[sub], kReduceWindow));
+    std::vector<int4> h(c->work[sub].n);
+    if (!h.empty())
+      HIP_CHECK(hipMemcpy(h.data(), c->work[sub].p, h.size() * sizeof(int4), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < h.size(); ++k) {
+      all.push_back(Item{(int)r, h[k].x, h[k].y, h[k].z, c->item_steps[sub][k], c->tof_on});
+      total_steps += c->item_steps[sub][k];
+    }
+  }
+  const uint64_t target = item_target ? std::max<uint64_t>(kMinItemSteps, item_target)
+                                      : std::max<uint64_t>(kMinItemSteps, total_steps / kBatchItems + 1);
+  std::sort(all.begin(), all.end(), [](const Item& u, const Item& v) {
+    return u.r != v.r ? u.r < v.r : (u.tile != v.tile ? u.tile < v.tile : u.a < v.a);
+  });
+  std::vector<Item> merged;
+  for (const Item& x : all) {
+    if (!merged.empty()) {
+      Item& m = merged.back();
+      if (m.r == x.r && m.tile == x.tile && m.e == x.a && m.steps + x.steps <= target) {
+        m.e = x.e;
+        m.steps += x.steps;
+        continue;
+      }
+    }
+    merged.push_back(x);
+  }
+  std::stable_sort(merged.begin(), merged.end(), [](const Item& u, const Item& v) { return u.steps > v.steps; });
+  b->descs[sub].alloc(n);
+  HIP_CHECK(hipMemcpy(b->descs[sub].p, d.data(), n * sizeof(ReconDesc), hipMemcpyHostToDevice));
+  b->reduce[sub].alloc(n);
+  HIP_CHECK(hipMemcpy(b->reduce[sub].p, rd.data(), n * sizeof(ReduceDesc), hipMemcpyHostToDevice));
+  b->reduce_blocks[sub] = rblocks;
+  for (int t = 0; t < 2; ++t) {
+    std::vector<int4> w;
+    for (const Item& x : merged)
+      if (x.tof == (t == 1)) w.push_back(make_int4(x.tile, x.a, x.e, x.r));
+    b->items[t][sub].alloc(std::max<size_t>(w.size(), 1));
+    if (!w.empty())
+      HIP_CHECK(hipMemcpy(b->items[t][sub].p, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice));
+    b->items[t][sub].n = w.size();
+  }
+}
+
+// The batch's work granularity, tuned as tune_items does for one context:
+// candidates the automatic rule and 2^14 .. 2^20 replayed steps per item,
+// applied to every reconstruction's cut (from their kept run summaries) and
+// to the merge; the first min(n_subsets, 4) subset updates of the whole
+// batch run once untimed and three times timed per candidate (the best
+// counts); the automatic rule
+// stays unless another is at least kTuneMargin faster.  Then every subset is
+// cut with the choice and the images, counters and subset positions reset.
+void tune_batch(petalo_batch* b) {
+  const size_t nt = std::min<size_t>(b->n_subsets, 4);
+  const bool graphs = b->use_graphs;
+  b->use_graphs = false;
+  auto recut = [&](uint64_t target, size_t upto) {
+    for (petalo_mlem* c : b->ctx) {
+      c->item_target = target;
+      for (size_t sub = 0; sub < upto; ++sub) cut_items(c, sub, c->cut_starts[sub], c->cut_summary[sub]);
+    }
+    for (size_t sub = 0; sub < upto; ++sub) batch_build_subset(b, sub, target);
+    b->item_target = target;
+  };
+  auto reset_position = [&] {
+    b->osem = petalo_osem_t{b->n_subsets, 1, 1};
+    for (petalo_mlem* c : b->ctx) c->osem = b->osem;
+  };
+  auto updates = [&] {
+    reset_position();
+    petalo_osem_t id;
+    for (size_t k = 0; k < nt; ++k)
+      if (petalo_batch_next(b, &id) != PETALO_OK) throw std::runtime_error(petalo_last_error());
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+  };
+  std::vector<size_t> seen;
+  uint64_t best = 0;
+  double best_ms = 1e300;
+  for (uint64_t target : {(uint64_t)0, (uint64_t)1 << 14, (uint64_t)1 << 15, (uint64_t)1 << 16, (uint64_t)1 << 17,
+                          (uint64_t)1 << 18, (uint64_t)1 << 19, (uint64_t)1 << 20}) {
+    recut(target, nt);
+    const size_t n_items = b->items[0][0].n + b->items[1][0].n;
+    if (std::find(seen.begin(), seen.end(), n_items) != seen.end()) continue;
+    seen.push_back(n_items);
+    updates();  // warm-up
+    double ms = 1e300;  // the best of three: a few updates time noisily
+    for (int rep = 0; rep < 3; ++rep) {
+      const auto t0 = std::chrono::steady_clock::now();
+      updates();
+      ms = std::min(ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    if (target == 0 || ms < best_ms * (1.0 - kTuneMargin)) {
+      best_ms = ms;
+      best = target;
+    }
+  }
+  recut(best, b->n_subsets);
+  for (petalo_mlem* c : b->ctx) {
+    const size_t nv = nvox(c->fov);
+    fill_f32(c->image.p, 1.0f, nv, b->stream);  // Image::ones (mlem.rs:17)
+    fill_f64(c->bp64.p, 0.0, nv, b->stream);
+    zero_counters(c->counters.p, b->stream);
+    c->keep_cuts = false;
+    std::vector<std::vector<uint64_t>>().swap(c->cut_starts);
+    std::vector<std::vector<uint32_t>>().swap(c->cut_summary);
+  }
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  reset_position();
+  b->use_graphs = graphs;
+}
+
+}  // namespace
+
 extern "C" {
 
 int petalo_batch_destroy(petalo_batch_t* b);
@@ -1323,11 +1565,13 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
     *out = nullptr;
     if (n == 0) return fail(PETALO_ERR_ARG, "a batch needs at least one reconstruction");
     if (!fovs || !lors || !n_lors) return fail(PETALO_ERR_ARG, "NULL argument");
-    petalo_mlem_options_t opt{PETALO_SCHEME_TILE, 1, 0, {0, 0, 0}, 0, PETALO_PRECISION_F32};
+    petalo_mlem_options_t opt{PETALO_SCHEME_TILE, 1, 0, {0, 0, 0}, 0, PETALO_PRECISION_F32, 0};
     if (options) opt = *options;
     if (opt.scheme == PETALO_SCHEME_ATOMIC) return fail(PETALO_ERR_ARG, "batches run the tiled scheme");
-    if (opt.autotune) return fail(PETALO_ERR_ARG, "options.autotune is not supported by petalo_batch_create");
     if (opt.precision != PETALO_PRECISION_F32) return fail(PETALO_ERR_ARG, "batches run the binary32 path");
+    // autotune: the tile shapes stay as given or defaulted; the work granularity is tuned
+    const bool tune = opt.autotune != 0 && opt.item_steps == 0;
+    opt.autotune = 0;
     opt.scheme = PETALO_SCHEME_TILE;
     for (size_t r = 0; r < n; ++r) {
       const std::string why = check_mlem_args(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],
@@ -1340,12 +1584,13 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
     try {
       b->device = opt.device;
       b->n_subsets = n_subsets;
+      b->item_target = (uint64_t)opt.item_steps;
       b->osem = petalo_osem_t{n_subsets, 1, 1};
       HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
       for (auto& e : b->ev) HIP_CHECK(hipEventCreate(&e));
       for (size_t r = 0; r < n; ++r)
         b->ctx.push_back(create_context(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],
-                                        sens ? sens[r] : nullptr, n_subsets, opt, b->stream));
+                                        sens ? sens[r] : nullptr, n_subsets, opt, b->stream, tune));
       for (petalo_mlem* c : b->ctx) {
         b->lds_f32 = std::max(b->lds_f32, ((size_t)c->tiles.tvox + 1) * sizeof(float));
         b->lds_f64 = std::max(b->lds_f64, (size_t)c->tiles.tvox * sizeof(double));
@@ -1368,66 +1613,8 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
         HIP_CHECK(hipMemcpy(b->update.p, u.data(), n * sizeof(UpdateDesc), hipMemcpyHostToDevice));
         b->update_blocks = blocks;
       }
-      for (size_t sub = 0; sub < n_subsets; ++sub) {
-        std::vector<ReconDesc> d(n);
-        std::vector<ReduceDesc> rd(n);
-        uint32_t rblocks = 0;
-        // (reconstruction, tile, first, end) with steps, from each context's items
-        struct Item { int r, tile, a, e; uint64_t steps; bool tof; };
-        std::vector<Item> all;
-        uint64_t total_steps = 0;
-        for (size_t r = 0; r < n; ++r) {
-          petalo_mlem* c = b->ctx[r];
-          d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->lpos[sub].p,
-                           c->dfov, c->dtof, c->tiles};
-          const uint32_t m = (uint32_t)c->set_size;
-          rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
-                             c->partial.p, c->rbuf.p, c->win[sub].p, c->pos2[sub].p, m, rblocks};
-          rblocks += c->n_blocks[sub];
-          b->reduce_lds[sub] = std::max(b->reduce_lds[sub], std::min(c->max_win[sub], kReduceWindow));
-          std::vector<int4> h(c->work[sub].n);
-          if (!h.empty())
-            HIP_CHECK(hipMemcpy(h.data(), c->work[sub].p, h.size() * sizeof(int4), hipMemcpyDeviceToHost));
-          for (size_t k = 0; k < h.size(); ++k) {
-            all.push_back(Item{(int)r, h[k].x, h[k].y, h[k].z, c->item_steps[sub][k], c->tof_on});
-            total_steps += c->item_steps[sub][k];
-          }
-        }
-        // Merge adjacent items of one tile up to the batch's item size, so
-        // that small reconstructions do not pay a tile load and flush per
-        // few runs; then largest first over the whole batch.
-        const uint64_t target = std::max<uint64_t>(kMinItemSteps, total_steps / kBatchItems + 1);
-        std::sort(all.begin(), all.end(), [](const Item& u, const Item& v) {
-          return u.r != v.r ? u.r < v.r : (u.tile != v.tile ? u.tile < v.tile : u.a < v.a);
-        });
-        std::vector<Item> merged;
-        for (const Item& x : all) {
-          if (!merged.empty()) {
-            Item& m = merged.back();
-            if (m.r == x.r && m.tile == x.tile && m.e == x.a && m.steps + x.steps <= target) {
-              m.e = x.e;
-              m.steps += x.steps;
-              continue;
-            }
-          }
-          merged.push_back(x);
-        }
-        std::stable_sort(merged.begin(), merged.end(), [](const Item& u, const Item& v) { return u.steps > v.steps; });
-        b->descs[sub].alloc(n);
-        HIP_CHECK(hipMemcpy(b->descs[sub].p, d.data(), n * sizeof(ReconDesc), hipMemcpyHostToDevice));
-        b->reduce[sub].alloc(n);
-        HIP_CHECK(hipMemcpy(b->reduce[sub].p, rd.data(), n * sizeof(ReduceDesc), hipMemcpyHostToDevice));
-        b->reduce_blocks[sub] = rblocks;
-        for (int t = 0; t < 2; ++t) {
-          std::vector<int4> w;
-          for (const Item& x : merged)
-            if (x.tof == (t == 1)) w.push_back(make_int4(x.tile, x.a, x.e, x.r));
-          b->items[t][sub].alloc(std::max<size_t>(w.size(), 1));
-          if (!w.empty())
-            HIP_CHECK(hipMemcpy(b->items[t][sub].p, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice));
-          b->items[t][sub].n = w.size();
-        }
-      }
+      for (size_t sub = 0; sub < n_subsets; ++sub) batch_build_subset(b, sub, (uint64_t)opt.item_steps);
+      if (tune) tune_batch(b);
       HIP_CHECK(hipStreamSynchronize(b->stream));
     } catch (...) {
       petalo_batch_destroy(b);
@@ -1436,6 +1623,18 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
     *out = b;
     return PETALO_OK;
   });
+}
+
+int petalo_batch_work_items(petalo_batch_t* b, uint64_t* item_steps, uint64_t* n_items_total) {
+  if (!b) return fail(PETALO_ERR_ARG, "batch is NULL");
+  if (item_steps) *item_steps = b->item_target;
+  if (n_items_total) {
+    uint64_t t = 0;
+    for (int k = 0; k < 2; ++k)
+      for (const auto& w : b->items[k]) t += w.n;
+    *n_items_total = t;
+  }
+  return PETALO_OK;
 }
 
 int petalo_batch_next(petalo_batch_t* b, petalo_osem_t* id_out) {

@@ -112,6 +112,7 @@ pub struct petalo_mlem_options_t {
     pub tile: [i32; 3],
     pub autotune: i32,
     pub precision: i32,
+    pub item_steps: i32,
 }
 
 /// Opaque MLEM/OSEM context.
@@ -274,6 +275,11 @@ extern "C" {
     ) -> c_int;
     pub fn petalo_mlem_info(ctx: *mut petalo_mlem_t, scheme: *mut i32, n_runs_total: *mut u64) -> c_int;
     pub fn petalo_mlem_tile_shape(ctx: *mut petalo_mlem_t, tile_out: *mut i32) -> c_int;
+    pub fn petalo_mlem_work_items(
+        ctx: *mut petalo_mlem_t,
+        item_steps_out: *mut u64,
+        n_items_total_out: *mut u64,
+    ) -> c_int;
     pub fn petalo_mlem_destroy(ctx: *mut petalo_mlem_t) -> c_int;
 
     pub fn petalo_batch_create(
@@ -288,6 +294,11 @@ extern "C" {
         options: *const petalo_mlem_options_t,
     ) -> c_int;
     pub fn petalo_batch_next(batch: *mut petalo_batch_t, id_out_or_null: *mut petalo_osem_t) -> c_int;
+    pub fn petalo_batch_work_items(
+        batch: *mut petalo_batch_t,
+        item_steps_out: *mut u64,
+        n_items_total_out: *mut u64,
+    ) -> c_int;
     pub fn petalo_batch_get_image(batch: *mut petalo_batch_t, r: usize, image_out: *mut f32) -> c_int;
     pub fn petalo_batch_set_options(batch: *mut petalo_batch_t, timing: c_int, use_graphs: c_int) -> c_int;
     pub fn petalo_batch_last_ms(batch: *mut petalo_batch_t, ms: *mut f64) -> c_int;

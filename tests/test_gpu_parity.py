@@ -483,7 +483,9 @@ def test_autotuned_context_matches_oracle(gpu):
     m = P.Mlem(fov, lors, tof=tof, n_subsets=2, autotune=True)
     scheme = m.info()["scheme"]
     shape = m.tile_shape()
-    _say(f"autotune chose scheme {scheme}, tile {shape}")
+    steps, n_items = m.work_items()
+    _say(f"autotune chose scheme {scheme}, tile {shape}, {steps} steps per work item ({n_items} items)")
+    assert steps == 0 or (1 << 14) <= steps <= (1 << 20)
     assert (scheme == P.SCHEME_ATOMIC and shape == (0, 0, 0)) or (scheme == P.SCHEME_TILE and min(shape) > 0)
     o = _run_oracle(full, n, lors, 4, tof=(42.47, 3.0), subsets=2)
     for k in range(4):
@@ -493,6 +495,38 @@ def test_autotuned_context_matches_oracle(gpu):
     d = P.Mlem(fov, lors, tof=tof, n_subsets=2, tile=(16, 16, 0))
     assert d.tile_shape()[:2] == (16, 16) and d.tile_shape()[2] > 0
     d.close()
+
+
+def test_work_item_granularity(gpu):
+    """The tiled scheme's work items re-cut to 2^14 and 2^19 replayed steps
+    each (and the automatic rule) give the same images: the cut changes which
+    workgroup replays a run, not what it adds (binary64 sums in another order,
+    rounded to binary32)."""
+    full, n = [120.0, 120.0, 90.0], [60, 60, 45]
+    fov = P.FOV(full, n)
+    lors = P.synth_phantom_lors(23, 400_000, 200.0)
+    tof = P.Tof(42.47, 3.0)
+    imgs, counts = {}, {}
+    for steps in (0, 1 << 14, 1 << 19):
+        m = P.Mlem(fov, lors, tof=tof, n_subsets=4, item_steps=steps)
+        got, n_items = m.work_items()
+        assert got == steps
+        counts[steps] = n_items
+        for _ in range(3):
+            img, _ = m.step()
+        imgs[steps] = img.data
+        m.close()
+    _say(f"work items per 4 subsets: {counts}")
+    assert counts[1 << 14] > counts[1 << 19]
+    # without a sensitivity image, voxels no LOR of a subset crosses become
+    # 0 and later 0 * inf = NaN, as in the reference: NaN-aware comparisons
+    for steps in (1 << 14, 1 << 19):
+        assert rel_l2_finite(imgs[steps], imgs[0]) <= 1e-6
+    o = _run_oracle(full, n, lors, 1, tof=(42.47, 3.0), subsets=4)
+    live = int(np.count_nonzero(np.isfinite(o[2]) & (o[2] != 0)))
+    _say(f"after 3 subset updates: {live} finite non-zero voxels of {fov.nvox}")
+    assert live >= fov.nvox // 4
+    assert rel_l2_finite(imgs[1 << 14], o[2]) <= 1e-5
 
 
 def test_graph_replays_match_direct_launches(gpu):
