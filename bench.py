@@ -7,13 +7,16 @@ cutoff 3), sensitivity image from 1e8 device-generated random LORs (mu = 0).
 One step = one MLEM iteration (1 subset) over all LORs.
 
 Prints ONE JSON line:
-  value      MLEM iterations/s (inputs resident in HBM before timing)
+  value      MLEM iterations/s (inputs resident in HBM before timing; the
+             updates replayed from their HIP graphs, back to back, after 20
+             warm-up iterations)
   roofline   the dominant kernel (the longer of the tiled forward kernel
              k_fwd_tile and the tiled backprojection k_scatter) -- its
              algorithmic bytes per launch (forward: 32 B per LOR + 4 B per
              pushed element; backprojection: 4 B per pushed element) / its
-             average duration from HIP events on the context's stream,
-             against 8 TB/s; traffic from the committed rocprofv3
+             average duration from HIP events on the context's stream over
+             as many updates again (launched directly, an event around each
+             kernel), against 8 TB/s; traffic from the committed rocprofv3
              FETCH_SIZE/WRITE_SIZE profile of the same workload
   cpu_baseline  the CPU oracle (binary32 restatement of the reference, all
              host threads, 10 000-LOR jobs with private buffers) on a
@@ -140,6 +143,63 @@ def measure_batched(P, steps, warmup, singles=True):
     return out
 
 
+def timed_kernels(P, ctx, n):
+    """Per-kernel device times of n more back-to-back updates: HIP events on
+    the context's stream around every kernel (the updates are then launched
+    directly instead of replayed from their graphs), read in batches of the
+    context's ring of 256 event sets."""
+    ctx.set_timing(True)
+    kern = []
+    for k in range(n):
+        ctx.step(copy_image=False)
+        if (k + 1) % 256 == 0:
+            kern += ctx.drain_kernel_timings(256)
+    P.lib().petalo_device_synchronize(0)
+    kern += ctx.drain_kernel_timings(256)
+    ctx.set_timing(False)
+    assert len(kern) == n
+    return kern
+
+
+def measure_workload(P, name, steps, warmup):
+    """One of the other workloads, timed as the headline is (steady state:
+    `warmup` untimed iterations, then `steps` back to back with HIP-event
+    kernel times), without a sensitivity image: its iterations/s, voxel
+    steps/s and the dominant kernel's roofline fraction."""
+    full, nvx, n_lors, fwhm, tof, _, subsets = WORKLOADS[name]
+    fov = P.FOV([full] * 3, [nvx] * 3)
+    lors = P.synth_phantom_lors(1, n_lors, fwhm)
+    t0 = time.perf_counter()
+    ctx = P.Mlem(fov, lors, tof=P.Tof(*tof) if tof else None, n_subsets=subsets)
+    create_ms = (time.perf_counter() - t0) * 1e3
+    ws = ctx.walk_stats()
+    for _ in range(warmup * subsets):
+        ctx.step(copy_image=False)
+    P.lib().petalo_device_synchronize(0)
+    t0 = time.perf_counter()
+    for _ in range(steps * subsets):
+        ctx.step(copy_image=False)
+    P.lib().petalo_device_synchronize(0)
+    el = time.perf_counter() - t0
+    kern = timed_kernels(P, ctx, steps * subsets)
+    ctx.close()
+    kt = {k: float(np.mean([d[k] for d in kern])) for k in kern[0]}
+    fwd_bytes = 32 * ws["n_lors"] + 4 * ws["n_nz"]
+    sct_bytes = 4 * ws["n_nz"]
+    cands = [("k_fwd_tile", fwd_bytes, kt["forward_tiles"]), ("k_scatter", sct_bytes, kt["scatter"])]
+    dom, dom_bytes, dom_ms = max(cands, key=lambda c: c[2])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    r = {"metric": METRICS[name], "value": steps / el, "unit": "iter/s", "steps": steps, "warmup": warmup,
+         "ms_per_iteration": el / steps * 1e3, "voxel_steps_per_s": ws["n_steps"] * steps / el,
+         "context_create_ms": create_ms, "lors": n_lors, "image": [nvx] * 3, "tof": tof,
+         "kernel_ms": kt,
+         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": dom_bytes, "kernel_ms": dom_ms}}
+    log(f"[bench] {name}: {r['value']:.1f} iter/s ({r['ms_per_iteration']:.3f} ms), {dom} at "
+        f"{achieved / HBM_PEAK_GBS:.3f} of the HBM roofline")
+    return r
+
+
 def run_batched(P, a):
     """The batched workload on its own line."""
     r = measure_batched(P, a.steps, a.warmup, singles=not a.no_cpu_baseline)
@@ -231,6 +291,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-lors", type=int, default=1_000_000)
     ap.add_argument("--no-sensitivity", action="store_true")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="skip the plain 60^3 and precision 200^3 workloads timed beside the headline")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run a plain device copy of this size first (counter calibration)")
     a = ap.parse_args()
@@ -313,22 +375,17 @@ def main():
     info = dict(ctx.info(), tile=list(ctx.tile_shape()), autotune=bool(a.autotune),
                 item_steps=steps_per_item, work_items=n_items)
     log(f"[bench] scheme {info}")
-    # HIP events bracket every kernel of every timed update; the updates run
-    # back to back (their timings are read once, after the timed region,
-    # or every 256 updates, the context's ring of event sets)
-    ctx.set_timing(True)
-    kern = []
+    # the timed region: the updates as a caller runs them, each replayed
+    # from its captured HIP graph, back to back
     P.lib().petalo_device_synchronize(0)
     t0 = time.perf_counter()
-    for k in range(a.steps * subsets):
+    for _ in range(a.steps * subsets):
         ctx.step(copy_image=False)
-        if (k + 1) % 256 == 0:
-            kern += ctx.drain_kernel_timings(256)
     P.lib().petalo_device_synchronize(0)
     elapsed = time.perf_counter() - t0
-    kern += ctx.drain_kernel_timings(256)
-    ctx.set_timing(False)
-    assert len(kern) == a.steps * subsets
+    # then as many updates again with HIP events around every kernel, for
+    # the kernel times and the roofline
+    kern = timed_kernels(P, ctx, a.steps * subsets)
     fwd_ms = [d["forward_tiles"] + d["forward_reduce"] for d in kern]
     sct_ms = [d["scatter"] for d in kern]
     proj_ms = [f + s_ for f, s_ in zip(fwd_ms, sct_ms)]
@@ -423,6 +480,11 @@ def main():
     batched = None
     if not a.no_batched and a.workload == "tof_mlem_120":
         batched = measure_batched(P, 2, 1)
+
+    # the parity-shape workloads, briefly, in the same run
+    others = None
+    if not a.no_other_workloads and a.workload == "tof_mlem_120":
+        others = {name: measure_workload(P, name, a.steps, a.warmup) for name in ("plain_mlem_60", "precision_200")}
 
     cpu = None
     parity = None
@@ -540,6 +602,7 @@ def main():
                        "create_ms": t_create * 1e3, "iter_per_s": 10.0 / (end_to_end_ms * 1e-3)},
         "rows": rows,
         "batched": batched,
+        "other_workloads": others,
         "projection_ms": proj_avg,
         "kernel_ms": kt,
         "forward_ms": float(np.mean(fwd_ms)),
