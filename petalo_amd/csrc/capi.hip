@@ -232,11 +232,7 @@ struct petalo_mlem {
   DevBuf<double> bp64;               // binary64 backprojection
   std::vector<std::vector<uint64_t>> item_steps;  // per subset: replayed steps of each work item
   uint64_t item_target = 0;          // replayed steps per work item; 0: the automatic rule (cut_items)
-  // kept while the autotuner re-cuts the work items: per subset, the first
-  // sorted run of every tile and (tile << 12) | length of every sorted run
-  bool keep_cuts = false;
-  std::vector<std::vector<uint64_t>> cut_starts;
-  std::vector<std::vector<uint32_t>> cut_summary;
+  std::vector<std::vector<uint64_t>> tile_start;  // per subset: first sorted run of every tile (ntiles + 1)
   // binary64 precision modes (PETALO_PRECISION_F64 / _F32_WALK)
   int precision = PETALO_PRECISION_F32;
   DevFov64 dfov64{};
@@ -370,7 +366,86 @@ __global__ void k_place_runs(const Run* __restrict__ in, const uint64_t* __restr
   const Run ru = in[src];
   out[j] = ru;
   pos[src] = lpos[j];
-  summary[j] = ((uint32_t)(sorted_keys[j] >> 32) << 12) | run_len(ru.code);
+  if (summary) summary[j] = ((uint32_t)(sorted_keys[j] >> 32) << 12) | run_len(ru.code);
+}
+
+// Irregular LORs of a subset (count 0xffffffff from the binning count pass):
+// flag[i] = 1 and their count set to 0; counts[n] = 0 for the exclusive scan.
+__global__ void k_flag_irregular(uint32_t* __restrict__ counts, uint32_t n, unsigned char* __restrict__ flag) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == n) counts[n] = 0;
+  if (i >= n) return;
+  const bool irr = counts[i] == 0xffffffffu;
+  flag[i] = irr ? 1 : 0;
+  if (irr) counts[i] = 0;
+}
+
+// *sum += counts[0 .. n), in 64 bits (zeroed by the caller).
+__global__ void k_sum_u32(const uint32_t* __restrict__ counts, uint32_t n, unsigned long long* __restrict__ sum) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  wave_add_u64(sum, i < n ? counts[i] : 0u);
+}
+
+// steps[j] = length of sorted run j (then scanned into a running total).
+__global__ void k_run_steps(const Run* __restrict__ runs, uint64_t n, uint64_t* __restrict__ steps) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) steps[j] = run_len(runs[j].code);
+}
+
+// out[k] = the running total of steps before sorted run at[k] (0 at 0).
+__global__ void k_steps_before(const uint64_t* __restrict__ incl, const uint64_t* __restrict__ at, uint32_t n,
+                               uint64_t* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = at[k] == 0 ? 0ull : incl[at[k] - 1];
+}
+
+// First sorted position of every tile; keys hold the tile from bit tshift
+// (values past ntiles, which a well-formed key never holds, are ignored).
+__global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uint32_t tshift, uint32_t ntiles,
+                              uint64_t* __restrict__ start) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint64_t t = keys[j] >> tshift;
+  if (t < ntiles && (j == 0 || (keys[j - 1] >> tshift) != t)) start[t] = j;
+}
+
+// One tile's share of the work items: its sorted runs [a, b), the running
+// step total before a, its steps, its item count and its first item.
+struct TileCut {
+  uint64_t a, b, base, steps;
+  uint32_t n, first, tile, pad;
+};
+
+// Work item k: tile t's chunk m (k = first + m) ends after the first run j
+// whose steps since the tile's start reach (m + 1)/n of the tile's (binary
+// search over the inclusive running total incl; the last chunk at b); a
+// chunk left empty (two boundaries on one run) has first == end.
+__global__ void k_cut_items(const uint64_t* __restrict__ incl, const TileCut* __restrict__ tc, uint32_t n_tiles,
+                            uint32_t n_items, int4* __restrict__ items, uint64_t* __restrict__ steps) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_items) return;
+  uint32_t lo = 0, hi = n_tiles;  // the tile whose items hold k
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (tc[mid].first <= k) lo = mid; else hi = mid;
+  }
+  const TileCut t = tc[lo];
+  const uint32_t m = k - t.first;
+  auto end_of = [&](uint32_t q) -> uint64_t {  // one past the last run of chunk q
+    if (q + 1 >= t.n) return t.b;
+    const double goal = (double)(q + 1) * (double)t.steps;
+    uint64_t l = t.a, h = t.b - 1;  // the first j in [a, b) with (incl[j] - base) n >= goal
+    while (l < h) {
+      const uint64_t mid = l + (h - l) / 2;
+      if ((double)(incl[mid] - t.base) * (double)t.n >= goal) h = mid; else l = mid + 1;
+    }
+    return l + 1;
+  };
+  const uint64_t e = end_of(m);
+  const uint64_t a = m == 0 ? t.a : end_of(m - 1);
+  const uint64_t b = e > a ? e : a;
+  items[k] = make_int4((int)t.tile, (int)a, (int)b, 0);
+  steps[k] = b > a ? incl[b - 1] - (a == 0 ? 0ull : incl[a - 1]) : 0ull;
 }
 
 __global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
@@ -380,35 +455,6 @@ __global__ void k_iota(uint32_t* __restrict__ x, uint64_t n) {
 
 inline unsigned grid64(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-// Work items of about `target` replayed steps each: every tile's sorted runs
-// cut into that many equal-step chunks, then ordered largest first; each with
-// its number of steps.
-std::vector<std::pair<uint64_t, int4>> tile_items(const std::vector<uint64_t>& starts,
-                                                  const std::vector<uint32_t>& summary, int32_t ntiles,
-                                                  uint64_t target) {
-  std::vector<std::pair<uint64_t, int4>> sized;
-  for (int k = 0; k < ntiles; ++k) {
-    const uint64_t a = starts[k], b = starts[k + 1];
-    if (a == b) continue;
-    uint64_t steps = 0;
-    for (uint64_t j = a; j < b; ++j) steps += summary[j] & kRunMaxSteps;
-    const uint64_t n_items = std::max<uint64_t>(1, (steps + target - 1) / target);
-    uint64_t acc = 0, prev = 0, begin = a, m = 1;
-    for (uint64_t j = a; j < b; ++j) {
-      acc += summary[j] & kRunMaxSteps;
-      if (j + 1 == b || acc * n_items >= m * steps) {
-        sized.push_back({acc - prev, make_int4(k, (int)begin, (int)(j + 1), 0)});
-        prev = acc;
-        begin = j + 1;
-        ++m;
-      }
-    }
-  }
-  std::stable_sort(sized.begin(), sized.end(),
-                   [](const std::pair<uint64_t, int4>& u, const std::pair<uint64_t, int4>& v) { return u.first > v.first; });
-  return sized;
-}
-
 void zero_counters(unsigned long long* p, hipStream_t s);
 
 // Work items of one subset: chunks of one tile's runs holding about equal
@@ -417,28 +463,75 @@ void zero_counters(unsigned long long* p, hipStream_t s);
 // many steps (and at least kMinItemSteps); automatically, kWorkItems in
 // total, and, since each item loads (forward) and flushes (scatter) its
 // whole tile, at least kItemStepsPerVoxel steps per LDS entry of the tile.
-void cut_items(petalo_mlem* c, size_t sub, const std::vector<uint64_t>& starts, const std::vector<uint32_t>& summary) {
+// Cut on the device from a running total of the sorted runs' lengths
+// (k_cut_items); the host orders the few thousand items.
+void cut_items(petalo_mlem* c, size_t sub) {
   const TileGeom& g = c->tiles;
-  uint64_t total_steps = 0;
-  for (uint32_t v : summary) total_steps += v & kRunMaxSteps;
-  const uint64_t target =
-      c->item_target ? std::max<uint64_t>(kMinItemSteps, c->item_target)
-                     : std::max<uint64_t>(std::max<uint64_t>(kMinItemSteps, total_steps / kWorkItems + 1),
-                                          kItemStepsPerVoxel * (uint64_t)g.tvox);
-  const std::vector<std::pair<uint64_t, int4>> sized = tile_items(starts, summary, g.ntiles, target);
-  std::vector<int4> items;
-  items.reserve(sized.size());
+  hipStream_t st = c->stream;
+  const uint64_t total = c->n_runs[sub];
+  const std::vector<uint64_t>& starts = c->tile_start[sub];
   c->item_steps[sub].clear();
-  for (auto& it : sized) {
-    items.push_back(it.second);
-    c->item_steps[sub].push_back(it.first);
+  std::vector<int4> items;
+  if (total) {
+    DevBuf<uint64_t> incl(total);
+    k_run_steps<<<grid64(total), kBlock, 0, st>>>(c->runs[sub].p, total, incl.p);
+    HIP_CHECK(hipGetLastError());
+    size_t tb = 0;
+    HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tb, incl.p, incl.p, (int)total, st));
+    DevBuf<unsigned char> tmp(std::max<size_t>(tb, 1));
+    HIP_CHECK(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, incl.p, incl.p, (int)total, st));
+    // running totals at the tile starts
+    const size_t nb = (size_t)g.ntiles + 1;
+    DevBuf<uint64_t> at(nb), before(nb);
+    HIP_CHECK(hipMemcpyAsync(at.p, starts.data(), nb * 8, hipMemcpyHostToDevice, st));
+    k_steps_before<<<grid_for(nb), kBlock, 0, st>>>(incl.p, at.p, (uint32_t)nb, before.p);
+    HIP_CHECK(hipGetLastError());
+    std::vector<uint64_t> bs(nb);
+    HIP_CHECK(hipMemcpyAsync(bs.data(), before.p, nb * 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    const uint64_t total_steps = bs[g.ntiles];
+    const uint64_t target =
+        c->item_target ? std::max<uint64_t>(kMinItemSteps, c->item_target)
+                       : std::max<uint64_t>(std::max<uint64_t>(kMinItemSteps, total_steps / kWorkItems + 1),
+                                            kItemStepsPerVoxel * (uint64_t)g.tvox);
+    std::vector<TileCut> tc;
+    uint32_t n_items = 0;
+    for (int32_t t = 0; t < g.ntiles; ++t) {
+      if (starts[t] == starts[t + 1]) continue;
+      const uint64_t steps = bs[t + 1] - bs[t];
+      const uint32_t n = (uint32_t)std::max<uint64_t>(1, (steps + target - 1) / target);
+      tc.push_back(TileCut{starts[t], starts[t + 1], bs[t], steps, n, n_items, (uint32_t)t, 0});
+      n_items += n;
+    }
+    if (n_items) {
+      DevBuf<TileCut> dtc(tc.size());
+      DevBuf<int4> dit(n_items);
+      DevBuf<uint64_t> dst(n_items);
+      HIP_CHECK(hipMemcpyAsync(dtc.p, tc.data(), tc.size() * sizeof(TileCut), hipMemcpyHostToDevice, st));
+      k_cut_items<<<grid_for(n_items), kBlock, 0, st>>>(incl.p, dtc.p, (uint32_t)tc.size(), n_items, dit.p, dst.p);
+      HIP_CHECK(hipGetLastError());
+      std::vector<int4> hit(n_items);
+      std::vector<uint64_t> hst(n_items);
+      HIP_CHECK(hipMemcpyAsync(hit.data(), dit.p, n_items * sizeof(int4), hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(hst.data(), dst.p, n_items * 8, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      std::vector<std::pair<uint64_t, int4>> sized;
+      for (uint32_t k = 0; k < n_items; ++k)
+        if (hit[k].z > hit[k].y) sized.push_back({hst[k], hit[k]});
+      std::stable_sort(sized.begin(), sized.end(), [](const std::pair<uint64_t, int4>& u,
+                                                      const std::pair<uint64_t, int4>& v) { return u.first > v.first; });
+      items.reserve(sized.size());
+      for (auto& it : sized) {
+        items.push_back(it.second);
+        c->item_steps[sub].push_back(it.first);
+      }
+    }
   }
   c->work[sub].alloc(std::max<size_t>(items.size(), 1));
   if (!items.empty())
-    HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice,
-                             c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->work[sub].p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, st));
   c->work[sub].n = items.size();
-  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipStreamSynchronize(st));
 }
 
 // Cuts the subset's LOR walks into per-tile runs, sorted by tile, and the
@@ -447,46 +540,55 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   const TileGeom& g = c->tiles;
   hipStream_t st = c->stream;
   uint64_t total = 0;
-  std::vector<uint32_t> irr;
-  std::vector<uint32_t> off(count + 1, 0);
+  size_t n_irr = 0;
+  c->run_off[sub].alloc(count + 1);
+  c->irregular[sub].alloc(std::max<size_t>(count, 1));
   if (count) {
-    DevBuf<uint32_t> nr(count);
+    // count pass; LORs with unbounded walks listed (in order) and given no
+    // runs; the run offsets scanned on the device
+    DevBuf<unsigned char> flag(count);
+    DevBuf<unsigned long long> sum(1);
+    DevBuf<uint32_t> n_sel(1);
     const Lor* L = c->lors.p + lo;
+    uint32_t* nr = c->run_off[sub].p;  // counts, then scanned in place
     if (c->tof_on)
-      k_bin_runs<true, false><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
+      k_bin_runs<true, false><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr,
                                                                   nullptr, nullptr, nullptr, 0u);
     else
-      k_bin_runs<false, false><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr.p,
+      k_bin_runs<false, false><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr,
                                                                    nullptr, nullptr, nullptr, 0u);
+    k_flag_irregular<<<grid_for(count + 1), kBlock, 0, st>>>(nr, (uint32_t)count, flag.p);
+    HIP_CHECK(hipMemsetAsync(sum.p, 0, 8, st));
+    k_sum_u32<<<grid_for(count), kBlock, 0, st>>>(nr, (uint32_t)count, sum.p);
     HIP_CHECK(hipGetLastError());
-    std::vector<uint32_t> h(count);
-    HIP_CHECK(hipMemcpyAsync(h.data(), nr.p, count * 4, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    for (size_t i = 0; i < count; ++i) {
-      off[i] = (uint32_t)total;
-      if (h[i] == 0xffffffffu) {
-        irr.push_back((uint32_t)i);
-      } else {
-        total += h[i];
-        if (total >= (1ull << 31)) throw std::runtime_error("2^31 or more tile runs in one subset");
-      }
-    }
-    off[count] = (uint32_t)total;
+    size_t sel_bytes = 0, scan_bytes = 0;
+    hipcub::CountingInputIterator<uint32_t> idx(0);
+    HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, idx, flag.p, c->irregular[sub].p, n_sel.p,
+                                            (int)count, st));
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, nr, nr, (int)count + 1, st));
+    DevBuf<unsigned char> tmp(std::max<size_t>(std::max(sel_bytes, scan_bytes), 1));
+    HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp.p, sel_bytes, idx, flag.p, c->irregular[sub].p, n_sel.p,
+                                            (int)count, st));
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan_bytes, nr, nr, (int)count + 1, st));
+    unsigned long long h_sum = 0;
+    uint32_t h_sel = 0;
+    HIP_CHECK(hipMemcpyAsync(&h_sum, sum.p, 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(&h_sel, n_sel.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));  // (also before the temporaries are freed)
+    if (h_sum >= (1ull << 31)) throw std::runtime_error("2^31 or more tile runs in one subset");
+    total = h_sum;
+    n_irr = h_sel;
+  } else {
+    HIP_CHECK(hipMemsetAsync(c->run_off[sub].p, 0, 4, st));
   }
-  c->run_off[sub].alloc(count + 1);
+  c->irregular[sub].n = n_irr;
   c->n_blocks[sub] = (uint32_t)((count + (1u << kWinShift) - 1) >> kWinShift);
   c->win[sub].alloc(c->n_blocks[sub] + 1);
   c->max_win[sub] = 0;
   HIP_CHECK(hipMemsetAsync(c->win[sub].p, 0, ((size_t)c->n_blocks[sub] + 1) * 4, st));  // empty windows
-  HIP_CHECK(hipMemcpyAsync(c->run_off[sub].p, off.data(), (count + 1) * 4, hipMemcpyHostToDevice, st));
-  c->irregular[sub].alloc(std::max<size_t>(irr.size(), 1));
-  if (!irr.empty())
-    HIP_CHECK(hipMemcpyAsync(c->irregular[sub].p, irr.data(), irr.size() * 4, hipMemcpyHostToDevice, st));
-  c->irregular[sub].n = irr.size();
   c->runs[sub].alloc(std::max<uint64_t>(total, 1));
   c->run_pos[sub].alloc(std::max<uint64_t>(total, 1));
   std::vector<uint64_t> starts(g.ntiles + 1, UINT64_MAX);
-  std::vector<uint32_t> summary(total);  // per sorted run: (tile << 12) | length
   if (total) {
     const Lor* L = c->lors.p + lo;
     DevBuf<Run> unsorted(total);
@@ -558,16 +660,15 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     for (size_t b = 0; b < c->n_blocks[sub]; ++b) mw = std::max(mw, hw[b + 1] - hw[b]);
     c->max_win[sub] = mw;
     HIP_CHECK(hipMemcpyAsync(c->win[sub].p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, st));
-    DevBuf<uint32_t> dsum(total);
     k_place_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, keys2.p, perm2.p, total, c->runs[sub].p,
-                                                   c->run_pos[sub].p, dsum.p, c->lpos[sub].p);
+                                                   c->run_pos[sub].p, nullptr, c->lpos[sub].p);
+    // first sorted run of every tile (UINT64_MAX where a tile has none)
+    DevBuf<uint64_t> dstart(g.ntiles);
+    HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, (size_t)g.ntiles * 8, st));
+    k_tile_starts<<<grid64(total), kBlock, 0, st>>>(keys2.p, total, 32u, (uint32_t)g.ntiles, dstart.p);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(summary.data(), dsum.p, total * 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, (size_t)g.ntiles * 8, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    for (uint64_t j = total; j-- > 0;) {
-      if ((summary[j] >> 12) >= (uint32_t)g.ntiles) throw std::runtime_error("run sort key without a valid tile");
-      starts[summary[j] >> 12] = j;
-    }
   }
   starts[g.ntiles] = total;
   for (int k = g.ntiles - 1; k >= 0; --k)
@@ -588,12 +689,9 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
                                                       c->pos4[sub].p);
   }
   HIP_CHECK(hipGetLastError());
-  cut_items(c, sub, starts, summary);
   c->n_runs[sub] = total;
-  if (c->keep_cuts) {
-    c->cut_starts[sub] = std::move(starts);
-    c->cut_summary[sub] = std::move(summary);
-  }
+  c->tile_start[sub] = std::move(starts);
+  cut_items(c, sub);
 }
 
 // 1/(p*ac) per LOR from the partials of the forward kernel, then the
@@ -812,12 +910,11 @@ std::string check_mlem_args(const petalo_fov_t* fov, const petalo_tof_t* tof, co
 // (a batch's), else its own.  Throws on failure.
 petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, const petalo_lor_t* lors,
                             size_t n_lors, const float* sensitivity, size_t n_subsets,
-                            const petalo_mlem_options_t& opt, hipStream_t shared, bool keep_cuts = false) {
+                            const petalo_mlem_options_t& opt, hipStream_t shared) {
     auto* c = new petalo_mlem();
     try {
       c->device = opt.device;
       c->item_target = (uint64_t)opt.item_steps;
-      c->keep_cuts = keep_cuts;
       c->fov = *fov;
       c->dfov = to_dev(*fov);
       c->dtof = to_dev(tof);
@@ -880,10 +977,7 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->max_win.resize(n_subsets, 0);
         c->irregular.resize(n_subsets);
         c->n_runs.assign(n_subsets, 0);
-        if (keep_cuts) {
-          c->cut_starts.resize(n_subsets);
-          c->cut_summary.resize(n_subsets);
-        }
+        c->tile_start.resize(n_subsets);
         c->rbuf.alloc(std::max<size_t>(c->set_size, 1));
         c->ac.alloc(std::max<size_t>(n_lors, 1));
         if (n_lors) {
@@ -989,23 +1083,23 @@ petalo_mlem_options_t autotune(const petalo_fov_t* fov, const petalo_tof_t* tof,
 }
 
 // The autotuner's second dimension: the work granularity of a tiled context,
-// timed on the context itself (created with keep_cuts).  Candidates: the
+// timed on the context itself.  Candidates: the
 // automatic rule of cut_items and targets of 2^14 .. 2^20 replayed steps per
 // work item, those that cut the first subset into a number of items no
-// earlier candidate gave; each re-cuts the items of the timed subsets (the
+// earlier candidate gave; each re-cuts (cut_items) the items of the timed subsets (the
 // first min(n_subsets, 4)), runs them once untimed and three times timed
 // (the best counts), and the automatic rule stays unless another is at least
 // kTuneMargin faster.  The
 // items of every subset are then cut with the choice, the image, counters
-// and subset position reset, and the run summaries released.
+// and subset position reset.
 void tune_items(petalo_mlem* c) {
-  if (c->scheme != PETALO_SCHEME_TILE || c->cut_summary.empty()) return;
+  if (c->scheme != PETALO_SCHEME_TILE) return;
   const size_t nt = std::min<size_t>(c->n_subsets, 4);
   const bool graphs = c->use_graphs;
   c->use_graphs = false;
   auto recut = [&](uint64_t target, size_t upto) {
     c->item_target = target;
-    for (size_t sub = 0; sub < upto; ++sub) cut_items(c, sub, c->cut_starts[sub], c->cut_summary[sub]);
+    for (size_t sub = 0; sub < upto; ++sub) cut_items(c, sub);
   };
   auto updates = [&]() {
     c->osem = petalo_osem_t{c->n_subsets, 1, 1};
@@ -1043,9 +1137,6 @@ void tune_items(petalo_mlem* c) {
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->osem = petalo_osem_t{c->n_subsets, 1, 1};
   c->use_graphs = graphs;
-  c->keep_cuts = false;
-  std::vector<std::vector<uint64_t>>().swap(c->cut_starts);
-  std::vector<std::vector<uint32_t>>().swap(c->cut_summary);
 }
 
 }  // namespace
@@ -1066,8 +1157,7 @@ int petalo_mlem_create(petalo_mlem_t** out, const petalo_fov_t* fov, const petal
     HIP_CHECK(hipSetDevice(opt.device));
     const bool tune = opt.scheme == PETALO_SCHEME_AUTO && opt.autotune;
     if (tune) opt = autotune(fov, tof, lors, n_lors, sensitivity, n_subsets, opt);
-    petalo_mlem* c = create_context(fov, tof, lors, n_lors, sensitivity, n_subsets, opt, nullptr,
-                                    tune && opt.item_steps == 0);
+    petalo_mlem* c = create_context(fov, tof, lors, n_lors, sensitivity, n_subsets, opt, nullptr);
     try {
       if (tune && opt.item_steps == 0) tune_items(c);
     } catch (...) {
@@ -1486,8 +1576,7 @@ void batch_build_subset(petalo_batch* b, size_t sub, uint64_t item_target) {
 
 // The batch's work granularity, tuned as tune_items does for one context:
 // candidates the automatic rule and 2^14 .. 2^20 replayed steps per item,
-// applied to every reconstruction's cut (from their kept run summaries) and
-// to the merge; the first min(n_subsets, 4) subset updates of the whole
+// applied to every reconstruction's cut and to the merge; the first min(n_subsets, 4) subset updates of the whole
 // batch run once untimed and three times timed per candidate (the best
 // counts); the automatic rule
 // stays unless another is at least kTuneMargin faster.  Then every subset is
@@ -1499,7 +1588,7 @@ void tune_batch(petalo_batch* b) {
   auto recut = [&](uint64_t target, size_t upto) {
     for (petalo_mlem* c : b->ctx) {
       c->item_target = target;
-      for (size_t sub = 0; sub < upto; ++sub) cut_items(c, sub, c->cut_starts[sub], c->cut_summary[sub]);
+      for (size_t sub = 0; sub < upto; ++sub) cut_items(c, sub);
     }
     for (size_t sub = 0; sub < upto; ++sub) batch_build_subset(b, sub, target);
     b->item_target = target;
@@ -1542,9 +1631,6 @@ void tune_batch(petalo_batch* b) {
     fill_f32(c->image.p, 1.0f, nv, b->stream);  // Image::ones (mlem.rs:17)
     fill_f64(c->bp64.p, 0.0, nv, b->stream);
     zero_counters(c->counters.p, b->stream);
-    c->keep_cuts = false;
-    std::vector<std::vector<uint64_t>>().swap(c->cut_starts);
-    std::vector<std::vector<uint32_t>>().swap(c->cut_summary);
   }
   HIP_CHECK(hipStreamSynchronize(b->stream));
   reset_position();
@@ -1571,7 +1657,7 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
     if (opt.precision != PETALO_PRECISION_F32) return fail(PETALO_ERR_ARG, "batches run the binary32 path");
     // autotune: the tile shapes stay as given or defaulted; the work granularity is tuned
     const bool tune = opt.autotune != 0 && opt.item_steps == 0;
-    opt.autotune = 0;
+    opt.autotune = 0;  // the contexts themselves are not tuned
     opt.scheme = PETALO_SCHEME_TILE;
     for (size_t r = 0; r < n; ++r) {
       const std::string why = check_mlem_args(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],
@@ -1590,7 +1676,7 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
       for (auto& e : b->ev) HIP_CHECK(hipEventCreate(&e));
       for (size_t r = 0; r < n; ++r)
         b->ctx.push_back(create_context(&fovs[r], tofs ? &tofs[r] : nullptr, lors[r], n_lors[r],
-                                        sens ? sens[r] : nullptr, n_subsets, opt, b->stream, tune));
+                                        sens ? sens[r] : nullptr, n_subsets, opt, b->stream));
       for (petalo_mlem* c : b->ctx) {
         b->lds_f32 = std::max(b->lds_f32, ((size_t)c->tiles.tvox + 1) * sizeof(float));
         b->lds_f64 = std::max(b->lds_f64, (size_t)c->tiles.tvox * sizeof(double));
@@ -1865,15 +1951,6 @@ __global__ void k_take_irregular(uint32_t* __restrict__ counts, uint32_t n, cons
 }
 
 
-// First sorted position of every tile; keys hold the tile from bit tshift
-// (values past ntiles, which a well-formed key never holds, are ignored).
-__global__ void k_tile_starts(const uint64_t* __restrict__ keys, uint64_t n, uint32_t tshift, uint32_t ntiles,
-                              uint64_t* __restrict__ start) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const uint64_t t = keys[j] >> tshift;
-  if (t < ntiles && (j == 0 || (keys[j - 1] >> tshift) != t)) start[t] = j;
-}
 
 // Tile counts up to which a sensitivity batch is binned straight into tile
 // order (k_bin_runs_seg keeps two cursors and the bounds of each tile's
