@@ -374,6 +374,9 @@ def main():
     steps_per_item, n_items = ctx.work_items()
     info = dict(ctx.info(), tile=list(ctx.tile_shape()), autotune=bool(a.autotune),
                 item_steps=steps_per_item, work_items=n_items)
+    unit, r_max, r_gm = ctx.accumulator()
+    info["accumulator"] = {"kind": "fixed64" if unit > 0 else "binary64", "unit": unit, "r_max": r_max,
+                           "r_geomean": r_gm}
     log(f"[bench] scheme {info}")
     # the timed region: the updates as a caller runs them, each replayed
     # from its captured HIP graph, back to back

@@ -105,7 +105,7 @@ typedef struct {
   int32_t tile[3];       /* tiled scheme: tile shape in voxels (x, y, z); a
                             zero takes the default (x, y: about 40, the axis
                             split evenly; z: as many slices as fit 10240 LDS
-                            entries).  At most 20480 LDS entries per tile,
+                            entries).  At most 20416 LDS entries per tile,
                             rows padded to an odd pitch.                      */
   int32_t autotune;      /* with scheme AUTO: 1 times the default tile, the
                             atomic scheme and tiles of about 24, 32, 48 and
@@ -171,6 +171,14 @@ int petalo_mlem_set_image(petalo_mlem_t* ctx, const float* image);
 int petalo_mlem_walk_stats(petalo_mlem_t* ctx, size_t subset, petalo_stats_t* out);
 /* LORs skipped by the index check during the last update. */
 int petalo_mlem_last_skipped(petalo_mlem_t* ctx, uint64_t* skipped);
+/* The last tiled update's backprojection accumulator: the value of one unit
+ * of its first work item's 64-bit fixed-point tile sums, or 0 where it summed
+ * in binary64 (some r = 1/(p*ac) negative or not finite, or the largest r too
+ * far above the geometric mean of the positive r for the item's unit), with
+ * the update's largest r and that geometric mean (either pointer may be
+ * NULL; NaN where no LOR had a positive r).  No counterpart in the reference
+ * (which sums in binary32); for tests and diagnostics. */
+int petalo_mlem_accumulator(petalo_mlem_t* ctx, double* unit_out, double* r_max_out, double* r_geomean_out);
 /* Per-kernel device time of the last update in ms, measured with HIP events
  * on the context's stream when timing is on. */
 int petalo_mlem_set_timing(petalo_mlem_t* ctx, int on);

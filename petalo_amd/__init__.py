@@ -171,6 +171,7 @@ _SIGS = {
     "petalo_mlem_set_image": (C.c_int, [_P, _P]),
     "petalo_mlem_walk_stats": (C.c_int, [_P, C.c_size_t, _P]),
     "petalo_mlem_last_skipped": (C.c_int, [_P, _P]),
+    "petalo_mlem_accumulator": (C.c_int, [_P, _P, _P, _P]),
     "petalo_mlem_set_timing": (C.c_int, [_P, C.c_int]),
     "petalo_mlem_last_timing": (C.c_int, [_P, _P, _P]),
     "petalo_mlem_last_phase_timing": (C.c_int, [_P, _P, _P, _P]),
@@ -425,6 +426,18 @@ class Mlem:
         v = C.c_uint64(0)
         _check(lib().petalo_mlem_last_skipped(self._ctx, C.byref(v)))
         return int(v.value)
+
+    def accumulator(self):
+        """The last tiled update's backprojection accumulator: (fixed-point
+        unit of its first work item, 0.0 for binary64 tile sums; largest r;
+        geometric mean of the positive r)."""
+        u, m, g = C.c_double(0.0), C.c_double(0.0), C.c_double(0.0)
+        _check(lib().petalo_mlem_accumulator(self._ctx, C.byref(u), C.byref(m), C.byref(g)))
+        return float(u.value), float(m.value), float(g.value)
+
+    def accumulator_unit(self):
+        """The last tiled update's fixed-point unit (0.0: binary64 tile sums)."""
+        return self.accumulator()[0]
 
     def set_timing(self, on=True):
         _check(lib().petalo_mlem_set_timing(self._ctx, 1 if on else 0))

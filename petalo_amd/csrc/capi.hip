@@ -69,7 +69,9 @@ constexpr uint32_t kReduceWindow = 20480;
 constexpr size_t kTimingRing = 256;
 // Largest tile (LDS entries, row padding included): its binary64 copy fills
 // the 160 KiB of LDS of a CU.
-constexpr int32_t kMaxTileVoxels = 20480;
+// LDS entries of the largest tile: binary64 tiles fill the 160 KB of LDS but
+// for 512 bytes that the tile kernels' own shared variables may take.
+constexpr int32_t kMaxTileVoxels = 20416;
 // Default tile budget: two workgroups' binary64 tiles per CU.
 constexpr int32_t kScatterEntries = 10240;
 
@@ -201,6 +203,7 @@ struct petalo_mlem {
   DevBuf<Lor> lors;
   DevBuf<float> image, bp, sens;
   DevBuf<unsigned long long> counters;
+  DevBuf<unsigned long long> rstat;  // r statistics of the tiled update (kRStatWords)
   petalo_osem_t osem{};
   bool timing = false;
   // timing: per timed update, events at its start, forward end, scatter end,
@@ -314,7 +317,7 @@ TileGeom tile_shape(const petalo_fov_t& fov, const int32_t shape[3]) {
 TileGeom make_tiles(const petalo_fov_t& fov, const int32_t shape[3]) {
   const TileGeom g = tile_shape(fov, shape);
   if (g.ntiles >= 65536) throw std::runtime_error("image too large for the tiled scheme (65536 tiles or more)");
-  if (g.tvox > kMaxTileVoxels) throw std::runtime_error("tile over 20480 LDS entries (row pitch included)");
+  if (g.tvox > kMaxTileVoxels) throw std::runtime_error("tile over 20416 LDS entries (row pitch included)");
   if (g.tx + g.ty + g.tz > (int32_t)kRunMaxSteps) throw std::runtime_error("tile too long for the run record");
   return g;
 }
@@ -704,11 +707,11 @@ void launch_reduce(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   if (c->pos2[sub].p)
     k_lor_reduce_win<false, true><<<c->n_blocks[sub], kWinThreads, (size_t)lds * sizeof(float), st>>>(
         c->ac.p + lo, (uint32_t)count, c->win[sub].p, lds, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
-        c->pos2[sub].p, c->partial.p, c->rbuf.p);
+        c->pos2[sub].p, c->partial.p, c->rbuf.p, c->rstat.p);
   else
     k_lor_reduce_win<false, false><<<c->n_blocks[sub], kWinThreads, (size_t)lds * sizeof(float), st>>>(
         c->ac.p + lo, (uint32_t)count, c->win[sub].p, lds, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
-        c->pos2[sub].p, c->partial.p, c->rbuf.p);
+        c->pos2[sub].p, c->partial.p, c->rbuf.p, c->rstat.p);
   HIP_CHECK(hipGetLastError());
   const size_t n_irr = c->irregular[sub].n;
   if (n_irr) {
@@ -726,6 +729,7 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count, bool timi
   if (count == 0) return;
   hipStream_t st = c->stream;
   const size_t nw = c->work[sub].n;
+  k_zero_u64<<<1, 320, 0, st>>>(c->rstat.p, kRStatWords);  // the reduction's r statistics
   if (nw) {
     const size_t lds = ((size_t)c->tiles.tvox + 1) * sizeof(float);
     if (c->tof_on)
@@ -745,12 +749,13 @@ void launch_tiled(petalo_mlem* c, size_t sub, size_t lo, size_t count, bool timi
     const size_t lds = (size_t)c->tiles.tvox * sizeof(double);
     const int4* sw = c->work[sub].p;
     const size_t nws = nw;
+    unsigned long long* rs = c->rstat.p;
     if (c->tof_on)
       k_scatter<true><<<(unsigned)nws, 1024, lds, st>>>(c->runs[sub].p, sw, c->rbuf.p, c->dfov, c->dtof,
-                                                       c->tiles, c->bp64.p);
+                                                       c->tiles, c->bp64.p, nullptr, rs);
     else
       k_scatter<false><<<(unsigned)nws, 1024, lds, st>>>(c->runs[sub].p, sw, c->rbuf.p, c->dfov,
-                                                        c->dtof, c->tiles, c->bp64.p);
+                                                        c->dtof, c->tiles, c->bp64.p, nullptr, rs);
     HIP_CHECK(hipGetLastError());
   }
 }
@@ -942,6 +947,8 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
       c->image.alloc(nv);
       c->bp.alloc(nv);
       c->counters.alloc(CNT_N);
+      c->rstat.alloc(kRStatWords);
+      k_zero_u64<<<1, 320, 0, c->stream>>>(c->rstat.p, kRStatWords);
       fill_f32(c->image.p, 1.0f, nv, c->stream);  // Image::ones (mlem.rs:17)
       fill_f32(c->bp.p, 0.0f, nv, c->stream);
       zero_counters(c->counters.p, c->stream);
@@ -1280,6 +1287,38 @@ int petalo_mlem_last_skipped(petalo_mlem_t* c, uint64_t* skipped) {
   });
 }
 
+int petalo_mlem_accumulator(petalo_mlem_t* c, double* unit, double* r_max, double* r_geomean) {
+  return guarded([&] {
+    if (!c || !unit) return fail(PETALO_ERR_ARG, "NULL argument");
+    *unit = 0.0;
+    if (r_max) *r_max = NAN;
+    if (r_geomean) *r_geomean = NAN;
+    if (!c->rstat.p) return PETALO_OK;  // not a tiled context
+    HIP_CHECK(hipSetDevice(c->device));
+    std::vector<unsigned long long> h(kRStatWords);
+    HIP_CHECK(hipMemcpyAsync(h.data(), c->rstat.p, kRStatWords * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    std::memcpy(unit, &h[kRUnitWord], 8);
+    float m = 0.0f;
+    double lsum = 0.0, n = 0.0;
+    for (int k = 0; k < kRSlots; ++k) {
+      uint32_t mb = (uint32_t)h[4 * k];
+      float mk;
+      std::memcpy(&mk, &mb, 4);
+      m = std::max(m, mk);
+      double lk;
+      std::memcpy(&lk, &h[4 * k + 2], 8);
+      lsum += lk;
+      n += (double)h[4 * k + 3];
+    }
+    if (n > 0) {
+      if (r_max) *r_max = m;
+      if (r_geomean) *r_geomean = std::exp2(lsum / n);
+    }
+    return PETALO_OK;
+  });
+}
+
 int petalo_mlem_set_timing(petalo_mlem_t* c, int on) {
   return guarded([&] {
     if (!c) return fail(PETALO_ERR_ARG, "context is NULL");
@@ -1525,11 +1564,12 @@ void batch_build_subset(petalo_batch* b, size_t sub, uint64_t item_target) {
   uint64_t total_steps = 0;
   for (size_t r = 0; r < n; ++r) {
     petalo_mlem* c = b->ctx[r];
-    d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->lpos[sub].p,
-                     c->dfov, c->dtof, c->tiles};
     const uint32_t m = (uint32_t)c->set_size;
+    d[r] = ReconDesc{c->runs[sub].p, c->image.p, c->partial.p, c->rbuf.p, c->bp64.p, c->lpos[sub].p,
+                     c->rstat.p, c->dfov, c->dtof, c->tiles};
     rd[r] = ReduceDesc{c->ac.p + sub * c->set_size, c->run_off[sub].p, c->run_pos[sub].p, c->pos4[sub].p,
-                       c->partial.p, c->rbuf.p, c->win[sub].p, c->pos2[sub].p, m, rblocks};
+                       c->partial.p, c->rbuf.p, c->win[sub].p, c->pos2[sub].p, c->rstat.p, m,
+                       rblocks};
     rblocks += c->n_blocks[sub];
     b->reduce_lds[sub] = std::max(b->reduce_lds[sub], std::min(c->max_win[sub], kReduceWindow));
     std::vector<int4> h(c->work[sub].n);
@@ -1692,7 +1732,7 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
         for (size_t r = 0; r < n; ++r) {
           petalo_mlem* c = b->ctx[r];
           const uint32_t nv = (uint32_t)nvox(c->fov);
-          u[r] = UpdateDesc{c->image.p, c->bp64.p, c->sens.p, c->counters.p, nv, blocks};
+          u[r] = UpdateDesc{c->image.p, c->bp64.p, c->sens.p, c->counters.p, c->rstat.p, nv, blocks};
           blocks += grid_for(nv);
         }
         b->update.alloc(n);
@@ -2091,6 +2131,7 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
       li2(by_length ? bmax : 0);
   DevBuf<float> rbuf;
   DevBuf<uint4> pos4;
+  DevBuf<unsigned long long> rstat(kRStatWords);  // each batch's r statistics
   if (!mu_zero) {
     rbuf.alloc(bmax);
     pos4.alloc(bmax);
@@ -2198,11 +2239,14 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
         k_fwd_tile<false><<<cap, 1024, ((size_t)g.tvox + 1) * sizeof(float), s>>>(un, items, dmu, f, notof, g, part,
                                                                                    order, nullptr);
         k_pack_pos4<<<grid_for(b), kBlock, 0, s>>>(offs.p, ps, b, pos4.p);
-        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, ps, pos4.p, part, rbuf.p);
+        // this batch's r statistics, for the backprojection's accumulator
+        k_zero_u64<<<1, 320, 0, s>>>(rstat.p, kRStatWords);
+        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, ps, pos4.p, part, rbuf.p,
+                                                             rstat.p);
         HIP_CHECK(hipGetLastError());
       }
       k_scatter<false><<<cap, 1024, (size_t)g.tvox * sizeof(double), s>>>(un, items, mu_zero ? nullptr : rbuf.p, f,
-                                                                          notof, g, acc, order);
+                                                                          notof, g, acc, order, rstat.p);
       HIP_CHECK(hipGetLastError());
     }
     if (h_irr) {

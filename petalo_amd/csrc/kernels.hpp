@@ -51,6 +51,81 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
   if ((threadIdx.x & 63) == 0 && s) atomicAdd(dst, s);
 }
 
+// Statistics of a subset update's r = 1/(p*ac) values (those of LORs with
+// tile runs), for the backprojection's choice of accumulator.  Workgroups of
+// the reduction add theirs to one of kRSlots slots of four words (so that no
+// address takes more than a few dozen atomics): the binary32 bits of the
+// largest finite non-negative r (atomic max), non-zero where some r is
+// negative or not finite, the binary64 bits of the sum of log2(r) over the
+// finite positive r, and their count (the geometric mean, unlike the mean, is
+// not carried by a few outliers).  Word kRSlots * 4 records the unit the
+// backprojection's first work item chose (binary64 bits; 0: binary64 sums).
+constexpr int kRSlots = 64;
+constexpr int kRStatWords = kRSlots * 4 + 1;
+constexpr int kRUnitWord = kRSlots * 4;
+
+struct RStats {
+  float rmax = 0.0f;  // largest finite non-negative r
+  bool bad = false;   // some r negative or not finite
+  double lsum = 0.0;  // sum of log2(r) over the finite positive ones
+  uint32_t n = 0;     // their count
+  __device__ __forceinline__ void note(float r) {
+    const bool ok = r >= 0.0f && r <= 3.402823466e38f;
+    bad |= !ok;
+    if (ok) rmax = fmaxf(rmax, r);
+    if (ok && r > 0.0f) {
+      lsum += (double)__log2f(r);
+      ++n;
+    }
+  }
+  // combines the statistics of the wavefront's lanes into every lane's
+  __device__ __forceinline__ void wave_combine() {
+    uint32_t m = __float_as_uint(rmax), b = bad ? 1u : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+      b |= (uint32_t)__shfl_xor((int)b, off, 64);
+      lsum += __shfl_xor(lsum, off, 64);
+      n += (uint32_t)__shfl_xor((int)n, off, 64);
+    }
+    rmax = __uint_as_float(m);
+    bad = b != 0;
+  }
+};
+
+// Adds the workgroup's statistics to slot blockIdx.x % kRSlots of st (at
+// most 16 wavefronts).  Every lane of the workgroup must call it.
+__device__ __forceinline__ void block_r_stats(unsigned long long* st, RStats rs) {
+  __shared__ float s_max[16];
+  __shared__ uint32_t s_bad[16], s_n[16];
+  __shared__ double s_lsum[16];
+  rs.wave_combine();
+  const uint32_t wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_max[wave] = rs.rmax;
+    s_bad[wave] = rs.bad ? 1u : 0u;
+    s_lsum[wave] = rs.lsum;
+    s_n[wave] = rs.n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = 0.0f;
+    uint32_t b = 0, n = 0;
+    double lsum = 0.0;
+    for (uint32_t k = 0; k < waves; ++k) {
+      m = fmaxf(m, s_max[k]);
+      b |= s_bad[k];
+      lsum += s_lsum[k];
+      n += s_n[k];
+    }
+    unsigned long long* slot = st + 4 * (blockIdx.x % kRSlots);
+    if (m > 0.0f) atomicMax(slot, (unsigned long long)__float_as_uint(m));
+    if (b) atomicOr(slot + 1, 1ull);
+    if (lsum != 0.0) unsafeAtomicAdd(reinterpret_cast<double*>(slot + 2), lsum);
+    if (n) atomicAdd(slot + 3, (unsigned long long)n);
+  }
+}
+
 __device__ __forceinline__ Lor load_lor(const Lor* __restrict__ lors, uint32_t i) {
   const float4* p = reinterpret_cast<const float4*>(lors + i);
   const float4 a = p[0];

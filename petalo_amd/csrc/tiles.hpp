@@ -729,7 +729,8 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
 // 2^kShift bytes: the voxel index and its per-axis strides are kept in bytes,
 // so a step's LDS address needs no scaling.
 template <int kShift>
-__host__ __device__ __forceinline__ void run_state(const Run& ru, const TileGeom& g, WalkS& w, WalkC& c) {
+__host__ __device__ __forceinline__ void run_state(const Run& ru, const TileGeom& g, WalkS& w, WalkC& c,
+                                                   uint32_t base = 0) {
   const int32_t dlx = 1 << kShift, dly = g.px << kShift, dlz = (g.px * g.ty) << kShift;
   w.nb_x = ru.nb_x;
   w.nb_y = ru.nb_y;
@@ -743,13 +744,25 @@ __host__ __device__ __forceinline__ void run_state(const Run& ru, const TileGeom
   c.d_x = (flips & 1u) ? -dlx : dlx;
   c.d_y = (flips & 2u) ? -dly : dly;
   c.d_z = (flips & 4u) ? -dlz : dlz;
-  w.index = (int32_t)(run_local(ru.code) << kShift);
+  w.index = (int32_t)(base + (run_local(ru.code) << kShift));
 }
 
-// Replays a run from its recorded state, calling add(byte_offset, w * r)
-// for every pushed step (offsets into a tile of binary64 elements).  The
-// voxel sequence and step lengths are walk #1's bit for bit; the TOF weight
-// is length * tof_shape * (666 * peak), with the constant folded into r once
+// LDS address (byte offset in the workgroup's LDS) of a pointer to shared
+// memory, and LDS pointers made from such addresses: a replay keeps its
+// voxel's LDS address itself, so that each step's access needs no add of the
+// tile's base.
+typedef __attribute__((address_space(3))) float lds_f32;
+typedef __attribute__((address_space(3))) double lds_f64;
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// Replays a run from its recorded state, calling add(lds_address, w * rr)
+// for every pushed step (addresses of elements of 8 bytes), where rr is the
+// run's factor: r times the Gaussian's 666 * peak with TOF, times the
+// accumulator's power-of-two scale (TileAcc).  The voxel sequence and step
+// lengths are walk #1's bit for bit; the TOF weight is
+// length * tof_shape * (666 * peak), with the constant folded into rr once
 // per run (a few ulp from walk #1's product).  Every step of a run lies
 // inside the TOF window, so the window test of step_weight is skipped: the
 // run spans the walk's first to last pushed step in the tile, pushed steps
@@ -759,20 +772,29 @@ __host__ __device__ __forceinline__ void run_state(const Run& ru, const TileGeom
 // not pushed.  GUARD false adds it anyway, which adds +0 when r is finite
 // (the caller's condition), saving the test.
 template <bool TOF, bool GUARD, class Add>
-__host__ __device__ __forceinline__ void replay_run(const Run& ru, float r, const DevTof& t, const TileGeom& g,
-                                                    Add&& add) {
+__host__ __device__ __forceinline__ void replay_run(const Run& ru, float rr, const DevTof& t, const TileGeom& g,
+                                                    uint32_t base, Add&& add) {
   WalkS w;
   WalkC c;
-  run_state<3>(ru, g, w, c);
+  run_state<3>(ru, g, w, c, base);
   const uint32_t m = run_len(ru.code);  // the run length bounds the replay
-  const float rr = TOF ? r * t.amp : r;
-  for (uint32_t k = 0; k < m; ++k) {
-    const Step s = next_boundary(w);
-    float wl = s.bnd - w.here;
+  auto step = [&] {
+    const float bnd = fminf(fminf(w.nb_x, w.nb_y), w.nb_z);
+    float wl = bnd - w.here;
     if (TOF) wl *= tof_shape(t, w.here, c.tof_peak);
     if (!GUARD || wl > 0.0f) add((uint32_t)w.index, wl * rr);
-    move(w, c, s);
+    cross(w, c, bnd);
+  };
+  // four steps per trip, written out: the compiler does not unroll a loop of
+  // a lane-dependent trip count around the masked crossing
+  uint32_t k = 0;
+  for (; k + 4 <= m; k += 4) {
+    step();
+    step();
+    step();
+    step();
   }
+  for (; k < m; ++k) step();
 }
 
 // Forward replay of a run against the LDS tile: p = sum of w * x over the
@@ -788,20 +810,27 @@ __device__ __forceinline__ float replay_project(const Run& ru, const DevTof& t, 
                                                 const float* __restrict__ timg) {
   WalkS w;
   WalkC c;
-  run_state<2>(ru, g, w, c);
-  const char* base = reinterpret_cast<const char*>(timg);
+  run_state<2>(ru, g, w, c, lds_addr(timg));
   const uint32_t m = run_len(ru.code);
   float p = 0.0f, w_prev = 0.0f, x_prev = 0.0f;
-  for (uint32_t k = 0; k < m; ++k) {
-    const Step s = next_boundary(w);
-    float wl = s.bnd - w.here;  // in-window: see replay_run
+  auto step = [&] {
+    const float bnd = fminf(fminf(w.nb_x, w.nb_y), w.nb_z);
+    float wl = bnd - w.here;  // in-window: see replay_run
     if (TOF) wl *= tof_shape(t, w.here, c.tof_peak);
-    const float x = *reinterpret_cast<const float*>(base + w.index);
+    const float x = *(const lds_f32*)(uintptr_t)(uint32_t)w.index;
     p = fmaf(w_prev, (!GUARD || w_prev > 0.0f) ? x_prev : 0.0f, p);
     w_prev = wl;
     x_prev = x;
-    move(w, c, s);
+    cross(w, c, bnd);
+  };
+  uint32_t k = 0;  // unrolled by hand, as in replay_run
+  for (; k + 4 <= m; k += 4) {
+    step();
+    step();
+    step();
+    step();
   }
+  for (; k < m; ++k) step();
   p = fmaf(w_prev, (!GUARD || w_prev > 0.0f) ? x_prev : 0.0f, p);
   return TOF ? t.amp * p : p;
 }
@@ -922,6 +951,7 @@ struct ReconDesc {
   const float* r;
   double* bp;
   const uint32_t* lpos;  // partial position of each sorted run
+  unsigned long long* rstats;  // r statistics of the update (kRStatWords)
   DevFov f;
   DevTof t;
   TileGeom g;
@@ -965,7 +995,8 @@ template <bool SENS>
 __device__ __forceinline__ void reduce_lors(const float* __restrict__ ac, uint32_t base, uint32_t first,
                                             uint32_t end, const uint32_t* __restrict__ run_off,
                                             const uint32_t* __restrict__ run_pos, const uint4* __restrict__ pos4,
-                                            const float* __restrict__ partial, float* __restrict__ r_out) {
+                                            const float* __restrict__ partial, float* __restrict__ r_out,
+                                            unsigned long long* rstats) {
   uint32_t idx[kReduceLors];
   bool on[kReduceLors];
   uint32_t q[kReduceLors][4];
@@ -985,6 +1016,7 @@ __device__ __forceinline__ void reduce_lors(const float* __restrict__ ac, uint32
   for (uint32_t u = 0; u < kReduceLors; ++u)
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) v[u][k] = q[u][k] != kNoRun ? partial[q[u][k] & ~kMoreRuns] : 0.0f;
+  RStats rs;
 #pragma unroll
   for (uint32_t u = 0; u < kReduceLors; ++u) {
     if (!on[u]) continue;
@@ -998,8 +1030,11 @@ __device__ __forceinline__ void reduce_lors(const float* __restrict__ ac, uint32
     }
     // MLEM: r = 1/(p*ac); sensitivity: p adapted to exp(-p), r = 1/exp(-p)
     // (projector.rs:67-74,144-149)
-    r_out[idx[u]] = SENS ? 1.0f / expf(-p) : 1.0f / (p * ac[idx[u]]);
+    const float rv = SENS ? 1.0f / expf(-p) : 1.0f / (p * ac[idx[u]]);
+    r_out[idx[u]] = rv;
+    if (q[u][0] != kNoRun) rs.note(rv);  // only LORs with runs backproject their r
   }
+  if (rstats) block_r_stats(rstats, rs);
 }
 
 // pos4 of LORs 0..n-1 from run_off and run_pos.
@@ -1037,14 +1072,15 @@ __global__ __launch_bounds__(256) void k_lor_reduce(const float* __restrict__ ac
                                                     const uint32_t* __restrict__ run_off,
                                                     const uint32_t* __restrict__ run_pos,
                                                     const uint4* __restrict__ pos4,
-                                                    const float* __restrict__ partial, float* __restrict__ r_out) {
+                                                    const float* __restrict__ partial, float* __restrict__ r_out,
+                                                    unsigned long long* rstats = nullptr) {
   const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
   const uint32_t share = (n + 7u) / 8u;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t first = slot * blockDim.x * kReduceLors + wave * 64u * kReduceLors + lane;
   const uint32_t base = xcd * share;
   const uint32_t end = base + share < n ? share : (base < n ? n - base : 0u);
-  reduce_lors<SENS>(ac, base, first, end, run_off, run_pos, pos4, partial, r_out);
+  reduce_lors<SENS>(ac, base, first, end, run_off, run_pos, pos4, partial, r_out, rstats);
 }
 
 // The same reduction for the (window block, tile) partial layout: all
@@ -1067,7 +1103,7 @@ __device__ __forceinline__ void reduce_window(const uint32_t b, const float* __r
                                               const uint32_t* __restrict__ run_off,
                                               const uint32_t* __restrict__ run_pos, const uint4* __restrict__ pos4,
                                               const uint2* __restrict__ pos2, const float* __restrict__ partial,
-                                              float* __restrict__ r_out) {
+                                              float* __restrict__ r_out, unsigned long long* rstats) {
   extern __shared__ float wpart[];
   const uint32_t ws = win[b], size = win[b + 1] - ws;
   const uint32_t stride = blockDim.x;
@@ -1108,6 +1144,7 @@ __device__ __forceinline__ void reduce_window(const uint32_t b, const float* __r
     }
     __syncthreads();
   }
+  RStats rs;
 #pragma unroll
   for (uint32_t u = 0; u < kWinLors; ++u) {
     const uint32_t i = (b << kWinShift) + threadIdx.x + u * stride;
@@ -1124,8 +1161,11 @@ __device__ __forceinline__ void reduce_window(const uint32_t b, const float* __r
       for (uint32_t k = run_off[i] + 4; k < e; ++k) p += staged ? wpart[run_pos[k] - ws] : partial[run_pos[k]];
     }
     // as reduce_lors (projector.rs:67-74,144-149)
-    r_out[i] = SENS ? 1.0f / expf(-p) : 1.0f / (p * a[u]);
+    const float rv = SENS ? 1.0f / expf(-p) : 1.0f / (p * a[u]);
+    r_out[i] = rv;
+    if (q[u][0] != kNoRun) rs.note(rv);  // only LORs with runs backproject their r
   }
+  if (rstats) block_r_stats(rstats, rs);
 }
 
 template <bool SENS, bool COMPACT>
@@ -1134,8 +1174,10 @@ __global__ __launch_bounds__(1024) void k_lor_reduce_win(const float* __restrict
                                                          const uint32_t* __restrict__ run_off,
                                                          const uint32_t* __restrict__ run_pos,
                                                          const uint4* __restrict__ pos4, const uint2* __restrict__ pos2,
-                                                         const float* __restrict__ partial, float* __restrict__ r_out) {
-  reduce_window<SENS, COMPACT>(blockIdx.x, ac, n, win, lds_entries, run_off, run_pos, pos4, pos2, partial, r_out);
+                                                         const float* __restrict__ partial, float* __restrict__ r_out,
+                                                         unsigned long long* rstats) {
+  reduce_window<SENS, COMPACT>(blockIdx.x, ac, n, win, lds_entries, run_off, run_pos, pos4, pos2, partial, r_out,
+                               rstats);
 }
 
 // Per-reconstruction tables of the batched small kernels; blocks of one
@@ -1149,6 +1191,7 @@ struct ReduceDesc {
   float* r;
   const uint32_t* win;  // window blocks (k_lor_reduce_win_batch)
   const uint2* pos2;    // compact positions, or null for pos4
+  unsigned long long* rstats;  // the reconstruction's r statistics (kRStatWords)
   uint32_t n;
   uint32_t block0;
 };
@@ -1158,6 +1201,7 @@ struct UpdateDesc {
   double* bp;
   const float* s;
   unsigned long long* counters;
+  unsigned long long* rstat;  // r statistics (kRStatWords)
   uint32_t nv;
   uint32_t block0;
 };
@@ -1182,10 +1226,10 @@ __global__ __launch_bounds__(1024) void k_lor_reduce_win_batch(const ReduceDesc*
   const ReduceDesc q = d[r];
   if (q.pos2)
     reduce_window<false, true>(blockIdx.x - q.block0, q.ac, q.n, q.win, lds_entries, q.run_off, q.run_pos, q.pos4,
-                               q.pos2, q.partial, q.r);
+                               q.pos2, q.partial, q.r, q.rstats);
   else
     reduce_window<false, false>(blockIdx.x - q.block0, q.ac, q.n, q.win, lds_entries, q.run_off, q.run_pos, q.pos4,
-                                q.pos2, q.partial, q.r);
+                                q.pos2, q.partial, q.r, q.rstats);
 }
 
 // LORs whose walk is not provably bounded: exact one-lane projection and
@@ -1243,16 +1287,90 @@ __global__ __launch_bounds__(256) void k_fwd_irregular(const Lor* __restrict__ l
 // loads through either table stay global loads).
 __device__ float kOneR[1] = {1.0f};
 
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
+// The backprojection tile's accumulator, the same for all lanes of a work
+// item.  Fixed point (unit > 0): every contribution v = w * r is scaled by a
+// power of two S and rounded to an integer -- in binary64, as the low 52 bits
+// of the encoding of v * S + 2^52 (one fused multiply-add; the sum's
+// exponent is fixed while v * S < 2^51) -- and added to a 64-bit integer with
+// ds_add_u64, which the LDS executes at about twice the rate of ds_add_f64
+// under a replay's scattered addresses (measured on gfx950; ds_add_f32 is an
+// order of magnitude slower still).  S is the largest power of two that keeps
+// the largest possible contribution -- the update's largest r, times the
+// Gaussian's peak factor, times 1.25 voxel diagonals (no step inside a voxel
+// is longer than its diagonal; the margin covers the boundaries' rounding) --
+// below 2^t with t = min(51, 64 - ceil(log2(runs of the item))): a run adds
+// at most once to a voxel, so no tile sum reaches 2^64.  Integer sums are
+// exact and independent of order; a contribution's rounding to the unit is
+// at most 2^-25 of a typical contribution (the guard below), less than its
+// own binary32 rounding.  Binary64 sums (unit == 0) where the scale does not
+// exist or could be too coarse: some r negative or not finite (a LOR with
+// p = 0, projector.rs:145), no positive r, or a largest r more than 2^(t-26)
+// times the geometric mean r (a few LORs of tiny projection would set a unit
+// too coarse for the others; two bits of the margin cover the step lengths
+// and Gaussian factors below their bound).  MLEM's LORs through voxels that
+// have decayed towards zero reach about 2^20 times the geometric mean r on the
+// 120^3 phantom workload; work items of up to 2^15 runs allow 2^23.
+static_assert(kRSlots == 64, "one statistics slot per lane of a wavefront");
+
+struct TileAcc {
+  double s;     // the scale S (fixed point)
+  double unit;  // 1 / S; 0 for binary64 sums
+};
+
+// The accumulator of a work item of n_item runs, from the statistics slots
+// at rstats (kRStatWords words; ignored with unit_r, where every r is 1).
+// Called by the first wavefront of the workgroup (all 64 lanes).
+__device__ __forceinline__ TileAcc tile_acc(const unsigned long long* rstats, bool unit_r, uint32_t n_item,
+                                            const DevFov& f, float amp) {
+  const TileAcc f64{0.0, 0.0};
+  float rmax = 1.0f;
+  double lmean = 0.0;  // mean log2(r)
+  if (!unit_r) {
+    if (!rstats) return f64;
+    const uint32_t lane = threadIdx.x & 63u;
+    RStats rs;
+    const unsigned long long* slot = rstats + 4 * lane;  // kRSlots == 64 slots, one per lane
+    rs.rmax = __uint_as_float((uint32_t)slot[0]);
+    rs.bad = slot[1] != 0;
+    rs.lsum = __longlong_as_double((long long)slot[2]);
+    rs.n = (uint32_t)slot[3];
+    rs.wave_combine();
+    if (rs.bad || rs.n == 0) return f64;
+    rmax = rs.rmax;
+    lmean = rs.lsum / (double)rs.n;
+  }
+  int lg = 0;
+  while (lg < 32 && (1u << lg) < n_item) ++lg;  // n_item <= 2^lg
+  const int t = min(51, 64 - lg);
+  const float bound = rmax * amp * (1.25f * sqrtf(f.vs_x * f.vs_x + f.vs_y * f.vs_y + f.vs_z * f.vs_z));
+  if (!(bound > 0.0f) || !(bound <= 3.402823466e38f) || !((double)__log2f(rmax) <= lmean + (double)(t - 26)))
+    return f64;
+  int k;
+  frexpf(bound, &k);  // bound < 2^k
+  const int e = t - k;
+  if (e > 1000 || e < -1000) return f64;
+  return TileAcc{ldexp(1.0, e), ldexp(1.0, -e)};
+}
+
+// round(v * s) for 0 <= v * s < 2^51, as a 64-bit integer
+__device__ __forceinline__ unsigned long long fixed_units(float v, double s) {
+  const double d = fma((double)v, s, 4503599627370496.0);  // 2^52 + v * s, rounded to an integer
+  return (unsigned long long)__double_as_longlong(d) & 0x000fffffffffffffull;
+}
+
 // A lane's runs j, j + stride, ... <= last of a backprojection work item,
 // added to the LDS tile with r = r_src[lor & r_mask].  Records are loaded
 // two runs ahead and r one run ahead, so no load's latency stalls a replay;
 // the first records are waited for before the loop, as in fwd_runs.
-// PERM: the runs are read through perm.
-template <bool TOF, bool PERM>
+// PERM: the runs are read through perm.  FIXED: the tile holds 64-bit fixed
+// point (TileAcc), where every r is finite, so zero-length steps add 0.
+template <bool TOF, bool PERM, bool FIXED>
 __device__ __forceinline__ void scatter_runs(const Run* __restrict__ runs, const uint32_t* __restrict__ perm,
                                              int32_t j, const int32_t last, const float* __restrict__ r_src,
-                                             const uint32_t r_mask, const DevTof& t, const TileGeom& g,
-                                             double* tile) {
+                                             const uint32_t r_mask, const double sfix, const DevTof& t,
+                                             const TileGeom& g, double* tile) {
   const int32_t stride = (int32_t)blockDim.x;
   const int32_t j1 = j + stride;
   const int32_t j1c = j1 < last ? j1 : last;
@@ -1260,8 +1378,14 @@ __device__ __forceinline__ void scatter_runs(const Run* __restrict__ runs, const
   Run nxt = runs[PERM ? perm[j1c] : j1c];
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   float r = r_src[cur.lor & r_mask];
-  char* base = reinterpret_cast<char*>(tile);
-  auto add = [&](uint32_t off, float v) { atomicAdd(reinterpret_cast<double*>(base + off), (double)v); };
+  const uint32_t base = lds_addr(tile);
+  auto add = [&](uint32_t at, float v) {
+    if (FIXED)
+      __hip_atomic_fetch_add((lds_u64*)(uintptr_t)at, fixed_units(v, sfix), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+      __hip_atomic_fetch_add((lds_f64*)(uintptr_t)at, (double)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
   for (;;) {
     const int32_t jn = j + stride;
     const int32_t jnn = jn + stride;
@@ -1269,10 +1393,11 @@ __device__ __forceinline__ void scatter_runs(const Run* __restrict__ runs, const
     const int32_t jnc = jnn < last ? jnn : last;
     const Run nn = runs[PERM ? perm[jnc] : jnc];
     // without a non-finite r in the wavefront, zero-length steps may add +0
-    if (__all(fabsf(r) <= 3.402823466e38f))
-      replay_run<TOF, false>(cur, r, t, g, add);
+    const float rr = TOF ? r * t.amp : r;
+    if (FIXED || __all(fabsf(r) <= 3.402823466e38f))
+      replay_run<TOF, false>(cur, rr, t, g, base, add);
     else
-      replay_run<TOF, true>(cur, r, t, g, add);
+      replay_run<TOF, true>(cur, rr, t, g, base, add);
     if (jn > last) break;
     cur = nxt;
     nxt = nn;
@@ -1282,25 +1407,44 @@ __device__ __forceinline__ void scatter_runs(const Run* __restrict__ runs, const
 }
 
 // r_in == nullptr backprojects the weights themselves (r = 1).  With perm,
-// sorted run j is runs[perm[j]] (runs left in LOR-major order).
+// sorted run j is runs[perm[j]] (runs left in LOR-major order).  rstats (the
+// update's r statistics, kRStatWords words) chooses the accumulator
+// (tile_acc); the first workgroup records its unit in word kRUnitWord.
 template <bool TOF>
 __device__ __forceinline__ void scatter_body(const Run* __restrict__ runs, const int4 wi,
                                              const float* __restrict__ r_in, const DevFov& f, const DevTof& t,
                                              const TileGeom& g, double* __restrict__ bp,
-                                             const uint32_t* __restrict__ perm) {
+                                             const uint32_t* __restrict__ perm, unsigned long long* rstats) {
   extern __shared__ double tile[];
+  __shared__ TileAcc s_acc;
   const int32_t tl = wi.x;
+  if (threadIdx.x < 64) {
+    const TileAcc a = tile_acc(rstats, r_in == nullptr, (uint32_t)(wi.z - wi.y), f, TOF ? t.amp : 1.0f);
+    if (threadIdx.x == 0) {
+      s_acc = a;
+      if (rstats && blockIdx.x == 0) rstats[kRUnitWord] = (unsigned long long)__double_as_longlong(a.unit);
+    }
+  }
+  // +0.0 and the integer 0 have the same bits
   for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) tile[k] = 0.0;
   __syncthreads();
+  const TileAcc acc = s_acc;
   const int32_t j = wi.y + (int32_t)threadIdx.x;
   if (j < wi.z) {
     // without r every run reads the one entry of kOneR
     const float* r_src = r_in ? r_in : kOneR;
     const uint32_t r_mask = r_in ? 0xffffffffu : 0u;
-    if (perm)
-      scatter_runs<TOF, true>(runs, perm, j, wi.z - 1, r_src, r_mask, t, g, tile);
-    else
-      scatter_runs<TOF, false>(runs, perm, j, wi.z - 1, r_src, r_mask, t, g, tile);
+    if (acc.unit > 0.0) {
+      if (perm)
+        scatter_runs<TOF, true, true>(runs, perm, j, wi.z - 1, r_src, r_mask, acc.s, t, g, tile);
+      else
+        scatter_runs<TOF, false, true>(runs, perm, j, wi.z - 1, r_src, r_mask, acc.s, t, g, tile);
+    } else {
+      if (perm)
+        scatter_runs<TOF, true, false>(runs, perm, j, wi.z - 1, r_src, r_mask, 0.0, t, g, tile);
+      else
+        scatter_runs<TOF, false, false>(runs, perm, j, wi.z - 1, r_src, r_mask, 0.0, t, g, tile);
+    }
   }
   __syncthreads();
   // flush the tile: consecutive lanes take consecutive x, so each wavefront
@@ -1309,8 +1453,10 @@ __device__ __forceinline__ void scatter_body(const Run* __restrict__ runs, const
   const int32_t tyi = (tl / g.ntx) % g.nty;
   const int32_t txi = tl % g.ntx;
   const int32_t x0 = txi * g.tx, y0 = tyi * g.ty, z0 = tzi * g.tz;
+  const unsigned long long* tile_u = reinterpret_cast<const unsigned long long*>(tile);
   for (TileCursor cu(g, (int32_t)threadIdx.x, (int32_t)blockDim.x); cu.k < g.tvox; cu.next(g, (int32_t)blockDim.x)) {
-    const double v = tile[cu.k];
+    // a fixed-point sum (below 2^62) converts to within 2^-53
+    const double v = acc.unit > 0.0 ? (double)tile_u[cu.k] * acc.unit : tile[cu.k];
     const int32_t x = x0 + cu.lx, y = y0 + cu.ly, z = z0 + cu.lz;
     if (v != 0.0 && cu.lx < g.tx && x < f.nx && y < f.ny && z < f.nz) unsafeAtomicAdd(bp + x + f.nx * (y + f.ny * z), v);
   }
@@ -1319,11 +1465,11 @@ __device__ __forceinline__ void scatter_body(const Run* __restrict__ runs, const
 template <bool TOF>
 __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, const int4* __restrict__ work,
                                                   const float* __restrict__ r_in, DevFov f, DevTof t, TileGeom g,
-                                                  double* __restrict__ bp,
-                                                  const uint32_t* __restrict__ perm = nullptr) {
+                                                  double* __restrict__ bp, const uint32_t* __restrict__ perm,
+                                                  unsigned long long* rstats) {
   const int4 wi = work[blockIdx.x];
   if (wi.z <= wi.y) return;  // an empty item (the grid may be an upper bound)
-  scatter_body<TOF>(runs, wi, r_in, f, t, g, bp, perm);
+  scatter_body<TOF>(runs, wi, r_in, f, t, g, bp, perm, rstats);
 }
 
 // Batched backprojection: work item w = (tile, first run, end run, reconstruction).
@@ -1332,7 +1478,7 @@ __global__ __launch_bounds__(1024) void k_scatter_batch(const ReconDesc* __restr
                                                         const int4* __restrict__ work) {
   const int4 wi = work[blockIdx.x];
   const ReconDesc& r = rd[wi.w];
-  scatter_body<TOF>(r.runs, wi, r.r, r.f, r.t, r.g, r.bp, nullptr);
+  scatter_body<TOF>(r.runs, wi, r.r, r.f, r.t, r.g, r.bp, nullptr, r.rstats);
 }
 
 // Batched MLEM update (as k_update_f64) over every reconstruction's image.
@@ -1347,9 +1493,11 @@ __global__ __launch_bounds__(256) void k_update_batch(const UpdateDesc* __restri
   q.x[i] = sj > 0.0f ? q.x[i] * (b * sj) : 0.0f;
 }
 
-// Zeroes the work counters of every reconstruction of a batch.
+// Zeroes the work counters and r statistics of every reconstruction of a batch.
 __global__ void k_zero_counters_batch(const UpdateDesc* __restrict__ d, uint32_t n_rec, uint32_t n_cnt) {
   for (uint32_t k = threadIdx.x; k < n_rec * n_cnt; k += blockDim.x) d[k / n_cnt].counters[k % n_cnt] = 0ull;
+  for (uint32_t k = threadIdx.x; k < n_rec * kRStatWords; k += blockDim.x)
+    d[k / kRStatWords].rstat[k % kRStatWords] = 0ull;
 }
 
 // MLEM update with the binary64 backprojection, which is reset:

@@ -257,6 +257,12 @@ extern "C" {
     pub fn petalo_mlem_set_image(ctx: *mut petalo_mlem_t, image: *const f32) -> c_int;
     pub fn petalo_mlem_walk_stats(ctx: *mut petalo_mlem_t, subset: usize, out: *mut petalo_stats_t) -> c_int;
     pub fn petalo_mlem_last_skipped(ctx: *mut petalo_mlem_t, skipped: *mut u64) -> c_int;
+    pub fn petalo_mlem_accumulator(
+        ctx: *mut petalo_mlem_t,
+        unit_out: *mut f64,
+        r_max_out: *mut f64,
+        r_geomean_out: *mut f64,
+    ) -> c_int;
     pub fn petalo_mlem_set_timing(ctx: *mut petalo_mlem_t, on: c_int) -> c_int;
     pub fn petalo_mlem_set_graphs(ctx: *mut petalo_mlem_t, on: c_int) -> c_int;
     pub fn petalo_mlem_last_timing(ctx: *mut petalo_mlem_t, project_ms: *mut f64, update_ms: *mut f64) -> c_int;
