@@ -203,192 +203,6 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
   return count;
 }
 
-// bin_lor without TOF, without stepping through the walk: each axis's
-// boundary sequence nb, nb + vl, nb + vl + vl, ... (the walk's values, bit for
-// bit: the walk adds vl to an axis's boundary each time it crosses it) is
-// generated on its own, and the axes are merged only where the walk leaves a
-// tile or the FOV.  The walk crosses boundaries in order of (value, axis x <
-// y < z), so a tile segment ends at the smallest (value, axis) among the
-// axes' segment-ending crossings (the crossing that leaves the tile along
-// that axis, or the axis's last one); the other axes cross the boundaries
-// that precede it in that order.  A segment's run is all of its steps when
-// its first and last steps have positive length (no TOF: those are the
-// pushed steps); otherwise the segment is stepped through as by bin_lor.
-// Adds the walk's step count to *n_steps.  Emits exactly bin_lor<false>'s
-// runs.
-template <class Emit>
-__host__ __device__ __forceinline__ uint32_t bin_lor_axes(const Lor& l, uint32_t i, const DevFov& f,
-                                                          const TileGeom& g, Emit&& emit,
-                                                          uint32_t* n_steps = nullptr) {
-  WalkC c;
-  WalkS w;
-  if (!walk_setup(l, f, c, w) || !bounded(w)) return 0;
-  // a degenerate LOR (p1 == p2) has NaN boundaries, which order no crossings
-  if (!(w.nb_x == w.nb_x && w.nb_y == w.nb_y && w.nb_z == w.nb_z && c.vl_x == c.vl_x && c.vl_y == c.vl_y &&
-        c.vl_z == c.vl_z))
-    return bin_lor<false>(l, i, f, no_tof(), g, emit, n_steps, nullptr);
-  int32_t cc[3], sgn[3], tile;
-  {
-    const int32_t nxy = f.nx * f.ny;
-    const int32_t z = w.index / nxy;
-    const int32_t rem = w.index - z * nxy;
-    const int32_t y = rem / f.nx;
-    const int32_t x = rem - y * f.nx;
-    sgn[0] = c.d_x < 0 ? -1 : 1;
-    sgn[1] = c.d_y < 0 ? -1 : 1;
-    sgn[2] = c.d_z < 0 ? -1 : 1;
-    cc[0] = sgn[0] > 0 ? g.tx - x % g.tx : x % g.tx + 1;
-    cc[1] = sgn[1] > 0 ? g.ty - y % g.ty : y % g.ty + 1;
-    cc[2] = sgn[2] > 0 ? g.tz - z % g.tz : z % g.tz + 1;
-    tile = x / g.tx + g.ntx * (y / g.ty + g.nty * (z / g.tz));
-  }
-  const float vl[3] = {c.vl_x, c.vl_y, c.vl_z};
-  const int32_t tt[3] = {g.tx, g.ty, g.tz};
-  const int32_t dtile[3] = {sgn[0], sgn[1] * g.ntx, sgn[2] * g.ntx * g.nty};
-  const uint32_t flips = (sgn[0] < 0 ? 1u : 0u) | (sgn[1] < 0 ? 2u : 0u) | (sgn[2] < 0 ? 4u : 0u);
-  float V[3] = {w.nb_x, w.nb_y, w.nb_z};  // next boundary per axis
-  int32_t r[3] = {w.r_x, w.r_y, w.r_z};   // crossings left per axis (the last one leaves the FOV)
-  float E[3] = {0.0f, 0.0f, 0.0f};        // value of the axis's segment-ending crossing
-  float Ep[3] = {0.0f, 0.0f, 0.0f};       // value of the crossing before it, -inf if none in the lookahead
-  bool have[3] = {false, false, false};
-  float here = w.here;
-  uint32_t count = 0;
-  uint32_t total = 0;
-  auto make_run = [&](const float nb[3], float h, const int32_t cq[3], uint32_t len) {
-    Coord q;
-    q.cx = cq[0];
-    q.cy = cq[1];
-    q.cz = cq[2];
-    q.sx = sgn[0];
-    q.sy = sgn[1];
-    q.sz = sgn[2];
-    Run ru;
-    ru.nb_x = nb[0];
-    ru.nb_y = nb[1];
-    ru.nb_z = nb[2];
-    ru.here = h;
-    ru.vl_x = c.vl_x;
-    ru.vl_y = c.vl_y;
-    ru.vl_z = c.vl_z;
-    ru.tof_peak = c.tof_peak;
-    ru.code = (local_of(g, q) << 15) | (flips << 12) | len;
-    ru.lor = i;
-    emit(ru, (uint32_t)tile);
-    ++count;
-  };
-  for (;;) {
-    int32_t m[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      m[a] = cc[a] < r[a] ? cc[a] : r[a];
-      if (!have[a]) {
-        float v = V[a], prev = -INFINITY;
-        for (int32_t j = 1; j < m[a]; ++j) {
-          prev = v;
-          v = v + vl[a];
-        }
-        E[a] = v;
-        Ep[a] = prev;
-        have[a] = true;
-      }
-    }
-    int e = 0;
-    if (E[1] < E[e]) e = 1;
-    if (E[2] < E[e]) e = 2;
-    float Ee = E[0], Epe = Ep[0];
-    int32_t me = m[0];
-#pragma unroll
-    for (int a = 1; a < 3; ++a)
-      if (a == e) {
-        Ee = E[a];
-        Epe = Ep[a];
-        me = m[a];
-      }
-    // crossings of the other axes before the end crossing
-    int32_t n[3];
-    float Vn[3];
-    float P = here > Epe ? here : Epe;  // start of the segment's last step
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if (a == e) {
-        n[a] = me;
-        Vn[a] = V[a];
-        continue;
-      }
-      float v = V[a];
-      int32_t k = 0;
-      const int32_t lim = m[a] - 1;
-      while (k < lim && (v < Ee || (v == Ee && a < e))) {
-        P = P > v ? P : v;
-        v = v + vl[a];
-        ++k;
-      }
-      n[a] = k;
-      Vn[a] = v;
-    }
-    const uint32_t steps = (uint32_t)(n[0] + n[1] + n[2]);
-    total += steps;
-    const float F = fminf(fminf(V[0], V[1]), V[2]);  // end of the segment's first step
-    if (F > here && Ee > P) {
-      make_run(V, here, cc, steps);
-    } else {
-      // step through the segment (ties at its ends): first and last pushed step
-      WalkS s = w;
-      s.nb_x = V[0];
-      s.nb_y = V[1];
-      s.nb_z = V[2];
-      s.here = here;
-      WalkC cs = c;
-      int32_t q3[3] = {cc[0], cc[1], cc[2]};
-      float nb0[3] = {0.0f, 0.0f, 0.0f};
-      float h0 = 0.0f;
-      int32_t c0[3] = {0, 0, 0};
-      int64_t first = -1, last = -1;
-      for (uint32_t k = 0; k < steps; ++k) {
-        const Step st = next_boundary(s);
-        if (st.bnd - s.here > 0.0f) {
-          if (first < 0) {
-            first = k;
-            nb0[0] = s.nb_x;
-            nb0[1] = s.nb_y;
-            nb0[2] = s.nb_z;
-            h0 = s.here;
-            c0[0] = q3[0];
-            c0[1] = q3[1];
-            c0[2] = q3[2];
-          }
-          last = k;
-        }
-        q3[0] -= st.ax ? 1 : 0;
-        q3[1] -= st.ay ? 1 : 0;
-        q3[2] -= st.az ? 1 : 0;
-        move(s, cs, st);
-      }
-      if (first >= 0) make_run(nb0, h0, c0, (uint32_t)(last - first + 1));
-    }
-    // the segment's crossings
-    bool out = false;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      r[a] -= n[a];
-      if (a == e) {
-        V[a] = Ee + vl[a];
-        out = r[a] == 0;
-        cc[a] = tt[a];
-        tile += dtile[a];
-        have[a] = false;
-      } else {
-        V[a] = Vn[a];
-        cc[a] -= n[a];
-      }
-    }
-    here = Ee;
-    if (out) break;
-  }
-  if (n_steps) *n_steps += total;
-  return count;
-}
-
 // The tiles in which bin_lor<false> emits a run for LOR i, in walk order,
 // without walking its steps: calls on_run(tile) for each and returns their
 // number.  The walk leaves a tile at the first, in (position, axis x < y <
@@ -525,6 +339,229 @@ __host__ __device__ __forceinline__ uint32_t tile_runs(const Lor& l, uint32_t i,
   return count;
 }
 
+// bin_lor without TOF by tile jumps (the fill pass's form of tile_runs):
+// emits exactly bin_lor<false>'s runs and adds its step count, without
+// generating every boundary.  Per tile segment the exit crossing is found as
+// tile_runs finds it (binary64 estimates, exact values where their error
+// intervals overlap), and the segment's exact state follows from repeated
+// binary32 additions done by repeat_add: the exit axis's crossing before its
+// last (Ep) and its last (Ee = Ep + vl), and for each other axis the number
+// n of its crossings that precede Ee in the walk's (value, axis) order --
+// from the binary64 quotient (Ee - V) / vl where that is decided beyond its
+// error, by counting the crossings one by one where it is not -- with the
+// last of them (P) and the next (P + vl).  V, the axes' next boundaries,
+// are so exact at every tile entry, and a run there is (V, here = the entry
+// crossing, the tile's local index, its step count) when its first step
+// (min V - here) and its last (Ee - max P) have positive length; a segment
+// with a zero-length first or last step (ties at its ends) is stepped
+// through for its first and last pushed steps.  Walks with a non-positive
+// boundary or voxel length (degenerate LORs) are stepped through by bin_lor.
+template <class Emit>
+__host__ __device__ __forceinline__ uint32_t bin_lor_jump(const Lor& l, uint32_t i, const DevFov& f,
+                                                          const TileGeom& g, Emit&& emit,
+                                                          uint32_t* n_steps = nullptr) {
+  WalkC c;
+  WalkS w;
+  if (!walk_setup(l, f, c, w) || !bounded(w)) return 0;
+  if (!(w.nb_x > 0.0f && w.nb_y > 0.0f && w.nb_z > 0.0f && c.vl_x > 0.0f && c.vl_y > 0.0f && c.vl_z > 0.0f))
+    return bin_lor<false>(l, i, f, no_tof(), g, emit, n_steps, nullptr);
+  const float nb[3] = {w.nb_x, w.nb_y, w.nb_z}, vl[3] = {c.vl_x, c.vl_y, c.vl_z};
+  int32_t cc[3], sgn[3], tile;
+  {
+    const int32_t nxy = f.nx * f.ny;
+    const int32_t z = w.index / nxy;
+    const int32_t rem = w.index - z * nxy;
+    const int32_t y = rem / f.nx;
+    const int32_t x = rem - y * f.nx;
+    sgn[0] = c.d_x < 0 ? -1 : 1;
+    sgn[1] = c.d_y < 0 ? -1 : 1;
+    sgn[2] = c.d_z < 0 ? -1 : 1;
+    cc[0] = sgn[0] > 0 ? g.tx - x % g.tx : x % g.tx + 1;
+    cc[1] = sgn[1] > 0 ? g.ty - y % g.ty : y % g.ty + 1;
+    cc[2] = sgn[2] > 0 ? g.tz - z % g.tz : z % g.tz + 1;
+    tile = x / g.tx + g.ntx * (y / g.ty + g.nty * (z / g.tz));
+  }
+  const int32_t tt[3] = {g.tx, g.ty, g.tz};
+  const int32_t dtile[3] = {sgn[0], sgn[1] * g.ntx, sgn[2] * g.ntx * g.nty};
+  const uint32_t flips = (sgn[0] < 0 ? 1u : 0u) | (sgn[1] < 0 ? 2u : 0u) | (sgn[2] < 0 ? 4u : 0u);
+  float V[3] = {w.nb_x, w.nb_y, w.nb_z};         // exact next boundary per axis
+  uint32_t cnt[3] = {0u, 0u, 0u};                // crossings of each axis so far
+  int32_t r[3] = {w.r_x, w.r_y, w.r_z};          // crossings left per axis (the last one leaves the FOV)
+  float here = w.here;
+  uint32_t count = 0, total = 0;
+  auto make_run = [&](const float s_nb[3], float h, const int32_t cq[3], uint32_t len) {
+    Coord q;
+    q.cx = cq[0];
+    q.cy = cq[1];
+    q.cz = cq[2];
+    q.sx = sgn[0];
+    q.sy = sgn[1];
+    q.sz = sgn[2];
+    Run ru;
+    ru.nb_x = s_nb[0];
+    ru.nb_y = s_nb[1];
+    ru.nb_z = s_nb[2];
+    ru.here = h;
+    ru.vl_x = c.vl_x;
+    ru.vl_y = c.vl_y;
+    ru.vl_z = c.vl_z;
+    ru.tof_peak = c.tof_peak;
+    ru.code = (local_of(g, q) << 15) | (flips << 12) | len;
+    ru.lor = i;
+    emit(ru, (uint32_t)tile);
+    ++count;
+  };
+  for (;;) {
+    // the segment-ending crossing of each axis: m[a]-th from now
+    int32_t m[3];
+    double b[3], err[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      m[a] = cc[a] < r[a] ? cc[a] : r[a];
+      b[a] = crossing_estimate(nb[a], vl[a], cnt[a] + (uint32_t)m[a] - 1u, err[a]);
+    }
+    int e = 0;
+    if (b[1] < b[e]) e = 1;
+    if (b[2] < b[e]) e = 2;
+    bool sure = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (a != e) sure = sure && b[e] + err[e] < b[a] - err[a];
+    if (!sure) {
+      float E[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) E[a] = repeat_add(V[a], vl[a], (uint32_t)m[a] - 1u);
+      e = 0;
+      if (E[1] < E[e]) e = 1;
+      if (E[2] < E[e]) e = 2;
+    }
+    float Ve = V[0], vle = vl[0];
+    int32_t me = m[0];
+#pragma unroll
+    for (int a = 1; a < 3; ++a)
+      if (a == e) {
+        Ve = V[a];
+        vle = vl[a];
+        me = m[a];
+      }
+    // the exit axis's last two crossings of the segment
+    const float Ep = me >= 2 ? repeat_add(Ve, vle, (uint32_t)me - 2u) : -INFINITY;
+    const float Ee = me >= 2 ? Ep + vle : Ve;
+    float P = here > Ep ? here : Ep;  // start of the segment's last step
+    int32_t n[3];
+    float Vn[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a == e) {
+        n[a] = me;
+        Vn[a] = V[a];
+        continue;
+      }
+      const int32_t lim = m[a] - 1;
+      const float v0 = V[a];
+      int32_t k = -1;  // the count, once decided
+      if (lim <= 0 || v0 > Ee) {
+        k = 0;
+      } else if (v0 < Ee) {
+        // crossings j = 0, 1, ... lie near v0 + j vl, each within j 2^-23 of its value
+        const double x = ((double)Ee - (double)v0) / (double)vl[a];
+        const double xc = ceil(x);
+        int32_t ke = xc < (double)lim ? (int32_t)xc : lim;  // crossings below Ee by the estimate
+        if (ke < 0) ke = 0;
+        bool ok = true;
+        if (ke > 0) {  // crossing ke - 1 below Ee
+          const double bj = (double)v0 + (double)(ke - 1) * (double)vl[a];
+          ok = ok && bj + (double)(ke - 1) * 0x1p-23 * bj < (double)Ee;
+        }
+        if (ke < lim) {  // crossing ke above Ee
+          const double bj = (double)v0 + (double)ke * (double)vl[a];
+          ok = ok && bj - (double)ke * 0x1p-23 * bj > (double)Ee;
+        }
+        if (ok) k = ke;
+      }
+      float v, pa = -INFINITY;
+      if (k >= 0) {
+        if (k > 0) {
+          pa = repeat_add(v0, vl[a], (uint32_t)k - 1u);
+          v = pa + vl[a];
+        } else {
+          v = v0;
+        }
+      } else {  // undecided: one crossing at a time, as the walk orders them
+        v = v0;
+        k = 0;
+        while (k < lim && (v < Ee || (v == Ee && a < e))) {
+          pa = v;
+          v = v + vl[a];
+          ++k;
+        }
+      }
+      P = P > pa ? P : pa;
+      n[a] = k;
+      Vn[a] = v;
+    }
+    const uint32_t steps = (uint32_t)(n[0] + n[1] + n[2]);
+    total += steps;
+    const float F = fminf(fminf(V[0], V[1]), V[2]);  // end of the segment's first step
+    if (F > here && Ee > P) {
+      make_run(V, here, cc, steps);
+    } else {
+      // step through the segment (ties at its ends): first and last pushed step
+      WalkS s = w;
+      s.nb_x = V[0];
+      s.nb_y = V[1];
+      s.nb_z = V[2];
+      s.here = here;
+      WalkC cs = c;
+      int32_t q3[3] = {cc[0], cc[1], cc[2]};
+      float nb0[3] = {0.0f, 0.0f, 0.0f};
+      float h0 = 0.0f;
+      int32_t c0[3] = {0, 0, 0};
+      int64_t first = -1, last = -1;
+      for (uint32_t k = 0; k < steps; ++k) {
+        const Step st = next_boundary(s);
+        if (st.bnd - s.here > 0.0f) {
+          if (first < 0) {
+            first = k;
+            nb0[0] = s.nb_x;
+            nb0[1] = s.nb_y;
+            nb0[2] = s.nb_z;
+            h0 = s.here;
+            c0[0] = q3[0];
+            c0[1] = q3[1];
+            c0[2] = q3[2];
+          }
+          last = k;
+        }
+        q3[0] -= st.ax ? 1 : 0;
+        q3[1] -= st.ay ? 1 : 0;
+        q3[2] -= st.az ? 1 : 0;
+        move(s, cs, st);
+      }
+      if (first >= 0) make_run(nb0, h0, c0, (uint32_t)(last - first + 1));
+    }
+    bool out = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      r[a] -= n[a];
+      cnt[a] += (uint32_t)n[a];
+      if (a == e) {
+        V[a] = Ee + vl[a];
+        out = r[a] == 0;
+        cc[a] = tt[a];
+        tile += dtile[a];
+      } else {
+        V[a] = Vn[a];
+        cc[a] -= n[a];
+      }
+    }
+    here = Ee;
+    if (out) break;
+  }
+  if (n_steps) *n_steps += total;
+  return count;
+}
+
 // Length field of a run's sort key (bits 0-11): longer runs first, in
 // buckets of 4 steps.  Lanes of a wavefront get runs of about equal length;
 // inside a bucket the (stable) sort keeps LOR order, so neighbouring LORs'
@@ -582,10 +619,9 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
       }
       ++k;
     };
-    // filling without TOF: the per-axis form (the same runs, 25% faster on
-    // the GPU; counting, where the step walk's single loop keeps the lanes
-    // together and also gives the pushed-step count, it is 7% slower)
-    const uint32_t count = (!TOF && FILL) ? bin_lor_axes(l, i, f, g, emit)
+    // filling without TOF: by tile jumps (the same runs; counting, the step
+    // walk also gives the pushed-step count)
+    const uint32_t count = (!TOF && FILL) ? bin_lor_jump(l, i, f, g, emit)
                                           : bin_lor<TOF>(l, i, f, t, g, emit, FILL ? nullptr : &steps,
                                                          FILL ? nullptr : &nz);
     if (FILL) {
@@ -685,7 +721,7 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
     };
     uint32_t count;
     if (FILL)
-      count = bin_lor_axes(l, i, f, g, emit);
+      count = bin_lor_jump(l, i, f, g, emit);
     else if (STATS)
       count = bin_lor<false>(l, i, f, notof, g, emit, &steps, &nz);
     else

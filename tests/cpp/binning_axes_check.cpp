@@ -1,6 +1,7 @@
-// Host-side check that the per-axis binning (bin_lor_axes) emits exactly the
-// runs and step counts of the step-by-step binning walk (bin_lor<false>),
-// bit for bit, and that the tile jump (tile_runs) finds the same tiles, over random LORs and LORs built to tie: axis-parallel, on
+// Host-side check that the fill pass's tile-jump binning (bin_lor_jump) emits
+// exactly the runs and step counts of the step-by-step binning walk
+// (bin_lor<false>), bit for bit, and that the count pass's tile jump
+// (tile_runs) finds the same tiles, over random LORs and LORs built to tie: axis-parallel, on
 // voxel planes, through voxel edges and corners, diagonal, degenerate
 // (p1 == p2), grazing a face, and the irregular LORs of the sensitivity
 // tests.  Usage: binning_axes_check [lors per case]; exit status 1 on any
@@ -99,7 +100,7 @@ int main(int argc, char** argv) {
       std::vector<R> a, b;
       uint32_t sa = 0, sb = 0;
       bin_lor<false>(l, (uint32_t)i, f, t, g, [&](const Run& ru, uint32_t tl) { a.push_back({ru, tl}); }, &sa, nullptr);
-      bin_lor_axes(l, (uint32_t)i, f, g, [&](const Run& ru, uint32_t tl) { b.push_back({ru, tl}); }, &sb);
+      bin_lor_jump(l, (uint32_t)i, f, g, [&](const Run& ru, uint32_t tl) { b.push_back({ru, tl}); }, &sb);
       WalkC c; WalkS w;
       const bool bnd = walk_setup(l, f, c, w) && bounded(w);
       if (!bnd) { sa = 0; }  // bin_lor counts nothing for unbounded walks either
@@ -128,7 +129,7 @@ int main(int argc, char** argv) {
                  g.tx, g.ty, g.tz, i, kind, a.size(), b.size(), sa, sb);
           for (size_t k = 0; k < a.size() || k < b.size(); ++k) {
             if (k < a.size()) printf("  A %zu: tile %u len %u local %u nb %.9g %.9g %.9g here %.9g\n", k, a[k].tile, run_len(a[k].ru.code), run_local(a[k].ru.code), a[k].ru.nb_x, a[k].ru.nb_y, a[k].ru.nb_z, a[k].ru.here);
-            if (k < b.size()) printf("  B %zu: tile %u len %u local %u nb %.9g %.9g %.9g here %.9g\n", k, b[k].tile, run_len(b[k].ru.code), run_local(b[k].ru.code), b[k].ru.nb_x, b[k].ru.nb_y, b[k].ru.nb_z, b[k].ru.here);
+            if (k < b.size()) printf("  J %zu: tile %u len %u local %u nb %.9g %.9g %.9g here %.9g\n", k, b[k].tile, run_len(b[k].ru.code), run_local(b[k].ru.code), b[k].ru.nb_x, b[k].ru.nb_y, b[k].ru.nb_z, b[k].ru.here);
           }
         }
         ++bad;

@@ -1,8 +1,8 @@
-"""The per-axis binning of the tiled scheme (bin_lor_axes, used to fill the
-runs of walks without TOF) and the tile jump (tile_runs, which counts them)
-against the step-by-step binning walk (bin_lor), compiled for the host from
-the library's own headers: identical runs (every field, bit for bit), step
-counts and tile sequences on 8 FOV/tile cases x 20 000 LORs; and the jump's
+"""The tile-jump binning of the tiled scheme (bin_lor_jump, which fills the
+runs of walks without TOF; tile_runs, which counts them) against the
+step-by-step binning walk (bin_lor), compiled for the host from the
+library's own headers: identical runs (every field, bit for bit), step
+counts and tile sequences on 8 FOV/tile cases x 20 000 LORs; and the jumps'
 repeated binary32 addition (repeat_add) against one addition at a time on
 400 000 cases (ties, binade crossings, long runs)."""
 import os
