@@ -113,12 +113,27 @@ def measure_batched(P, steps, warmup, singles=True):
     for _ in range(subsets):
         b.step()
         ms_direct.append(b.last_ms())
+    # the persistent schedule (one launch per subset update taking every
+    # phase's tasks from a queue), timed as the default one
+    b.set_options(timing=False, graphs=True)
+    b.set_schedule(True)
+    for _ in range(subsets):
+        b.step()
+    P.lib().petalo_device_synchronize(0)
+    t0 = time.perf_counter()
+    for _ in range(steps * subsets):
+        b.step()
+    P.lib().petalo_device_synchronize(0)
+    persistent_ms = (time.perf_counter() - t0) / steps * 1e3
+    b.image(0)  # fails if a wait of the persistent launch ran into its time limit
     b.close()
     full_ms = el / steps * 1e3
-    log(f"[bench] batched: {full_ms:.2f} ms per full iteration of all {n_rec}")
+    log(f"[bench] batched: {full_ms:.2f} ms per full iteration of all {n_rec} "
+        f"({persistent_ms:.2f} ms with the persistent schedule)")
     out = {"value": steps / el, "unit": "iter/s", "steps": steps, "warmup": warmup,
            "ms_per_full_iteration": full_ms, "subset_update_ms_graphs": full_ms / subsets,
            "subset_update_ms_direct_launches": float(np.mean(ms_direct)), "create_s": create_s,
+           "persistent_schedule_ms_per_full_iteration": persistent_ms,
            "config": {"workload": "batched_tof_osem_32", "reconstructions": n_rec, "subsets": subsets,
                       "voxels": [int(v) for v in sizes], "n_lors": [int(c) for c in counts],
                       "total_lors": int(counts.sum()), "tof_sigma_ps": 42.47, "tof_cutoff": 3.0}}

@@ -250,6 +250,14 @@ int petalo_batch_get_image(petalo_batch_t* batch, size_t r, float* image_out);
 /* timing: HIP events around each petalo_batch_next (petalo_batch_last_ms);
  * use_graphs: replay captured graphs (default 1) or launch directly. */
 int petalo_batch_set_options(petalo_batch_t* batch, int timing, int use_graphs);
+/* persistent = 1: each subset update of the batch is one persistent launch
+ * whose workgroups take every reconstruction's forward, reduction,
+ * backprojection and update tasks from an atomic queue (each waiting, time
+ * bounded, for the previous phase of its own reconstruction); 0: one launch
+ * per phase.  No counterpart in the reference (one mlem iterator per call,
+ * src/mlem.rs:8-31).  A wait past its limit makes petalo_batch_get_image
+ * fail with PETALO_ERR_HIP. */
+int petalo_batch_set_schedule(petalo_batch_t* batch, int persistent);
 int petalo_batch_last_ms(petalo_batch_t* batch, double* ms);
 int petalo_batch_destroy(petalo_batch_t* batch);
 

@@ -186,6 +186,7 @@ _SIGS = {
     "petalo_mlem_work_items": (C.c_int, [_P, _P, _P]),
     "petalo_batch_get_image": (C.c_int, [_P, C.c_size_t, _P]),
     "petalo_batch_set_options": (C.c_int, [_P, C.c_int, C.c_int]),
+    "petalo_batch_set_schedule": (C.c_int, [_P, C.c_int]),
     "petalo_batch_last_ms": (C.c_int, [_P, _P]),
     "petalo_batch_destroy": (C.c_int, [_P]),
     "petalo_make_sensitivity_image": (C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_uint64, C.c_float,
@@ -547,6 +548,11 @@ class Batch:
 
     def set_options(self, timing=False, graphs=True):
         _check(lib().petalo_batch_set_options(self._b, 1 if timing else 0, 1 if graphs else 0))
+
+    def set_schedule(self, persistent=True):
+        """persistent: one launch per subset update taking every phase's
+        tasks from a queue; False: one launch per phase."""
+        _check(lib().petalo_batch_set_schedule(self._b, 1 if persistent else 0))
 
     def last_ms(self):
         v = C.c_double(0)

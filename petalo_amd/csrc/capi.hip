@@ -793,7 +793,7 @@ void ensure_lds_attributes() {
                        (const void*)k_scatter_batch<true>,  (const void*)k_scatter_batch<false>,
                        (const void*)k_fwd_tile_batch<true>, (const void*)k_fwd_tile_batch<false>,
                        (const void*)k_lor_reduce_win<false, true>, (const void*)k_lor_reduce_win<false, false>,
-                       (const void*)k_lor_reduce_win_batch};
+                       (const void*)k_lor_reduce_win_batch, (const void*)k_batch_persistent};
   for (const void* f : fns) HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   if (done.size() <= (size_t)dev) done.resize(dev + 1, false);
   done[dev] = true;
@@ -1502,17 +1502,65 @@ struct petalo_batch {
   bool timing = false, have_timing = false;
   hipEvent_t ev[2] = {nullptr, nullptr};
   uint64_t item_target = 0;               // replayed steps per merged item; 0: the automatic rule
+  // persistent schedule (k_batch_persistent): per subset its task list and
+  // the tasks per (reconstruction, phase); one sync buffer for all
+  bool persistent = false;
+  std::vector<DevBuf<int4>> ptasks;
+  std::vector<DevBuf<uint32_t>> pneed;
+  std::vector<size_t> plds;               // per subset: dynamic LDS bytes of the launch
+  std::vector<uint32_t> preduce_lds;      // per subset: reduction window entries staged in LDS
+  DevBuf<uint32_t> psync;
+  unsigned pgrid = 0;                     // resident workgroups of the launch
 };
 
 namespace {
 
 // Work items of a whole batch per subset update.
 constexpr uint64_t kBatchItems = 3072;
+// Reduction window entries the persistent launch stages in LDS (79 KB).
+constexpr uint32_t kPersistReduceEntries = 79 * 256;
+
+// Zeroes the persistent schedule's sync words but the sticky error.
+__global__ void k_zero_sync(uint32_t* __restrict__ sync, uint32_t n) {
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x)
+    if (k != kSyncSticky) sync[k] = 0u;
+}
+
+// The rare unbounded walks of every reconstruction, one lane each.
+void batch_irregular(petalo_batch* b, size_t sub) {
+  for (petalo_mlem* c : b->ctx) {
+    const size_t n_irr = c->irregular[sub].n;
+    if (!n_irr) continue;
+    const Lor* L = c->lors.p + sub * c->set_size;
+    if (c->tof_on)
+      k_fwd_irregular<true><<<grid_for(n_irr), kBlock, 0, b->stream>>>(L, c->irregular[sub].p, (uint32_t)n_irr,
+                                                                       c->dfov, c->dtof, c->image.p, c->bp64.p,
+                                                                       c->counters.p);
+    else
+      k_fwd_irregular<false><<<grid_for(n_irr), kBlock, 0, b->stream>>>(L, c->irregular[sub].p, (uint32_t)n_irr,
+                                                                        c->dfov, c->dtof, c->image.p, c->bp64.p,
+                                                                        c->counters.p);
+  }
+  HIP_CHECK(hipGetLastError());
+}
 
 void batch_enqueue(petalo_batch* b, size_t sub) {
   hipStream_t st = b->stream;
   const uint32_t n = (uint32_t)b->ctx.size();
   k_zero_counters_batch<<<1, 256, 0, st>>>(b->update.p, n, CNT_N);
+  if (b->persistent) {
+    k_zero_sync<<<1, 256, 0, st>>>(b->psync.p, kSyncDone + 3 * n);
+    batch_irregular(b, sub);
+    const size_t nt = b->ptasks[sub].n;
+    if (nt) {
+      const unsigned grid = (unsigned)std::min<size_t>(nt, b->pgrid);
+      k_batch_persistent<<<grid, 1024, b->plds[sub], st>>>(b->ptasks[sub].p, (uint32_t)nt, b->descs[sub].p,
+                                                          b->reduce[sub].p, b->update.p, b->pneed[sub].p,
+                                                          b->psync.p, b->preduce_lds[sub]);
+      HIP_CHECK(hipGetLastError());
+    }
+    return;
+  }
   const ReconDesc* rd = b->descs[sub].p;
   if (b->items[1][sub].n)
     k_fwd_tile_batch<true><<<(unsigned)b->items[1][sub].n, 1024, b->lds_f32, st>>>(rd, b->items[1][sub].p);
@@ -1523,18 +1571,7 @@ void batch_enqueue(petalo_batch* b, size_t sub) {
     k_lor_reduce_win_batch<<<b->reduce_blocks[sub], kWinThreads, (size_t)b->reduce_lds[sub] * sizeof(float), st>>>(
         b->reduce[sub].p, n, b->reduce_lds[sub]);
   HIP_CHECK(hipGetLastError());
-  for (petalo_mlem* c : b->ctx) {
-    // the rare unbounded walks, one lane each
-    const size_t n_irr = c->irregular[sub].n;
-    if (!n_irr) continue;
-    const Lor* L = c->lors.p + sub * c->set_size;
-    if (c->tof_on)
-      k_fwd_irregular<true><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
-                                                                c->dtof, c->image.p, c->bp64.p, c->counters.p);
-    else
-      k_fwd_irregular<false><<<grid_for(n_irr), kBlock, 0, st>>>(L, c->irregular[sub].p, (uint32_t)n_irr, c->dfov,
-                                                                 c->dtof, c->image.p, c->bp64.p, c->counters.p);
-  }
+  batch_irregular(b, sub);
   if (b->items[1][sub].n)
     k_scatter_batch<true><<<(unsigned)b->items[1][sub].n, 1024, b->lds_f64, st>>>(rd, b->items[1][sub].p);
   if (b->items[0][sub].n)
@@ -1598,6 +1635,49 @@ void batch_build_subset(petalo_batch* b, size_t sub, uint64_t item_target) {
     merged.push_back(x);
   }
   std::stable_sort(merged.begin(), merged.end(), [](const Item& u, const Item& v) { return u.steps > v.steps; });
+  // the persistent schedule's tasks: reconstructions by decreasing work, each
+  // one's forward items (largest first), then its reduction blocks, its
+  // backprojection items and its update chunks, phase after phase
+  {
+    std::vector<uint64_t> rsteps(n, 0);
+    for (const Item& x : merged) rsteps[x.r] += x.steps;
+    std::vector<size_t> order(n);
+    for (size_t r = 0; r < n; ++r) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](size_t u, size_t v) { return rsteps[u] > rsteps[v]; });
+    std::vector<std::vector<int4>> per(n);
+    for (const Item& x : merged) per[x.r].push_back(make_int4(x.tile, x.a, x.e, x.tof ? 4 : 0));
+    std::vector<int4> tasks;
+    std::vector<uint32_t> need(3 * n, 0u);
+    auto head = [](uint32_t kind, size_t r, bool tof) { return (int)(kind | (tof ? 4u : 0u) | ((uint32_t)r << 3)); };
+    for (size_t r : order)
+      for (const int4& w : per[r]) {
+        tasks.push_back(make_int4(head(kTaskForward, r, w.w != 0), w.x, w.y, w.z));
+        ++need[3 * r + kTaskForward];
+      }
+    for (size_t r : order)
+      for (uint32_t blk = 0; blk < b->ctx[r]->n_blocks[sub]; ++blk) {
+        tasks.push_back(make_int4(head(kTaskReduce, r, false), (int)blk, 0, 0));
+        ++need[3 * r + kTaskReduce];
+      }
+    for (size_t r : order)
+      for (const int4& w : per[r]) {
+        tasks.push_back(make_int4(head(kTaskScatter, r, w.w != 0), w.x, w.y, w.z));
+        ++need[3 * r + kTaskScatter];
+      }
+    constexpr uint32_t kUpdateChunk = 16384;  // voxels per update task
+    for (size_t r : order) {
+      const uint32_t nv = (uint32_t)nvox(b->ctx[r]->fov);
+      for (uint32_t v = 0; v < nv; v += kUpdateChunk)
+        tasks.push_back(make_int4(head(kTaskUpdate, r, false), (int)v, (int)std::min(nv, v + kUpdateChunk), 0));
+    }
+    if (n >= (1u << 29)) throw std::runtime_error("too many reconstructions for the persistent schedule");
+    b->ptasks[sub].alloc(std::max<size_t>(tasks.size(), 1));
+    if (!tasks.empty())
+      HIP_CHECK(hipMemcpy(b->ptasks[sub].p, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
+    b->ptasks[sub].n = tasks.size();
+    b->pneed[sub].alloc(3 * n);
+    HIP_CHECK(hipMemcpy(b->pneed[sub].p, need.data(), need.size() * 4, hipMemcpyHostToDevice));
+  }
   b->descs[sub].alloc(n);
   HIP_CHECK(hipMemcpy(b->descs[sub].p, d.data(), n * sizeof(ReconDesc), hipMemcpyHostToDevice));
   b->reduce[sub].alloc(n);
@@ -1612,6 +1692,11 @@ void batch_build_subset(petalo_batch* b, size_t sub, uint64_t item_target) {
       HIP_CHECK(hipMemcpy(b->items[t][sub].p, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice));
     b->items[t][sub].n = w.size();
   }
+  // the persistent launch stages reduction windows of up to 79 KB (larger
+  // ones are gathered from memory), so that two workgroups fit a CU's LDS
+  // with the kernel's own shared variables
+  b->preduce_lds[sub] = std::min<uint32_t>(b->reduce_lds[sub], kPersistReduceEntries);
+  b->plds[sub] = std::max(std::max(b->lds_f32, b->lds_f64), (size_t)b->preduce_lds[sub] * sizeof(float));
 }
 
 // The batch's work granularity, tuned as tune_items does for one context:
@@ -1722,6 +1807,19 @@ int petalo_batch_create(petalo_batch_t** out, size_t n, const petalo_fov_t* fovs
         b->lds_f64 = std::max(b->lds_f64, (size_t)c->tiles.tvox * sizeof(double));
       }
       b->descs.resize(n_subsets);
+      b->ptasks.resize(n_subsets);
+      b->pneed.resize(n_subsets);
+      b->plds.assign(n_subsets, 0);
+      b->preduce_lds.assign(n_subsets, 0);
+      b->psync.alloc(kSyncDone + 3 * n);
+      HIP_CHECK(hipMemset(b->psync.p, 0, (kSyncDone + 3 * n) * sizeof(uint32_t)));
+      {
+        // resident workgroups of the persistent launch (its largest LDS is
+        // set per subset; two 1024-lane workgroups per CU at most)
+        int cus = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+        b->pgrid = (unsigned)std::max(1, 2 * cus);
+      }
       b->reduce.resize(n_subsets);
       b->reduce_blocks.assign(n_subsets, 0);
       b->reduce_lds.assign(n_subsets, 0);
@@ -1796,7 +1894,11 @@ int petalo_batch_get_image(petalo_batch_t* b, size_t r, float* out) {
     HIP_CHECK(hipSetDevice(b->device));
     petalo_mlem* c = b->ctx[r];
     HIP_CHECK(hipMemcpyAsync(out, c->image.p, nvox(c->fov) * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    uint32_t sticky = 0;
+    HIP_CHECK(hipMemcpyAsync(&sticky, b->psync.p + kSyncSticky, 4, hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (sticky)
+      return fail(PETALO_ERR_HIP, "a persistent-schedule update waited past its time limit: its images are not valid");
     return PETALO_OK;
   });
 }
@@ -1807,6 +1909,23 @@ int petalo_batch_set_options(petalo_batch_t* b, int timing, int use_graphs) {
   b->use_graphs = use_graphs != 0;
   b->have_timing = false;
   return PETALO_OK;
+}
+
+int petalo_batch_set_schedule(petalo_batch_t* b, int persistent) {
+  return guarded([&] {
+    if (!b) return fail(PETALO_ERR_ARG, "batch is NULL");
+    HIP_CHECK(hipSetDevice(b->device));
+    if ((persistent != 0) != b->persistent) {
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      for (auto& g : b->graphs)
+        if (g) {
+          (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
+      b->persistent = persistent != 0;
+    }
+    return PETALO_OK;
+  });
 }
 
 int petalo_batch_last_ms(petalo_batch_t* b, double* ms) {

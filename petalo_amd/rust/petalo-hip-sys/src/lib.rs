@@ -307,6 +307,7 @@ extern "C" {
     ) -> c_int;
     pub fn petalo_batch_get_image(batch: *mut petalo_batch_t, r: usize, image_out: *mut f32) -> c_int;
     pub fn petalo_batch_set_options(batch: *mut petalo_batch_t, timing: c_int, use_graphs: c_int) -> c_int;
+    pub fn petalo_batch_set_schedule(batch: *mut petalo_batch_t, persistent: c_int) -> c_int;
     pub fn petalo_batch_last_ms(batch: *mut petalo_batch_t, ms: *mut f64) -> c_int;
     pub fn petalo_batch_destroy(batch: *mut petalo_batch_t) -> c_int;
 
