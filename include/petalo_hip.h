@@ -346,6 +346,13 @@ int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const fl
                                            const petalo_fov_t* out_fov_or_null, float detector_length_mm,
                                            float r_min_mm, float dr_mm, float dz_mm, float da_mm, int32_t adjust,
                                            uint64_t seed, float* out, petalo_stats_t* stats_or_null);
+/* The two calls above keep their device buffers and stream (a few GB for
+ * 2^24-LOR batches: the LORs, their runs and the scans) per device between
+ * calls, which saves about 4 ms of allocation and release per call.  No
+ * counterpart in the reference (make_sensitivity_image runs once per
+ * process there): frees every kept set, on every device.  Calls running at
+ * the same time keep theirs. */
+int petalo_release_workspaces(void);
 /* petalo_make_sensitivity_image for caller-given LORs on one FOV, allocating
  * nothing.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops).

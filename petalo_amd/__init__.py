@@ -193,6 +193,7 @@ _SIGS = {
                                                 C.c_float, _P, _P]),
     "petalo_sensitivity_lors": (C.c_int, [_P, C.c_size_t, C.c_uint64, C.c_float, C.c_float, _P]),
     "petalo_sensitivity_image_noalloc": (C.c_int, [_P, _P, _P, C.c_size_t, _P, _P]),
+    "petalo_release_workspaces": (C.c_int, []),
     "petalo_density_to_attenuation": (C.c_int, [_P, C.c_size_t, C.c_float]),
     "petalo_system_matrix_rows": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, _P, _P, _P, _P]),
     "petalo_image3d_write": (C.c_int, [C.c_char_p, _P, _P]),
@@ -603,6 +604,12 @@ def make_sensitivity_image(attenuation, n_lors, detector_length=None, detector_d
         lp, int(n_lors), int(seed), L, D, _ptr(out), C.byref(st) if return_stats else None))
     img = Image(out_fov, out)
     return (img, _stats(st)) if return_stats else img
+
+
+def release_workspaces():
+    """Free the device buffers the sensitivity calls keep between calls (one
+    set per device, a few GB for their 2^24-LOR batches)."""
+    _check(lib().petalo_release_workspaces())
 
 
 def make_sensitivity_image_discrete(attenuation, detector_length, r_min, dr, dz, da,

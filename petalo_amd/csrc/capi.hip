@@ -11,7 +11,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <numeric>
@@ -2063,15 +2065,18 @@ namespace {
 constexpr uint32_t kSensBatch = 1u << 24;
 // Work item size of the sensitivity kernels, in runs.
 constexpr uint32_t kSensItemRuns = 8192;
+// Most length classes of a sensitivity batch's segment binning.
+constexpr uint32_t kSensClasses = 4;
 
 // Unbounded walks (count 0xffffffff): copied out for the one-lane kernel,
 // their count set to 0.  counts[n] is set to 0 for the exclusive scan.
 // Sort key of a sensitivity LOR: about the number of steps of its walk (the
 // crossings of each axis before the first axis runs out, from the boundary
 // arithmetic; an estimate, for ordering only), so that the binning kernel's
-// lanes, which take several LORs each, get walks of similar lengths.
-__global__ void k_walk_len_keys(const Lor* __restrict__ lors, uint32_t n, DevFov f, uint32_t* __restrict__ keys,
-                                uint32_t* __restrict__ idx) {
+// lanes, which take several LORs each, get walks of similar lengths.  The
+// estimate >> shift, at most 255: one 8-bit radix pass sorts the batch.
+__global__ void k_walk_len_keys(const Lor* __restrict__ lors, uint32_t n, DevFov f, uint32_t shift,
+                                uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Lor l = load_lor(lors, i);
@@ -2089,7 +2094,7 @@ __global__ void k_walk_len_keys(const Lor* __restrict__ lors, uint32_t n, DevFov
       est += q;
     }
   }
-  keys[i] = (uint32_t)fminf(est, 65535.0f);
+  keys[i] = min((uint32_t)fminf(est, 65535.0f) >> shift, 255u);
   idx[i] = i;
 }
 
@@ -2112,8 +2117,9 @@ __global__ void k_take_irregular(uint32_t* __restrict__ counts, uint32_t n, cons
 
 
 // Tile counts up to which a sensitivity batch is binned straight into tile
-// order (k_bin_runs_seg keeps two cursors and the bounds of each tile's
-// segment in LDS); above it the runs are binned LOR-major and sorted by tile.
+// order (k_bin_runs_seg keeps a cursor and the end of each (tile, length
+// class) segment in LDS); above it the runs are binned LOR-major and sorted
+// by tile.
 constexpr uint32_t kSegMaxTiles = 2048;
 
 // Work items of a sensitivity batch binned by k_bin_runs_seg: tile t's runs
@@ -2174,13 +2180,68 @@ __global__ __launch_bounds__(1024) void k_seg_items(const uint32_t* __restrict__
 // every step instead of jumping from tile to tile).
 using LorSource = std::function<void(Lor*, uint64_t, uint32_t, hipStream_t)>;
 
-// A device buffer that only grows (kept across the batches of one call).
+// A device buffer that only grows.
 template <class T> struct Grow {
   DevBuf<T> b;
   T* ensure(size_t n) {
     if (n > b.n) b.alloc(n + n / 4);
     return b.p;
   }
+};
+
+// Device buffers and stream of the tiled sensitivity image, kept per device
+// between calls: the batch buffers of 2^24 LORs and the runs of a batch
+// come to a few GB, and allocating and freeing them took about 4 ms of a
+// 50 ms call.  A call leases its device's set (SensLease), or a fresh one
+// when a concurrent call holds it; a set given back when the slot is taken,
+// or after an error, is freed.  petalo_release_workspaces frees the kept
+// sets.
+struct SensWork {
+  hipStream_t stream = nullptr;
+  Grow<float> dmu, dout, rbuf, partial;
+  Grow<double> acc;
+  Grow<unsigned long long> cnt, rstat;
+  Grow<Lor> dl, lb, lraw, irr;
+  Grow<uint32_t> counts, offs, n_irr, seg, soff, lk, lk2, li, li2, perm, perm2, pos;
+  Grow<uint4> pos4;
+  Grow<unsigned char> lsort_tmp, scan_tmp, sort_tmp;
+  Grow<uint64_t> dstart, keys, keys2;
+  Grow<Run> unsorted;
+  Grow<int4> work;
+  SensWork() { HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+  ~SensWork() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  SensWork(const SensWork&) = delete;
+  SensWork& operator=(const SensWork&) = delete;
+};
+
+constexpr int kSensDevices = 64;
+std::mutex g_sens_mu;
+std::unique_ptr<SensWork> g_sens_work[kSensDevices];
+
+struct SensLease {
+  int dev = -1;
+  std::unique_ptr<SensWork> w;
+  SensLease() {
+    int d = 0;
+    HIP_CHECK(hipGetDevice(&d));
+    if (d >= 0 && d < kSensDevices) {
+      std::lock_guard<std::mutex> lk(g_sens_mu);
+      dev = d;
+      w = std::move(g_sens_work[d]);
+    }
+    if (!w) w.reset(new SensWork());
+  }
+  ~SensLease() {
+    // after an error the set's stream may hold failed work: drop it
+    if (dev < 0 || std::uncaught_exceptions() > 0) return;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(g_sens_mu);
+    if (!g_sens_work[dev]) g_sens_work[dev] = std::move(w);
+  }
+  SensLease(const SensLease&) = delete;
+  SensLease& operator=(const SensLease&) = delete;
 };
 
 // Work items of a sensitivity batch, built on the device from the tile
@@ -2218,19 +2279,44 @@ __global__ void k_sens_items(const uint64_t* __restrict__ start, uint32_t ntiles
   for (uint32_t k = n_items + threadIdx.x; k < cap; k += blockDim.x) items[k] = make_int4(0, 0, 0, 0);
 }
 
-void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const float* mu_host, const float* dmu,
+// Whether every value of mu is zero (either sign); scanned in blocks, so a
+// non-zero image ends the scan early.
+bool all_zero(const float* mu, size_t n) {
+  const size_t kBlk = 4096;
+  for (size_t a = 0; a < n; a += kBlk) {
+    const size_t e = std::min(n, a + kBlk);
+    uint32_t any = 0;
+    for (size_t k = a; k < e; ++k) {
+      uint32_t b;
+      std::memcpy(&b, mu + k, 4);
+      any |= b & 0x7fffffffu;
+    }
+    if (any) return false;
+  }
+  return true;
+}
+
+// mu_zero: the attenuation image (dmu on the device) is zero everywhere.
+void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_zero, const float* dmu,
                 uint64_t n_lors, const LorSource& produce, double* acc, unsigned long long* cnt, bool by_length,
                 bool want_stats) {
+  hipStream_t s = ws.stream;
   const int32_t no_shape[3] = {0, 0, 0};
   const TileGeom g = make_tiles(fov, no_shape);
-  const size_t na = nvox(fov);
-  bool mu_zero = true;
-  for (size_t k = 0; k < na && mu_zero; ++k) mu_zero = mu_host[k] == 0.0f;
   // tile order straight from the binning (segments), or LOR-major runs sorted by tile
   const bool segs = (uint32_t)g.ntiles <= kSegMaxTiles;
   const uint32_t nt = (uint32_t)g.ntiles;
-  // runs this long or longer fill their segment from the start, shorter ones from the end
-  const uint32_t long_run = (uint32_t)(g.tx + g.ty + g.tz) / 4u;
+  // length classes of the segment binning: as many as fit 8 bytes per (tile,
+  // class) in 32 KB of the fill pass's LDS, at most kSensClasses
+  RunClasses rc;
+  if (segs) {
+    while (rc.n < kSensClasses && (size_t)nt * rc.n * 2 <= 4096) rc.n *= 2;
+    rc.scale = (double)rc.n / (double)(g.tx + g.ty + g.tz);
+  }
+  const uint32_t ncls = rc.n;
+  // walk lengths are below nx + ny + nz steps: the shift that brings them under 256
+  uint32_t len_shift = 0;
+  while ((((uint64_t)fov.n[0] + fov.n[1] + fov.n[2]) >> len_shift) > 255u) ++len_shift;
   const uint32_t shift = mu_zero ? 32u : kLorBlockShift;
   const uint32_t tshift = mu_zero ? 12u : 32u;
   int tile_bits = 1;
@@ -2240,140 +2326,140 @@ void sens_tiled(hipStream_t s, const petalo_fov_t& fov, const DevFov& f, const f
   ensure_lds_attributes();
   const size_t bmax = (size_t)std::min<uint64_t>(n_lors, kSensBatch);
   const size_t ng_max = bin_grid(bmax);
-  const size_t n_seg = segs ? (size_t)nt * ng_max + 1 : 0;
+  const size_t n_seg = segs ? (size_t)nt * ncls * ng_max + 1 : 0;
   // buffers of the whole call: batch-sized ones now, run-sized ones grown
   // to the largest batch (no allocation, and one wait, per batch)
   // by length, the segment binning reads lraw through the sort's order and needs no copy in lb
-  DevBuf<Lor> lb(by_length && segs ? 0 : bmax), lraw(by_length ? bmax : 0), irr(bmax);
-  DevBuf<uint32_t> counts(bmax + 1), offs(bmax + 1), n_irr(1), seg(n_seg), soff(n_seg);
-  DevBuf<uint32_t> lk(by_length ? bmax : 0), lk2(by_length ? bmax : 0), li(by_length ? bmax : 0),
-      li2(by_length ? bmax : 0);
-  DevBuf<float> rbuf;
-  DevBuf<uint4> pos4;
-  DevBuf<unsigned long long> rstat(kRStatWords);  // each batch's r statistics
-  if (!mu_zero) {
-    rbuf.alloc(bmax);
-    pos4.alloc(bmax);
-  }
+  Lor* lb = ws.lb.ensure(by_length && segs ? 0 : bmax);
+  Lor* lraw = ws.lraw.ensure(by_length ? bmax : 0);
+  Lor* irr = ws.irr.ensure(bmax);
+  uint32_t* counts = ws.counts.ensure(bmax + 1);
+  uint32_t* offs = ws.offs.ensure(bmax + 1);
+  uint32_t* n_irr = ws.n_irr.ensure(1);
+  uint32_t* seg = ws.seg.ensure(n_seg);
+  uint32_t* soff = ws.soff.ensure(n_seg);
+  uint32_t* lk = ws.lk.ensure(by_length ? bmax : 0);
+  uint32_t* lk2 = ws.lk2.ensure(by_length ? bmax : 0);
+  uint32_t* li = ws.li.ensure(by_length ? bmax : 0);
+  uint32_t* li2 = ws.li2.ensure(by_length ? bmax : 0);
+  float* rbuf = ws.rbuf.ensure(mu_zero ? 0 : bmax);
+  uint4* pos4 = ws.pos4.ensure(mu_zero ? 0 : bmax);
+  unsigned long long* rstat = ws.rstat.ensure(kRStatWords);  // each batch's r statistics
   size_t lsort_bytes = 0, scan_bytes = 0, seg_scan_bytes = 0;
   if (by_length)
-    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, lsort_bytes, lk.p, lk2.p, li.p, li2.p, (int)bmax, 0, 16, s));
-  HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts.p, offs.p, (int)bmax + 1, s));
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, lsort_bytes, lk, lk2, li, li2, (int)bmax, 0, 8, s));
+  HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts, offs, (int)bmax + 1, s));
   if (segs) {
     if (n_seg >= (1ull << 31)) throw std::runtime_error("too many (workgroup, tile) segments in a sensitivity batch");
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, seg_scan_bytes, seg.p, soff.p, (int)n_seg, s));
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, seg_scan_bytes, seg, soff, (int)n_seg, s));
     scan_bytes = std::max(scan_bytes, seg_scan_bytes);
   }
-  DevBuf<unsigned char> lsort_tmp(std::max<size_t>(lsort_bytes, 1)), scan_tmp(std::max<size_t>(scan_bytes, 1));
-  DevBuf<uint64_t> dstart(segs ? 0 : g.ntiles);
-  Grow<Run> unsorted;
-  Grow<uint64_t> keys, keys2;
-  Grow<uint32_t> perm, perm2, pos;
-  Grow<float> partial;
-  Grow<unsigned char> sort_tmp;
-  Grow<int4> work;
+  unsigned char* lsort_tmp = ws.lsort_tmp.ensure(std::max<size_t>(lsort_bytes, 1));
+  unsigned char* scan_tmp = ws.scan_tmp.ensure(std::max<size_t>(scan_bytes, 1));
+  uint64_t* dstart = ws.dstart.ensure(segs ? 0 : g.ntiles);
   for (uint64_t first = 0; first < n_lors; first += kSensBatch) {
     const uint32_t b = (uint32_t)std::min<uint64_t>(kSensBatch, n_lors - first);
     const uint32_t ng = bin_grid(b);
-    produce(by_length ? lraw.p : lb.p, first, b, s);
+    produce(by_length ? lraw : lb, first, b, s);
     HIP_CHECK(hipGetLastError());
     if (by_length) {
       // the batch in order of walk length (the image is a sum over LORs)
-      k_walk_len_keys<<<grid_for(b), kBlock, 0, s>>>(lraw.p, b, f, lk.p, li.p);
+      k_walk_len_keys<<<grid_for(b), kBlock, 0, s>>>(lraw, b, f, len_shift, lk, li);
       HIP_CHECK(hipGetLastError());
       size_t sb = lsort_bytes;
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(lsort_tmp.p, sb, lk.p, lk2.p, li.p, li2.p, (int)b, 0, 16, s));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(lsort_tmp, sb, lk, lk2, li, li2, (int)b, 0, 8, s));
       // the segment binning reads the batch through the order; the sorted
       // binning walks a reordered copy
-      if (!segs) k_gather_lors_by<<<grid_for(b), kBlock, 0, s>>>(lraw.p, li2.p, b, lb.p);
+      if (!segs) k_gather_lors_by<<<grid_for(b), kBlock, 0, s>>>(lraw, li2, b, lb);
       HIP_CHECK(hipGetLastError());
     }
-    const Lor* src_lors = (by_length && segs) ? lraw.p : lb.p;
-    const uint32_t* lor_order = (by_length && segs) ? li2.p : nullptr;
-    const size_t nsb = (size_t)nt * ng + 1;  // this batch's segments and the closing zero
+    const Lor* src_lors = (by_length && segs) ? lraw : lb;
+    const uint32_t* lor_order = (by_length && segs) ? li2 : nullptr;
+    const size_t nsb = (size_t)nt * ncls * ng + 1;  // this batch's segments and the closing zero
     if (segs) {
-      HIP_CHECK(hipMemsetAsync(seg.p + (nsb - 1), 0, 4, s));
+      HIP_CHECK(hipMemsetAsync(seg + (nsb - 1), 0, 4, s));
+      const size_t lds = (size_t)nt * ncls * 4;
       if (want_stats)
-        k_bin_runs_seg<false, true><<<ng, kBlock, (size_t)nt * 4, s>>>(src_lors, lor_order, b, f, g, counts.p, seg.p,
-                                                                       nullptr, nullptr, nullptr, nullptr, 0u, cnt);
+        k_bin_runs_seg<false, true><<<ng, kBlock, lds, s>>>(src_lors, lor_order, b, f, g, rc, counts, seg, nullptr,
+                                                            nullptr, nullptr, nullptr, cnt);
       else
-        k_bin_runs_seg<false, false><<<ng, kBlock, (size_t)nt * 4, s>>>(src_lors, lor_order, b, f, g, counts.p, seg.p,
-                                                                        nullptr, nullptr, nullptr, nullptr, 0u, cnt);
+        k_bin_runs_seg<false, false><<<ng, kBlock, lds, s>>>(src_lors, lor_order, b, f, g, rc, counts, seg, nullptr,
+                                                             nullptr, nullptr, nullptr, cnt);
     } else {
-      k_bin_runs<false, false><<<ng, kBlock, 0, s>>>(lb.p, b, f, notof, g, counts.p, nullptr, nullptr, nullptr,
+      k_bin_runs<false, false><<<ng, kBlock, 0, s>>>(lb, b, f, notof, g, counts, nullptr, nullptr, nullptr,
                                                       shift, cnt);
     }
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemsetAsync(n_irr.p, 0, 4, s));
-    k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts.p, b, src_lors, lor_order, irr.p, n_irr.p);
+    HIP_CHECK(hipMemsetAsync(n_irr, 0, 4, s));
+    k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts, b, src_lors, lor_order, irr, n_irr);
     HIP_CHECK(hipGetLastError());
     uint32_t total = 0, h_irr = 0;
     if (!segs || !mu_zero) {
       size_t tb = scan_bytes;
-      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tb, counts.p, offs.p, (int)b + 1, s));
+      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb, counts, offs, (int)b + 1, s));
     }
     if (segs) {
       size_t tb = scan_bytes;
-      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, tb, seg.p, soff.p, (int)nsb, s));
-      HIP_CHECK(hipMemcpyAsync(&total, soff.p + (nsb - 1), 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb, seg, soff, (int)nsb, s));
+      HIP_CHECK(hipMemcpyAsync(&total, soff + (nsb - 1), 4, hipMemcpyDeviceToHost, s));
     } else {
-      HIP_CHECK(hipMemcpyAsync(&total, offs.p + b, 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipMemcpyAsync(&total, offs + b, 4, hipMemcpyDeviceToHost, s));
     }
-    HIP_CHECK(hipMemcpyAsync(&h_irr, n_irr.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&h_irr, n_irr, 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));  // the batch's run count sizes the run buffers
     if (total >= (1u << 31)) throw std::runtime_error("2^31 or more tile runs in one sensitivity batch");
     if (total) {
       const uint32_t cap = total / kSensItemRuns + nt + 1;
-      int4* items = work.ensure(cap);
-      Run* un = unsorted.ensure(total);
+      int4* items = ws.work.ensure(cap);
+      Run* un = ws.unsorted.ensure(total);
       // sorted position of every LOR-major run (attenuation present)
-      uint32_t* ps = mu_zero ? nullptr : pos.ensure(total);
+      uint32_t* ps = mu_zero ? nullptr : ws.pos.ensure(total);
       uint32_t* order = nullptr;  // runs in tile order are un[order[j]]; nullptr: un[j]
       if (segs) {
-        k_bin_runs_seg<true, false><<<ng, kBlock, (size_t)nt * 16, s>>>(src_lors, lor_order, b, f, g, nullptr, seg.p,
-                                                                        soff.p, offs.p, un, ps, long_run, cnt);
-        k_seg_items<<<1, 1024, 0, s>>>(soff.p, ng, nt, kSensItemRuns, cap, items);
+        k_bin_runs_seg<true, false><<<ng, kBlock, (size_t)nt * ncls * 8, s>>>(src_lors, lor_order, b, f, g, rc, nullptr,
+                                                                              seg, soff, offs, un, ps, cnt);
+        // tile t's runs: segments [t * ncls * ng, (t + 1) * ncls * ng)
+        k_seg_items<<<1, 1024, 0, s>>>(soff, ncls * ng, nt, kSensItemRuns, cap, items);
         HIP_CHECK(hipGetLastError());
       } else {
-        uint64_t* k1 = keys.ensure(total);
-        uint64_t* k2 = keys2.ensure(total);
-        uint32_t* p1 = perm.ensure(total);
-        uint32_t* p2 = perm2.ensure(total);
-        k_bin_runs<false, true><<<ng, kBlock, 0, s>>>(lb.p, b, f, notof, g, nullptr, offs.p, un, k1, shift, cnt);
+        uint64_t* k1 = ws.keys.ensure(total);
+        uint64_t* k2 = ws.keys2.ensure(total);
+        uint32_t* p1 = ws.perm.ensure(total);
+        uint32_t* p2 = ws.perm2.ensure(total);
+        k_bin_runs<false, true><<<ng, kBlock, 0, s>>>(lb, b, f, notof, g, nullptr, offs, un, k1, shift, cnt);
         k_iota<<<grid64(total), kBlock, 0, s>>>(p1, total);
         HIP_CHECK(hipGetLastError());
         size_t sort_bytes = 0;
         HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k1, k2, p1, p2, (int)total, 0, end_bit, s));
-        unsigned char* stmp = sort_tmp.ensure(std::max<size_t>(sort_bytes, 1));
+        unsigned char* stmp = ws.sort_tmp.ensure(std::max<size_t>(sort_bytes, 1));
         HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp, sort_bytes, k1, k2, p1, p2, (int)total, 0, end_bit, s));
         if (ps) k_invert_perm<<<grid64(total), kBlock, 0, s>>>(p2, total, ps);
-        HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, (size_t)g.ntiles * 8, s));
-        k_tile_starts<<<grid64(total), kBlock, 0, s>>>(k2, total, tshift, nt, dstart.p);
-        k_sens_items<<<1, 256, 0, s>>>(dstart.p, nt, total, kSensItemRuns, cap, items);
+        HIP_CHECK(hipMemsetAsync(dstart, 0xff, (size_t)g.ntiles * 8, s));
+        k_tile_starts<<<grid64(total), kBlock, 0, s>>>(k2, total, tshift, nt, dstart);
+        k_sens_items<<<1, 256, 0, s>>>(dstart, nt, total, kSensItemRuns, cap, items);
         HIP_CHECK(hipGetLastError());
         order = p2;
       }
       if (!mu_zero) {
-        float* part = partial.ensure(total);
+        float* part = ws.partial.ensure(total);
         k_fwd_tile<false><<<cap, 1024, ((size_t)g.tvox + 1) * sizeof(float), s>>>(un, items, dmu, f, notof, g, part,
                                                                                    order, nullptr);
-        k_pack_pos4<<<grid_for(b), kBlock, 0, s>>>(offs.p, ps, b, pos4.p);
+        k_pack_pos4<<<grid_for(b), kBlock, 0, s>>>(offs, ps, b, pos4);
         // this batch's r statistics, for the backprojection's accumulator
-        k_zero_u64<<<1, 320, 0, s>>>(rstat.p, kRStatWords);
-        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs.p, ps, pos4.p, part, rbuf.p,
-                                                             rstat.p);
+        k_zero_u64<<<1, 320, 0, s>>>(rstat, kRStatWords);
+        k_lor_reduce<true><<<reduce_grid(b), kBlock, 0, s>>>(nullptr, b, offs, ps, pos4, part, rbuf,
+                                                             rstat);
         HIP_CHECK(hipGetLastError());
       }
-      k_scatter<false><<<cap, 1024, (size_t)g.tvox * sizeof(double), s>>>(un, items, mu_zero ? nullptr : rbuf.p, f,
-                                                                          notof, g, acc, order, rstat.p);
+      k_scatter<false><<<cap, 1024, (size_t)g.tvox * sizeof(double), s>>>(un, items, mu_zero ? nullptr : rbuf, f,
+                                                                          notof, g, acc, order, rstat);
       HIP_CHECK(hipGetLastError());
     }
     if (h_irr) {
-      k_sens<false, false><<<grid_for(h_irr), kBlock, 0, s>>>(irr.p, h_irr, 0, 0.0f, 0.0f, f, dmu, f, acc, cnt);
+      k_sens<false, false><<<grid_for(h_irr), kBlock, 0, s>>>(irr, h_irr, 0, 0.0f, 0.0f, f, dmu, f, acc, cnt);
       HIP_CHECK(hipGetLastError());
     }
   }
-  HIP_CHECK(hipStreamSynchronize(s));  // the buffers above are released at scope end
 }
 
 }  // namespace
@@ -2419,40 +2505,43 @@ int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const fl
     const DevFov fa = to_dev(*att_fov);
     const DevFov fo = to_dev(out_fov ? *out_fov : *att_fov);
     const size_t na = nvox(*att_fov), no = nvox(out_fov ? *out_fov : *att_fov);
-    hipStream_t s;
-    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    struct StreamGuard {
-      hipStream_t s;
-      ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } sg{s};
-    DevBuf<float> dmu(na), dout(no);
-    DevBuf<double> acc(no);
-    DevBuf<unsigned long long> cnt(CNT_N);
-    HIP_CHECK(hipMemcpyAsync(dmu.p, mu, na * sizeof(float), hipMemcpyHostToDevice, s));
-    fill_f64(acc.p, 0.0, no, s);
-    zero_counters(cnt.p, s);
+    SensLease lease;  // the device's kept buffers and stream
+    SensWork& ws = *lease.w;
+    hipStream_t s = ws.stream;
+    float* dmu = ws.dmu.ensure(na);
+    float* dout = ws.dout.ensure(no);
+    double* acc = ws.acc.ensure(no);
+    unsigned long long* cnt = ws.cnt.ensure(CNT_N);
+    // an image of zeros is cleared on the device rather than copied
+    const bool mu_zero = all_zero(mu, na);
+    if (mu_zero)
+      HIP_CHECK(hipMemsetAsync(dmu, 0, na * sizeof(float), s));
+    else
+      HIP_CHECK(hipMemcpyAsync(dmu, mu, na * sizeof(float), hipMemcpyHostToDevice, s));
+    fill_f64(acc, 0.0, no, s);
+    zero_counters(cnt, s);
     LorSource src = [&](Lor* dst, uint64_t first, uint32_t n, hipStream_t st) {
       k_pair_lors<<<grid_for(n), kBlock, 0, st>>>(first, n, el, seed, dst);
     };
     if (!out_fov) {
-      sens_tiled(s, *att_fov, fa, mu, dmu.p, n_pairs, src, acc.p, cnt.p, false, stats != nullptr);
+      sens_tiled(ws, *att_fov, fa, mu_zero, dmu, n_pairs, src, acc, cnt, false, stats != nullptr);
     } else {
       // two grids: one lane per LOR, binary64 global atomics
       DevBuf<Lor> lb((size_t)std::min<uint64_t>(n_pairs, kSensBatch));
       for (uint64_t first = 0; first < n_pairs; first += kSensBatch) {
         const uint32_t b = (uint32_t)std::min<uint64_t>(kSensBatch, n_pairs - first);
         src(lb.p, first, b, s);
-        k_sens<false, true><<<grid_for(b), kBlock, 0, s>>>(lb.p, b, 0, 0.0f, 0.0f, fa, dmu.p, fo, acc.p, cnt.p);
+        k_sens<false, true><<<grid_for(b), kBlock, 0, s>>>(lb.p, b, 0, 0.0f, 0.0f, fa, dmu, fo, acc, cnt);
         HIP_CHECK(hipGetLastError());
       }
     }
     // normalised by n (n - 1) / 2 over the element count (discrete.rs:16-20)
     const double norm = (double)npts * (double)(npts - 1) / 2.0;
-    k_sens_finalize<<<grid_for(no), kBlock, 0, s>>>(acc.p, dout.p, (float)norm, no);
+    k_sens_finalize<<<grid_for(no), kBlock, 0, s>>>(acc, dout, (float)norm, no);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(out, dout.p, no * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(out, dout, no * sizeof(float), hipMemcpyDeviceToHost, s));
     unsigned long long h[CNT_N];
-    HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (h[CNT_RUN_MISMATCH])
       throw std::runtime_error("tile binning: fill and count passes disagree on " + std::to_string(h[CNT_RUN_MISMATCH]) +
@@ -2479,23 +2568,26 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
     const DevFov fa = to_dev(*att_fov);
     const DevFov fo = to_dev(out_fov ? *out_fov : *att_fov);
     const size_t na = nvox(*att_fov), no = nvox(out_fov ? *out_fov : *att_fov);
-    hipStream_t s;
-    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    struct StreamGuard {
-      hipStream_t s;
-      ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } sg{s};
-    DevBuf<float> dmu(na), dout(no);
-    DevBuf<double> acc(no);
-    DevBuf<unsigned long long> cnt(CNT_N);
+    SensLease lease;  // the device's kept buffers and stream
+    SensWork& ws = *lease.w;
+    hipStream_t s = ws.stream;
+    float* dmu = ws.dmu.ensure(na);
+    float* dout = ws.dout.ensure(no);
+    double* acc = ws.acc.ensure(no);
+    unsigned long long* cnt = ws.cnt.ensure(CNT_N);
     DevBuf<Lor> dl;
-    HIP_CHECK(hipMemcpyAsync(dmu.p, mu, na * sizeof(float), hipMemcpyHostToDevice, s));
+    // an image of zeros is cleared on the device rather than copied
+    const bool mu_zero = all_zero(mu, na);
+    if (mu_zero)
+      HIP_CHECK(hipMemsetAsync(dmu, 0, na * sizeof(float), s));
+    else
+      HIP_CHECK(hipMemcpyAsync(dmu, mu, na * sizeof(float), hipMemcpyHostToDevice, s));
     if (lors && n_lors) {
       dl.alloc(n_lors);
       HIP_CHECK(hipMemcpyAsync(dl.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice, s));
     }
-    fill_f64(acc.p, 0.0, no, s);
-    zero_counters(cnt.p, s);
+    fill_f64(acc, 0.0, no, s);
+    zero_counters(cnt, s);
     const bool two = out_fov != nullptr;
     if (n_lors) {
       const unsigned g = grid_for(n_lors);
@@ -2507,21 +2599,21 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
           HIP_CHECK(hipMemcpyAsync(dst, dl.p + first, (size_t)cnt_ * sizeof(Lor), hipMemcpyDeviceToDevice, st));
         })
                              : LorSource([&](Lor* dst, uint64_t first, uint32_t cnt_, hipStream_t st) {
-          k_sens_lors<<<sens_lors_grid(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst, cnt.p);
+          k_sens_lors<<<sens_lors_grid(cnt_), kBlock, 0, st>>>(cnt_, (uint32_t)first, seed, fa, L, Rr, dst, cnt);
         });
-        sens_tiled(s, *att_fov, fa, mu, dmu.p, n_lors, src, acc.p, cnt.p, true, stats != nullptr);
+        sens_tiled(ws, *att_fov, fa, mu_zero, dmu, n_lors, src, acc, cnt, true, stats != nullptr);
       } else {
         // two FOVs: one lane per LOR, binary64 global atomics
-        if (lors) k_sens<false, true><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
-        else      k_sens<true, true><<<g, kBlock, 0, s>>>(nullptr, n, seed, L, R, fa, dmu.p, fo, acc.p, cnt.p);
+        if (lors) k_sens<false, true><<<g, kBlock, 0, s>>>(dl.p, n, seed, L, R, fa, dmu, fo, acc, cnt);
+        else      k_sens<true, true><<<g, kBlock, 0, s>>>(nullptr, n, seed, L, R, fa, dmu, fo, acc, cnt);
       }
       HIP_CHECK(hipGetLastError());
     }
-    k_sens_finalize<<<grid_for(no), kBlock, 0, s>>>(acc.p, dout.p, (float)n_lors, no);
+    k_sens_finalize<<<grid_for(no), kBlock, 0, s>>>(acc, dout, (float)n_lors, no);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(out, dout.p, no * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(out, dout, no * sizeof(float), hipMemcpyDeviceToHost, s));
     unsigned long long h[CNT_N];
-    HIP_CHECK(hipMemcpyAsync(h, cnt.p, sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (h[CNT_RUN_MISMATCH])
       throw std::runtime_error("tile binning: fill and count passes disagree on " + std::to_string(h[CNT_RUN_MISMATCH]) +
@@ -2532,6 +2624,25 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
     if (stats)
       *stats = petalo_stats_t{(uint64_t)n_lors, h[CNT_STEPS], h[CNT_NZ], h[CNT_SKIPPED], h[CNT_MISSED],
                               h[CNT_RUNAWAY], h[CNT_SHIFTED]};
+    return PETALO_OK;
+  });
+}
+
+int petalo_release_workspaces(void) {
+  return guarded([&] {
+    std::unique_ptr<SensWork> kept[kSensDevices];
+    {
+      std::lock_guard<std::mutex> lk(g_sens_mu);
+      for (int d = 0; d < kSensDevices; ++d) kept[d] = std::move(g_sens_work[d]);
+    }
+    int cur = 0;
+    HIP_CHECK(hipGetDevice(&cur));
+    for (int d = 0; d < kSensDevices; ++d) {
+      if (!kept[d]) continue;
+      HIP_CHECK(hipSetDevice(d));  // each set is freed on its own device
+      kept[d].reset();
+    }
+    HIP_CHECK(hipSetDevice(cur));
     return PETALO_OK;
   });
 }

@@ -203,6 +203,31 @@ __host__ __device__ __forceinline__ uint32_t bin_lor(const Lor& l, uint32_t i, c
   return count;
 }
 
+// Length classes of runs, by which the sensitivity batches order each
+// tile's runs (so that a replaying wavefront's lanes get runs of about one
+// length): n classes of an estimated step count s, class min(n - 1, s *
+// scale) with scale = n / (tx + ty + tz).  The count pass (tile_runs) and the
+// fill pass (bin_lor_jump) must put every run in the same class, so s comes
+// from values both compute alike: the binary64 estimates of the crossings
+// that enter (h) and leave (b) the tile, s = (b - h) * (1/vl_x + 1/vl_y +
+// 1/vl_z); a walk both step through (degenerate LORs) uses its run's exact
+// length.
+struct RunClasses {
+  uint32_t n = 1;
+  double scale = 0.0;
+  __host__ __device__ __forceinline__ uint32_t of_steps(double st) const {
+    const double c = st * scale;
+    return c >= 1.0 ? (c < (double)(n - 1) ? (uint32_t)c : n - 1) : 0u;
+  }
+  __host__ __device__ __forceinline__ uint32_t of_crossings(double h, double b, double inv_vl_sum) const {
+    return of_steps((b - h) * inv_vl_sum);
+  }
+};
+
+__host__ __device__ __forceinline__ double inv_vl_sum(const WalkC& c) {
+  return (double)(1.0f / c.vl_x) + (double)(1.0f / c.vl_y) + (double)(1.0f / c.vl_z);
+}
+
 // The tiles in which bin_lor<false> emits a run for LOR i, in walk order,
 // without walking its steps: calls on_run(tile) for each and returns their
 // number.  The walk leaves a tile at the first, in (position, axis x < y <
@@ -228,12 +253,15 @@ __host__ __device__ __forceinline__ double crossing_estimate(float nb, float vl,
 
 template <class OnRun>
 __host__ __device__ __forceinline__ uint32_t tile_runs(const Lor& l, uint32_t i, const DevFov& f, const TileGeom& g,
-                                                       OnRun&& on_run) {
+                                                       const RunClasses& rc, OnRun&& on_run) {
   WalkC c;
   WalkS w;
   if (!walk_setup(l, f, c, w) || !bounded(w)) return 0;
   if (!(w.nb_x > 0.0f && w.nb_y > 0.0f && w.nb_z > 0.0f && c.vl_x > 0.0f && c.vl_y > 0.0f && c.vl_z > 0.0f))
-    return bin_lor<false>(l, i, f, no_tof(), g, [&](const Run&, uint32_t tl) { on_run(tl); });
+    return bin_lor<false>(l, i, f, no_tof(), g, [&](const Run& ru, uint32_t tl) {
+      on_run(tl, rc.of_steps((double)run_len(ru.code)));
+    });
+  const double ivs = inv_vl_sum(c);
   const float nb[3] = {w.nb_x, w.nb_y, w.nb_z}, vl[3] = {c.vl_x, c.vl_y, c.vl_z};
   int32_t tile;
   uint32_t idx[3], last[3];
@@ -312,7 +340,7 @@ __host__ __device__ __forceinline__ uint32_t tile_runs(const Lor& l, uint32_t i,
       push = Be > hx;
     }
     if (push) {
-      on_run((uint32_t)tile);
+      on_run((uint32_t)tile, rc.of_crossings(h, bee, ivs));
       ++count;
     }
     if (oe) break;
@@ -358,13 +386,18 @@ __host__ __device__ __forceinline__ uint32_t tile_runs(const Lor& l, uint32_t i,
 // boundary or voxel length (degenerate LORs) are stepped through by bin_lor.
 template <class Emit>
 __host__ __device__ __forceinline__ uint32_t bin_lor_jump(const Lor& l, uint32_t i, const DevFov& f,
-                                                          const TileGeom& g, Emit&& emit,
+                                                          const TileGeom& g, const RunClasses& rc, Emit&& emit,
                                                           uint32_t* n_steps = nullptr) {
   WalkC c;
   WalkS w;
   if (!walk_setup(l, f, c, w) || !bounded(w)) return 0;
   if (!(w.nb_x > 0.0f && w.nb_y > 0.0f && w.nb_z > 0.0f && c.vl_x > 0.0f && c.vl_y > 0.0f && c.vl_z > 0.0f))
-    return bin_lor<false>(l, i, f, no_tof(), g, emit, n_steps, nullptr);
+    return bin_lor<false>(l, i, f, no_tof(), g, [&](const Run& ru, uint32_t tl) {
+      emit(ru, tl, rc.of_steps((double)run_len(ru.code)));
+    }, n_steps, nullptr);
+  const double ivs = inv_vl_sum(c);
+  const float ivl[3] = {1.0f / c.vl_x, 1.0f / c.vl_y, 1.0f / c.vl_z};
+  double h_est = 0.0;  // estimate of the crossing that entered the tile, as tile_runs keeps it
   const float nb[3] = {w.nb_x, w.nb_y, w.nb_z}, vl[3] = {c.vl_x, c.vl_y, c.vl_z};
   int32_t cc[3], sgn[3], tile;
   {
@@ -389,6 +422,7 @@ __host__ __device__ __forceinline__ uint32_t bin_lor_jump(const Lor& l, uint32_t
   int32_t r[3] = {w.r_x, w.r_y, w.r_z};          // crossings left per axis (the last one leaves the FOV)
   float here = w.here;
   uint32_t count = 0, total = 0;
+  uint32_t cls = 0;  // the segment's length class
   auto make_run = [&](const float s_nb[3], float h, const int32_t cq[3], uint32_t len) {
     Coord q;
     q.cx = cq[0];
@@ -408,7 +442,7 @@ __host__ __device__ __forceinline__ uint32_t bin_lor_jump(const Lor& l, uint32_t
     ru.tof_peak = c.tof_peak;
     ru.code = (local_of(g, q) << 15) | (flips << 12) | len;
     ru.lor = i;
-    emit(ru, (uint32_t)tile);
+    emit(ru, (uint32_t)tile, cls);
     ++count;
   };
   for (;;) {
@@ -437,13 +471,16 @@ __host__ __device__ __forceinline__ uint32_t bin_lor_jump(const Lor& l, uint32_t
     }
     float Ve = V[0], vle = vl[0];
     int32_t me = m[0];
+    double be = b[0];
 #pragma unroll
     for (int a = 1; a < 3; ++a)
       if (a == e) {
         Ve = V[a];
         vle = vl[a];
         me = m[a];
+        be = b[a];
       }
+    cls = rc.of_crossings(h_est, be, ivs);
     // the exit axis's last two crossings of the segment
     const float Ep = me >= 2 ? repeat_add(Ve, vle, (uint32_t)me - 2u) : -INFINITY;
     const float Ee = me >= 2 ? Ep + vle : Ve;
@@ -463,8 +500,10 @@ __host__ __device__ __forceinline__ uint32_t bin_lor_jump(const Lor& l, uint32_t
       if (lim <= 0 || v0 > Ee) {
         k = 0;
       } else if (v0 < Ee) {
-        // crossings j = 0, 1, ... lie near v0 + j vl, each within j 2^-23 of its value
-        const double x = ((double)Ee - (double)v0) / (double)vl[a];
+        // crossings j = 0, 1, ... lie near v0 + j vl, each within j 2^-23 of
+        // its value; x estimates their number below Ee (an error in x only
+        // sends a count to the checks below or to the one-by-one count)
+        const double x = (double)((Ee - v0) * ivl[a]);
         const double xc = ceil(x);
         int32_t ke = xc < (double)lim ? (int32_t)xc : lim;  // crossings below Ee by the estimate
         if (ke < 0) ke = 0;
@@ -556,6 +595,7 @@ __host__ __device__ __forceinline__ uint32_t bin_lor_jump(const Lor& l, uint32_t
       }
     }
     here = Ee;
+    h_est = be;
     if (out) break;
   }
   if (n_steps) *n_steps += total;
@@ -621,7 +661,8 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
     };
     // filling without TOF: by tile jumps (the same runs; counting, the step
     // walk also gives the pushed-step count)
-    const uint32_t count = (!TOF && FILL) ? bin_lor_jump(l, i, f, g, emit)
+    const uint32_t count = (!TOF && FILL) ? bin_lor_jump(l, i, f, g, RunClasses{},
+                                                         [&](const Run& ru, uint32_t tile, uint32_t) { emit(ru, tile); })
                                           : bin_lor<TOF>(l, i, f, t, g, emit, FILL ? nullptr : &steps,
                                                          FILL ? nullptr : &nz);
     if (FILL) {
@@ -656,41 +697,38 @@ __global__ __launch_bounds__(256) void k_bin_runs(const Lor* __restrict__ lors, 
 
 // Binning without TOF straight into tile order, for the sensitivity image.
 // LOR i is lors[order[i]] (lors[i] without order).  Workgroup w of a grid of
-// G takes the same LORs in both passes (as
-// k_bin_runs), and the runs of its LORs in tile t form segment (t, w) of a
-// tile-major layout: segment sizes seg[t * G + w], their exclusive prefix
-// sums soff.  Count (FILL false): n_runs[i] as k_bin_runs<false, false>,
-// and seg from a per-workgroup LDS histogram; STATS walks every step to
-// count the steps and pushed steps too (bin_lor), otherwise the tiles come
-// from the tile jump (tile_runs) and only the miss and shifted-entry
-// counters are kept.  Fill (FILL true): each run goes to the next free
-// position of its segment, runs of long_run steps or more from its start and
-// shorter ones from its end (two LDS cursors per tile), so that the lanes of
-// a replay mostly get runs of one class (inside a class positions follow no
-// order), and,
-// with run_pos, LOR i's k-th run position to run_pos[offset[i] + k].  A run
-// past its segment's end is not written (its run_pos entry is 0), a segment
-// left short is padded with empty runs, and every segment whose fill differs
-// from its count adds to counters[CNT_RUN_MISMATCH].  Dynamic LDS: 4 * ntiles
-// bytes (count) or 16 * ntiles bytes (fill).
+// G takes the same LORs in both passes (as k_bin_runs), and its runs in tile
+// t of length class c (RunClasses rc, C = rc.n classes) form segment
+// ((t * C + C - 1 - c) * G + w) of a tile-major layout: in each tile the
+// longest class first, and lanes that replay neighbouring runs of a tile get
+// runs of about one length.  Segment sizes go to seg, their exclusive prefix
+// sums are soff.  Count (FILL false): n_runs[i] as k_bin_runs<false, false>,
+// and seg from a per-workgroup LDS histogram of the tile jumps (tile_runs);
+// STATS also walks every step (bin_lor) for the step and pushed-step
+// counters.  Fill (FILL true): each run (bin_lor_jump, the same tiles and
+// classes) goes to the next free position of its segment (positions within
+// a segment follow no order) and, with run_pos, LOR i's k-th run position to
+// run_pos[offset[i] + k].  A run past its segment's end is not written (its
+// run_pos entry is 0), a segment left short is padded with empty runs, and
+// every segment whose fill differs from its count adds to
+// counters[CNT_RUN_MISMATCH].  Dynamic LDS: 4 * ntiles * C bytes (count) or
+// 8 * ntiles * C bytes (fill).
 template <bool FILL, bool STATS>
 __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lors, const uint32_t* __restrict__ order,
-                                                      uint32_t n, DevFov f, TileGeom g, uint32_t* __restrict__ n_runs,
-                                                      uint32_t* __restrict__ seg, const uint32_t* __restrict__ soff,
+                                                      uint32_t n, DevFov f, TileGeom g, RunClasses rc,
+                                                      uint32_t* __restrict__ n_runs, uint32_t* __restrict__ seg,
+                                                      const uint32_t* __restrict__ soff,
                                                       const uint32_t* __restrict__ offset, Run* __restrict__ runs,
-                                                      uint32_t* __restrict__ run_pos, uint32_t long_run,
+                                                      uint32_t* __restrict__ run_pos,
                                                       unsigned long long* __restrict__ counters) {
-  // count: [0, nt) runs per tile; fill: [0, nt) front cursor, [nt, 2 nt)
-  // back cursor, [2 nt, 3 nt) segment start, [3 nt, 4 nt) segment end
+  // count: [0, ns) runs per segment; fill: [0, ns) cursor, [ns, 2 ns) segment end
   extern __shared__ uint32_t cur[];
-  const uint32_t nt = (uint32_t)g.ntiles, ng = gridDim.x, wg = blockIdx.x;
-  for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+  const uint32_t ncls = rc.n, ns = (uint32_t)g.ntiles * ncls, ng = gridDim.x, wg = blockIdx.x;
+  for (uint32_t t = threadIdx.x; t < ns; t += blockDim.x) {
     if (FILL) {
-      const uint32_t a = soff[(size_t)t * ng + wg], e = a + seg[(size_t)t * ng + wg];
+      const uint32_t a = soff[(size_t)t * ng + wg];
       cur[t] = a;
-      cur[nt + t] = e;
-      cur[2 * nt + t] = a;
-      cur[3 * nt + t] = e;
+      cur[ns + t] = a + seg[(size_t)t * ng + wg];
     } else {
       cur[t] = 0;
     }
@@ -705,29 +743,22 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
     const Lor l = load_lor(lors, order ? order[i] : i);
     const uint32_t base = (FILL && run_pos) ? offset[i] : 0;
     const uint32_t room = (FILL && run_pos) ? offset[i + 1] - base : 0;
-    uint32_t k = 0;
-    auto emit = [&](const Run& ru, uint32_t tile) {
-      if (FILL) {
-        // long runs fill the segment from its start, short ones from its end
-        const bool lng = run_len(ru.code) >= long_run;
-        const uint32_t at = lng ? atomicAdd(&cur[tile], 1u) : atomicSub(&cur[nt + tile], 1u) - 1u;
-        const bool fits = at >= cur[2 * nt + tile] && at < cur[3 * nt + tile];
+    uint32_t count;
+    if (FILL) {
+      uint32_t k = 0;
+      count = bin_lor_jump(l, i, f, g, rc, [&](const Run& ru, uint32_t tile, uint32_t cls) {
+        const uint32_t sg = tile * ncls + (ncls - 1u - cls);
+        const uint32_t at = atomicAdd(&cur[sg], 1u);
+        const bool fits = at < cur[ns + sg];
         if (fits) runs[at] = ru;
         if (run_pos && k < room) run_pos[base + k] = fits ? at : 0u;
-      } else {
-        atomicAdd(&cur[tile], 1u);
-      }
-      ++k;
-    };
-    uint32_t count;
-    if (FILL)
-      count = bin_lor_jump(l, i, f, g, emit);
-    else if (STATS)
-      count = bin_lor<false>(l, i, f, notof, g, emit, &steps, &nz);
-    else
-      count = tile_runs(l, i, f, g, [&](uint32_t tile) { atomicAdd(&cur[tile], 1u); });
-    if (FILL && run_pos) mismatch += count != room ? 1u : 0u;
-    if (!FILL) {
+        ++k;
+      });
+      if (run_pos) mismatch += count != room ? 1u : 0u;
+    } else {
+      count = tile_runs(l, i, f, g, rc,
+                        [&](uint32_t tile, uint32_t cls) { atomicAdd(&cur[tile * ncls + (ncls - 1u - cls)], 1u); });
+      if (STATS) bin_lor<false>(l, i, f, notof, g, [](const Run&, uint32_t) {}, &steps, &nz);
       WalkC c;
       WalkS w;
       bool sh = false;
@@ -738,12 +769,12 @@ __global__ __launch_bounds__(256) void k_bin_runs_seg(const Lor* __restrict__ lo
     }
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+  for (uint32_t t = threadIdx.x; t < ns; t += blockDim.x) {
     if (FILL) {
-      // the two cursors meet exactly when the segment holds its count
-      mismatch += cur[t] != cur[nt + t] ? 1u : 0u;
+      // the cursor ends at the segment's end exactly when the segment holds its count
+      mismatch += cur[t] != cur[ns + t] ? 1u : 0u;
       // a segment left short holds empty runs (length 0, LOR 0), not stale records
-      for (uint32_t x = cur[t]; x < cur[nt + t]; ++x) runs[x] = Run{};
+      for (uint32_t x = cur[t]; x < cur[ns + t]; ++x) runs[x] = Run{};
     } else {
       seg[(size_t)t * ng + wg] = cur[t];
     }

@@ -348,6 +348,7 @@ extern "C" {
         out: *mut f32,
         stats_or_null: *mut petalo_stats_t,
     ) -> c_int;
+    pub fn petalo_release_workspaces() -> c_int;
     pub fn petalo_sensitivity_image_noalloc(
         fov: *const petalo_fov_t,
         mu: *const f32,

@@ -1,7 +1,7 @@
 // Host-side check that the fill pass's tile-jump binning (bin_lor_jump) emits
 // exactly the runs and step counts of the step-by-step binning walk
 // (bin_lor<false>), bit for bit, and that the count pass's tile jump
-// (tile_runs) finds the same tiles, over random LORs and LORs built to tie: axis-parallel, on
+// (tile_runs) finds the same tiles and length classes, over random LORs and LORs built to tie: axis-parallel, on
 // voxel planes, through voxel edges and corners, diagonal, degenerate
 // (p1 == p2), grazing a face, and the irregular LORs of the sensitivity
 // tests.  Usage: binning_axes_check [lors per case]; exit status 1 on any
@@ -54,6 +54,7 @@ int main(int argc, char** argv) {
   }
   printf("repeat_add: %ld cases, repeat_add mismatches %ld\n", adds, bad_add);
   uint64_t steps_a = 0, steps_b = 0;
+  long cls_hist[9] = {0}, bad_cls = 0;
   struct Case { float full[3]; uint32_t n[3]; int32_t tile[3]; };
   std::vector<Case> cases = {
     {{30, 30, 30}, {6, 6, 6}, {6, 6, 6}}, {{30, 30, 30}, {6, 6, 6}, {2, 3, 4}},
@@ -98,9 +99,16 @@ int main(int argc, char** argv) {
       }
       l.dt = 0.0f; l.ac = 1.0f;
       std::vector<R> a, b;
+      std::vector<uint32_t> cls_fill;
       uint32_t sa = 0, sb = 0;
+      RunClasses rc;
+      rc.n = 8;
+      rc.scale = 8.0 / (double)(g.tx + g.ty + g.tz);
       bin_lor<false>(l, (uint32_t)i, f, t, g, [&](const Run& ru, uint32_t tl) { a.push_back({ru, tl}); }, &sa, nullptr);
-      bin_lor_jump(l, (uint32_t)i, f, g, [&](const Run& ru, uint32_t tl) { b.push_back({ru, tl}); }, &sb);
+      bin_lor_jump(l, (uint32_t)i, f, g, rc, [&](const Run& ru, uint32_t tl, uint32_t cl) {
+        b.push_back({ru, tl});
+        cls_fill.push_back(cl);
+      }, &sb);
       WalkC c; WalkS w;
       const bool bnd = walk_setup(l, f, c, w) && bounded(w);
       if (!bnd) { sa = 0; }  // bin_lor counts nothing for unbounded walks either
@@ -110,10 +118,26 @@ int main(int argc, char** argv) {
       bool ok = a.size() == b.size() && sa == sb;
       for (size_t k = 0; ok && k < a.size(); ++k) ok = same(a[k], b[k]);
       // the tile jump: the same tiles, in walk order
-      std::vector<uint32_t> tj;
-      const uint32_t nj = tile_runs(l, (uint32_t)i, f, g, [&](uint32_t tl) { tj.push_back(tl); });
-      bool tok = nj == tj.size() && tj.size() == a.size();
-      for (size_t k = 0; tok && k < a.size(); ++k) tok = tj[k] == a[k].tile;
+      // the tile jump: the same tiles, in walk order, in the fill's length classes
+      std::vector<uint32_t> tj, cls_count;
+      const uint32_t nj = tile_runs(l, (uint32_t)i, f, g, rc, [&](uint32_t tl, uint32_t cl) {
+        tj.push_back(tl);
+        cls_count.push_back(cl);
+      });
+      bool tok = nj == tj.size() && tj.size() == a.size() && cls_count.size() == cls_fill.size();
+      for (size_t k = 0; tok && k < a.size(); ++k) tok = tj[k] == a[k].tile && cls_count[k] == cls_fill[k];
+      for (size_t k = 0; k < cls_count.size(); ++k) {
+        ++cls_hist[cls_count[k] < 8 ? cls_count[k] : 8];
+        // for LORs in general position (no tied crossings, which the class
+        // counts as steps) the class is the run's length scaled, give or
+        // take the estimate's error: the crossings of one axis over a
+        // distance D number D / vl within one, so the run's length lies
+        // within 3 steps of the estimate
+        const double len = (double)run_len(a[k].ru.code), c = (double)cls_count[k];
+        if (kind == 0 && i >= 10 && k < a.size() &&
+            (c > (len + 3.0) * rc.scale + 1e-9 || (cls_count[k] < 7 && c + 1.0 < (len - 3.0) * rc.scale - 1e-9)))
+          ++bad_cls;
+      }
       if (!tok) {
         if (bad_tiles < 5) {
           printf("TILE MISMATCH case %ux%ux%u tile %dx%dx%d lor %d kind %d: runs %zu vs %zu\n", cs.n[0], cs.n[1], cs.n[2],
@@ -122,6 +146,9 @@ int main(int argc, char** argv) {
             printf("  step walk %d  jump %d\n", k < a.size() ? (int)a[k].tile : -1, k < tj.size() ? (int)tj[k] : -1);
         }
         ++bad_tiles;
+        if (bad_tiles <= 5 && tj.size() == a.size())
+          for (size_t k = 0; k < cls_count.size() && k < cls_fill.size(); ++k)
+            printf("  class count %u fill %u (len %u)\n", cls_count[k], cls_fill[k], run_len(a[k].ru.code));
       }
       if (!ok) {
         if (bad < 5) {
@@ -138,5 +165,8 @@ int main(int argc, char** argv) {
   }
   printf("lors with runs %ld, runs %ld, steps %llu vs %llu, mismatches %ld, tile mismatches %ld\n", lors_hit, runs,
          (unsigned long long)steps_a, (unsigned long long)steps_b, bad, bad_tiles);
-  return (bad || bad_tiles || bad_add) ? 1 : 0;
+  printf("length classes:");
+  for (int k = 0; k < 9; ++k) printf(" %ld", cls_hist[k]);
+  printf(", class far from the run length %ld\n", bad_cls);
+  return (bad || bad_tiles || bad_add || bad_cls) ? 1 : 0;
 }

@@ -2,7 +2,10 @@
 runs of walks without TOF; tile_runs, which counts them) against the
 step-by-step binning walk (bin_lor), compiled for the host from the
 library's own headers: identical runs (every field, bit for bit), step
-counts and tile sequences on 8 FOV/tile cases x 20 000 LORs; and the jumps'
+counts and tile sequences on 8 FOV/tile cases x 20 000 LORs, the same run
+length class in both jumps (the sensitivity segments are sized by one and
+filled by the other) and, for LORs in general position, a class that
+matches the run's length to within the estimate's 3 steps; and the jumps'
 repeated binary32 addition (repeat_add) against one addition at a time on
 400 000 cases (ties, binade crossings, long runs)."""
 import os
@@ -28,3 +31,4 @@ def test_axes_binning_matches_step_walk():
         assert out.returncode == 0, out.stdout
         assert "repeat_add mismatches 0" in out.stdout
         assert ", mismatches 0, tile mismatches 0" in out.stdout
+        assert "class far from the run length 0" in out.stdout

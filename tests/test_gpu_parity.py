@@ -259,6 +259,31 @@ def test_sensitivity_batches_agree_with_one_lane_path(gpu):
     assert e <= 1e-6
 
 
+def test_sensitivity_kept_workspace(gpu):
+    """The sensitivity calls keep their device buffers between calls: a call
+    after a larger one (buffers kept, bigger than needed), after a smaller
+    one on another FOV (buffers grown), and after release_workspaces (buffers
+    made anew) gives the image of a first call, and each matches the oracle."""
+    att = P.FOV([180.0] * 3, [30] * 3)
+    small = P.FOV([120.0, 120.0, 90.0], [20, 20, 15])
+    mu = P.density_to_attenuation(P.synth_density(att), 0.096)
+    mu0 = P.Image(small, np.zeros(small.nvox, np.float32))
+    n, seed, L, D = 150_000, 8, 300.0, 700.0
+    P.release_workspaces()
+    first = P.make_sensitivity_image(mu, n, L, D, seed=seed).data.copy()
+    imgs = []
+    for k in range(3):
+        P.make_sensitivity_image(mu0, [400_000, 20_000, 20_000][k], L, D, seed=k)
+        if k == 2:
+            P.release_workspaces()
+        imgs.append(P.make_sensitivity_image(mu, n, L, D, seed=seed).data.copy())
+    ref, _ = O.sensitivity(att.full_size, att.n, mu.data, P.sensitivity_lors(att, n, seed, L, D), threads=16)
+    errs = [rel_l2(x, first) for x in imgs]
+    _say(f"kept workspace: relL2 to the first call {['%.1e' % e for e in errs]}, first vs oracle {rel_l2(first, ref):.1e}")
+    assert max(errs) <= 1e-7
+    assert rel_l2(first, ref) <= 1e-5
+
+
 @pytest.mark.parametrize("attenuated", [False, True])
 def test_sensitivity_many_tiles(gpu, attenuated):
     """A 480x480x240 image has 12x12x40 = 5760 tiles, more than the binning
