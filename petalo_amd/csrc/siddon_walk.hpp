@@ -409,13 +409,15 @@ __host__ __device__ __forceinline__ bool advance_packed(WalkS& w, const WalkC c,
 // integer, 2^23 <= V < 2^24) and d / u = q + r (q an integer, 0 <= r < 1,
 // exact: a power-of-two scaling).  If V + q <= 2^24 - 1, the exact sum
 // (V + q + r) u lies below 2^(e+1), where binary32 values are the multiples
-// of u, so it rounds to (V + Q) u with Q = q + (r > 1/2) -- for r != 1/2;
-// r == 1/2 rounds to the even neighbour, which depends on V, and is done by
-// one hardware addition.  So k additions jump V to V + kQ while the start of
-// the last one, V + (k-1)Q, still satisfies the bound; the addition that
-// leaves the binade is done by the hardware, and the next binade repeats
-// the argument.  v < d (only at the start of a walk) takes hardware
-// additions until v >= d, which keeps d / u below 2^24.
+// of u, so it rounds to (V + Q) u with Q = q + (r > 1/2) -- for r != 1/2.
+// r == 1/2 rounds to the even neighbour: after one hardware addition V is
+// even, and from an even V each addition lands on V + q + (q odd), even
+// again, so Q = q + (q & 1) from then on.  So k additions jump V to V + kQ
+// while the start of the last one, V + (k-1)Q, still satisfies the bound;
+// the addition that leaves the binade is done by the hardware in the same
+// round, and the next binade repeats the argument.  v < d (only at the
+// start of a walk) takes hardware additions until v >= d, which keeps
+// d / u below 2^24.
 // floor(num / den) for num < 2^24, 0 < den < 2^24: a binary32 estimate
 // (within a few units: both operands are exact, the reciprocal and the
 // product are each within an ulp) corrected in integers.
@@ -432,10 +434,7 @@ __host__ __device__ __forceinline__ uint32_t div_small(uint32_t num, uint32_t de
 }
 
 __host__ __device__ __forceinline__ float repeat_add(float v, float d, uint32_t m) {
-  if (m <= 4u) {  // a few additions cost less than the binade arithmetic
-    for (uint32_t j = 0; j < m; ++j) v = v + d;
-    return v;
-  }
+  if (m == 0u) return v;
   if (!(v < INFINITY && d < INFINITY)) return v + d;
   while (m > 0) {
     if (!(v >= d)) {
@@ -445,22 +444,35 @@ __host__ __device__ __forceinline__ float repeat_add(float v, float d, uint32_t 
     }
     uint32_t bits;
     memcpy(&bits, &v, 4);
-    const int32_t e = (int32_t)(bits >> 23) - 127;  // v >= d > 0 is normal here
-    const uint32_t V = (bits & 0x7fffffu) | 0x800000u;
+    int32_t e = (int32_t)(bits >> 23) - 127;  // v >= d > 0 is normal here
+    uint32_t V = (bits & 0x7fffffu) | 0x800000u;
     const float t = ldexpf(d, 23 - e);  // d / u, exact (or below 1/2 when it underflows)
     const float qf = floorf(t);
     const float r = t - qf;
     const uint32_t q = (uint32_t)qf;
-    if (r == 0.5f || V + q > 0xffffffu) {  // a tie, or an addition that leaves the binade
+    uint32_t Q = q + (r > 0.5f ? 1u : 0u);
+    if (r == 0.5f) {  // a tie: one hardware addition makes V even
+      v = v + d;
+      --m;
+      uint32_t b2;
+      memcpy(&b2, &v, 4);
+      if (m == 0u || (int32_t)(b2 >> 23) - 127 != e) continue;  // done, or left the binade
+      V = (b2 & 0x7fffffu) | 0x800000u;
+      Q = q + (q & 1u);
+    }
+    if (V + q > 0xffffffu) {  // the next addition leaves the binade
       v = v + d;
       --m;
       continue;
     }
-    const uint32_t Q = q + (r > 0.5f ? 1u : 0u);
     uint32_t k = m;
     if (Q != 0u && (uint64_t)V + (uint64_t)(m - 1u) * Q + q > 0xffffffu) k = div_small(0xffffffu - q - V, Q) + 1u;
     v = ldexpf((float)(V + k * Q), e - 23);  // V + kQ <= 2^24: exact
     m -= k;
+    if (m > 0u) {  // the addition that leaves the binade
+      v = v + d;
+      --m;
+    }
   }
   return v;
 }

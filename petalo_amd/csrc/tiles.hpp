@@ -496,40 +496,25 @@ __host__ __device__ __forceinline__ uint32_t bin_lor_jump(const Lor& l, uint32_t
       }
       const int32_t lim = m[a] - 1;
       const float v0 = V[a];
-      int32_t k = -1;  // the count, once decided
-      if (lim <= 0 || v0 > Ee) {
-        k = 0;
-      } else if (v0 < Ee) {
-        // crossings j = 0, 1, ... lie near v0 + j vl, each within j 2^-23 of
-        // its value; x estimates their number below Ee (an error in x only
-        // sends a count to the checks below or to the one-by-one count)
-        const double x = (double)((Ee - v0) * ivl[a]);
-        const double xc = ceil(x);
-        int32_t ke = xc < (double)lim ? (int32_t)xc : lim;  // crossings below Ee by the estimate
-        if (ke < 0) ke = 0;
-        bool ok = true;
-        if (ke > 0) {  // crossing ke - 1 below Ee
-          const double bj = (double)v0 + (double)(ke - 1) * (double)vl[a];
-          ok = ok && bj + (double)(ke - 1) * 0x1p-23 * bj < (double)Ee;
+      // crossings of axis a before Ee in the walk's (value, axis) order, at most lim
+      auto before = [&](float x) { return x < Ee || (x == Ee && a < e); };
+      int32_t k = 0;
+      float v = v0, pa = -INFINITY;
+      if (lim > 0 && before(v0)) {
+        // crossing j lies near v0 + j vl: about ceil((Ee - v0) / vl) of them
+        // precede Ee.  The estimate is checked on the exact crossings it
+        // names: lowered while crossing ke - 1 does not precede Ee, raised
+        // while crossing ke does.
+        const float xe = ceilf((Ee - v0) * ivl[a]);
+        int32_t ke = xe >= 1.0f ? (xe < (float)lim ? (int32_t)xe : lim) : 1;
+        for (;;) {
+          pa = repeat_add(v0, vl[a], (uint32_t)ke - 1u);
+          if (ke == 1 || before(pa)) break;
+          --ke;
         }
-        if (ke < lim) {  // crossing ke above Ee
-          const double bj = (double)v0 + (double)ke * (double)vl[a];
-          ok = ok && bj - (double)ke * 0x1p-23 * bj > (double)Ee;
-        }
-        if (ok) k = ke;
-      }
-      float v, pa = -INFINITY;
-      if (k >= 0) {
-        if (k > 0) {
-          pa = repeat_add(v0, vl[a], (uint32_t)k - 1u);
-          v = pa + vl[a];
-        } else {
-          v = v0;
-        }
-      } else {  // undecided: one crossing at a time, as the walk orders them
-        v = v0;
-        k = 0;
-        while (k < lim && (v < Ee || (v == Ee && a < e))) {
+        v = pa + vl[a];
+        k = ke;
+        while (k < lim && before(v)) {
           pa = v;
           v = v + vl[a];
           ++k;

@@ -286,9 +286,9 @@ def test_sensitivity_kept_workspace(gpu):
 
 @pytest.mark.parametrize("attenuated", [False, True])
 def test_sensitivity_many_tiles(gpu, attenuated):
-    """A 480x480x240 image has 12x12x40 = 5760 tiles, more than the binning
-    into per-(tile, workgroup) segments keeps cursors for: its runs are binned
-    LOR-major and sorted by tile instead.  Image and counters match the oracle,
+    """A 480x480x240 image has 12x12x20 = 2880 sensitivity tiles (40x40x12),
+    more than the binning into per-(tile, workgroup) segments keeps cursors
+    for: its runs are binned LOR-major and sorted by tile instead.  Image and counters match the oracle,
     with and without attenuation."""
     att = P.FOV([480.0, 480.0, 240.0], [480, 480, 240])
     mu = P.Image(att, np.full(att.nvox, 0.0005 if attenuated else 0.0, np.float32))
