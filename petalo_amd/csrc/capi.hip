@@ -795,7 +795,8 @@ void ensure_lds_attributes() {
                        (const void*)k_scatter_batch<true>,  (const void*)k_scatter_batch<false>,
                        (const void*)k_fwd_tile_batch<true>, (const void*)k_fwd_tile_batch<false>,
                        (const void*)k_lor_reduce_win<false, true>, (const void*)k_lor_reduce_win<false, false>,
-                       (const void*)k_lor_reduce_win_batch, (const void*)k_batch_persistent};
+                       (const void*)k_lor_reduce_win_batch, (const void*)k_batch_persistent,
+                       (const void*)k_scatter_unit32};
   for (const void* f : fns) HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   if (done.size() <= (size_t)dev) done.resize(dev + 1, false);
   done[dev] = true;
@@ -2320,6 +2321,26 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
     rc.scale = (double)rc.n / (double)(g.tx + g.ty + g.tz);
   }
   const uint32_t ncls = rc.n;
+  // without attenuation every r is 1: the scatter's tiles sum in 32-bit fixed
+  // point, a step adding round(w * 2^e) units with e such that an item's
+  // kSensItemRuns runs of at most 1.25 voxel diagonals each stay below 2^32
+  bool unit32 = false;
+  float unit32_s = 0.0f;
+  double unit32_unit = 0.0;
+  if (mu_zero) {
+    int lg = 0;
+    while ((1u << lg) < kSensItemRuns) ++lg;
+    const float diag = std::sqrt(f.vs_x * f.vs_x + f.vs_y * f.vs_y + f.vs_z * f.vs_z);
+    const float bound = 1.25f * diag;
+    int k = 0;
+    std::frexp(bound, &k);  // bound < 2^k
+    const int e = (32 - lg) - k;
+    if (bound > 0.0f && std::isfinite(bound) && e > -120 && e < 120) {
+      unit32 = true;
+      unit32_s = std::ldexp(1.0f, e);
+      unit32_unit = std::ldexp(1.0, -e);
+    }
+  }
   // walk lengths are below nx + ny + nz steps: the shift that brings them under 256
   uint32_t len_shift = 0;
   while ((((uint64_t)fov.n[0] + fov.n[1] + fov.n[2]) >> len_shift) > 255u) ++len_shift;
@@ -2338,7 +2359,9 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
   // by length, the segment binning reads lraw through the sort's order and needs no copy in lb
   Lor* lb = ws.lb.ensure(by_length && segs ? 0 : bmax);
   Lor* lraw = ws.lraw.ensure(by_length ? bmax : 0);
-  Lor* irr = ws.irr.ensure(bmax);
+  // unbounded walks of several batches: room for one batch's and 2^16 more
+  const size_t irr_cap = bmax + (1u << 16);
+  Lor* irr = ws.irr.ensure(irr_cap);
   uint32_t* counts = ws.counts.ensure(bmax + 1);
   uint32_t* offs = ws.offs.ensure(bmax + 1);
   uint32_t* n_irr = ws.n_irr.ensure(1);
@@ -2363,6 +2386,7 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
   unsigned char* lsort_tmp = ws.lsort_tmp.ensure(std::max<size_t>(lsort_bytes, 1));
   unsigned char* scan_tmp = ws.scan_tmp.ensure(std::max<size_t>(scan_bytes, 1));
   uint64_t* dstart = ws.dstart.ensure(segs ? 0 : g.ntiles);
+  uint32_t irr_pending = 0;  // unbounded walks held in irr
   for (uint64_t first = 0; first < n_lors; first += kSensBatch) {
     const uint32_t b = (uint32_t)std::min<uint64_t>(kSensBatch, n_lors - first);
     const uint32_t ng = bin_grid(b);
@@ -2396,7 +2420,17 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
                                                       shift, cnt);
     }
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemsetAsync(n_irr, 0, 4, s));
+    // unbounded walks collect in irr over the batches (a handful per 2^24
+    // random LORs, and the one-lane kernel costs its longest walk whatever
+    // their number): projected when irr could not take another batch's, and
+    // after the last batch
+    if (irr_pending + b > irr_cap) {
+      k_sens<false, false><<<grid_for(irr_pending), kBlock, 0, s>>>(irr, irr_pending, 0, 0.0f, 0.0f, f,
+                                                                     mu_zero ? nullptr : dmu, f, acc, cnt);
+      HIP_CHECK(hipGetLastError());
+      irr_pending = 0;
+    }
+    if (irr_pending == 0) HIP_CHECK(hipMemsetAsync(n_irr, 0, 4, s));
     k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts, b, src_lors, lor_order, irr, n_irr);
     HIP_CHECK(hipGetLastError());
     uint32_t total = 0, h_irr = 0;
@@ -2457,14 +2491,20 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
                                                              rstat);
         HIP_CHECK(hipGetLastError());
       }
-      k_scatter<false><<<cap, 1024, (size_t)g.tvox * sizeof(double), s>>>(un, items, mu_zero ? nullptr : rbuf, f,
-                                                                          notof, g, acc, order, rstat);
+      if (unit32)
+        k_scatter_unit32<<<cap, 1024, (size_t)g.tvox * sizeof(uint32_t), s>>>(un, items, f, g, acc, order, unit32_s,
+                                                                              unit32_unit);
+      else
+        k_scatter<false><<<cap, 1024, (size_t)g.tvox * sizeof(double), s>>>(un, items, mu_zero ? nullptr : rbuf, f,
+                                                                            notof, g, acc, order, rstat);
       HIP_CHECK(hipGetLastError());
     }
-    if (h_irr) {
-      k_sens<false, false><<<grid_for(h_irr), kBlock, 0, s>>>(irr, h_irr, 0, 0.0f, 0.0f, f, dmu, f, acc, cnt);
-      HIP_CHECK(hipGetLastError());
-    }
+    irr_pending = h_irr;  // the running count
+  }
+  if (irr_pending) {
+    k_sens<false, false><<<grid_for(irr_pending), kBlock, 0, s>>>(irr, irr_pending, 0, 0.0f, 0.0f, f,
+                                                                   mu_zero ? nullptr : dmu, f, acc, cnt);
+    HIP_CHECK(hipGetLastError());
   }
 }
 

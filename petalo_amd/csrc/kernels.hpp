@@ -337,6 +337,16 @@ __device__ __forceinline__ bool sens_attempt(uint64_t seed, uint32_t i, uint32_t
   sincosf(t1, &s1, &c1);
   sincosf(t2, &s2, &c2);
   l = Lor{R * c1, R * s1, z1, R * c2, R * s2, z2, 0.0f, 1.0f};
+  // most draws miss; a segment that passes the FOV's circumscribed cylinder,
+  // or lies beyond its z range, by a clear margin (0.2% of the radius,
+  // 0.1% of the half-length: far above the rounding of these few products
+  // and of fov_entry's own arithmetic) misses the FOV, and fov_entry would
+  // say so
+  const float dx = l.x2 - l.x1, dy = l.y2 - l.y1;
+  const float cr = l.x1 * dy - l.y1 * dx;  // (distance of the line from the z axis) * |(dx, dy)|
+  if (cr * cr > (f.hw_x * f.hw_x + f.hw_y * f.hw_y) * 1.004f * (dx * dx + dy * dy)) return false;
+  const float zg = f.hw_z * 1.001f;
+  if ((z1 > zg && z2 > zg) || (z1 < -zg && z2 < -zg)) return false;
   float ex, ey, ez;
   return fov_entry(f, l.x1, l.y1, l.z1, l.x2, l.y2, l.z2, ex, ey, ez);
 }
@@ -397,7 +407,8 @@ __global__ __launch_bounds__(256) void k_sens_lors(uint32_t n, uint32_t first, u
 // Sensitivity backprojection (project_one_lor_sens, projector.rs:72-74):
 // forward walk on the attenuation FOV, p = sum w*mu, adapted to exp(-p);
 // backward walk on the output FOV (TWO) or the same row, adding
-// w * (1/exp(-p)) into binary64 accumulators.
+// w * (1/exp(-p)) into binary64 accumulators.  mu == nullptr stands for an
+// attenuation image of zeros (p = 0; the forward walk still counts).
 template <bool GEN, bool TWO>
 __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint32_t n, uint64_t seed, float L,
                                               float R, DevFov fa, const float* __restrict__ mu, DevFov fo,
@@ -437,7 +448,7 @@ __global__ __launch_bounds__(256) void k_sens(const Lor* __restrict__ lors, uint
           if (!started) { start = w; start_step = c_steps; started = true; }
           ++c_nz;
           if ((uint32_t)w.index >= (uint32_t)fa.nvox) { bad = true; break; }
-          p += wt * mu[w.index];
+          if (mu) p += wt * mu[w.index];  // mu == nullptr: no attenuation (p stays 0)
         }
         ++c_steps;
         if (advance(w, c, s)) break;

@@ -812,7 +812,8 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // Replays a run from its recorded state, calling add(lds_address, w * rr)
 // for every pushed step (addresses of elements of 8 bytes), where rr is the
 // run's factor: r times the Gaussian's 666 * peak with TOF, times the
-// accumulator's power-of-two scale (TileAcc).  The voxel sequence and step
+// accumulator's power-of-two scale (TileAcc); the tile's entries are
+// 2^kShift bytes.  The voxel sequence and step
 // lengths are walk #1's bit for bit; the TOF weight is
 // length * tof_shape * (666 * peak), with the constant folded into rr once
 // per run (a few ulp from walk #1's product).  Every step of a run lies
@@ -823,12 +824,12 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // there, so wl > 0 exactly when the step length is; a step of length 0 is
 // not pushed.  GUARD false adds it anyway, which adds +0 when r is finite
 // (the caller's condition), saving the test.
-template <bool TOF, bool GUARD, class Add>
+template <bool TOF, bool GUARD, int kShift = 3, class Add>
 __host__ __device__ __forceinline__ void replay_run(const Run& ru, float rr, const DevTof& t, const TileGeom& g,
                                                     uint32_t base, Add&& add) {
   WalkS w;
   WalkC c;
-  run_state<3>(ru, g, w, c, base);
+  run_state<kShift>(ru, g, w, c, base);
   const uint32_t m = run_len(ru.code);  // the run length bounds the replay
   auto step = [&] {
     const float bnd = fminf(fminf(w.nb_x, w.nb_y), w.nb_z);
@@ -1340,6 +1341,7 @@ __global__ __launch_bounds__(256) void k_fwd_irregular(const Lor* __restrict__ l
 __device__ float kOneR[1] = {1.0f};
 
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 // The backprojection tile's accumulator, the same for all lanes of a work
 // item.  Fixed point (unit > 0): every contribution v = w * r is scaled by a
@@ -1416,13 +1418,18 @@ __device__ __forceinline__ unsigned long long fixed_units(float v, double s) {
 // added to the LDS tile with r = r_src[lor & r_mask].  Records are loaded
 // two runs ahead and r one run ahead, so no load's latency stalls a replay;
 // the first records are waited for before the loop, as in fwd_runs.
-// PERM: the runs are read through perm.  FIXED: the tile holds 64-bit fixed
-// point (TileAcc), where every r is finite, so zero-length steps add 0.
-template <bool TOF, bool PERM, bool FIXED>
+// PERM: the runs are read through perm.  ACC: the tile holds binary64 sums
+// (kAccF64), 64-bit fixed point (kAccU64, TileAcc), where every r is finite,
+// so zero-length steps add 0, or 32-bit fixed point (kAccU32, unit r: the
+// scale sfix keeps a work item's sums below 2^32).
+constexpr int kAccF64 = 0, kAccU64 = 1, kAccU32 = 2;
+template <bool TOF, bool PERM, int ACC>
 __device__ __forceinline__ void scatter_runs(const Run* __restrict__ runs, const uint32_t* __restrict__ perm,
                                              int32_t j, const int32_t last, const float* __restrict__ r_src,
                                              const uint32_t r_mask, const double sfix, const DevTof& t,
-                                             const TileGeom& g, double* tile) {
+                                             const TileGeom& g, void* tile) {
+  constexpr bool FIXED = ACC != kAccF64;
+  const float sf32 = (float)sfix;
   const int32_t stride = (int32_t)blockDim.x;
   const int32_t j1 = j + stride;
   const int32_t j1c = j1 < last ? j1 : last;
@@ -1432,7 +1439,10 @@ __device__ __forceinline__ void scatter_runs(const Run* __restrict__ runs, const
   float r = r_src[cur.lor & r_mask];
   const uint32_t base = lds_addr(tile);
   auto add = [&](uint32_t at, float v) {
-    if (FIXED)
+    if (ACC == kAccU32)  // v * sf32 is exact (a power of two), below 2^32
+      __hip_atomic_fetch_add((lds_u32*)(uintptr_t)at, __float2uint_rn(v * sf32), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    else if (ACC == kAccU64)
       __hip_atomic_fetch_add((lds_u64*)(uintptr_t)at, fixed_units(v, sfix), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
     else
@@ -1446,10 +1456,11 @@ __device__ __forceinline__ void scatter_runs(const Run* __restrict__ runs, const
     const Run nn = runs[PERM ? perm[jnc] : jnc];
     // without a non-finite r in the wavefront, zero-length steps may add +0
     const float rr = TOF ? r * t.amp : r;
+    constexpr int kShift = ACC == kAccU32 ? 2 : 3;
     if (FIXED || __all(fabsf(r) <= 3.402823466e38f))
-      replay_run<TOF, false>(cur, rr, t, g, base, add);
+      replay_run<TOF, false, kShift>(cur, rr, t, g, base, add);
     else
-      replay_run<TOF, true>(cur, rr, t, g, base, add);
+      replay_run<TOF, true, kShift>(cur, rr, t, g, base, add);
     if (jn > last) break;
     cur = nxt;
     nxt = nn;
@@ -1488,14 +1499,14 @@ __device__ __forceinline__ void scatter_body(const Run* __restrict__ runs, const
     const uint32_t r_mask = r_in ? 0xffffffffu : 0u;
     if (acc.unit > 0.0) {
       if (perm)
-        scatter_runs<TOF, true, true>(runs, perm, j, wi.z - 1, r_src, r_mask, acc.s, t, g, tile);
+        scatter_runs<TOF, true, kAccU64>(runs, perm, j, wi.z - 1, r_src, r_mask, acc.s, t, g, tile);
       else
-        scatter_runs<TOF, false, true>(runs, perm, j, wi.z - 1, r_src, r_mask, acc.s, t, g, tile);
+        scatter_runs<TOF, false, kAccU64>(runs, perm, j, wi.z - 1, r_src, r_mask, acc.s, t, g, tile);
     } else {
       if (perm)
-        scatter_runs<TOF, true, false>(runs, perm, j, wi.z - 1, r_src, r_mask, 0.0, t, g, tile);
+        scatter_runs<TOF, true, kAccF64>(runs, perm, j, wi.z - 1, r_src, r_mask, 0.0, t, g, tile);
       else
-        scatter_runs<TOF, false, false>(runs, perm, j, wi.z - 1, r_src, r_mask, 0.0, t, g, tile);
+        scatter_runs<TOF, false, kAccF64>(runs, perm, j, wi.z - 1, r_src, r_mask, 0.0, t, g, tile);
     }
   }
   __syncthreads();
@@ -1522,6 +1533,44 @@ __global__ __launch_bounds__(1024) void k_scatter(const Run* __restrict__ runs, 
   const int4 wi = work[blockIdx.x];
   if (wi.z <= wi.y) return;  // an empty item (the grid may be an upper bound)
   scatter_body<TOF>(runs, wi, r_in, f, t, g, bp, perm, rstats);
+}
+
+// Backprojection of the weights themselves (r = 1, no TOF: the sensitivity
+// image without attenuation) with the tile in 32-bit fixed point: half the
+// LDS of the 64-bit tile, so two workgroups share a CU at tiles of up to
+// 20416 entries.  Each step adds round(w * s) units; s = 2^e comes from the
+// host with every item of at most n runs and every step of at most the
+// voxel diagonal d: n * (1.25 d) * s < 2^32, so no sum overflows.  A step's
+// rounding is at most half a unit (unit = 1 / s, about 2^-16 of d for
+// 16384-run items), which the flush's binary64 sum then carries exactly.
+__global__ __launch_bounds__(1024) void k_scatter_unit32(const Run* __restrict__ runs, const int4* __restrict__ work,
+                                                         DevFov f, TileGeom g, double* __restrict__ bp,
+                                                         const uint32_t* __restrict__ perm, float s, double unit) {
+  extern __shared__ uint32_t tile32[];
+  const int4 wi = work[blockIdx.x];
+  if (wi.z <= wi.y) return;  // an empty item (the grid may be an upper bound)
+  for (int32_t k = threadIdx.x; k < g.tvox; k += blockDim.x) tile32[k] = 0u;
+  __syncthreads();
+  const int32_t j = wi.y + (int32_t)threadIdx.x;
+  const DevTof notof = no_tof();
+  if (j < wi.z) {
+    if (perm)
+      scatter_runs<false, true, kAccU32>(runs, perm, j, wi.z - 1, kOneR, 0u, (double)s, notof, g, tile32);
+    else
+      scatter_runs<false, false, kAccU32>(runs, perm, j, wi.z - 1, kOneR, 0u, (double)s, notof, g, tile32);
+  }
+  __syncthreads();
+  const int32_t tl = wi.x;
+  const int32_t tzi = tl / (g.ntx * g.nty);
+  const int32_t tyi = (tl / g.ntx) % g.nty;
+  const int32_t txi = tl % g.ntx;
+  const int32_t x0 = txi * g.tx, y0 = tyi * g.ty, z0 = tzi * g.tz;
+  for (TileCursor cu(g, (int32_t)threadIdx.x, (int32_t)blockDim.x); cu.k < g.tvox; cu.next(g, (int32_t)blockDim.x)) {
+    const uint32_t u = tile32[cu.k];
+    const int32_t x = x0 + cu.lx, y = y0 + cu.ly, z = z0 + cu.lz;
+    if (u != 0u && cu.lx < g.tx && x < f.nx && y < f.ny && z < f.nz)
+      unsafeAtomicAdd(bp + x + f.nx * (y + f.ny * z), (double)u * unit);
+  }
 }
 
 // Batched backprojection: work item w = (tile, first run, end run, reconstruction).

@@ -209,7 +209,8 @@ def test_sensitivity_two_fovs(gpu):
 
 
 def test_sensitivity_no_attenuation(gpu):
-    """mu = 0 everywhere (the tiled path skips the forward half: p = 0 exactly)."""
+    """mu = 0 everywhere (the tiled path skips the forward half: p = 0
+    exactly): image and counters against the binary32 oracle."""
     att = P.FOV([180.0] * 3, [60] * 3)
     mu = P.Image(att, np.zeros(att.nvox, np.float32))
     n, seed, L, D = 300_000, 3, 300.0, 700.0
@@ -221,6 +222,33 @@ def test_sensitivity_no_attenuation(gpu):
     e, e2 = rel_l2(img.data, ref), rel_l2(img2.data, ref)
     _say(f"sensitivity mu=0: relL2 {e:.2e}, without work counters {e2:.2e}")
     assert e <= 1e-5 and e2 <= 1e-5
+
+
+def test_sensitivity_unit_weights_fixed_point_bound(gpu):
+    """Without attenuation the tiles sum the weights in 32-bit fixed point,
+    each step rounded to at most 2^-17 of a voxel diagonal.  Against the same
+    walks' weights summed in binary64 (the rows, bit-identical with the
+    oracle's), every voxel lies within (its step count) x diagonal x 2^-17 / N
+    plus the binary32 output's rounding; with 18 steps per voxel on
+    average here, the image is within 3e-6 of those sums (the rounding
+    averages out as the steps per voxel grow)."""
+    att = P.FOV([180.0] * 3, [60] * 3)
+    mu = P.Image(att, np.zeros(att.nvox, np.float32))
+    n = 40_000
+    lors = P.sensitivity_lors(att, n, 11, 300.0, 700.0)
+    img = P.make_sensitivity_image(mu, n, lors=lors).data.astype(np.float64)
+    idx, w, count, _ = P.system_matrix_rows(att, lors)
+    assert int(count.max()) <= idx.shape[1]
+    keep = np.arange(idx.shape[1])[None, :] < count[:, None]
+    vi, vw = idx[keep], w[keep].astype(np.float64)
+    exact = np.bincount(vi, weights=vw, minlength=att.nvox) / n
+    steps = np.bincount(vi, minlength=att.nvox)
+    diag = float(np.sqrt(np.sum((np.asarray(att.full_size) / np.asarray(att.n)) ** 2)))
+    bound = steps * diag * 2.0 ** -17 / n + np.abs(exact) * 2.0 ** -23 + 1e-30
+    worst = float(np.max(np.abs(img - exact) / bound))
+    _say(f"unit weights: relL2 to the binary64 sums {rel_l2(img, exact):.2e}, worst voxel at {worst:.2f} of its bound")
+    assert worst <= 1.0
+    assert rel_l2(img, exact) <= 3e-6
 
 
 def test_sensitivity_irregular_lors(gpu):
