@@ -415,27 +415,20 @@ __host__ __device__ __forceinline__ bool advance_packed(WalkS& w, const WalkC c,
 // again, so Q = q + (q & 1) from then on.  So k additions jump V to V + kQ
 // while the start of the last one, V + (k-1)Q, still satisfies the bound;
 // the addition that leaves the binade is done by the hardware in the same
-// round, and the next binade repeats the argument.  v < d (only at the
-// start of a walk) takes hardware additions until v >= d, which keeps
-// d / u below 2^24.
-// floor(num / den) for num < 2^24, 0 < den < 2^24: a binary32 estimate
-// (within a few units: both operands are exact, the reciprocal and the
-// product are each within an ulp) corrected in integers.
-__host__ __device__ __forceinline__ uint32_t div_small(uint32_t num, uint32_t den) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const float rcp = __builtin_amdgcn_rcpf((float)den);
-#else
-  const float rcp = 1.0f / (float)den;
-#endif
-  uint32_t k = (uint32_t)((float)num * rcp);
-  while (k * den > num) --k;
-  while ((k + 1u) * den <= num) ++k;
-  return k;
-}
-
+// round, and the next binade repeats the argument.  Values below 8 d (the
+// start of a walk, where binades hold few crossings) take hardware additions
+// first, which also keeps d / u below 2^24.  The integers V, q, Q and k are
+// carried in binary32 (all below 2^24, so exact) and combined by fused
+// multiply-adds.
 __host__ __device__ __forceinline__ float repeat_add(float v, float d, uint32_t m) {
   if (m == 0u) return v;
   if (!(v < INFINITY && d < INFINITY)) return v + d;
+  // below 8 d a binade holds few crossings (the start of a walk): hardware
+  // additions, at most 16, so that the rounds below cross few binades
+  for (int j = 0; j < 16 && m > 0u && v * 0.125f < d; ++j) {
+    v = v + d;
+    --m;
+  }
   while (m > 0) {
     if (!(v >= d)) {
       v = v + d;
@@ -444,31 +437,40 @@ __host__ __device__ __forceinline__ float repeat_add(float v, float d, uint32_t 
     }
     uint32_t bits;
     memcpy(&bits, &v, 4);
-    int32_t e = (int32_t)(bits >> 23) - 127;  // v >= d > 0 is normal here
-    uint32_t V = (bits & 0x7fffffu) | 0x800000u;
+    const int32_t e = (int32_t)(bits >> 23) - 127;  // v >= d > 0 is normal here
+    float Vf = (float)((bits & 0x7fffffu) | 0x800000u);
     const float t = ldexpf(d, 23 - e);  // d / u, exact (or below 1/2 when it underflows)
     const float qf = floorf(t);
     const float r = t - qf;
-    const uint32_t q = (uint32_t)qf;
-    uint32_t Q = q + (r > 0.5f ? 1u : 0u);
+    float Qf = qf + (r > 0.5f ? 1.0f : 0.0f);
     if (r == 0.5f) {  // a tie: one hardware addition makes V even
       v = v + d;
       --m;
       uint32_t b2;
       memcpy(&b2, &v, 4);
       if (m == 0u || (int32_t)(b2 >> 23) - 127 != e) continue;  // done, or left the binade
-      V = (b2 & 0x7fffffu) | 0x800000u;
-      Q = q + (q & 1u);
+      Vf = (float)((b2 & 0x7fffffu) | 0x800000u);
+      Qf = qf + (float)((uint32_t)qf & 1u);
     }
-    if (V + q > 0xffffffu) {  // the next addition leaves the binade
+    // integers below 2^24 are exact in binary32, and so is every sum or fused
+    // product-sum here that stays at or below 2^24; above, the comparisons
+    // with 2^24 - 1 still come out right (the rounded value stays above it)
+    if (Vf + qf > 16777215.0f) {  // the next addition leaves the binade
       v = v + d;
       --m;
       continue;
     }
-    uint32_t k = m;
-    if (Q != 0u && (uint64_t)V + (uint64_t)(m - 1u) * Q + q > 0xffffffu) k = div_small(0xffffffu - q - V, Q) + 1u;
-    v = ldexpf((float)(V + k * Q), e - 23);  // V + kQ <= 2^24: exact
-    m -= k;
+    float kf = (float)m;
+    if (fmaf(kf - 1.0f, Qf, Vf + qf) > 16777215.0f) {
+      // k - 1 = floor((2^24 - 1 - q - V) / Q): the rounded quotient is at
+      // most one above it, and never below
+      const float num = (16777215.0f - qf) - Vf;
+      float kk = floorf(num / Qf);
+      if (fmaf(kk, Qf, -num) > 0.0f) kk -= 1.0f;
+      kf = kk + 1.0f;
+    }
+    v = ldexpf(fmaf(kf, Qf, Vf), e - 23);  // V + kQ <= 2^24: exact
+    m -= (uint32_t)kf;
     if (m > 0u) {  // the addition that leaves the binade
       v = v + d;
       --m;
