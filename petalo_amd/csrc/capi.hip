@@ -385,10 +385,18 @@ __global__ void k_flag_irregular(uint32_t* __restrict__ counts, uint32_t n, unsi
   if (irr) counts[i] = 0;
 }
 
-// *sum += counts[0 .. n), in 64 bits (zeroed by the caller).
+// *sum += counts[0 .. n), in 64 bits (zeroed by the caller).  Each lane sums
+// a grid-strided share and each wavefront adds once, so that the one address
+// takes a few thousand atomics rather than one per 64 counts.  Launch with
+// k_sum_grid(n) blocks of kBlock.
+inline unsigned k_sum_grid(size_t n) { return std::min<unsigned>(grid_for(n), 1024u); }
 __global__ void k_sum_u32(const uint32_t* __restrict__ counts, uint32_t n, unsigned long long* __restrict__ sum) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  wave_add_u64(sum, i < n ? counts[i] : 0u);
+  unsigned long long s = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    s += counts[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(sum, s);
 }
 
 // steps[j] = length of sorted run j (then scanned into a running total).
@@ -564,7 +572,7 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
                                                                    nullptr, nullptr, nullptr, 0u);
     k_flag_irregular<<<grid_for(count + 1), kBlock, 0, st>>>(nr, (uint32_t)count, flag.p);
     HIP_CHECK(hipMemsetAsync(sum.p, 0, 8, st));
-    k_sum_u32<<<grid_for(count), kBlock, 0, st>>>(nr, (uint32_t)count, sum.p);
+    k_sum_u32<<<k_sum_grid(count), kBlock, 0, st>>>(nr, (uint32_t)count, sum.p);
     HIP_CHECK(hipGetLastError());
     size_t sel_bytes = 0, scan_bytes = 0;
     hipcub::CountingInputIterator<uint32_t> idx(0);
@@ -2303,9 +2311,11 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
                 uint64_t n_lors, const LorSource& produce, double* acc, unsigned long long* cnt, bool by_length,
                 bool want_stats) {
   hipStream_t s = ws.stream;
-  // the default transaxial tile, as deep as the LDS holds: the sensitivity
-  // scatter runs one workgroup per CU anyway, and deeper tiles cut the runs
-  // per LOR (for 120^3: 40 x 40 x 12, 4.1 runs per LOR against 5.3 at depth 6)
+  // the default transaxial tile, as deep as a 64-bit LDS tile allows (the
+  // unit-weight scatter's 32-bit tile then still fits two workgroups per CU,
+  // the 64-bit one one): deeper tiles cut the runs per LOR, and the binning
+  // scales with them (for 120^3: 40 x 40 x 12, 4.1 runs per LOR against 5.3
+  // at the reconstruction's depth 6)
   const int32_t no_shape[3] = {0, 0, 0};
   const TileGeom g0 = tile_shape(fov, no_shape);
   const int32_t sens_shape[3] = {0, 0, even_split(fov.n[2], std::max(1, kMaxTileVoxels / (g0.px * g0.ty)))};
