@@ -2228,7 +2228,10 @@ struct SensWork {
 
 constexpr int kSensDevices = 64;
 std::mutex g_sens_mu;
-std::unique_ptr<SensWork> g_sens_work[kSensDevices];
+// never destroyed: a set still kept when the process exits is left to the
+// driver, as freeing device memory from static destructors may run after the
+// HIP runtime has shut down
+std::unique_ptr<SensWork>* const g_sens_work = new std::unique_ptr<SensWork>[kSensDevices];
 
 struct SensLease {
   int dev = -1;
