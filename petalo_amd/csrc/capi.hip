@@ -19,6 +19,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/petalo_hip.h"
@@ -146,17 +147,80 @@ __global__ void k_lor_ac(const Lor* __restrict__ lors, size_t n, float* __restri
 
 // The context's LORs on the device: uploaded once, then (sort) each subset's
 // LORs reordered as above; the remainder past the last subset is kept as is.
+}  // namespace
+
+namespace petalo {
+
+// Host-to-device upload through pinned staging (declared in capi_internal.hpp).
+constexpr size_t kUploadChunk = 16u << 20;  // bytes per pinned buffer
+constexpr size_t kUploadMin = 32u << 20;    // smaller copies are plain
+void upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  static std::mutex mu;
+  // never freed: kept for the process (and not released by static destructors,
+  // which may run after the HIP runtime has shut down)
+  static void* pin[2] = {nullptr, nullptr};
+  std::unique_lock<std::mutex> lk(mu, std::try_to_lock);
+  if (bytes < kUploadMin || !lk.owns_lock()) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return;
+  }
+  for (int b = 0; b < 2; ++b)
+    if (!pin[b]) HIP_CHECK(hipHostMalloc(&pin[b], kUploadChunk, hipHostMallocDefault));
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  for (int b = 0; b < 2; ++b) HIP_CHECK(hipEventCreateWithFlags(&ev.e[b], hipEventDisableTiming));
+  const unsigned hc = std::thread::hardware_concurrency();
+  const unsigned n_threads = std::max(1u, std::min(8u, hc ? hc : 1u));
+  bool pending[2] = {false, false};
+  size_t off = 0;
+  int b = 0;
+  try {
+    while (off < bytes) {
+      const size_t n = std::min(kUploadChunk, bytes - off);
+      if (pending[b]) HIP_CHECK(hipEventSynchronize(ev.e[b]));  // the buffer's previous DMA is done
+      const char* from = static_cast<const char*>(src) + off;
+      char* to = static_cast<char*>(pin[b]);
+      const size_t per = (n + n_threads - 1) / n_threads;
+      std::vector<std::thread> th;
+      for (unsigned k = 1; k < n_threads; ++k) {
+        const size_t a = std::min(n, (size_t)k * per), e = std::min(n, a + per);
+        if (a < e) th.emplace_back([=] { std::memcpy(to + a, from + a, e - a); });
+      }
+      std::memcpy(to, from, std::min(n, per));
+      for (std::thread& t : th) t.join();
+      HIP_CHECK(hipMemcpyAsync(static_cast<char*>(dst) + off, pin[b], n, hipMemcpyHostToDevice, st));
+      HIP_CHECK(hipEventRecord(ev.e[b], st));
+      pending[b] = true;
+      off += n;
+      b ^= 1;
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(st);  // no DMA may still read a buffer the next upload refills
+    throw;
+  }
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace petalo
+
+namespace {
+
 void place_lors(DevBuf<Lor>& out, hipStream_t st, const petalo_lor_t* lors, size_t n_lors, size_t n_subsets,
                 size_t m, bool tof, bool sort, const petalo_fov_t& fov, const TileGeom& g) {
   out.alloc(std::max<size_t>(n_lors, 1));
   if (!n_lors) return;
   if (!sort || m < 2) {
-    HIP_CHECK(hipMemcpyAsync(out.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+    upload_h2d(out.p, lors, n_lors * sizeof(Lor), st);
     return;
   }
   DevBuf<Lor> raw(n_lors);
-  HIP_CHECK(hipMemcpyAsync(raw.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice, st));
+  upload_h2d(raw.p, lors, n_lors * sizeof(Lor), st);
   OrderParams p;
   for (int a = 0; a < 3; ++a) {
     p.full[a] = fov.full_size_mm[a];
@@ -2022,7 +2086,7 @@ int petalo_system_matrix_rows(const petalo_fov_t* fov, const petalo_tof_t* tof, 
     DevBuf<int32_t> di(std::max<size_t>(n_lors * cap, 1));
     DevBuf<float> dw(std::max<size_t>(n_lors * cap, 1));
     DevBuf<uint32_t> dc(n_lors), ds(n_lors);
-    HIP_CHECK(hipMemcpy(dl.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice));
+    upload_h2d(dl.p, lors, n_lors * sizeof(Lor), nullptr);
     if (t.on)
       k_rows<true><<<grid_for(n_lors), kBlock>>>(dl.p, (uint32_t)n_lors, f, t, (uint32_t)cap, di.p, dw.p, dc.p, ds.p);
     else
@@ -2643,7 +2707,7 @@ int petalo_make_sensitivity_image(const petalo_fov_t* att_fov, const float* mu, 
       HIP_CHECK(hipMemcpyAsync(dmu, mu, na * sizeof(float), hipMemcpyHostToDevice, s));
     if (lors && n_lors) {
       dl.alloc(n_lors);
-      HIP_CHECK(hipMemcpyAsync(dl.p, lors, n_lors * sizeof(Lor), hipMemcpyHostToDevice, s));
+      upload_h2d(dl.p, lors, n_lors * sizeof(Lor), s);
     }
     fill_f64(acc, 0.0, no, s);
     zero_counters(cnt, s);

@@ -139,7 +139,7 @@ int petalo_ingest_lors(const float* records, size_t n_records, const petalo_inge
     const uint32_t m = (uint32_t)n;
     DevBuf<H5Lor> rec(m), smeared(m), kept(m);
     DevBuf<uint32_t> keep(m + 1), pos(m + 1);
-    HIP_CHECK(hipMemcpyAsync(rec.p, records + lo * 11, (size_t)m * sizeof(H5Lor), hipMemcpyHostToDevice, s));
+    upload_h2d(rec.p, records + lo * 11, (size_t)m * sizeof(H5Lor), s);
     k_ingest<<<grid_for(m), kBlock, 0, s>>>(rec.p, lo, m, make_ingest(p), smeared.p, keep.p);
     HIP_CHECK(hipMemsetAsync(keep.p + m, 0, 4, s));
     size_t tmp_bytes = 0;

@@ -224,6 +224,38 @@ def test_sensitivity_no_attenuation(gpu):
     assert e <= 1e-5 and e2 <= 1e-5
 
 
+def test_large_lor_upload(gpu):
+    """LOR arrays of 32 MB and more reach the device through the pinned
+    upload ring (16 MB chunks filled by host threads): 1.3e6 given LORs
+    (41.6 MB, three chunks, the last partial) give the image of the same
+    LORs generated on the device; and a context of 1.1e6 LORs that miss the
+    FOV followed by 2e5 phantom LORs (the latter all in the last chunks)
+    gives the update of a context of the 2e5 alone (6.4 MB, a plain copy) --
+    equal to 1e-9."""
+    att = P.FOV([180.0] * 3, [30] * 3)
+    mu = P.density_to_attenuation(P.synth_density(att), 0.096)
+    n, seed, L, D = 1_300_000, 23, 300.0, 700.0
+    gen = P.sensitivity_lors(att, n, seed, L, D)
+    a = P.make_sensitivity_image(mu, n, L, D, seed=seed).data
+    b = P.make_sensitivity_image(mu, n, lors=gen).data
+    e = rel_l2(b, a)
+    fov = P.FOV([180.0] * 3, [40] * 3)
+    real = P.synth_phantom_lors(5, 200_000, 200.0)
+    miss = np.zeros(1_100_000, P.LOR_DTYPE)
+    miss["p1"] = (1000.0, 1000.0, 1000.0)
+    miss["p2"] = (1000.0, 1001.0, 1000.0)
+    miss["ac"] = 1.0
+    c1 = P.Mlem(fov, real, tof=P.Tof(42.47, 3.0))
+    c2 = P.Mlem(fov, np.concatenate([miss, real]), tof=P.Tof(42.47, 3.0))
+    i1, _ = c1.step()
+    i2, _ = c2.step()
+    e2 = rel_l2(i2.data, i1.data)
+    c1.close()
+    c2.close()
+    _say(f"large uploads: sensitivity given vs generated {e:.1e}, contexts {e2:.1e}")
+    assert e <= 1e-9 and e2 <= 1e-9
+
+
 def test_sensitivity_unit_weights_fixed_point_bound(gpu):
     """Without attenuation the tiles sum the weights in 32-bit fixed point,
     each step rounded to at most 2^-17 of a voxel diagonal.  Against the same
