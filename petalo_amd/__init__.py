@@ -122,7 +122,8 @@ def scattergram_spec(phi=0, z=(0, 0.0), dz=(0, 0.0), r=(0, 0.0), dt=(0, 0.0)):
 
 def ingest_lors(records, params, scattergram=None):
     """reco_info/lors records (H5LOR_DTYPE or n x 11 float32) -> (LORs, records read)
-    (src/io/hdf5.rs:97-228)."""
+    (src/io/hdf5.rs:97-228).  The LORs are a view of a buffer sized for every
+    record (the kept ones first)."""
     rec = np.ascontiguousarray(records)
     rec = rec.view(np.float32).reshape(-1, 11) if rec.dtype != np.float32 else rec.reshape(-1, 11)
     rec = np.ascontiguousarray(rec, np.float32)
@@ -132,7 +133,7 @@ def ingest_lors(records, params, scattergram=None):
     _check(lib().petalo_ingest_lors(_ptr(rec), len(rec), C.byref(params),
                                     C.byref(scattergram) if scattergram is not None else None,
                                     _ptr(out), len(out), C.byref(n_out), C.byref(n_read)))
-    return out[:n_out.value].copy(), int(n_read.value)
+    return out[:n_out.value], int(n_read.value)
 
 
 class _Config(C.Structure):

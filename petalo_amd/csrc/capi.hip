@@ -151,49 +151,62 @@ __global__ void k_lor_ac(const Lor* __restrict__ lors, size_t n, float* __restri
 
 namespace petalo {
 
-// Host-to-device upload through pinned staging (declared in capi_internal.hpp).
-constexpr size_t kUploadChunk = 16u << 20;  // bytes per pinned buffer
-constexpr size_t kUploadMin = 32u << 20;    // smaller copies are plain
+// Host transfers through pinned staging (declared in capi_internal.hpp): a
+// process-wide ring of two pinned buffers, one transfer at a time.
+constexpr size_t kStageChunk = 16u << 20;  // bytes per pinned buffer
+constexpr size_t kStageMin = 32u << 20;    // smaller copies are plain
+struct StageRing {
+  std::mutex mu;
+  void* pin[2] = {nullptr, nullptr};
+};
+// never freed: kept for the process (and not released by static destructors,
+// which may run after the HIP runtime has shut down)
+StageRing* const g_stage = new StageRing();
+
+struct StageEvents {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  StageEvents() {
+    for (hipEvent_t& x : e) HIP_CHECK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  }
+  ~StageEvents() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
+// memcpy of n bytes split over up to 8 host threads (the caller's included).
+void parallel_copy(char* to, const char* from, size_t n) {
+  const unsigned hc = std::thread::hardware_concurrency();
+  const unsigned n_threads = std::max(1u, std::min(8u, hc ? hc : 1u));
+  const size_t per = (n + n_threads - 1) / n_threads;
+  std::vector<std::thread> th;
+  for (unsigned k = 1; k < n_threads; ++k) {
+    const size_t a = std::min(n, (size_t)k * per), e = std::min(n, a + per);
+    if (a < e) th.emplace_back([=] { std::memcpy(to + a, from + a, e - a); });
+  }
+  std::memcpy(to, from, std::min(n, per));
+  for (std::thread& t : th) t.join();
+}
+
 void upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
-  static std::mutex mu;
-  // never freed: kept for the process (and not released by static destructors,
-  // which may run after the HIP runtime has shut down)
-  static void* pin[2] = {nullptr, nullptr};
-  std::unique_lock<std::mutex> lk(mu, std::try_to_lock);
-  if (bytes < kUploadMin || !lk.owns_lock()) {
+  std::unique_lock<std::mutex> lk(g_stage->mu, std::try_to_lock);
+  if (bytes < kStageMin || !lk.owns_lock()) {
     HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
     HIP_CHECK(hipStreamSynchronize(st));
     return;
   }
+  void** pin = g_stage->pin;
   for (int b = 0; b < 2; ++b)
-    if (!pin[b]) HIP_CHECK(hipHostMalloc(&pin[b], kUploadChunk, hipHostMallocDefault));
-  struct Events {
-    hipEvent_t e[2] = {nullptr, nullptr};
-    ~Events() {
-      for (hipEvent_t x : e)
-        if (x) (void)hipEventDestroy(x);
-    }
-  } ev;
-  for (int b = 0; b < 2; ++b) HIP_CHECK(hipEventCreateWithFlags(&ev.e[b], hipEventDisableTiming));
-  const unsigned hc = std::thread::hardware_concurrency();
-  const unsigned n_threads = std::max(1u, std::min(8u, hc ? hc : 1u));
+    if (!pin[b]) HIP_CHECK(hipHostMalloc(&pin[b], kStageChunk, hipHostMallocDefault));
+  StageEvents ev;
   bool pending[2] = {false, false};
   size_t off = 0;
   int b = 0;
   try {
     while (off < bytes) {
-      const size_t n = std::min(kUploadChunk, bytes - off);
+      const size_t n = std::min(kStageChunk, bytes - off);
       if (pending[b]) HIP_CHECK(hipEventSynchronize(ev.e[b]));  // the buffer's previous DMA is done
-      const char* from = static_cast<const char*>(src) + off;
-      char* to = static_cast<char*>(pin[b]);
-      const size_t per = (n + n_threads - 1) / n_threads;
-      std::vector<std::thread> th;
-      for (unsigned k = 1; k < n_threads; ++k) {
-        const size_t a = std::min(n, (size_t)k * per), e = std::min(n, a + per);
-        if (a < e) th.emplace_back([=] { std::memcpy(to + a, from + a, e - a); });
-      }
-      std::memcpy(to, from, std::min(n, per));
-      for (std::thread& t : th) t.join();
+      parallel_copy(static_cast<char*>(pin[b]), static_cast<const char*>(src) + off, n);
       HIP_CHECK(hipMemcpyAsync(static_cast<char*>(dst) + off, pin[b], n, hipMemcpyHostToDevice, st));
       HIP_CHECK(hipEventRecord(ev.e[b], st));
       pending[b] = true;
@@ -201,10 +214,42 @@ void upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
       b ^= 1;
     }
   } catch (...) {
-    (void)hipStreamSynchronize(st);  // no DMA may still read a buffer the next upload refills
+    (void)hipStreamSynchronize(st);  // no DMA may still read a buffer the next transfer refills
     throw;
   }
   HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void download_d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  std::unique_lock<std::mutex> lk(g_stage->mu, std::try_to_lock);
+  if (bytes < kStageMin || !lk.owns_lock()) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return;
+  }
+  void** pin = g_stage->pin;
+  for (int b = 0; b < 2; ++b)
+    if (!pin[b]) HIP_CHECK(hipHostMalloc(&pin[b], kStageChunk, hipHostMallocDefault));
+  StageEvents ev;
+  auto dma = [&](size_t off, int b) {
+    const size_t n = std::min(kStageChunk, bytes - off);
+    HIP_CHECK(hipMemcpyAsync(pin[b], static_cast<const char*>(src) + off, n, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipEventRecord(ev.e[b], st));
+  };
+  try {
+    dma(0, 0);
+    int b = 0;
+    for (size_t off = 0; off < bytes; off += kStageChunk, b ^= 1) {
+      // the other buffer was copied out in the previous round: refill it
+      if (off + kStageChunk < bytes) dma(off + kStageChunk, b ^ 1);
+      HIP_CHECK(hipEventSynchronize(ev.e[b]));
+      parallel_copy(static_cast<char*>(dst) + off, static_cast<const char*>(pin[b]),
+                    std::min(kStageChunk, bytes - off));
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(st);  // no DMA may still write a buffer the next transfer reads
+    throw;
+  }
 }
 
 }  // namespace petalo

@@ -15,6 +15,11 @@ namespace petalo {
 // copy from pageable memory); small ones, and those that find the ring in
 // use by another thread, are plain copies.  Throws HipError on failure.
 void upload_h2d(void* dst, const void* src, size_t bytes, hipStream_t st);
+// The reverse: device src to pageable host memory at dst, the DMA of one
+// pinned buffer overlapping the host threads' copy out of the other (which
+// also spreads the first-touch page faults of a fresh destination over the
+// threads).  Work on st before the call is ordered before the copy.
+void download_d2h(void* dst, const void* src, size_t bytes, hipStream_t st);
 
 // Records the message returned by petalo_last_error() on this thread.
 void set_error(const std::string& msg);

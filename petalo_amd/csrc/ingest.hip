@@ -166,7 +166,7 @@ int petalo_ingest_lors(const float* records, size_t n_records, const petalo_inge
       k_to_lor<<<grid_for(n_kept), kBlock, 0, s>>>(kept.p, n_kept, sg, have_sg ? trues.p : nullptr,
                                                    have_sg ? scatters.p : nullptr, lors.p);
       HIP_CHECK(hipGetLastError());
-      HIP_CHECK(hipMemcpyAsync(out, lors.p, (size_t)n_kept * sizeof(Lor), hipMemcpyDeviceToHost, s));
+      download_d2h(out, lors.p, (size_t)n_kept * sizeof(Lor), s);
     }
     HIP_CHECK(hipStreamSynchronize(s));
     *n_out = n_kept;

@@ -696,6 +696,33 @@ def test_batch_matches_single_reconstructions(gpu):
         s.close()
 
 
+def test_ingest_large_transfers(gpu):
+    """Ingestion of 1.2e6 records in one call (52.8 MB of records up, about
+    38 MB of LORs down: both through the pinned transfer ring) equals the
+    same records ingested as four event ranges of 3e5 (plain copies),
+    concatenated: every draw is keyed by the record's index."""
+    rng = np.random.default_rng(78)
+    n = 1_200_000
+    rec = np.zeros(n, P.H5LOR_DTYPE)
+    a1, a2 = rng.uniform(0, 2 * np.pi, n), rng.uniform(0, 2 * np.pi, n)
+    rec["x1"], rec["y1"], rec["z1"] = 360 * np.cos(a1), 360 * np.sin(a1), rng.uniform(-160, 160, n)
+    rec["x2"], rec["y2"], rec["z2"] = 360 * np.cos(a2), 360 * np.sin(a2), rng.uniform(-160, 160, n)
+    rec["dt"] = np.arange(n, dtype=np.float32) * np.float32(1e-3)
+    rec["q1"], rec["q2"] = rng.uniform(0, 3000, n), rng.uniform(0, 3000, n)
+    rec["E1"], rec["E2"] = rng.normal(511, 5, n), rng.normal(511, 5, n)
+
+    def params(lo, hi):
+        return P.ingest_params(events=(lo, hi), smear_energy_fwhm=0.12, z_max=150.0,
+                               discretize=(350.0, 20.0, 3.0, 3.0), adjust=P.ADJUST_RANDOM_Z_PHI, seed=7)
+
+    whole, nw = P.ingest_lors(rec, params(0, n))
+    parts = [P.ingest_lors(rec, params(k, k + 300_000))[0] for k in range(0, n, 300_000)]
+    joined = np.concatenate(parts)
+    _say(f"ingest in one call: {len(whole)} LORs ({len(whole) * 32 / 2**20:.1f} MiB), in four: {len(joined)}")
+    assert nw == n and len(whole) * 32 >= 32 << 20
+    assert whole.tobytes() == joined.tobytes()
+
+
 def test_ingest_matches_oracle(gpu):
     """LOR ingestion (cuts, energy smearing, RandomZPhi and Random element
     adjustment, scattergram) on the device against the oracle's restatement
