@@ -2139,11 +2139,11 @@ int petalo_system_matrix_rows(const petalo_fov_t* fov, const petalo_tof_t* tof, 
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipDeviceSynchronize());
     if (cap) {
-      HIP_CHECK(hipMemcpy(idx, di.p, n_lors * cap * sizeof(int32_t), hipMemcpyDeviceToHost));
-      HIP_CHECK(hipMemcpy(w, dw.p, n_lors * cap * sizeof(float), hipMemcpyDeviceToHost));
+      download_d2h(idx, di.p, n_lors * cap * sizeof(int32_t), nullptr);
+      download_d2h(w, dw.p, n_lors * cap * sizeof(float), nullptr);
     }
-    HIP_CHECK(hipMemcpy(count, dc.p, n_lors * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(steps, ds.p, n_lors * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    download_d2h(count, dc.p, n_lors * sizeof(uint32_t), nullptr);
+    download_d2h(steps, ds.p, n_lors * sizeof(uint32_t), nullptr);
     return PETALO_OK;
   });
 }

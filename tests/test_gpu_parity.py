@@ -696,6 +696,24 @@ def test_batch_matches_single_reconstructions(gpu):
         s.close()
 
 
+def test_rows_large_transfer(gpu):
+    """System-matrix rows of 1e5 LORs in one call (150 MB of indices and as
+    much of weights come down through the pinned transfer ring) equal the
+    same rows computed 5000 LORs at a time (plain copies); entries past a
+    row's count are not written by the library and are not compared."""
+    fov = P.FOV([180.0] * 3, [60] * 3)
+    lors = P.synth_phantom_lors(8, 100_000, 200.0)
+    tof = P.Tof(42.47, 3.0)
+    idx, w, cnt, st = P.system_matrix_rows(fov, lors, tof=tof)
+    parts = [P.system_matrix_rows(fov, lors[k:k + 5000], tof=tof, cap=idx.shape[1]) for k in range(0, 100_000, 5000)]
+    assert idx.nbytes >= 32 << 20
+    assert np.array_equal(cnt, np.concatenate([q[2] for q in parts]))
+    assert np.array_equal(st, np.concatenate([q[3] for q in parts]))
+    valid = np.arange(idx.shape[1])[None, :] < np.minimum(cnt, idx.shape[1])[:, None]
+    assert np.array_equal(idx[valid], np.concatenate([q[0] for q in parts])[valid])
+    assert np.array_equal(w.view(np.uint32)[valid], np.concatenate([q[1] for q in parts]).view(np.uint32)[valid])
+
+
 def test_ingest_large_transfers(gpu):
     """Ingestion of 1.2e6 records in one call (52.8 MB of records up, about
     38 MB of LORs down: both through the pinned transfer ring) equals the
