@@ -579,6 +579,28 @@ inline unsigned grid64(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 
 void zero_counters(unsigned long long* p, hipStream_t s);
 
+// A device buffer that only grows.
+template <class T> struct Grow {
+  DevBuf<T> b;
+  T* ensure(size_t n) {
+    if (n > b.n) b.alloc(n + n / 4);
+    return b.p;
+  }
+};
+
+// Temporary device buffers of a context's setup (build_runs, cut_items),
+// reused across its subsets: allocating and freeing a dozen buffers per
+// subset cost more than the setup's kernels when the subsets are small.
+struct SetupScratch {
+  Grow<unsigned char> flag, tmp, wtmp;
+  Grow<unsigned long long> sum, chk;
+  Grow<uint32_t> n_sel, perm, perm2, order;
+  Grow<Run> unsorted;
+  Grow<uint64_t> keys, keys2, wkey, wkey2, dstart, incl, at, before, dst;
+  Grow<TileCut> dtc;
+  Grow<int4> dit;
+};
+
 // Work items of one subset: chunks of one tile's runs holding about equal
 // numbers of replayed steps, largest first so that the last workgroups to
 // start are the short ones.  With c->item_target the chunks hold about that
@@ -587,7 +609,9 @@ void zero_counters(unsigned long long* p, hipStream_t s);
 // whole tile, at least kItemStepsPerVoxel steps per LDS entry of the tile.
 // Cut on the device from a running total of the sorted runs' lengths
 // (k_cut_items); the host orders the few thousand items.
-void cut_items(petalo_mlem* c, size_t sub) {
+void cut_items(petalo_mlem* c, size_t sub, SetupScratch* scratch = nullptr) {
+  SetupScratch local;
+  SetupScratch& S = scratch ? *scratch : local;
   const TileGeom& g = c->tiles;
   hipStream_t st = c->stream;
   const uint64_t total = c->n_runs[sub];
@@ -595,21 +619,22 @@ void cut_items(petalo_mlem* c, size_t sub) {
   c->item_steps[sub].clear();
   std::vector<int4> items;
   if (total) {
-    DevBuf<uint64_t> incl(total);
-    k_run_steps<<<grid64(total), kBlock, 0, st>>>(c->runs[sub].p, total, incl.p);
+    uint64_t* incl = S.incl.ensure(total);
+    k_run_steps<<<grid64(total), kBlock, 0, st>>>(c->runs[sub].p, total, incl);
     HIP_CHECK(hipGetLastError());
     size_t tb = 0;
-    HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tb, incl.p, incl.p, (int)total, st));
-    DevBuf<unsigned char> tmp(std::max<size_t>(tb, 1));
-    HIP_CHECK(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, incl.p, incl.p, (int)total, st));
+    HIP_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tb, incl, incl, (int)total, st));
+    unsigned char* tmp = S.tmp.ensure(std::max<size_t>(tb, 1));
+    HIP_CHECK(hipcub::DeviceScan::InclusiveSum(tmp, tb, incl, incl, (int)total, st));
     // running totals at the tile starts
     const size_t nb = (size_t)g.ntiles + 1;
-    DevBuf<uint64_t> at(nb), before(nb);
-    HIP_CHECK(hipMemcpyAsync(at.p, starts.data(), nb * 8, hipMemcpyHostToDevice, st));
-    k_steps_before<<<grid_for(nb), kBlock, 0, st>>>(incl.p, at.p, (uint32_t)nb, before.p);
+    uint64_t* at = S.at.ensure(nb);
+    uint64_t* before = S.before.ensure(nb);
+    HIP_CHECK(hipMemcpyAsync(at, starts.data(), nb * 8, hipMemcpyHostToDevice, st));
+    k_steps_before<<<grid_for(nb), kBlock, 0, st>>>(incl, at, (uint32_t)nb, before);
     HIP_CHECK(hipGetLastError());
     std::vector<uint64_t> bs(nb);
-    HIP_CHECK(hipMemcpyAsync(bs.data(), before.p, nb * 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(bs.data(), before, nb * 8, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     const uint64_t total_steps = bs[g.ntiles];
     const uint64_t target =
@@ -626,16 +651,16 @@ void cut_items(petalo_mlem* c, size_t sub) {
       n_items += n;
     }
     if (n_items) {
-      DevBuf<TileCut> dtc(tc.size());
-      DevBuf<int4> dit(n_items);
-      DevBuf<uint64_t> dst(n_items);
-      HIP_CHECK(hipMemcpyAsync(dtc.p, tc.data(), tc.size() * sizeof(TileCut), hipMemcpyHostToDevice, st));
-      k_cut_items<<<grid_for(n_items), kBlock, 0, st>>>(incl.p, dtc.p, (uint32_t)tc.size(), n_items, dit.p, dst.p);
+      TileCut* dtc = S.dtc.ensure(tc.size());
+      int4* dit = S.dit.ensure(n_items);
+      uint64_t* dst = S.dst.ensure(n_items);
+      HIP_CHECK(hipMemcpyAsync(dtc, tc.data(), tc.size() * sizeof(TileCut), hipMemcpyHostToDevice, st));
+      k_cut_items<<<grid_for(n_items), kBlock, 0, st>>>(incl, dtc, (uint32_t)tc.size(), n_items, dit, dst);
       HIP_CHECK(hipGetLastError());
       std::vector<int4> hit(n_items);
       std::vector<uint64_t> hst(n_items);
-      HIP_CHECK(hipMemcpyAsync(hit.data(), dit.p, n_items * sizeof(int4), hipMemcpyDeviceToHost, st));
-      HIP_CHECK(hipMemcpyAsync(hst.data(), dst.p, n_items * 8, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(hit.data(), dit, n_items * sizeof(int4), hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(hst.data(), dst, n_items * 8, hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipStreamSynchronize(st));
       std::vector<std::pair<uint64_t, int4>> sized;
       for (uint32_t k = 0; k < n_items; ++k)
@@ -658,7 +683,9 @@ void cut_items(petalo_mlem* c, size_t sub) {
 
 // Cuts the subset's LOR walks into per-tile runs, sorted by tile, and the
 // per-workgroup work items over them; lists the LORs with unbounded walks.
-void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
+void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count, SetupScratch* scratch = nullptr) {
+  SetupScratch local;
+  SetupScratch& S = scratch ? *scratch : local;
   const TileGeom& g = c->tiles;
   hipStream_t st = c->stream;
   uint64_t total = 0;
@@ -668,9 +695,9 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   if (count) {
     // count pass; LORs with unbounded walks listed (in order) and given no
     // runs; the run offsets scanned on the device
-    DevBuf<unsigned char> flag(count);
-    DevBuf<unsigned long long> sum(1);
-    DevBuf<uint32_t> n_sel(1);
+    unsigned char* flag = S.flag.ensure(count);
+    unsigned long long* sum = S.sum.ensure(1);
+    uint32_t* n_sel = S.n_sel.ensure(1);
     const Lor* L = c->lors.p + lo;
     uint32_t* nr = c->run_off[sub].p;  // counts, then scanned in place
     if (c->tof_on)
@@ -679,23 +706,23 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     else
       k_bin_runs<false, false><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nr,
                                                                    nullptr, nullptr, nullptr, 0u);
-    k_flag_irregular<<<grid_for(count + 1), kBlock, 0, st>>>(nr, (uint32_t)count, flag.p);
-    HIP_CHECK(hipMemsetAsync(sum.p, 0, 8, st));
-    k_sum_u32<<<k_sum_grid(count), kBlock, 0, st>>>(nr, (uint32_t)count, sum.p);
+    k_flag_irregular<<<grid_for(count + 1), kBlock, 0, st>>>(nr, (uint32_t)count, flag);
+    HIP_CHECK(hipMemsetAsync(sum, 0, 8, st));
+    k_sum_u32<<<k_sum_grid(count), kBlock, 0, st>>>(nr, (uint32_t)count, sum);
     HIP_CHECK(hipGetLastError());
     size_t sel_bytes = 0, scan_bytes = 0;
     hipcub::CountingInputIterator<uint32_t> idx(0);
-    HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, idx, flag.p, c->irregular[sub].p, n_sel.p,
+    HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, idx, flag, c->irregular[sub].p, n_sel,
                                             (int)count, st));
     HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, nr, nr, (int)count + 1, st));
-    DevBuf<unsigned char> tmp(std::max<size_t>(std::max(sel_bytes, scan_bytes), 1));
-    HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp.p, sel_bytes, idx, flag.p, c->irregular[sub].p, n_sel.p,
+    unsigned char* tmp = S.tmp.ensure(std::max<size_t>(std::max(sel_bytes, scan_bytes), 1));
+    HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, sel_bytes, idx, flag, c->irregular[sub].p, n_sel,
                                             (int)count, st));
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, scan_bytes, nr, nr, (int)count + 1, st));
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, nr, nr, (int)count + 1, st));
     unsigned long long h_sum = 0;
     uint32_t h_sel = 0;
-    HIP_CHECK(hipMemcpyAsync(&h_sum, sum.p, 8, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipMemcpyAsync(&h_sel, n_sel.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(&h_sum, sum, 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(&h_sel, n_sel, 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));  // (also before the temporaries are freed)
     if (h_sum >= (1ull << 31)) throw std::runtime_error("2^31 or more tile runs in one subset");
     total = h_sum;
@@ -713,36 +740,38 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   std::vector<uint64_t> starts(g.ntiles + 1, UINT64_MAX);
   if (total) {
     const Lor* L = c->lors.p + lo;
-    DevBuf<Run> unsorted(total);
-    DevBuf<uint64_t> keys(total), keys2(total);
-    DevBuf<uint32_t> perm(total), perm2(total);
-    DevBuf<unsigned long long> chk(CNT_N);
-    zero_counters(chk.p, st);
+    Run* unsorted = S.unsorted.ensure(total);
+    uint64_t* keys = S.keys.ensure(total);
+    uint64_t* keys2 = S.keys2.ensure(total);
+    uint32_t* perm = S.perm.ensure(total);
+    uint32_t* perm2 = S.perm2.ensure(total);
+    unsigned long long* chk = S.chk.ensure(CNT_N);
+    zero_counters(chk, st);
     if (c->tof_on)
       k_bin_runs<true, true><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
-                                                                 c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift,
-                                                                 chk.p);
+                                                                 c->run_off[sub].p, unsorted, keys, kLorBlockShift,
+                                                                 chk);
     else
       k_bin_runs<false, true><<<bin_grid(count), kBlock, 0, st>>>(L, (uint32_t)count, c->dfov, c->dtof, g, nullptr,
-                                                                  c->run_off[sub].p, unsorted.p, keys.p, kLorBlockShift,
-                                                                  chk.p);
+                                                                  c->run_off[sub].p, unsorted, keys, kLorBlockShift,
+                                                                  chk);
     HIP_CHECK(hipGetLastError());
     {
       unsigned long long h[CNT_N];
-      HIP_CHECK(hipMemcpyAsync(h, chk.p, sizeof h, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(h, chk, sizeof h, hipMemcpyDeviceToHost, st));
       HIP_CHECK(hipStreamSynchronize(st));
       if (h[CNT_RUN_MISMATCH])
         throw std::runtime_error("tile binning: fill and count passes disagree on " + std::to_string(h[CNT_RUN_MISMATCH]) +
                                  " LORs");
     }
-    k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
+    k_iota<<<grid64(total), kBlock, 0, st>>>(perm, total);
     int end_bit = 33;
     while ((1 << (end_bit - 32)) < g.ntiles) ++end_bit;
     size_t tmp_bytes = 0;
-    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total, 0,
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, perm, perm2, (int)total, 0,
                                                  end_bit, st));
-    DevBuf<unsigned char> tmp(std::max<size_t>(tmp_bytes, 1));
-    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, perm.p, perm2.p, (int)total, 0,
+    unsigned char* tmp = S.tmp.ensure(std::max<size_t>(tmp_bytes, 1));
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, perm, perm2, (int)total, 0,
                                                  end_bit, st));
     // partial layout by (window block, tile): a stable sort of the sorted
     // runs by that key gives each run's stored position
@@ -753,21 +782,22 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     c->lpos[sub].alloc(total);
     std::vector<uint32_t> hw(c->n_blocks[sub] + 1, UINT32_MAX);
     {
-      DevBuf<uint64_t> wkey(total), wkey2(total);
-      DevBuf<uint32_t> order(total);
-      k_window_keys<<<grid64(total), kBlock, 0, st>>>(unsorted.p, keys2.p, perm2.p, total, (uint32_t)tile_bits,
-                                                      wkey.p);
-      k_iota<<<grid64(total), kBlock, 0, st>>>(perm.p, total);
+      uint64_t* wkey = S.wkey.ensure(total);
+      uint64_t* wkey2 = S.wkey2.ensure(total);
+      uint32_t* order = S.order.ensure(total);
+      k_window_keys<<<grid64(total), kBlock, 0, st>>>(unsorted, keys2, perm2, total, (uint32_t)tile_bits,
+                                                      wkey);
+      k_iota<<<grid64(total), kBlock, 0, st>>>(perm, total);
       HIP_CHECK(hipGetLastError());
       size_t wbytes = 0;
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, wbytes, wkey.p, wkey2.p, perm.p, order.p, (int)total, 0,
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, wbytes, wkey, wkey2, perm, order, (int)total, 0,
                                                    tile_bits + blk_bits, st));
-      DevBuf<unsigned char> wtmp(std::max<size_t>(wbytes, 1));
-      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(wtmp.p, wbytes, wkey.p, wkey2.p, perm.p, order.p, (int)total, 0,
+      unsigned char* wtmp = S.wtmp.ensure(std::max<size_t>(wbytes, 1));
+      HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(wtmp, wbytes, wkey, wkey2, perm, order, (int)total, 0,
                                                    tile_bits + blk_bits, st));
-      k_invert_perm<<<grid64(total), kBlock, 0, st>>>(order.p, total, c->lpos[sub].p);
+      k_invert_perm<<<grid64(total), kBlock, 0, st>>>(order, total, c->lpos[sub].p);
       HIP_CHECK(hipMemcpyAsync(c->win[sub].p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, st));
-      k_window_starts<<<grid64(total), kBlock, 0, st>>>(wkey2.p, total, (uint32_t)tile_bits, c->n_blocks[sub],
+      k_window_starts<<<grid64(total), kBlock, 0, st>>>(wkey2, total, (uint32_t)tile_bits, c->n_blocks[sub],
                                                         c->win[sub].p);
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipMemcpyAsync(hw.data(), c->win[sub].p, hw.size() * 4, hipMemcpyDeviceToHost, st));
@@ -782,14 +812,14 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
     for (size_t b = 0; b < c->n_blocks[sub]; ++b) mw = std::max(mw, hw[b + 1] - hw[b]);
     c->max_win[sub] = mw;
     HIP_CHECK(hipMemcpyAsync(c->win[sub].p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice, st));
-    k_place_runs<<<grid64(total), kBlock, 0, st>>>(unsorted.p, keys2.p, perm2.p, total, c->runs[sub].p,
+    k_place_runs<<<grid64(total), kBlock, 0, st>>>(unsorted, keys2, perm2, total, c->runs[sub].p,
                                                    c->run_pos[sub].p, nullptr, c->lpos[sub].p);
     // first sorted run of every tile (UINT64_MAX where a tile has none)
-    DevBuf<uint64_t> dstart(g.ntiles);
-    HIP_CHECK(hipMemsetAsync(dstart.p, 0xff, (size_t)g.ntiles * 8, st));
-    k_tile_starts<<<grid64(total), kBlock, 0, st>>>(keys2.p, total, 32u, (uint32_t)g.ntiles, dstart.p);
+    uint64_t* dstart = S.dstart.ensure(g.ntiles);
+    HIP_CHECK(hipMemsetAsync(dstart, 0xff, (size_t)g.ntiles * 8, st));
+    k_tile_starts<<<grid64(total), kBlock, 0, st>>>(keys2, total, 32u, (uint32_t)g.ntiles, dstart);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(starts.data(), dstart.p, (size_t)g.ntiles * 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(starts.data(), dstart, (size_t)g.ntiles * 8, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
   }
   starts[g.ntiles] = total;
@@ -813,7 +843,7 @@ void build_runs(petalo_mlem* c, size_t sub, size_t lo, size_t count) {
   HIP_CHECK(hipGetLastError());
   c->n_runs[sub] = total;
   c->tile_start[sub] = std::move(starts);
-  cut_items(c, sub);
+  cut_items(c, sub, &S);
 }
 
 // 1/(p*ac) per LOR from the partials of the forward kernel, then the
@@ -1114,8 +1144,9 @@ petalo_mlem* create_context(const petalo_fov_t* fov, const petalo_tof_t* tof, co
         c->bp64.alloc(nv);
         fill_f64(c->bp64.p, 0.0, nv, c->stream);
         uint64_t max_runs = 1;
+        SetupScratch scratch;
         for (size_t sub = 0; sub < n_subsets; ++sub) {
-          build_runs(c, sub, sub * c->set_size, c->set_size);
+          build_runs(c, sub, sub * c->set_size, c->set_size, &scratch);
           max_runs = std::max<uint64_t>(max_runs, c->n_runs[sub]);
         }
         c->partial.alloc(max_runs);
@@ -2299,14 +2330,6 @@ __global__ __launch_bounds__(1024) void k_seg_items(const uint32_t* __restrict__
 // every step instead of jumping from tile to tile).
 using LorSource = std::function<void(Lor*, uint64_t, uint32_t, hipStream_t)>;
 
-// A device buffer that only grows.
-template <class T> struct Grow {
-  DevBuf<T> b;
-  T* ensure(size_t n) {
-    if (n > b.n) b.alloc(n + n / 4);
-    return b.p;
-  }
-};
 
 // Device buffers and stream of the tiled sensitivity image, kept per device
 // between calls: the batch buffers of 2^24 LORs and the runs of a batch
