@@ -346,8 +346,8 @@ int petalo_make_sensitivity_image_discrete(const petalo_fov_t* att_fov, const fl
                                            const petalo_fov_t* out_fov_or_null, float detector_length_mm,
                                            float r_min_mm, float dr_mm, float dz_mm, float da_mm, int32_t adjust,
                                            uint64_t seed, float* out, petalo_stats_t* stats_or_null);
-/* The two calls above keep their device buffers and stream (a few GB for
- * 2^24-LOR batches: the LORs, their runs and the scans) per device between
+/* The two calls above keep their device buffers and stream (about 10 GB for
+ * 2^25-LOR batches at 120^3: the LORs, their runs and the scans) per device between
  * calls, which saves about 4 ms of allocation and release per call.  No
  * counterpart in the reference (make_sensitivity_image runs once per
  * process there): frees every kept set, on every device.  Calls running at

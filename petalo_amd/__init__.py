@@ -609,7 +609,7 @@ def make_sensitivity_image(attenuation, n_lors, detector_length=None, detector_d
 
 def release_workspaces():
     """Free the device buffers the sensitivity calls keep between calls (one
-    set per device, a few GB for their 2^24-LOR batches)."""
+    set per device, about 10 GB for their 2^25-LOR batches)."""
     _check(lib().petalo_release_workspaces())
 
 

@@ -2211,7 +2211,7 @@ int petalo_sensitivity_lors(const petalo_fov_t* fov, size_t n_lors, uint64_t see
 namespace {
 
 // Sensitivity image on the tiled scheme, LORs in batches of kSensBatch.
-constexpr uint32_t kSensBatch = 1u << 24;
+constexpr uint32_t kSensBatch = 1u << 25;
 // Work item size of the sensitivity kernels, in runs: each item clears and
 // flushes its whole tile, so items are larger than the reconstruction's.
 constexpr uint32_t kSensItemRuns = 16384;
@@ -2332,9 +2332,9 @@ using LorSource = std::function<void(Lor*, uint64_t, uint32_t, hipStream_t)>;
 
 
 // Device buffers and stream of the tiled sensitivity image, kept per device
-// between calls: the batch buffers of 2^24 LORs and the runs of a batch
-// come to a few GB, and allocating and freeing them took about 4 ms of a
-// 50 ms call.  A call leases its device's set (SensLease), or a fresh one
+// between calls: the batch buffers of 2^25 LORs and the runs of a batch
+// come to about 10 GB at 120^3, and allocating and freeing them took about
+// 4 ms of a 50 ms call.  A call leases its device's set (SensLease), or a fresh one
 // when a concurrent call holds it; a set given back when the slot is taken,
 // or after an error, is freed.  petalo_release_workspaces frees the kept
 // sets.
@@ -2565,7 +2565,7 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
                                                       shift, cnt);
     }
     HIP_CHECK(hipGetLastError());
-    // unbounded walks collect in irr over the batches (a handful per 2^24
+    // unbounded walks collect in irr over the batches (a handful per 2^25
     // random LORs, and the one-lane kernel costs its longest walk whatever
     // their number): projected when irr could not take another batch's, and
     // after the last batch

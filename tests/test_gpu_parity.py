@@ -310,7 +310,7 @@ def test_sensitivity_batches_agree_with_one_lane_path(gpu):
     att = P.FOV([180.0] * 3, [30] * 3)
     rho = P.synth_density(att)
     mu = P.density_to_attenuation(rho, 0.096)
-    n, seed, L, D = (1 << 24) + 4099, 21, 300.0, 700.0
+    n, seed, L, D = (1 << 25) + 4099, 21, 300.0, 700.0
     a, sa = P.make_sensitivity_image(mu, n, L, D, seed=seed, return_stats=True)
     b, sb = P.make_sensitivity_image(mu, n, L, D, output_fov=att, seed=seed, return_stats=True)
     e = rel_l2(a.data, b.data)
@@ -388,7 +388,7 @@ def test_discrete_sensitivity_matches_oracle(gpu, adjust):
 
 
 def test_discrete_sensitivity_two_fovs_and_batches(gpu):
-    """More pairs than one batch of the tiled path (2^24): the tiled image
+    """More pairs than one batch of the tiled path (2^25): the tiled image
     agrees with the one-lane path (selected by an explicit output FOV)."""
     att = P.FOV([120.0] * 3, [24] * 3)
     mu = P.density_to_attenuation(P.synth_density(att), 0.096)
@@ -397,7 +397,7 @@ def test_discrete_sensitivity_two_fovs_and_batches(gpu):
     b, sb = P.make_sensitivity_image_discrete(mu, 100.0, output_fov=att, return_stats=True, seed=3, **geom)
     e = rel_l2(a.data, b.data)
     _say(f"discrete sensitivity {sa['n_lors']} pairs: tiled vs one-lane relL2 {e:.2e}")
-    assert sa["n_lors"] > (1 << 24) and sa["n_lors"] == sb["n_lors"]
+    assert sa["n_lors"] > (1 << 25) and sa["n_lors"] == sb["n_lors"]
     assert sa["n_steps"] == sb["n_steps"] and sa["n_nz"] == sb["n_nz"]
     assert e <= 1e-6
 
