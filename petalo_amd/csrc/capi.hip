@@ -2254,14 +2254,31 @@ __global__ void k_gather_lors_by(const Lor* __restrict__ in, const uint32_t* __r
   if (j < n) out[j] = in[idx[j]];
 }
 
+// irr holds cap LORs; *n_irr counts every unbounded walk taken, also those
+// past cap (the caller then takes the batch's again, k_retake_irregular).
 __global__ void k_take_irregular(uint32_t* __restrict__ counts, uint32_t n, const Lor* __restrict__ lors,
-                                 const uint32_t* __restrict__ order, Lor* __restrict__ irr,
+                                 const uint32_t* __restrict__ order, Lor* __restrict__ irr, uint32_t cap,
                                  uint32_t* __restrict__ n_irr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == n) counts[n] = 0;
   if (i >= n || counts[i] != 0xffffffffu) return;
   counts[i] = 0;
-  irr[atomicAdd(n_irr, 1u)] = lors[order ? order[i] : i];
+  const uint32_t at = atomicAdd(n_irr, 1u);
+  if (at < cap) irr[at] = lors[order ? order[i] : i];
+}
+
+// The unbounded walks of LORs 0 .. n - 1 (lors[order[i]]) found again from
+// the walk setup, as the binning kernels flag them, into irr (room for cap).
+__global__ void k_retake_irregular(const Lor* __restrict__ lors, const uint32_t* __restrict__ order, uint32_t n,
+                                   DevFov f, Lor* __restrict__ irr, uint32_t cap, uint32_t* __restrict__ n_irr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Lor l = load_lor(lors, order ? order[i] : i);
+  WalkC c;
+  WalkS w;
+  if (!walk_setup(l, f, c, w) || bounded(w)) return;
+  const uint32_t at = atomicAdd(n_irr, 1u);
+  if (at < cap) irr[at] = l;
 }
 
 
@@ -2504,9 +2521,10 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
   // by length, the segment binning reads lraw through the sort's order and needs no copy in lb
   Lor* lb = ws.lb.ensure(by_length && segs ? 0 : bmax);
   Lor* lraw = ws.lraw.ensure(by_length ? bmax : 0);
-  // unbounded walks of several batches: room for one batch's and 2^16 more
-  const size_t irr_cap = bmax + (1u << 16);
-  Lor* irr = ws.irr.ensure(irr_cap);
+  // unbounded walks of several batches (a handful per 2^25 random LORs):
+  // grown when a batch brings more than the room left
+  Lor* irr = ws.irr.ensure(1u << 16);
+  size_t irr_cap = ws.irr.b.n;
   uint32_t* counts = ws.counts.ensure(bmax + 1);
   uint32_t* offs = ws.offs.ensure(bmax + 1);
   uint32_t* n_irr = ws.n_irr.ensure(1);
@@ -2565,18 +2583,12 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
                                                       shift, cnt);
     }
     HIP_CHECK(hipGetLastError());
-    // unbounded walks collect in irr over the batches (a handful per 2^25
-    // random LORs, and the one-lane kernel costs its longest walk whatever
-    // their number): projected when irr could not take another batch's, and
-    // after the last batch
-    if (irr_pending + b > irr_cap) {
-      k_sens<false, false><<<grid_for(irr_pending), kBlock, 0, s>>>(irr, irr_pending, 0, 0.0f, 0.0f, f,
-                                                                     mu_zero ? nullptr : dmu, f, acc, cnt);
-      HIP_CHECK(hipGetLastError());
-      irr_pending = 0;
-    }
+    // unbounded walks collect in irr over the batches (the one-lane kernel
+    // costs its longest walk whatever their number) and are projected after
+    // the last batch, or when a batch brings more than irr has room for
     if (irr_pending == 0) HIP_CHECK(hipMemsetAsync(n_irr, 0, 4, s));
-    k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts, b, src_lors, lor_order, irr, n_irr);
+    k_take_irregular<<<grid_for((size_t)b + 1), kBlock, 0, s>>>(counts, b, src_lors, lor_order, irr,
+                                                                 (uint32_t)irr_cap, n_irr);
     HIP_CHECK(hipGetLastError());
     uint32_t total = 0, h_irr = 0;
     if (!segs || !mu_zero) {
@@ -2643,6 +2655,28 @@ void sens_tiled(SensWork& ws, const petalo_fov_t& fov, const DevFov& f, bool mu_
         k_scatter<false><<<cap, 1024, (size_t)g.tvox * sizeof(double), s>>>(un, items, mu_zero ? nullptr : rbuf, f,
                                                                             notof, g, acc, order, rstat);
       HIP_CHECK(hipGetLastError());
+    }
+    if (h_irr > irr_cap) {
+      // irr overflowed: project the earlier batches' walks (intact below
+      // irr_pending), make room for this batch's and take them again
+      if (irr_pending) {
+        k_sens<false, false><<<grid_for(irr_pending), kBlock, 0, s>>>(irr, irr_pending, 0, 0.0f, 0.0f, f,
+                                                                       mu_zero ? nullptr : dmu, f, acc, cnt);
+        HIP_CHECK(hipGetLastError());
+      }
+      const uint32_t mine = h_irr - irr_pending;
+      HIP_CHECK(hipStreamSynchronize(s));  // before irr is reallocated
+      irr = ws.irr.ensure(mine);
+      irr_cap = ws.irr.b.n;
+      n_irr = ws.n_irr.ensure(1);
+      HIP_CHECK(hipMemsetAsync(n_irr, 0, 4, s));
+      k_retake_irregular<<<grid_for(b), kBlock, 0, s>>>(src_lors, lor_order, b, f, irr, (uint32_t)irr_cap, n_irr);
+      HIP_CHECK(hipGetLastError());
+      uint32_t again = 0;
+      HIP_CHECK(hipMemcpyAsync(&again, n_irr, 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      if (again != mine) throw std::runtime_error("sensitivity: unbounded walks counted differently when taken again");
+      h_irr = mine;
     }
     irr_pending = h_irr;  // the running count
   }

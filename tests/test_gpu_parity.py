@@ -304,6 +304,29 @@ def test_sensitivity_irregular_lors(gpu):
     assert rel_l2(img.data, ref) <= 1e-5
 
 
+def test_sensitivity_many_unbounded_walks(gpu):
+    """500 000 given LORs in planes of constant z (walks the binning cannot
+    bound, projected one lane per LOR; about a fifth of them cross the FOV)
+    among 100 000 ordinary ones: more than the 2^16 that the buffer of such
+    walks holds at first, so it grows and they are taken again.  Image and counters agree with the one-lane
+    path (selected by an explicit output FOV)."""
+    att = P.FOV([180.0] * 3, [30] * 3)
+    mu = P.density_to_attenuation(P.synth_density(att), 0.096)
+    rng = np.random.default_rng(19)
+    flat = random_lors(rng, 500_000, radius=350.0, zhalf=80.0)
+    flat["p2"][:, 2] = flat["p1"][:, 2]
+    lors = np.concatenate([random_lors(rng, 100_000, radius=350.0, zhalf=80.0), flat])
+    for m in (mu, P.Image(att, np.zeros(att.nvox, np.float32))):
+        a, sa = P.make_sensitivity_image(m, len(lors), lors=lors, return_stats=True)
+        b, sb = P.make_sensitivity_image(m, len(lors), lors=lors, output_fov=att, return_stats=True)
+        e = rel_l2(a.data, b.data)
+        _say(f"unbounded walks: tiled vs one-lane relL2 {e:.2e}; steps {sa['n_steps']} {sb['n_steps']}, "
+             f"missed {sa['n_missed']} of {len(lors)}")
+        for k in ("n_steps", "n_nz", "n_skipped", "n_missed"):
+            assert sa[k] == sb[k], k
+        assert e <= 1e-6
+
+
 def test_sensitivity_batches_agree_with_one_lane_path(gpu):
     """More LORs than one batch of the tiled path: it agrees with the
     one-lane kernel (selected by passing the output FOV explicitly)."""
